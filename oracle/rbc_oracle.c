@@ -1,0 +1,933 @@
+/*
+ * rbc_oracle.c -- FP64 CPU restatement of gpu_flow's Rayleigh-Benard time stepper.
+ *
+ * TEST INFRASTRUCTURE ONLY (parity checker and timed CPU baseline).  The product library
+ * never links or calls this file.
+ *
+ * What it restates, function by function (paths relative to gpu_flow/code/):
+ *   orc_lambda              util/coeff.cu:1232-1257 (12-digit roots for k<=8, else pi(k+1/2))
+ *   orc_coeff_tables        util/coeff.h:17-67 definitions (integrals over z in [-1/2,1/2]),
+ *                           evaluated by Gauss-Legendre quadrature, plus the reference's
+ *                           derived tables Ibbb/Iaad (coeff.cu:1197-1209) and its Iac quirk
+ *                           (even-even off-diagonals identically zero, coeff.cu:112).
+ *   orc_nonlinear           operator/nonlinear_operator_rayleigh_noslip.cu:83-1394, 1426-1650
+ *                           (the O(L^2) pairwise-product form, 20 products per (i,j) pair)
+ *   orc_inverse_B           operator/inverse_B_matrix.cu:3-62, 71-173, 199-328
+ *   orc_implicit            operator/linear_implicit_operator.cu:12-227, 242-322, 350-490
+ *   orc_step                timestepping/implicit_explicit_stepping.cu:32-97 + util/util.cu
+ *                           make_real (order of main.cu:612-616)
+ *   orc_dealias             timestepping/implicit_explicit_stepping.cu:163-182
+ *   orc_make_real           util/util.cu make_real / make_real_signal
+ *   orc_fft_*               cuFFT 2-D real transforms as used by the reference (plan2d(N,M),
+ *                           C2R unnormalised, R2C forward); radix-2, written here.
+ *   GSL LU (gsl_linalg_LU_decomp/_invert) is restated as partial-pivot Doolittle LU.
+ */
+#define _GNU_SOURCE
+#include "rbc_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex cplx;
+
+/* ======================================================================================== */
+/* lambda_k (coeff.cu:1232-1257)                                                            */
+/* ======================================================================================== */
+double orc_lambda(int k) {
+    /* The roots of the Chandrasekhar boundary condition, as the reference tabulates them
+     * (lambda_k / pi to 12 digits for k <= 8; the asymptote pi (k + 1/2) beyond). */
+    static const double roots_over_pi[9] = {0.0,            1.505618731142, 2.499752670074,
+                                            3.500010679436, 4.499999538484, 5.500000019944,
+                                            6.499999999138, 7.500000000037, 8.499999999998};
+    if ((unsigned)k > 8u) return M_PI * (k + 0.5);
+    return M_PI * roots_over_pi[k];
+}
+
+/* C_k and derivatives (coeff.h:31-36; coeff.cu chandra*), 1-based k. */
+static void chandra_all(int k, double z, double out[4]) {
+    double l = orc_lambda(k);
+    if (k % 2 != 0) {
+        double ch = cosh(0.5 * l), c = cos(0.5 * l);
+        double a = cosh(l * z) / ch, b = cos(l * z) / c;
+        double as = sinh(l * z) / ch, bs = sin(l * z) / c;
+        out[0] = a - b;
+        out[1] = l * (as + bs);
+        out[2] = l * l * (a + b);
+        out[3] = l * l * l * (as - bs);
+    } else {
+        double sh = sinh(0.5 * l), s = sin(0.5 * l);
+        double a = sinh(l * z) / sh, b = sin(l * z) / s;
+        double ac = cosh(l * z) / sh, bc = cos(l * z) / s;
+        out[0] = a - b;
+        out[1] = l * (ac - bc);
+        out[2] = l * l * (a + b);
+        out[3] = l * l * l * (ac + bc);
+    }
+}
+
+/* S_k = sin(k pi (z+1/2)) and derivatives (coeff.h:27). */
+static void sine_all(int k, double z, double out[3]) {
+    double kp = k * M_PI, a = kp * (z + 0.5);
+    out[0] = sin(a);
+    out[1] = kp * cos(a);
+    out[2] = -kp * kp * sin(a);
+}
+
+int orc_gauss_legendre(int n, double* z, double* w) {
+    if (n < 1) return -1;
+    for (int i = 0; i < n; ++i) {
+        double x = cos(M_PI * (i + 0.75) / (n + 0.5));
+        double dp = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = x;
+            for (int k = 2; k <= n; ++k) {
+                double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            if (n == 1) { p1 = x; p0 = 1.0; }
+            dp = n * (x * p1 - p0) / (x * x - 1.0);
+            double dx = p1 / dp;
+            x -= dx;
+            if (fabs(dx) < 1e-16) break;
+        }
+        double p0 = 1.0, p1 = x;
+        for (int k = 2; k <= n; ++k) {
+            double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        if (n == 1) { p1 = x; p0 = 1.0; }
+        dp = n * (x * p1 - p0) / (x * x - 1.0);
+        /* node x_i descending in i; store ascending */
+        z[n - 1 - i] = 0.5 * x;
+        w[n - 1 - i] = 0.5 * 2.0 / ((1.0 - x * x) * dp * dp);
+    }
+    return 0;
+}
+
+int orc_coeff_tables(int L, double* two, double* three) {
+    if (L < 1 || L > 128) return -1;
+    const int nq = 4 * L + 64;
+    double* z = malloc(sizeof(double) * nq);
+    double* w = malloc(sizeof(double) * nq);
+    /* basis values: a=C, b=C', c=C'', d=C''', s1=S, s2=S' */
+    double* B = malloc(sizeof(double) * 6 * (size_t)L * nq);
+    if (!z || !w || !B) { free(z); free(w); free(B); return -2; }
+    orc_gauss_legendre(nq, z, w);
+    double *a = B, *b = B + (size_t)L * nq, *c = B + 2 * (size_t)L * nq, *d = B + 3 * (size_t)L * nq;
+    double *s1 = B + 4 * (size_t)L * nq, *s2 = B + 5 * (size_t)L * nq;
+    for (int m = 0; m < L; ++m)
+        for (int q = 0; q < nq; ++q) {
+            double cv[4], sv[3];
+            chandra_all(m + 1, z[q], cv);
+            sine_all(m + 1, z[q], sv);
+            a[m * nq + q] = cv[0]; b[m * nq + q] = cv[1];
+            c[m * nq + q] = cv[2]; d[m * nq + q] = cv[3];
+            s1[m * nq + q] = sv[0]; s2[m * nq + q] = sv[1];
+        }
+    const size_t LL = (size_t)L * L, LLL = LL * L;
+    /* 2-factor: Iac = <C_i C''_j>, Iab = <C_i C'_j>, Ia1 = <C_i S_j>, Ia2 = <C_i S'_j> */
+    const double* p2[ORC_NTWO][2] = {{a, c}, {a, b}, {a, s1}, {a, s2}};
+    for (int t = 0; t < ORC_NTWO; ++t)
+        for (int j = 0; j < L; ++j)
+            for (int i = 0; i < L; ++i) {
+                double s = 0.0;
+                for (int q = 0; q < nq; ++q) s += w[q] * p2[t][0][i * nq + q] * p2[t][1][j * nq + q];
+                two[t * LL + i + (size_t)j * L] = s;
+            }
+    /* Iac and Ia1 feed the per-wavenumber linear operators.  The reference evaluates them in
+     * closed form (coeff.cu:77-116 and :146-178); those forms assume lambda_k is an exact
+     * root, which the 12-digit table is not, so they differ from the integrals at the 1e-12
+     * level.  They are restated here as written.  Iac quirk (coeff.cu:112): the even-even
+     * off-diagonal entries are computed from (lambdacotj - lambdacotj) and are exactly 0. */
+    {
+        double* Iac = two + ORC_IAC * LL;
+        double* Ia1 = two + ORC_IA1 * LL;
+        memset(Iac, 0, sizeof(double) * LL);
+        memset(Ia1, 0, sizeof(double) * LL);
+        for (int i = 1; i <= L; i += 2) {
+            double li = orc_lambda(i), lti = li * tan(0.5 * li), li2 = li * li;
+            Iac[(i - 1) + (size_t)(i - 1) * L] = -2.0 * lti - lti * lti;
+            for (int j = i + 2; j <= L; j += 2) {
+                double lj = orc_lambda(j), ltj = lj * tan(0.5 * lj), lj2 = lj * lj;
+                double rij = 8.0 * li2 * lj2 * (ltj - lti) / (li2 * li2 - lj2 * lj2);
+                Iac[(i - 1) + (size_t)(j - 1) * L] = rij;
+                Iac[(j - 1) + (size_t)(i - 1) * L] = rij;
+            }
+        }
+        for (int i = 2; i <= L; i += 2) {
+            double li = orc_lambda(i), lci = li / tan(0.5 * li);
+            Iac[(i - 1) + (size_t)(i - 1) * L] = 2.0 * lci - lci * lci;
+            /* off-diagonal even-even entries stay 0 (the quirk) */
+        }
+        for (int i = 1; i <= L; i += 2) {
+            double zi = orc_lambda(i) / M_PI, zi2 = zi * zi;
+            for (int j = 1; j <= L; j += 2) {
+                int j2 = j * j;
+                Ia1[(i - 1) + (size_t)(j - 1) * L] = -4.0 * j * zi2 / (M_PI * (j2 * j2 - zi2 * zi2));
+            }
+        }
+        for (int i = 2; i <= L; i += 2) {
+            double zi = orc_lambda(i) / M_PI, zi2 = zi * zi;
+            for (int j = 2; j <= L; j += 2) {
+                int j2 = j * j;
+                Ia1[(i - 1) + (size_t)(j - 1) * L] = 4.0 * j * zi2 / (M_PI * (j2 * j2 - zi2 * zi2));
+            }
+        }
+    }
+    /* The reference tables are only filled for the parity combinations its loops visit;
+     * entries outside them are identically zero (they vanish analytically as well). */
+
+    /* 3-factor tables (direct ones) */
+    const double* p3[ORC_NTHREE][3] = {
+        {a, c, s1},    /* Iac1 = <C_i C''_j S_k> */
+        {b, b, s1},    /* Ibb1 = <C'_i C'_j S_k> */
+        {a, a, b},     /* Iaab = <C_i C_j C'_k> */
+        {NULL, NULL, NULL}, /* Iaad: derived below */
+        {a, b, c},     /* Iabc = <C_i C'_j C''_k> */
+        {a, s1, s1},   /* Ia11 */
+        {a, s1, s2},   /* Ia12 = <C_i S_j S'_k> */
+        {s1, s1, s1},  /* I111 */
+        {NULL, NULL, NULL}, /* Ibbb: derived below */
+        {b, s1, s1},   /* Ib11 = <C'_i S_j S_k> */
+        {a, a, s1},    /* Iaa1 = <C_i C_j S_k> */
+        {NULL, NULL, NULL}, /* Iab2: derived below (rab2 = -rbb1 - rac1, coeff.cu:729) */
+    };
+    (void)d;
+    for (int t = 0; t < ORC_NTHREE; ++t) {
+        if (!p3[t][0]) continue;
+        double* T = three + t * LLL;
+        for (int k = 0; k < L; ++k)
+            for (int j = 0; j < L; ++j)
+                for (int i = 0; i < L; ++i) {
+                    const double *x = p3[t][0] + i * nq, *y = p3[t][1] + j * nq, *u = p3[t][2] + k * nq;
+                    double s = 0.0;
+                    for (int q = 0; q < nq; ++q) s += w[q] * x[q] * y[q] * u[q];
+                    T[i + (size_t)j * L + (size_t)k * LL] = s;
+                }
+    }
+    double* Iabc = three + ORC_IABC * LLL;
+    for (int k = 0; k < L; ++k)
+        for (int j = 0; j < L; ++j)
+            for (int i = 0; i < L; ++i) {
+                size_t ijk = i + (size_t)j * L + (size_t)k * LL;
+                /* Iaad(i,j,k) = -Iabc(i,j,k) - Iabc(j,i,k)   (coeff.cu:1197-1200) */
+                three[ORC_IAAD * LLL + ijk] = -Iabc[ijk] - Iabc[j + (size_t)i * L + (size_t)k * LL];
+                /* Ibbb(i,j,k) = -Iabc(i,j,k) - Iabc(i,k,j)   (coeff.cu:1206-1209) */
+                three[ORC_IBBB * LLL + ijk] = -Iabc[ijk] - Iabc[i + (size_t)k * L + (size_t)j * LL];
+                /* Iab2 = -Ibb1 - Iac1 (coeff.cu:729 and siblings) */
+                three[ORC_IAB2 * LLL + ijk] = -three[ORC_IBB1 * LLL + ijk] - three[ORC_IAC1 * LLL + ijk];
+            }
+    free(z); free(w); free(B);
+    return 0;
+}
+
+/* ======================================================================================== */
+/* FFT: radix-2 complex, and the 2-D real transforms (cuFFT plan2d(N, M) semantics)          */
+/* ======================================================================================== */
+static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+/* in-place complex FFT of length n with stride 1; sign -1 forward, +1 inverse; unnormalised */
+static void fft1(cplx* x, int n, int sign) {
+    for (int i = 1, j = 0; i < n; ++i) {
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) { cplx t = x[i]; x[i] = x[j]; x[j] = t; }
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        int h = len >> 1;
+        for (int k = 0; k < h; ++k) {
+            double ang = sign * 2.0 * M_PI * k / len;
+            cplx wk = cos(ang) + I * sin(ang);
+            for (int i = 0; i < n; i += len) {
+                cplx u = x[i + k], v = x[i + k + h] * wk;
+                x[i + k] = u + v;
+                x[i + k + h] = u - v;
+            }
+        }
+    }
+}
+
+int orc_fft_r2c(int M, int N, const double* phys, double* spec_) {
+    if (!is_pow2(M) || !is_pow2(N)) return -1;
+    const int mox = M / 2 + 1;
+    cplx* spec = (cplx*)spec_;
+    cplx* row = malloc(sizeof(cplx) * (M > N ? M : N));
+    for (int y = 0; y < N; ++y) {
+        for (int x = 0; x < M; ++x) row[x] = phys[(size_t)y * M + x];
+        fft1(row, M, -1);
+        for (int k = 0; k < mox; ++k) spec[(size_t)y * mox + k] = row[k];
+    }
+    for (int k = 0; k < mox; ++k) {
+        for (int y = 0; y < N; ++y) row[y] = spec[(size_t)y * mox + k];
+        fft1(row, N, -1);
+        for (int y = 0; y < N; ++y) spec[(size_t)y * mox + k] = row[y];
+    }
+    free(row);
+    return 0;
+}
+
+int orc_fft_c2r(int M, int N, const double* spec_, double* phys) {
+    if (!is_pow2(M) || !is_pow2(N)) return -1;
+    const int mox = M / 2 + 1;
+    const cplx* spec = (const cplx*)spec_;
+    cplx* tmp = malloc(sizeof(cplx) * (size_t)mox * N);
+    cplx* row = malloc(sizeof(cplx) * (M > N ? M : N));
+    for (int k = 0; k < mox; ++k) {
+        for (int y = 0; y < N; ++y) row[y] = spec[(size_t)y * mox + k];
+        fft1(row, N, +1);
+        for (int y = 0; y < N; ++y) tmp[(size_t)y * mox + k] = row[y];
+    }
+    for (int y = 0; y < N; ++y) {
+        /* Hermitian extension along x; the imaginary parts of X_0 and X_{M/2} do not reach
+         * the real output (standard C2R). */
+        for (int k = 0; k < mox; ++k) row[k] = tmp[(size_t)y * mox + k];
+        for (int k = 1; k < M / 2; ++k) row[M - k] = conj(row[k]);
+        fft1(row, M, +1);
+        for (int x = 0; x < M; ++x) phys[(size_t)y * M + x] = creal(row[x]);
+    }
+    free(tmp);
+    free(row);
+    return 0;
+}
+
+/* ======================================================================================== */
+/* Small dense LU (gsl_linalg_LU_decomp / gsl_linalg_LU_invert restated)                     */
+/* ======================================================================================== */
+/* a: n x n row-major, replaced by its inverse.  Returns 0, or -1 if singular. */
+static int lu_invert(int n, double* a, double* work /* n*n */, int* perm) {
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int j = 0; j < n - 1; ++j) {
+        int p = j;
+        double amax = fabs(a[j * n + j]);
+        for (int i = j + 1; i < n; ++i)
+            if (fabs(a[i * n + j]) > amax) { amax = fabs(a[i * n + j]); p = i; }
+        if (p != j) {
+            for (int c = 0; c < n; ++c) { double t = a[j * n + c]; a[j * n + c] = a[p * n + c]; a[p * n + c] = t; }
+            int t = perm[j]; perm[j] = perm[p]; perm[p] = t;
+        }
+        double piv = a[j * n + j];
+        if (piv != 0.0)
+            for (int i = j + 1; i < n; ++i) {
+                double l = a[i * n + j] / piv;
+                a[i * n + j] = l;
+                for (int c = j + 1; c < n; ++c) a[i * n + c] -= l * a[j * n + c];
+            }
+    }
+    for (int i = 0; i < n; ++i)
+        if (a[i * n + i] == 0.0) return -1;
+    /* solve LU x = P e_col for each column */
+    for (int col = 0; col < n; ++col) {
+        double* x = work + col * n; /* column stored contiguously */
+        for (int i = 0; i < n; ++i) x[i] = (perm[i] == col) ? 1.0 : 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < i; ++k) x[i] -= a[i * n + k] * x[k];
+        for (int i = n - 1; i >= 0; --i) {
+            for (int k = i + 1; k < n; ++k) x[i] -= a[i * n + k] * x[k];
+            x[i] /= a[i * n + i];
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int col = 0; col < n; ++col) a[i * n + col] = work[col * n + i];
+    return 0;
+}
+
+/* ======================================================================================== */
+/* Context                                                                                  */
+/* ======================================================================================== */
+struct orc_ctx {
+    int M, N, L, mox, nthreads;
+    double Lx, Ly, Ra, Pr, dt;
+    size_t plane, field; /* mox*N, mox*N*L complex */
+    double* two;
+    double* three;
+    cplx *th, *f, *g, *F, *G;          /* state */
+    cplx *r1[5];                        /* AB2 history */
+    int have_hist;
+    double *imp;  /* per (col,row): 4*L*L implicit blocks TT, Tf, fT, ff */
+    double *invBf; /* per (col,row): L*L */
+};
+
+#define TWO(c, t, i, j) ((c)->two[(size_t)(t) * (c)->L * (c)->L + (i) + (size_t)(j) * (c)->L])
+#define THREE(c, t, i, j, k) \
+    ((c)->three[(size_t)(t) * (c)->L * (c)->L * (c)->L + (i) + (size_t)(j) * (c)->L + (size_t)(k) * (c)->L * (c)->L])
+
+static double delta(int a, int b) { return a == b ? 1.0 : 0.0; }
+
+/* Builds the per-wavenumber factors exactly as linear_implicit_operator.cu:47-214 and
+ * inverse_B_matrix.cu:98-164 do (FP64, PRECISION_DOUBLE build). */
+static int build_factors(orc_ctx* c) {
+    const int L = c->L, n2 = 2 * L;
+    double* ident = malloc(sizeof(double) * n2 * n2);
+    double* invB = malloc(sizeof(double) * n2 * n2);
+    double* tmpB = malloc(sizeof(double) * n2 * n2);
+    double* Lm = malloc(sizeof(double) * n2 * n2);
+    double* C = malloc(sizeof(double) * n2 * n2);
+    double* work = malloc(sizeof(double) * n2 * n2);
+    int* perm = malloc(sizeof(int) * n2);
+    int rc = 0;
+    for (int k = 0; k < c->mox && !rc; ++k) {
+        double mode_x = k * 2.0 * M_PI / c->Lx;
+        for (int l = 0; l < c->N && !rc; ++l) {
+            double mode_y = 2.0 * M_PI / c->Ly;
+            if (l < (c->N / 2) + 1) mode_y *= l;
+            else mode_y *= (c->N - l);
+            double mode_sq = mode_x * mode_x + mode_y * mode_y;
+            size_t pidx = (size_t)l * c->mox + k;
+            /* ---------------- inverse_B_matrix (f block) ---------------- */
+            double* ib = c->invBf + pidx * L * L;
+            if (k == 0 && l == 0) {
+                memset(ib, 0, sizeof(double) * L * L);
+            } else {
+                for (int m = 0; m < L; ++m)
+                    for (int m1 = 0; m1 < L; ++m1)
+                        ib[m * L + m1] = mode_sq * (mode_sq * delta(m, m1) - TWO(c, ORC_IAC, m, m1)) / c->Pr;
+                if (lu_invert(L, ib, work, perm)) rc = -3;
+            }
+            /* ---------------- linear_implicit_operator ---------------- */
+            memset(ident, 0, sizeof(double) * n2 * n2);
+            for (int i = 0; i < n2; ++i) ident[i * n2 + i] = 1.0;
+            memset(invB, 0, sizeof(double) * n2 * n2);
+            if (k == 0 && l == 0) {
+                for (int m = L; m < n2; ++m) ident[m * n2 + m] = 0.0;
+                for (int m = 0; m < L; ++m) invB[m * n2 + m] = 2.0;
+            } else {
+                memset(tmpB, 0, sizeof(double) * n2 * n2);
+                for (int m = 0; m < L; ++m) {
+                    tmpB[m * n2 + m] = 0.5;
+                    for (int m1 = 0; m1 < L; ++m1)
+                        tmpB[(m + L) * n2 + m1 + L] = mode_sq * (mode_sq * delta(m, m1) - TWO(c, ORC_IAC, m, m1)) / c->Pr;
+                }
+                memcpy(invB, tmpB, sizeof(double) * n2 * n2);
+                if (lu_invert(n2, invB, work, perm)) rc = -3;
+            }
+            memset(Lm, 0, sizeof(double) * n2 * n2);
+            for (int m = 0; m < L; ++m)
+                for (int m1 = 0; m1 < L; ++m1) {
+                    double pm = (m + 1) * M_PI;
+                    double lam = orc_lambda(m + 1), lam2 = lam * lam;
+                    Lm[m * n2 + m1] = (-0.5) * (mode_sq + pm * pm) * delta(m, m1);
+                    Lm[m * n2 + m1 + L] = mode_sq * TWO(c, ORC_IA1, m1, m);
+                    Lm[(m + L) * n2 + m1] = c->Ra * mode_sq * TWO(c, ORC_IA1, m, m1);
+                    Lm[(m + L) * n2 + m1 + L] =
+                        -mode_sq * ((lam2 * lam2 + mode_sq * mode_sq) * delta(m, m1) - 2.0 * mode_sq * TWO(c, ORC_IAC, m, m1));
+                }
+            /* C = ident - dt * invB * L */
+            for (int i = 0; i < n2; ++i)
+                for (int j = 0; j < n2; ++j) {
+                    double s = 0.0;
+                    for (int q = 0; q < n2; ++q) s += invB[i * n2 + q] * Lm[q * n2 + j];
+                    C[i * n2 + j] = -c->dt * s + ident[i * n2 + j];
+                }
+            double* out = c->imp + pidx * 4 * L * L;
+            if (k == 0 && l == 0) {
+                double* up = tmpB; /* L x L */
+                for (int m = 0; m < L; ++m)
+                    for (int m1 = 0; m1 < L; ++m1) up[m * L + m1] = C[m * n2 + m1];
+                if (lu_invert(L, up, work, perm)) rc = -3;
+                memset(out, 0, sizeof(double) * 4 * L * L);
+                for (int m = 0; m < L; ++m)
+                    for (int m1 = 0; m1 < L; ++m1) out[m * L + m1] = up[m * L + m1];
+            } else {
+                if (lu_invert(n2, C, work, perm)) rc = -3;
+                for (int m = 0; m < L; ++m)
+                    for (int m1 = 0; m1 < L; ++m1) {
+                        out[0 * L * L + m * L + m1] = C[m * n2 + m1];
+                        out[1 * L * L + m * L + m1] = C[m * n2 + m1 + L];
+                        out[2 * L * L + m * L + m1] = C[(m + L) * n2 + m1];
+                        out[3 * L * L + m * L + m1] = C[(m + L) * n2 + m1 + L];
+                    }
+            }
+        }
+    }
+    free(ident); free(invB); free(tmpB); free(Lm); free(C); free(work); free(perm);
+    return rc;
+}
+
+orc_ctx* orc_create(int M, int N, int L, double Lx, double Ly, double Ra, double Pr, double dt,
+                    int nthreads) {
+    if (!is_pow2(M) || !is_pow2(N) || L < 1 || L > 128 || !(Lx > 0) || !(Ly > 0) || !(Pr > 0)) return NULL;
+    orc_ctx* c = calloc(1, sizeof(orc_ctx));
+    if (!c) return NULL;
+    c->M = M; c->N = N; c->L = L; c->mox = M / 2 + 1;
+    c->Lx = Lx; c->Ly = Ly; c->Ra = Ra; c->Pr = Pr; c->dt = dt;
+    c->nthreads = nthreads > 0 ? nthreads : 1;
+    c->plane = (size_t)c->mox * N;
+    c->field = c->plane * L;
+    c->two = calloc((size_t)ORC_NTWO * L * L, sizeof(double));
+    c->three = calloc((size_t)ORC_NTHREE * L * L * L, sizeof(double));
+    c->th = calloc(c->field, sizeof(cplx));
+    c->f = calloc(c->field, sizeof(cplx));
+    c->g = calloc(c->field, sizeof(cplx));
+    c->F = calloc(L, sizeof(cplx));
+    c->G = calloc(L, sizeof(cplx));
+    for (int i = 0; i < 5; ++i) c->r1[i] = calloc(i < 3 ? c->field : (size_t)L, sizeof(cplx));
+    c->imp = calloc(c->plane * 4 * L * L, sizeof(double));
+    c->invBf = calloc(c->plane * L * L, sizeof(double));
+    if (!c->two || !c->three || !c->th || !c->f || !c->g || !c->F || !c->G || !c->imp || !c->invBf ||
+        !c->r1[0] || !c->r1[4] || orc_coeff_tables(L, c->two, c->three) || build_factors(c)) {
+        orc_destroy(c);
+        return NULL;
+    }
+    return c;
+}
+
+void orc_destroy(orc_ctx* c) {
+    if (!c) return;
+    free(c->two); free(c->three); free(c->th); free(c->f); free(c->g); free(c->F); free(c->G);
+    for (int i = 0; i < 5; ++i) free(c->r1[i]);
+    free(c->imp); free(c->invBf);
+    free(c);
+}
+
+int orc_set_state(orc_ctx* c, const double* th, const double* f, const double* g, const double* F,
+                  const double* G) {
+    if (th) memcpy(c->th, th, sizeof(cplx) * c->field);
+    if (f) memcpy(c->f, f, sizeof(cplx) * c->field);
+    if (g) memcpy(c->g, g, sizeof(cplx) * c->field);
+    if (F) memcpy(c->F, F, sizeof(cplx) * c->L);
+    if (G) memcpy(c->G, G, sizeof(cplx) * c->L);
+    return 0;
+}
+
+int orc_get_state(orc_ctx* c, double* th, double* f, double* g, double* F, double* G) {
+    if (th) memcpy(th, c->th, sizeof(cplx) * c->field);
+    if (f) memcpy(f, c->f, sizeof(cplx) * c->field);
+    if (g) memcpy(g, c->g, sizeof(cplx) * c->field);
+    if (F) memcpy(F, c->F, sizeof(cplx) * c->L);
+    if (G) memcpy(G, c->G, sizeof(cplx) * c->L);
+    return 0;
+}
+
+/* ======================================================================================== */
+/* dealias and make_real                                                                    */
+/* ======================================================================================== */
+void orc_dealias(int columns, int rows, int matrices, double* data_) {
+    cplx* data = (cplx*)data_;
+    for (int m = 0; m < matrices; ++m)
+        for (int r = 0; r < rows; ++r)
+            for (int col = 0; col < columns; ++col) {
+                size_t idx = col + (size_t)r * columns + (size_t)m * columns * rows;
+                if (columns - col <= (1.0 / 3.0) * columns) data[idx] = 0.0;
+                if (fabs(0.5 * rows - r) <= (1.0 / 3.0) * 0.5 * rows) data[idx] = 0.0;
+            }
+}
+
+void orc_make_real(int columns, int rows, int matrices, double* data_) {
+    cplx* data = (cplx*)data_;
+    for (int m = 0; m < matrices; ++m) {
+        cplx* p = data + (size_t)m * columns * rows;
+        p[0] = creal(p[0]);
+        for (int r = 1; r < rows / 2; ++r) p[(size_t)r * columns] = conj(p[(size_t)(rows - r) * columns]);
+        p[(size_t)(rows / 2) * columns] = creal(p[(size_t)(rows / 2) * columns]);
+    }
+}
+
+/* ======================================================================================== */
+/* Nonlinear operator (nonlinear_operator_rayleigh_noslip.cu:83-1394)                       */
+/* ======================================================================================== */
+enum { D_X, D_Y, D_XX, D_YY, D_XY, D_LAP, D_XLAP, D_YLAP, D_ID };
+
+/* Apply one of the reference's horizontal derivative kernels (:1428-1583) to a plane. */
+static void apply_deriv(const orc_ctx* c, int kind, const cplx* in, cplx* out) {
+    const int mox = c->mox, N = c->N;
+    const double Lx = c->Lx, Ly = c->Ly;
+    for (int r = 0; r < N; ++r) {
+        int ky = r;
+        if (ky > N / 2) ky -= N;
+        for (int col = 0; col < mox; ++col) {
+            size_t i = (size_t)r * mox + col;
+            int kx = col;
+            cplx v = in[i];
+            switch (kind) {
+                case D_X: { double fac = kx * (2 * M_PI / Lx); v = I * fac * v; break; }
+                case D_Y: { double fac = ky * (2 * M_PI / Ly); v = I * fac * v; break; }
+                case D_XX: { double fac = kx * (2 * M_PI / Lx); fac = -fac * fac; v *= fac; break; }
+                case D_YY: { double fac = ky * (2 * M_PI / Ly); fac = -fac * fac; v *= fac; break; }
+                case D_XY: { double fac = -4 * M_PI * M_PI * (kx * ky) / (Lx * Ly); v *= fac; break; }
+                case D_LAP: {
+                    double fx = kx / Lx, fy = ky / Ly;
+                    double fac = (-4 * M_PI * M_PI) * (fx * fx + fy * fy);
+                    v *= fac;
+                    break;
+                }
+                case D_XLAP: case D_YLAP: {
+                    double fac = (kind == D_XLAP) ? kx * (2 * M_PI / Lx) : ky * (2 * M_PI / Ly);
+                    v = I * fac * v;
+                    double fx = kx / Lx, fy = ky / Ly;
+                    v *= (-4 * M_PI * M_PI) * (fx * fx + fy * fy);
+                    break;
+                }
+                default: break;
+            }
+            out[i] = v;
+        }
+    }
+}
+
+/* The 20 pairwise products of nonlinear_operator_rayleigh_noslip.cu:293-466, as
+ * (left field, right field) with 0..7 = u11 u21 u31 u12 u22 theta dxtheta dytheta. */
+enum { U11, U21, U31, U12, U22, TH, DXTH, DYTH, NPHYS };
+static const int PROD[20][2] = {
+    {U11, U11}, {U11, U21}, {U11, U31}, {U11, U12}, {U11, U22}, {U21, U21}, {U21, U31},
+    {U21, U12}, {U21, U22}, {U31, U31}, {U31, U12}, {U31, U22}, {U12, U12}, {U12, U22},
+    {U22, U22}, {U11, DXTH}, {U12, DXTH}, {U21, DYTH}, {U22, DYTH}, {U31, TH}};
+enum { P11_11, P11_21, P11_31, P11_12, P11_22, P21_21, P21_31, P21_12, P21_22, P31_31, P31_12,
+       P31_22, P12_12, P12_22, P22_22, P11_DX, P12_DX, P21_DY, P22_DY, P31_TH };
+
+/* Output targets */
+enum { OUT_TH, OUT_F, OUT_G };
+
+/* One summand: out[target][k] += sign*mult * C(i,j,k) * D(P) with C given by a table lookup
+ * whose index order is spelled out per summand (see nonlinear...cu:479-1067). */
+typedef struct { int target, prod, deriv, table, order; double mult; } summand;
+/* order: 0 -> (i,j,k) 1 -> (i,k,j) 2 -> (k,i,j) 3 -> (j,i,k) 4 -> (j,k,i) */
+static const summand SUMS[] = {
+    /* f (:479-849) */
+    {OUT_F, P11_11, D_XX, ORC_IBBB, 0, -1.0},   /* 1 */
+    {OUT_F, P11_12, D_XX, ORC_IBB1, 1, -2.0},   /* 2 */
+    {OUT_F, P12_12, D_XX, ORC_IB11, 2, -1.0},   /* 3 */
+    {OUT_F, P11_21, D_XY, ORC_IBBB, 3, -2.0},   /* 4 */
+    {OUT_F, P21_12, D_XY, ORC_IBB1, 1, -2.0},   /* 5 */
+    {OUT_F, P11_22, D_XY, ORC_IBB1, 1, -2.0},   /* 6 */
+    {OUT_F, P12_22, D_XY, ORC_IB11, 2, -2.0},   /* 7 */
+    {OUT_F, P11_31, D_X, ORC_IABC, 3, 1.0},     /* 8 */
+    {OUT_F, P31_12, D_X, ORC_IAC1, 1, 1.0},     /* 9 */
+    {OUT_F, P21_21, D_YY, ORC_IBBB, 0, -1.0},   /* 14 */
+    {OUT_F, P21_22, D_YY, ORC_IBB1, 1, -2.0},   /* 15 */
+    {OUT_F, P22_22, D_YY, ORC_IB11, 2, -1.0},   /* 16 */
+    {OUT_F, P21_31, D_Y, ORC_IABC, 3, 1.0},     /* 17 */
+    {OUT_F, P31_22, D_Y, ORC_IAC1, 1, 1.0},     /* 18 */
+    {OUT_F, P11_31, D_XLAP, ORC_IAAB, 4, -1.0}, /* 19 */
+    {OUT_F, P31_12, D_XLAP, ORC_IAA1, 1, -1.0}, /* 21 */
+    {OUT_F, P21_31, D_YLAP, ORC_IAAB, 4, -1.0}, /* 22 */
+    {OUT_F, P31_22, D_YLAP, ORC_IAA1, 1, -1.0}, /* 23 */
+    {OUT_F, P31_31, D_LAP, ORC_IAAB, 0, 1.0},   /* 24: C=-Iaab, added as -C */
+    /* g (:858-1067 part) */
+    {OUT_G, P11_11, D_XY, ORC_IBB1, 0, 1.0},    /* 1 */
+    {OUT_G, P11_12, D_XY, ORC_IB11, 0, 2.0},    /* 2 */
+    {OUT_G, P12_12, D_XY, ORC_I111, 0, 1.0},    /* 3 */
+    {OUT_G, P11_21, D_YY, ORC_IBB1, 0, 1.0},    /* 4 */
+    {OUT_G, P21_12, D_YY, ORC_IB11, 0, 1.0},    /* 5 */
+    {OUT_G, P11_22, D_YY, ORC_IB11, 0, 1.0},    /* 6 */
+    {OUT_G, P12_22, D_YY, ORC_I111, 0, 1.0},    /* 7 */
+    {OUT_G, P11_31, D_Y, ORC_IAB2, 3, -1.0},    /* 8 */
+    {OUT_G, P31_12, D_Y, ORC_IA12, 0, -1.0},    /* 9 */
+    {OUT_G, P11_21, D_XX, ORC_IBB1, 0, -1.0},   /* 10 */
+    {OUT_G, P21_12, D_XX, ORC_IB11, 0, -1.0},   /* 11 */
+    {OUT_G, P11_22, D_XX, ORC_IB11, 0, -1.0},   /* 12 */
+    {OUT_G, P12_22, D_XX, ORC_I111, 0, -1.0},   /* 13 */
+    {OUT_G, P21_21, D_XY, ORC_IBB1, 0, -1.0},   /* 14 */
+    {OUT_G, P21_22, D_XY, ORC_IB11, 0, -2.0},   /* 15 */
+    {OUT_G, P22_22, D_XY, ORC_I111, 0, -1.0},   /* 16 */
+    {OUT_G, P21_31, D_X, ORC_IAB2, 3, 1.0},     /* 17: C=-Iab2(j,i,k), added as -C */
+    {OUT_G, P31_22, D_X, ORC_IA12, 0, 1.0},     /* 18 */
+    /* theta (:1006-1067) */
+    {OUT_TH, P11_DX, D_ID, ORC_IB11, 0, 1.0},
+    {OUT_TH, P12_DX, D_ID, ORC_I111, 0, 1.0},
+    {OUT_TH, P21_DY, D_ID, ORC_IB11, 0, 1.0},
+    {OUT_TH, P22_DY, D_ID, ORC_I111, 0, 1.0},
+    {OUT_TH, P31_TH, D_ID, ORC_IA12, 1, 1.0},
+};
+#define NSUMS ((int)(sizeof(SUMS) / sizeof(SUMS[0])))
+
+static double coef(const orc_ctx* c, const summand* s, int i, int j, int k) {
+    switch (s->order) {
+        case 0: return THREE(c, s->table, i, j, k);
+        case 1: return THREE(c, s->table, i, k, j);
+        case 2: return THREE(c, s->table, k, i, j);
+        case 3: return THREE(c, s->table, j, i, k);
+        default: return THREE(c, s->table, j, k, i);
+    }
+}
+
+int orc_nonlinear(orc_ctx* c, const double* th_, const double* f_, const double* g_, const double* F_,
+                  const double* G_, double* oth, double* of, double* og, double* oF, double* oG) {
+    const int L = c->L, M = c->M, N = c->N, mox = c->mox;
+    const size_t plane = c->plane, X = (size_t)M * N;
+    const cplx *th = (const cplx*)th_, *f = (const cplx*)f_, *g = (const cplx*)g_;
+    const cplx *F = (const cplx*)F_, *G = (const cplx*)G_;
+    /* physical fields: NPHYS x L planes of M*N reals (part 1, :96-233) */
+    double* phys = malloc(sizeof(double) * NPHYS * (size_t)L * X);
+    cplx* spec = malloc(sizeof(cplx) * plane);
+    if (!phys || !spec) { free(phys); free(spec); return -2; }
+    for (int m = 0; m < L; ++m) {
+        const cplx* fm = f + m * plane;
+        const cplx* gm = g + m * plane;
+        const cplx* tm = th + m * plane;
+        apply_deriv(c, D_X, fm, spec);
+        orc_fft_c2r(M, N, (double*)spec, phys + ((size_t)U11 * L + m) * X);
+        apply_deriv(c, D_Y, fm, spec);
+        orc_fft_c2r(M, N, (double*)spec, phys + ((size_t)U21 * L + m) * X);
+        apply_deriv(c, D_LAP, fm, spec);
+        for (size_t q = 0; q < plane; ++q) spec[q] *= -1.0;
+        orc_fft_c2r(M, N, (double*)spec, phys + ((size_t)U31 * L + m) * X);
+        apply_deriv(c, D_Y, gm, spec);
+        spec[0] += F[m];
+        orc_fft_c2r(M, N, (double*)spec, phys + ((size_t)U12 * L + m) * X);
+        apply_deriv(c, D_X, gm, spec);
+        for (size_t q = 0; q < plane; ++q) spec[q] *= -1.0;
+        spec[0] += G[m];
+        orc_fft_c2r(M, N, (double*)spec, phys + ((size_t)U22 * L + m) * X);
+        orc_fft_c2r(M, N, (const double*)tm, phys + ((size_t)TH * L + m) * X);
+        apply_deriv(c, D_X, tm, spec);
+        orc_fft_c2r(M, N, (double*)spec, phys + ((size_t)DXTH * L + m) * X);
+        apply_deriv(c, D_Y, tm, spec);
+        orc_fft_c2r(M, N, (double*)spec, phys + ((size_t)DYTH * L + m) * X);
+    }
+    free(spec);
+
+    const int nt = c->nthreads;
+    /* per-thread accumulators: theta, f, g (L planes each) and F, G (L) */
+    const size_t acc_sz = 3 * (size_t)L * plane + 2 * (size_t)L;
+    cplx* acc = calloc((size_t)nt * acc_sz, sizeof(cplx));
+    if (!acc) { free(phys); return -2; }
+    const double scale = 1.0 / (double)X;
+#pragma omp parallel num_threads(nt)
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        cplx* my = acc + (size_t)tid * acc_sz;
+        cplx* myF = my + 3 * (size_t)L * plane;
+        cplx* myG = myF + L;
+        double* buf = malloc(sizeof(double) * X);
+        cplx* P = malloc(sizeof(cplx) * 20 * plane);
+        cplx* D = malloc(sizeof(cplx) * plane);
+#pragma omp for schedule(dynamic, 1)
+        for (int pair = 0; pair < L * L; ++pair) {
+            const int j = pair / L, i = pair % L;
+            for (int p = 0; p < 20; ++p) {
+                const double* a = phys + ((size_t)PROD[p][0] * L + i) * X;
+                const double* b = phys + ((size_t)PROD[p][1] * L + j) * X;
+                for (size_t q = 0; q < X; ++q) buf[q] = scale * a[q] * b[q];
+                orc_fft_r2c(M, N, buf, (double*)(P + (size_t)p * plane));
+            }
+            for (int s = 0; s < NSUMS; ++s) {
+                const summand* S = &SUMS[s];
+                const cplx* src = P + (size_t)S->prod * plane;
+                if (S->deriv != D_ID) { apply_deriv(c, S->deriv, src, D); src = D; }
+                for (int k = 0; k < L; ++k) {
+                    double C = coef(c, S, i, j, k);
+                    if (C == 0.0) continue;
+                    double fac = S->mult * C;
+                    cplx* out = my + ((size_t)S->target * L + k) * plane;
+                    for (size_t q = 0; q < plane; ++q) out[q] += src[q] * fac;
+                }
+            }
+            /* F and G (:920-1003): DC parts only */
+            for (int k = 0; k < L; ++k) {
+                double C1 = -THREE(c, ORC_IAB2, j, i, k);
+                double C2 = -THREE(c, ORC_IA12, i, j, k);
+                if (C1 != 0.0) {
+                    myF[k] += P[(size_t)P11_31 * plane] * C1;
+                    myG[k] += P[(size_t)P21_31 * plane] * C1;
+                }
+                if (C2 != 0.0) {
+                    myF[k] += P[(size_t)P31_12 * plane] * C2;
+                    myG[k] += P[(size_t)P31_22 * plane] * C2;
+                }
+            }
+        }
+        free(buf); free(P); free(D);
+    }
+    free(phys);
+    /* reduce in thread order, then scale (:1366-1370) */
+    cplx* outs[3] = {(cplx*)oth, (cplx*)of, (cplx*)og};
+    const double sc[3] = {-1.0, -1.0 / c->Pr, -1.0 / c->Pr};
+    for (int t = 0; t < 3; ++t)
+        for (size_t q = 0; q < (size_t)L * plane; ++q) {
+            cplx s = 0.0;
+            for (int th_i = 0; th_i < nt; ++th_i) s += acc[(size_t)th_i * acc_sz + (size_t)t * L * plane + q];
+            outs[t][q] = s * sc[t];
+        }
+    for (int k = 0; k < L; ++k) {
+        cplx sF = 0.0, sG = 0.0;
+        for (int th_i = 0; th_i < nt; ++th_i) {
+            sF += acc[(size_t)th_i * acc_sz + 3 * (size_t)L * plane + k];
+            sG += acc[(size_t)th_i * acc_sz + 3 * (size_t)L * plane + L + k];
+        }
+        ((cplx*)oF)[k] = sF * (-1.0 / c->Pr);
+        ((cplx*)oG)[k] = sG * (-1.0 / c->Pr);
+    }
+    free(acc);
+    (void)mox;
+    return 0;
+}
+
+/* ======================================================================================== */
+/* inverse B and implicit operators                                                         */
+/* ======================================================================================== */
+static double wave_sq(const orc_ctx* c, int col, int row) {
+    /* get_wave_number_x/_y of the GPU kernels (inverse_B_matrix.cu:187-194) */
+    double kx = (M_PI + M_PI) * (col / c->Lx);
+    int r = row;
+    if (r >= (c->N / 2) + 1) r -= c->N;
+    double ky = (M_PI + M_PI) * (r / c->Ly);
+    return kx * kx + ky * ky;
+}
+
+int orc_inverse_B(orc_ctx* c, const double* th_, const double* f_, const double* g_, const double* F_,
+                  const double* G_, double* oth_, double* of_, double* og_, double* oF_, double* oG_) {
+    const int L = c->L;
+    const size_t plane = c->plane;
+    const cplx *th = (const cplx*)th_, *f = (const cplx*)f_, *g = (const cplx*)g_;
+    cplx *oth = (cplx*)oth_, *of = (cplx*)of_, *og = (cplx*)og_;
+    for (size_t q = 0; q < (size_t)L * plane; ++q) oth[q] = th[q] * 2.0;
+    for (int m = 0; m < L; ++m)
+        for (size_t p = 0; p < plane; ++p) {
+            const double* op = c->invBf + p * L * L + (size_t)m * L;
+            cplx s = 0.0;
+            for (int m1 = 0; m1 < L; ++m1) s += op[m1] * f[(size_t)m1 * plane + p];
+            of[(size_t)m * plane + p] = s;
+        }
+    for (int m = 0; m < L; ++m)
+        for (int r = 0; r < c->N; ++r)
+            for (int col = 0; col < c->mox; ++col) {
+                size_t i = (size_t)m * plane + (size_t)r * c->mox + col;
+                double msq = wave_sq(c, col, r);
+                og[i] = (msq != 0.0) ? g[i] * ((-2.0) * c->Pr / msq) : 0.0;
+            }
+    for (int m = 0; m < L; ++m) {
+        ((cplx*)oF_)[m] = 2.0 * c->Pr * creal(((const cplx*)F_)[m]);
+        ((cplx*)oG_)[m] = 2.0 * c->Pr * creal(((const cplx*)G_)[m]);
+    }
+    return 0;
+}
+
+int orc_implicit(orc_ctx* c, const double* th_, const double* f_, const double* g_, const double* F_,
+                 const double* G_, double* oth_, double* of_, double* og_, double* oF_, double* oG_) {
+    const int L = c->L;
+    const size_t plane = c->plane, LL = (size_t)L * L;
+    const cplx *th = (const cplx*)th_, *f = (const cplx*)f_, *g = (const cplx*)g_;
+    cplx *oth = (cplx*)oth_, *of = (cplx*)of_, *og = (cplx*)og_;
+    for (size_t p = 0; p < plane; ++p) {
+        const double* blk = c->imp + p * 4 * LL;
+        for (int m = 0; m < L; ++m) {
+            cplx st = 0.0, sf = 0.0;
+            for (int m1 = 0; m1 < L; ++m1) {
+                cplx it = th[(size_t)m1 * plane + p], iff = f[(size_t)m1 * plane + p];
+                st += blk[0 * LL + m * L + m1] * it + blk[1 * LL + m * L + m1] * iff;
+                sf += blk[2 * LL + m * L + m1] * it + blk[3 * LL + m * L + m1] * iff;
+            }
+            oth[(size_t)m * plane + p] = st;
+            of[(size_t)m * plane + p] = sf;
+        }
+    }
+    for (int m = 0; m < L; ++m)
+        for (int r = 0; r < c->N; ++r)
+            for (int col = 0; col < c->mox; ++col) {
+                size_t i = (size_t)m * plane + (size_t)r * c->mox + col;
+                double msq = wave_sq(c, col, r);
+                if (msq != 0.0) {
+                    double mp = (m + 1) * M_PI;
+                    double factor = 1.0 / (1.0 + c->dt * c->Pr * (msq + mp * mp));
+                    og[i] = factor * g[i];
+                } else {
+                    og[i] = 0.0;
+                }
+            }
+    for (int m = 0; m < L; ++m) {
+        double mp = (m + 1) * M_PI;
+        double factor = 1.0 / (1.0 + c->dt * c->Pr * mp * mp);
+        ((cplx*)oF_)[m] = factor * creal(((const cplx*)F_)[m]);
+        ((cplx*)oG_)[m] = factor * creal(((const cplx*)G_)[m]);
+    }
+    return 0;
+}
+
+int orc_implicit_factor(orc_ctx* c, int col, int row, double* out) {
+    if (col < 0 || col >= c->mox || row < 0 || row >= c->N) return -1;
+    memcpy(out, c->imp + ((size_t)row * c->mox + col) * 4 * c->L * c->L, sizeof(double) * 4 * c->L * c->L);
+    return 0;
+}
+
+/* ======================================================================================== */
+/* time step (implicit_explicit_stepping.cu:32-97, main.cu:612-616)                         */
+/* ======================================================================================== */
+void orc_reset_history(orc_ctx* c) { c->have_hist = 0; }
+
+int orc_step(orc_ctx* c, int nsteps) {
+    const int L = c->L;
+    const size_t fs = c->field;
+    cplx* N5[5];
+    cplx* R5[5];
+    for (int i = 0; i < 5; ++i) {
+        size_t n = i < 3 ? fs : (size_t)L;
+        N5[i] = malloc(sizeof(cplx) * n);
+        R5[i] = malloc(sizeof(cplx) * n);
+    }
+    cplx* X[5] = {c->th, c->f, c->g, c->F, c->G};
+    const double dt = c->dt;
+    int rc = 0;
+    for (int s = 0; s < nsteps && !rc; ++s) {
+        for (int i = 0; i < 3; ++i) orc_dealias(c->mox, c->N, L, (double*)X[i]);
+        rc = orc_nonlinear(c, (double*)X[0], (double*)X[1], (double*)X[2], (double*)X[3], (double*)X[4],
+                           (double*)N5[0], (double*)N5[1], (double*)N5[2], (double*)N5[3], (double*)N5[4]);
+        if (rc) break;
+        orc_inverse_B(c, (double*)N5[0], (double*)N5[1], (double*)N5[2], (double*)N5[3], (double*)N5[4],
+                      (double*)R5[0], (double*)R5[1], (double*)R5[2], (double*)R5[3], (double*)R5[4]);
+        for (int i = 0; i < 5; ++i) {
+            size_t n = i < 3 ? fs : (size_t)L;
+            if (!c->have_hist) {
+                for (size_t q = 0; q < n; ++q) X[i][q] += R5[i][q] * dt;
+            } else {
+                for (size_t q = 0; q < n; ++q) X[i][q] += R5[i][q] * (1.5 * dt);
+                for (size_t q = 0; q < n; ++q) X[i][q] += c->r1[i][q] * (-0.5 * dt);
+            }
+            memcpy(c->r1[i], R5[i], sizeof(cplx) * n);
+        }
+        c->have_hist = 1;
+        for (int i = 0; i < 3; ++i) orc_dealias(c->mox, c->N, L, (double*)X[i]);
+        orc_implicit(c, (double*)X[0], (double*)X[1], (double*)X[2], (double*)X[3], (double*)X[4],
+                     (double*)N5[0], (double*)N5[1], (double*)N5[2], (double*)N5[3], (double*)N5[4]);
+        for (int i = 0; i < 5; ++i) memcpy(X[i], N5[i], sizeof(cplx) * (i < 3 ? fs : (size_t)L));
+        for (int i = 0; i < 3; ++i) orc_make_real(c->mox, c->N, L, (double*)X[i]);
+    }
+    for (int i = 0; i < 5; ++i) { free(N5[i]); free(R5[i]); }
+    return rc;
+}
+
+/* ======================================================================================== */
+/* Nusselt numbers (new diagnostic; T = -z + theta, so -dT/dz = 1 - d theta/dz)            */
+/* ======================================================================================== */
+int orc_nusselt(orc_ctx* c, double* nu3) {
+    const int L = c->L;
+    double sb = 0.0, st = 0.0;
+    for (int m = 0; m < L; ++m) {
+        double a = creal(c->th[(size_t)m * c->plane]);
+        double mp = (m + 1) * M_PI;
+        sb += mp * a;                               /* dS_m/dz at z=-1/2 = m pi */
+        st += mp * ((m + 1) % 2 ? -1.0 : 1.0) * a;  /* at z=+1/2 = m pi cos(m pi) */
+    }
+    nu3[0] = 1.0 - sb;
+    nu3[1] = 1.0 - st;
+    /* <w theta> over the box: w_q(z) = sum_m C_m(z) k^2 f_m,q ; theta_q(z) = sum_n S_n theta_n,q;
+     * z-integral <C_m S_n> = Ia1(m,n); horizontal mean by Parseval (kx>0 counted twice). */
+    double s = 0.0;
+    for (int r = 0; r < c->N; ++r)
+        for (int col = 0; col < c->mox; ++col) {
+            size_t p = (size_t)r * c->mox + col;
+            double k2 = wave_sq(c, col, r);
+            double wgt = (col == 0 || 2 * col == c->M) ? 1.0 : 2.0;
+            for (int m = 0; m < L; ++m) {
+                cplx wm = k2 * c->f[(size_t)m * c->plane + p];
+                for (int n = 0; n < L; ++n) {
+                    double ia = TWO(c, ORC_IA1, m, n);
+                    if (ia == 0.0) continue;
+                    s += wgt * ia * creal(wm * conj(c->th[(size_t)n * c->plane + p]));
+                }
+            }
+        }
+    nu3[2] = 1.0 + s;
+    return 0;
+}

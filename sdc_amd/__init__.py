@@ -1,0 +1,9 @@
+"""sdc_amd: MI355X-native Rayleigh-Benard time stepper (drop-in for gpu_flow's hot path).
+
+The compute path is libsdc_rbc.so (hand-written HIP for gfx950 behind the C-ABI in
+include/rbc_abi.h); this package holds its sources (csrc/), the build script and the
+Python host binding.
+"""
+from .rbc import Stepper, RbcError, load_library, LIB_PATH  # noqa: F401
+
+__all__ = ["Stepper", "RbcError", "load_library", "LIB_PATH"]

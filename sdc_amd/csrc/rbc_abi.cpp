@@ -1,0 +1,572 @@
+// rbc_abi.cpp -- C-ABI of libsdc_rbc.so: context lifetime, state exchange in the
+// reference layout, the time step pipeline, operator-level calls and diagnostics.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "rbc_internal.hpp"
+
+using namespace rbc;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct HipError {
+    hipError_t e;
+    const char* what;
+};
+
+#define HIPCHK(x)                                        \
+    do {                                                 \
+        hipError_t _e = (x);                             \
+        if (_e != hipSuccess) throw HipError{_e, #x};    \
+    } while (0)
+
+}  // namespace
+
+struct rbc_ctx {
+    rbc_params p{};
+    Geometry geo;
+    DevGeom dg{};
+    Factors fac;
+    int prec = RBC_F64;
+    size_t rsz = 8;  // bytes per real on the device
+    hipStream_t stream = nullptr;
+    Buffers b{};
+    void* ia1 = nullptr;  // L*L doubles (Nusselt)
+    std::vector<void*> allocs;
+    bool have_hist = false;
+    std::string err;
+    std::vector<unsigned char> stage;  // host staging for one compact field
+    hipEvent_t ev[RBC_NSTAGES + 1]{};
+    bool events = false;
+
+    size_t csz() const { return 2 * rsz; }
+    void* dalloc(size_t bytes) {
+        void* ptr = nullptr;
+        HIPCHK(hipMalloc(&ptr, bytes > 0 ? bytes : 16));
+        allocs.push_back(ptr);
+        return ptr;
+    }
+    ~rbc_ctx() {
+        if (stream) hipStreamSynchronize(stream);
+        for (void* a : allocs) hipFree(a);
+        if (events)
+            for (auto& e : ev) hipEventDestroy(e);
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <typename T>
+void upload_real(rbc_ctx* c, void* dst, const std::vector<double>& src) {
+    std::vector<T> tmp(src.size());
+    for (size_t i = 0; i < src.size(); ++i) tmp[i] = (T)src[i];
+    HIPCHK(hipMemcpy(dst, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+void upload_reals(rbc_ctx* c, void* dst, const std::vector<double>& src) {
+    if (c->rsz == 8) upload_real<double>(c, dst, src);
+    else upload_real<float>(c, dst, src);
+}
+
+// reference layout (mox x N x L complex doubles) -> compact [p][m] in device precision
+template <typename T>
+void pack_field(const rbc_ctx* c, const double* src, T* dst) {
+    const Geometry& g = c->geo;
+    const int L = g.L, NYc = g.NYc(), mox = g.mox;
+    for (int kx = 0; kx < g.KX; ++kx)
+        for (int kc = 0; kc < NYc; ++kc) {
+            const size_t p = (size_t)kx * NYc + kc;
+            const int row = g.rows[kc];
+            for (int m = 0; m < L; ++m) {
+                const size_t s = ((size_t)m * g.N + row) * mox + kx;
+                dst[2 * (p * L + m)] = (T)src[2 * s];
+                dst[2 * (p * L + m) + 1] = (T)src[2 * s + 1];
+            }
+        }
+}
+
+template <typename T>
+void unpack_field(const rbc_ctx* c, const T* src, double* dst) {
+    const Geometry& g = c->geo;
+    const int L = g.L, NYc = g.NYc(), mox = g.mox;
+    std::memset(dst, 0, sizeof(double) * 2 * (size_t)mox * g.N * L);
+    for (int kx = 0; kx < g.KX; ++kx)
+        for (int kc = 0; kc < NYc; ++kc) {
+            const size_t p = (size_t)kx * NYc + kc;
+            const int row = g.rows[kc];
+            for (int m = 0; m < L; ++m) {
+                const size_t s = ((size_t)m * g.N + row) * mox + kx;
+                dst[2 * s] = (double)src[2 * (p * L + m)];
+                dst[2 * s + 1] = (double)src[2 * (p * L + m) + 1];
+            }
+        }
+}
+
+void put_field(rbc_ctx* c, const double* host, void* dev) {
+    const size_t n = (size_t)c->geo.P() * c->geo.L * c->csz();
+    c->stage.resize(n);
+    if (c->rsz == 8) pack_field<double>(c, host, (double*)c->stage.data());
+    else pack_field<float>(c, host, (float*)c->stage.data());
+    HIPCHK(hipMemcpyAsync(dev, c->stage.data(), n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+void get_field(rbc_ctx* c, const void* dev, double* host) {
+    const size_t n = (size_t)c->geo.P() * c->geo.L * c->csz();
+    c->stage.resize(n);
+    HIPCHK(hipMemcpyAsync(c->stage.data(), dev, n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->rsz == 8) unpack_field<double>(c, (const double*)c->stage.data(), host);
+    else unpack_field<float>(c, (const float*)c->stage.data(), host);
+}
+
+void put_vec(rbc_ctx* c, const double* host, void* dev) {
+    const int L = c->geo.L;
+    std::vector<unsigned char> tmp((size_t)L * c->csz());
+    for (int i = 0; i < 2 * L; ++i) {
+        if (c->rsz == 8) ((double*)tmp.data())[i] = host[i];
+        else ((float*)tmp.data())[i] = (float)host[i];
+    }
+    HIPCHK(hipMemcpyAsync(dev, tmp.data(), tmp.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+void get_vec(rbc_ctx* c, const void* dev, double* host) {
+    const int L = c->geo.L;
+    std::vector<unsigned char> tmp((size_t)L * c->csz());
+    HIPCHK(hipMemcpyAsync(tmp.data(), dev, tmp.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < 2 * L; ++i) host[i] = (c->rsz == 8) ? ((double*)tmp.data())[i] : ((float*)tmp.data())[i];
+}
+
+// ---- pipeline, dispatched on precision -------------------------------------------------
+template <typename T>
+void nonlinear_into(rbc_ctx* c, void* const X[3], const void* XF, const void* XG, void* const Nout[3], void* NF,
+                    void* NG, int stage_base) {
+    auto mark = [&](int i) {
+        if (stage_base >= 0) HIPCHK(hipEventRecord(c->ev[stage_base + i], c->stream));
+    };
+    Launch<T>::eval(c->dg, c->b, X, XF, XG, c->stream);
+    mark(1);
+    Launch<T>::fft_y_inv(c->dg, c->b, c->stream);
+    mark(2);
+    Launch<T>::rows(c->dg, c->b, c->stream);
+    mark(3);
+    Launch<T>::fft_y_fwd(c->dg, c->b, c->stream);
+    mark(4);
+    Launch<T>::project(c->dg, c->b, c->p.Pr, Nout, NF, NG, c->stream);
+    mark(5);
+}
+
+template <typename T>
+void one_step(rbc_ctx* c, bool timed) {
+    StepCoef sc;
+    if (c->have_hist) {
+        sc.a_cur = 1.5 * c->p.dt;
+        sc.a_prev = -0.5 * c->p.dt;
+        sc.use_prev = 1;
+    } else {
+        sc.a_cur = c->p.dt;
+        sc.a_prev = 0.0;
+        sc.use_prev = 0;
+    }
+    if (timed) HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    nonlinear_into<T>(c, c->b.X, c->b.XF, c->b.XG, c->b.Nn, c->b.NF, c->b.NG, timed ? 0 : -1);
+    Launch<T>::update(c->dg, c->b, c->fac.ngroups(), c->fac.L0, c->fac.L1, c->fac.stride, c->p.Pr, c->p.dt, sc,
+                      UPD_STEP, c->b.X, c->b.Nn, c->b.Rp, c->stream);
+    if (timed) HIPCHK(hipEventRecord(c->ev[6], c->stream));
+    Launch<T>::update_FG(c->dg, c->p.Pr, c->p.dt, sc, UPD_STEP, c->b.XF, c->b.XG, c->b.NF, c->b.NG, c->b.RpF,
+                         c->b.RpG, c->stream);
+    if (timed) HIPCHK(hipEventRecord(c->ev[7], c->stream));
+    Launch<T>::make_real(c->dg, c->b, c->b.X, c->stream);
+    if (timed) HIPCHK(hipEventRecord(c->ev[8], c->stream));
+    HIPCHK(hipGetLastError());
+    c->have_hist = true;
+}
+
+void run_steps(rbc_ctx* c, int n) {
+    for (int i = 0; i < n; ++i) {
+        if (c->rsz == 8) one_step<double>(c, false);
+        else one_step<float>(c, false);
+    }
+}
+
+int fail(rbc_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+int hip_fail(rbc_ctx* c, const HipError& e) {
+    return fail(c, RBC_ERR_HIP, std::string(e.what) + ": " + hipGetErrorString(e.e));
+}
+
+const char* kStageNames[RBC_NSTAGES] = {"eval", "fft_y_inv", "rows_products", "fft_y_fwd",
+                                        "project", "update", "update_FG", "make_real"};
+
+}  // namespace
+
+extern "C" {
+
+int rbc_create(const rbc_params* p, rbc_ctx** out) {
+    if (!p || !out) return fail(nullptr, RBC_ERR_ARG, "null argument");
+    *out = nullptr;
+    auto pow2 = [](int n) { return n >= 8 && n <= 4096 && (n & (n - 1)) == 0; };
+    if (!pow2(p->M) || !pow2(p->N)) return fail(nullptr, RBC_ERR_ARG, "M and N must be powers of two in [8, 4096]");
+    if (p->L < 1 || p->L > 64) return fail(nullptr, RBC_ERR_ARG, "L must be in [1, 64]");
+    if (!(p->Lx > 0) || !(p->Ly > 0) || !(p->Pr > 0) || !(p->dt > 0) || !std::isfinite(p->Ra))
+        return fail(nullptr, RBC_ERR_ARG, "Lx, Ly, Pr, dt must be positive and Ra finite");
+    if (p->precision < RBC_F64 || p->precision > RBC_BF16X3) return fail(nullptr, RBC_ERR_ARG, "bad precision");
+    if (p->nq < 0 || p->nq > 1024) return fail(nullptr, RBC_ERR_ARG, "nq out of range");
+    {
+        // LDS budgets of the row kernel (8 M complex) and the y-FFT tiles (2 N complex)
+        const size_t cs = (p->precision == RBC_F64) ? 16 : 8;
+        if (8 * (size_t)p->M * cs > 65536 || 2 * (size_t)p->N * cs > 65536)
+            return fail(nullptr, RBC_ERR_ARG, "grid too large for the LDS budget at this precision");
+    }
+    std::unique_ptr<rbc_ctx> c(new rbc_ctx());
+    c->p = *p;
+    c->prec = p->precision;
+    c->rsz = (p->precision == RBC_F64) ? 8 : 4;
+    try {
+        int ndev = 0;
+        HIPCHK(hipGetDeviceCount(&ndev));
+        if (p->device < 0 || p->device >= ndev) return fail(nullptr, RBC_ERR_ARG, "no such HIP device");
+        HIPCHK(hipSetDevice(p->device));
+        c->geo = make_geometry(p->M, p->N, p->L);
+        const Geometry& g = c->geo;
+        // kept rows must be [0, a] U [N-a, N-1] (true for the reference mask at every N)
+        const int NYc = g.NYc(), a = (NYc - 1) / 2;
+        for (int i = 0; i < NYc; ++i) {
+            const int want = (i <= a) ? i : p->N - (NYc - i);
+            if (NYc % 2 == 0 || g.rows[i] != want) return fail(nullptr, RBC_ERR_ARG, "unexpected dealias row set");
+        }
+        double qerr = 0.0;
+        int nq = p->nq;
+        if (nq == 0) nq = exact_level_count(p->L, p->precision == RBC_F64 ? 1e-13 : 1e-8, &qerr);
+        c->p.nq = nq;
+        DevGeom& d = c->dg;
+        d.M = p->M;
+        d.N = p->N;
+        d.L = p->L;
+        d.nq = nq;
+        d.KX = g.KX;
+        d.KXp = (g.KX + 3) & ~3;
+        d.NYc = NYc;
+        d.P = g.P();
+        d.logM = 0;
+        while ((1 << d.logM) < p->M) ++d.logM;
+        d.logN = 0;
+        while ((1 << d.logN) < p->N) ++d.logN;
+        d.Lx = p->Lx;
+        d.Ly = p->Ly;
+
+        std::string err;
+        if (build_factors(g, p->Lx, p->Ly, p->Ra, p->Pr, p->dt, c->fac, err)) return fail(nullptr, RBC_ERR_NUMERIC, err);
+
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
+        c->events = true;
+
+        const size_t cs = c->csz(), P = d.P, L = d.L;
+        Buffers& b = c->b;
+        for (int i = 0; i < 3; ++i) {
+            b.X[i] = c->dalloc(P * L * cs);
+            b.Nn[i] = c->dalloc(P * L * cs);
+            b.Rp[i] = c->dalloc(P * L * cs);
+            b.Xo[i] = c->dalloc(P * L * cs);
+            b.Oo[i] = c->dalloc(P * L * cs);
+        }
+        b.XF = c->dalloc(L * cs);
+        b.XG = c->dalloc(L * cs);
+        b.NF = c->dalloc(L * cs);
+        b.NG = c->dalloc(L * cs);
+        b.RpF = c->dalloc(L * cs);
+        b.RpG = c->dalloc(L * cs);
+        b.XoF = c->dalloc(L * cs);
+        b.XoG = c->dalloc(L * cs);
+        b.OoF = c->dalloc(L * cs);
+        b.OoG = c->dalloc(L * cs);
+        b.lev = c->dalloc((size_t)6 * nq * P * cs);
+        b.hin = c->dalloc((size_t)6 * nq * d.N * d.KXp * cs);
+        b.hout = c->dalloc((size_t)7 * nq * d.N * d.KXp * cs);
+        b.prod = c->dalloc((size_t)7 * nq * P * cs);
+        // zero-initialise state and padding once (not in any step: nothing is memset later)
+        for (int i = 0; i < 3; ++i) {
+            HIPCHK(hipMemset(b.X[i], 0, P * L * cs));
+            HIPCHK(hipMemset(b.Rp[i], 0, P * L * cs));
+        }
+        HIPCHK(hipMemset(b.XF, 0, L * cs));
+        HIPCHK(hipMemset(b.XG, 0, L * cs));
+        HIPCHK(hipMemset(b.RpF, 0, L * cs));
+        HIPCHK(hipMemset(b.RpG, 0, L * cs));
+        HIPCHK(hipMemset(b.hin, 0, (size_t)6 * nq * d.N * d.KXp * cs));
+        HIPCHK(hipMemset(b.hout, 0, (size_t)7 * nq * d.N * d.KXp * cs));
+
+        // vertical tables at the Gauss-Legendre levels
+        std::vector<double> z, w, bv;
+        gauss_legendre(nq, z, w);
+        basis_values(p->L, z, bv);
+        auto B = [&](int t, int m, int l) { return bv[((size_t)t * L + m) * nq + l]; };
+        std::vector<double> ev((size_t)nq * 4 * L), pr((size_t)nq * 5 * L);
+        for (int l = 0; l < nq; ++l)
+            for (int m = 0; m < (int)L; ++m) {
+                ev[((size_t)l * 4 + 0) * L + m] = B(1, m, l);  // C'
+                ev[((size_t)l * 4 + 1) * L + m] = B(0, m, l);  // C
+                ev[((size_t)l * 4 + 2) * L + m] = B(4, m, l);  // S
+                ev[((size_t)l * 4 + 3) * L + m] = B(5, m, l);  // S'
+                pr[((size_t)l * 5 + 0) * L + m] = w[l] * B(1, m, l);
+                pr[((size_t)l * 5 + 1) * L + m] = w[l] * B(2, m, l);
+                pr[((size_t)l * 5 + 2) * L + m] = w[l] * B(0, m, l);
+                pr[((size_t)l * 5 + 3) * L + m] = w[l] * B(4, m, l);
+                pr[((size_t)l * 5 + 4) * L + m] = w[l] * B(5, m, l);
+            }
+        b.evtab = c->dalloc(ev.size() * c->rsz);
+        b.prtab = c->dalloc(pr.size() * c->rsz);
+        upload_reals(c.get(), b.evtab, ev);
+        upload_reals(c.get(), b.prtab, pr);
+        // twiddles exp(-2 pi i k / n), built in FP64
+        auto tw = [&](int n) {
+            std::vector<double> t((size_t)n);  // n/2 complex
+            for (int k = 0; k < n / 2; ++k) {
+                t[2 * k] = std::cos(2.0 * M_PI * k / n);
+                t[2 * k + 1] = -std::sin(2.0 * M_PI * k / n);
+            }
+            return t;
+        };
+        auto tM = tw(p->M), tN = tw(p->N);
+        b.twM = c->dalloc(tM.size() * c->rsz);
+        b.twN = c->dalloc(tN.size() * c->rsz);
+        upload_reals(c.get(), b.twM, tM);
+        upload_reals(c.get(), b.twN, tN);
+        b.kx = c->dalloc(P * c->rsz);
+        b.ky = c->dalloc(P * c->rsz);
+        b.k2 = c->dalloc(P * c->rsz);
+        upload_reals(c.get(), b.kx, c->fac.kx);
+        upload_reals(c.get(), b.ky, c->fac.ky);
+        upload_reals(c.get(), b.k2, c->fac.k2);
+        b.rows = (int*)c->dalloc(g.rows.size() * sizeof(int));
+        HIPCHK(hipMemcpy(b.rows, g.rows.data(), g.rows.size() * sizeof(int), hipMemcpyHostToDevice));
+        b.gstart = (int*)c->dalloc(c->fac.group_start.size() * sizeof(int));
+        b.gmem = (int*)c->dalloc(c->fac.members.size() * sizeof(int));
+        HIPCHK(hipMemcpy(b.gstart, c->fac.group_start.data(), c->fac.group_start.size() * sizeof(int),
+                         hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b.gmem, c->fac.members.data(), c->fac.members.size() * sizeof(int), hipMemcpyHostToDevice));
+        b.gmat = c->dalloc(c->fac.mats.size() * c->rsz);
+        upload_reals(c.get(), b.gmat, c->fac.mats);
+        b.nred = (int)((P + 255) / 256) + 2;
+        b.red = (double*)c->dalloc((size_t)b.nred * sizeof(double));
+        std::vector<double> iac, ia1;
+        closed_form_iac_ia1(p->L, iac, ia1);
+        c->ia1 = c->dalloc(ia1.size() * sizeof(double));
+        HIPCHK(hipMemcpy(c->ia1, ia1.data(), ia1.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipDeviceSynchronize());
+    } catch (const HipError& e) {
+        return fail(nullptr, RBC_ERR_HIP, std::string(e.what) + ": " + hipGetErrorString(e.e));
+    } catch (const std::bad_alloc&) {
+        return fail(nullptr, RBC_ERR_NOMEM, "host allocation failed");
+    }
+    *out = c.release();
+    return RBC_OK;
+}
+
+void rbc_destroy(rbc_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->p.device);
+    delete c;
+}
+
+const char* rbc_last_error(const rbc_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int rbc_get_info(const rbc_ctx* c, int* nq, int* kx_kept, int* ky_kept, int* n_unique_k) {
+    if (!c) return RBC_ERR_ARG;
+    if (nq) *nq = c->dg.nq;
+    if (kx_kept) *kx_kept = c->dg.KX;
+    if (ky_kept) *ky_kept = c->dg.NYc;
+    if (n_unique_k) *n_unique_k = c->fac.ngroups();
+    return RBC_OK;
+}
+
+int rbc_set_state(rbc_ctx* c, const double* th, const double* f, const double* g, const double* F, const double* G) {
+    if (!c) return RBC_ERR_ARG;
+    try {
+        hipSetDevice(c->p.device);
+        if (th) put_field(c, th, c->b.X[0]);
+        if (f) put_field(c, f, c->b.X[1]);
+        if (g) put_field(c, g, c->b.X[2]);
+        if (F) put_vec(c, F, c->b.XF);
+        if (G) put_vec(c, G, c->b.XG);
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+int rbc_get_state(rbc_ctx* c, double* th, double* f, double* g, double* F, double* G) {
+    if (!c) return RBC_ERR_ARG;
+    try {
+        hipSetDevice(c->p.device);
+        if (th) get_field(c, c->b.X[0], th);
+        if (f) get_field(c, c->b.X[1], f);
+        if (g) get_field(c, c->b.X[2], g);
+        if (F) get_vec(c, c->b.XF, F);
+        if (G) get_vec(c, c->b.XG, G);
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+int rbc_step(rbc_ctx* c, int nsteps) {
+    if (!c || nsteps < 0) return RBC_ERR_ARG;
+    try {
+        hipSetDevice(c->p.device);
+        run_steps(c, nsteps);
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+int rbc_reset_history(rbc_ctx* c) {
+    if (!c) return RBC_ERR_ARG;
+    c->have_hist = false;
+    return RBC_OK;
+}
+
+int rbc_synchronize(rbc_ctx* c) {
+    if (!c) return RBC_ERR_ARG;
+    try {
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+static int op_call(rbc_ctx* c, int which, const double* th, const double* f, const double* g, const double* F,
+                   const double* G, double* oth, double* of, double* og, double* oF, double* oG) {
+    if (!c || !th || !f || !g || !F || !G || !oth || !of || !og || !oF || !oG) return RBC_ERR_ARG;
+    try {
+        hipSetDevice(c->p.device);
+        Buffers& b = c->b;
+        put_field(c, th, b.Xo[0]);
+        put_field(c, f, b.Xo[1]);
+        put_field(c, g, b.Xo[2]);
+        put_vec(c, F, b.XoF);
+        put_vec(c, G, b.XoG);
+        StepCoef sc{0.0, 0.0, 0};
+        void** outs = b.Oo;
+        void *oFp = b.OoF, *oGp = b.OoG;
+        if (which == 0) {
+            if (c->rsz == 8) nonlinear_into<double>(c, b.Xo, b.XoF, b.XoG, b.Oo, b.OoF, b.OoG, -1);
+            else nonlinear_into<float>(c, b.Xo, b.XoF, b.XoG, b.Oo, b.OoF, b.OoG, -1);
+        } else if (which == 1) {  // inverse B: N in Xo -> R in Oo
+            if (c->rsz == 8) {
+                Launch<double>::update(c->dg, b, c->fac.ngroups(), c->fac.L0, c->fac.L1, c->fac.stride, c->p.Pr,
+                                       c->p.dt, sc, UPD_INVB, b.Xo, b.Xo, b.Oo, c->stream);
+                Launch<double>::update_FG(c->dg, c->p.Pr, c->p.dt, sc, UPD_INVB, b.XoF, b.XoG, b.XoF, b.XoG, b.OoF,
+                                          b.OoG, c->stream);
+            } else {
+                Launch<float>::update(c->dg, b, c->fac.ngroups(), c->fac.L0, c->fac.L1, c->fac.stride, c->p.Pr,
+                                      c->p.dt, sc, UPD_INVB, b.Xo, b.Xo, b.Oo, c->stream);
+                Launch<float>::update_FG(c->dg, c->p.Pr, c->p.dt, sc, UPD_INVB, b.XoF, b.XoG, b.XoF, b.XoG, b.OoF,
+                                         b.OoG, c->stream);
+            }
+        } else {  // implicit: in place on Xo
+            if (c->rsz == 8) {
+                Launch<double>::update(c->dg, b, c->fac.ngroups(), c->fac.L0, c->fac.L1, c->fac.stride, c->p.Pr,
+                                       c->p.dt, sc, UPD_IMPLICIT, b.Xo, b.Xo, b.Oo, c->stream);
+                Launch<double>::update_FG(c->dg, c->p.Pr, c->p.dt, sc, UPD_IMPLICIT, b.XoF, b.XoG, b.XoF, b.XoG,
+                                          b.OoF, b.OoG, c->stream);
+            } else {
+                Launch<float>::update(c->dg, b, c->fac.ngroups(), c->fac.L0, c->fac.L1, c->fac.stride, c->p.Pr,
+                                      c->p.dt, sc, UPD_IMPLICIT, b.Xo, b.Xo, b.Oo, c->stream);
+                Launch<float>::update_FG(c->dg, c->p.Pr, c->p.dt, sc, UPD_IMPLICIT, b.XoF, b.XoG, b.XoF, b.XoG,
+                                         b.OoF, b.OoG, c->stream);
+            }
+            outs = b.Xo;
+            oFp = b.XoF;
+            oGp = b.XoG;
+        }
+        HIPCHK(hipGetLastError());
+        get_field(c, outs[0], oth);
+        get_field(c, outs[1], of);
+        get_field(c, outs[2], og);
+        get_vec(c, oFp, oF);
+        get_vec(c, oGp, oG);
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+int rbc_nonlinear(rbc_ctx* c, const double* th, const double* f, const double* g, const double* F, const double* G,
+                  double* oth, double* of, double* og, double* oF, double* oG) {
+    return op_call(c, 0, th, f, g, F, G, oth, of, og, oF, oG);
+}
+int rbc_inverse_B(rbc_ctx* c, const double* th, const double* f, const double* g, const double* F, const double* G,
+                  double* oth, double* of, double* og, double* oF, double* oG) {
+    return op_call(c, 1, th, f, g, F, G, oth, of, og, oF, oG);
+}
+int rbc_implicit(rbc_ctx* c, const double* th, const double* f, const double* g, const double* F, const double* G,
+                 double* oth, double* of, double* og, double* oF, double* oG) {
+    return op_call(c, 2, th, f, g, F, G, oth, of, og, oF, oG);
+}
+
+int rbc_nusselt(rbc_ctx* c, double* nu3) {
+    if (!c || !nu3) return RBC_ERR_ARG;
+    try {
+        hipSetDevice(c->p.device);
+        if (c->rsz == 8) Launch<double>::nusselt(c->dg, c->b, c->ia1, c->stream);
+        else Launch<float>::nusselt(c->dg, c->b, c->ia1, c->stream);
+        HIPCHK(hipGetLastError());
+        std::vector<double> red(c->b.nred);
+        HIPCHK(hipMemcpyAsync(red.data(), c->b.red, red.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        const int nblk = c->b.nred - 2;
+        double s = 0.0;
+        for (int i = 0; i < nblk; ++i) s += red[i];
+        nu3[0] = 1.0 - red[nblk];
+        nu3[1] = 1.0 - red[nblk + 1];
+        nu3[2] = 1.0 + s;
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+int rbc_profile_steps(rbc_ctx* c, int nsteps, double* ms) {
+    if (!c || !ms || nsteps < 1) return RBC_ERR_ARG;
+    try {
+        hipSetDevice(c->p.device);
+        for (int i = 0; i < RBC_NSTAGES; ++i) ms[i] = 0.0;
+        for (int s = 0; s < nsteps; ++s) {
+            if (c->rsz == 8) one_step<double>(c, true);
+            else one_step<float>(c, true);
+            HIPCHK(hipEventSynchronize(c->ev[RBC_NSTAGES]));
+            for (int i = 0; i < RBC_NSTAGES; ++i) {
+                float t = 0.f;
+                HIPCHK(hipEventElapsedTime(&t, c->ev[i], c->ev[i + 1]));
+                ms[i] += t;
+            }
+        }
+        for (int i = 0; i < RBC_NSTAGES; ++i) ms[i] /= nsteps;
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+const char* rbc_stage_name(int i) { return (i >= 0 && i < RBC_NSTAGES) ? kStageNames[i] : ""; }
+
+}  // extern "C"

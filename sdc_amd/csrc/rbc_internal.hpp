@@ -1,0 +1,149 @@
+// rbc_internal.hpp -- internal declarations shared by the host setup, the kernels and the
+// C-ABI of libsdc_rbc.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "rbc_abi.h"
+
+namespace rbc {
+
+// ---------------------------------------------------------------------------------------
+// FP64 host setup (rbc_setup.cpp)
+// ---------------------------------------------------------------------------------------
+
+// lambda_k of the Chandrasekhar functions as the reference tabulates them
+// (util/coeff.cu:1232-1257): 12-digit roots for k <= 8, pi (k + 1/2) beyond.
+double lambda_k(int k);
+
+// Gauss-Legendre nodes/weights on [-1/2, 1/2], ascending.
+void gauss_legendre(int n, std::vector<double>& z, std::vector<double>& w);
+
+// Basis values at nodes z: out[(t*L + m)*nz + q] for t = 0..5 = C, C', C'', C''', S, S'.
+void basis_values(int L, const std::vector<double>& z, std::vector<double>& out);
+
+// Closed-form 2-factor integrals used by the linear operators, restated from
+// util/coeff.cu:77-116 (Iac, including its even-even off-diagonal zeros) and :146-178 (Ia1).
+// t[i + j*L].
+void closed_form_iac_ia1(int L, std::vector<double>& Iac, std::vector<double>& Ia1);
+
+// Smallest number of collocation levels (multiple of 8) whose Gauss-Legendre rule
+// reproduces the Galerkin triple products of L modes to relative accuracy tol.
+int exact_level_count(int L, double tol, double* achieved);
+
+// Kept-mode geometry of the reference dealias mask (implicit_explicit_stepping.cu:163-182).
+struct Geometry {
+    int M, N, L, mox;
+    int KX;                 // kept kx columns 0..KX-1
+    std::vector<int> rows;  // kept rows (ascending), NYc entries
+    int NYc() const { return (int)rows.size(); }
+    int P() const { return KX * NYc(); }  // compact mode count; p = kx*NYc + kyc
+};
+Geometry make_geometry(int M, int N, int L);
+
+// Per-|k|^2 factors, split by vertical parity class (class 0: 0-based even m, class 1: odd).
+// For each group: [Cinv class0 (n0 x n0, column-major), Cinv class1, Binv_f class0 (L0 x L0),
+// Binv_f class1], n_c = 2*L_c, unknown order within a class: theta modes then f modes.
+struct Factors {
+    int L0, L1, stride;                 // class sizes and doubles per group
+    std::vector<double> mats;           // ngroups * stride
+    std::vector<int> group_start;       // ngroups + 1
+    std::vector<int> members;           // P, compact mode indices grouped
+    std::vector<double> k2;             // P, |k|^2 per compact mode
+    std::vector<double> kx, ky;         // P, wavenumbers per compact mode
+    int ngroups() const { return (int)group_start.size() - 1; }
+};
+// Follows linear_implicit_operator.cu:47-214 and inverse_B_matrix.cu:98-164.
+// Returns 0 or a negative status (singular matrix).
+int build_factors(const Geometry& g, double Lx, double Ly, double Ra, double Pr, double dt,
+                  Factors& out, std::string& err);
+
+// ---------------------------------------------------------------------------------------
+// Device-side launchers (rbc_kernels.hip), templated on the real type T.
+// ---------------------------------------------------------------------------------------
+struct DevGeom {
+    int M, N, L, nq, KX, KXp, NYc, P;
+    int logM, logN;
+    double Lx, Ly;
+};
+
+struct Buffers {
+    // mode space, [p][m] complex (m fastest)
+    void* X[3];
+    void* XF;
+    void* XG;  // L complex
+    void* Nn[3];
+    void* NF;
+    void* NG;
+    void* Rp[3];
+    void* RpF;
+    void* RpG;
+    // operator-level staging (same shapes as X / Nn)
+    void* Xo[3];
+    void* XoF;
+    void* XoG;
+    void* Oo[3];
+    void* OoF;
+    void* OoG;
+    // level space
+    void* lev;    // 6 * nq * P complex  [field][l][p]
+    void* hin;    // 6 * nq * N * KXp complex  [field][l][y][kx]
+    void* hout;   // 7 * nq * N * KXp complex  [prod][l][y][kx]
+    void* prod;   // 7 * nq * P complex  [prod][l][p]
+    // tables
+    void* evtab;  // nq * 4 * L : C', C, S, S' at level l   [l][t][m]
+    void* prtab;  // nq * 5 * L : w*C', w*C'', w*C, w*S, w*S'  [l][t][m]
+    void* twM;    // M/2 complex: exp(-2 pi i k / M)
+    void* twN;    // N/2 complex
+    void* kx;     // P real
+    void* ky;     // P real
+    void* k2;     // P real
+    int* rows;    // NYc kept rows
+    int* gstart;  // ngroups + 1
+    int* gmem;    // P
+    void* gmat;   // ngroups * stride real
+    double* red;  // reduction scratch (doubles)
+    int nred;
+};
+
+struct StepCoef {
+    double a_cur, a_prev;  // X += a_cur*R ; X += a_prev*R_prev
+    int use_prev;
+};
+
+enum UpdateMode { UPD_STEP = 0, UPD_INVB = 1, UPD_IMPLICIT = 2 };
+
+template <typename T>
+struct Launch {
+    // stage 1: vertical evaluation -> six level fields (spectral)
+    static void eval(const DevGeom& g, const Buffers& b, void* const X[3], const void* XF,
+                     const void* XG, hipStream_t s);
+    // stage 2: inverse y-FFT of level fields into hybrid (kx spectral, y physical) rows
+    static void fft_y_inv(const DevGeom& g, const Buffers& b, hipStream_t s);
+    // stage 3: x C2R, pointwise products, x R2C per (level, y) row
+    static void rows(const DevGeom& g, const Buffers& b, hipStream_t s);
+    // stage 4: forward y-FFT of the products back to compact spectral layout
+    static void fft_y_fwd(const DevGeom& g, const Buffers& b, hipStream_t s);
+    // stage 5: vertical projection -> N (theta, f, g, F, G), scaled as the reference does
+    static void project(const DevGeom& g, const Buffers& b, double Pr, void* const Nout[3],
+                        void* NFo, void* NGo, hipStream_t s);
+    // stage 6: per-|k|^2 group update: inverse B, AB2 combine, implicit Euler
+    static void update(const DevGeom& g, const Buffers& b, int ngroups, int L0, int L1,
+                       int stride, double Pr, double dt, StepCoef sc, int mode,
+                       void* const Xio[3], void* const Nin[3], void* const Rp[3], hipStream_t s);
+    // F and G (mean flow) update
+    static void update_FG(const DevGeom& g, double Pr, double dt, StepCoef sc, int mode,
+                          void* XF, void* XG, const void* NF, const void* NG, void* RpF,
+                          void* RpG, hipStream_t s);
+    // make_real on the kx = 0 column (util.cu make_real)
+    static void make_real(const DevGeom& g, const Buffers& b, void* const X[3], hipStream_t s);
+    // gather/scatter between the reference layout (host-staged, T complex) and [p][m]
+    static void nusselt(const DevGeom& g, const Buffers& b, const void* ia1, hipStream_t s);
+};
+
+}  // namespace rbc

@@ -1,0 +1,159 @@
+"""GPU parity of libsdc_rbc (through the C-ABI) against the FP64 CPU oracle.
+
+Tolerances: FP64 path <= 1e-10 relative after 100 steps (and ~1e-12 per operator call);
+FP32 path relative errors as stated per test."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sdc_amd.initial import kept_mask, low_mode_theta, random_theta
+from tests.helpers import random_state, rel_err
+
+pytestmark = pytest.mark.gpu
+
+C1 = dict(M=32, N=32, L=8, Lx=4.032, Ly=4.032, Ra=5000.0, Pr=1.0, dt=1e-3)
+
+
+def _gpu(cfg, precision="f64", **kw):
+    from sdc_amd import Stepper
+    return Stepper(cfg["M"], cfg["N"], cfg["L"], cfg["Lx"], cfg["Ly"], cfg["Ra"], cfg["Pr"], cfg["dt"],
+                   precision=precision, **kw)
+
+
+def _orc(cfg, nthreads=8):
+    return O.Stepper(cfg["M"], cfg["N"], cfg["L"], cfg["Lx"], cfg["Ly"], cfg["Ra"], cfg["Pr"], cfg["dt"],
+                     nthreads=nthreads)
+
+
+def _cmp(a, b, keep):
+    errs = []
+    for i in range(3):
+        errs.append(rel_err(a[i][:, keep], b[i][:, keep]))
+    for i in (3, 4):
+        errs.append(rel_err(a[i], b[i]) if np.abs(b[i]).max() > 0 else float(np.abs(a[i]).max()))
+    return errs
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _build():
+    O.build()
+
+
+def test_c1_operators_fp64():
+    print("case: C1 fp64 operators", flush=True)
+    st = random_state(C1["M"], C1["N"], C1["L"], seed=11, amp=1e-2)
+    keep = kept_mask(C1["M"], C1["N"])
+    with _gpu(C1) as G:
+        o = _orc(C1)
+        e = _cmp(G.calculate_nonlinear(*st), o.nonlinear(*st), keep)
+        assert max(e) < 1e-12, e
+        e = _cmp(G.calculate_inverse(*st), o.inverse_B(*st), keep)
+        assert max(e) < 1e-14, e
+        e = _cmp(G.calculate_implicit(*st), o.implicit(*st), keep)
+        assert max(e) < 1e-13, e
+
+
+def test_c1_100_steps_fp64_parity():
+    """North-star criterion: FP64 fields within 1e-10 relative after 100 steps (C1)."""
+    print("case: C1 fp64 100 steps", flush=True)
+    st = low_mode_theta(C1["M"], C1["N"], C1["L"], seed=1, amp=1e-3)
+    # a developed-looking start: add small f, g, F, G so every path is exercised
+    extra = random_state(C1["M"], C1["N"], C1["L"], seed=2, amp=1e-3)
+    st = [st[0] + extra[0], extra[1], extra[2], extra[3], extra[4]]
+    keep = kept_mask(C1["M"], C1["N"])
+    with _gpu(C1) as G:
+        o = _orc(C1)
+        G.set_state(*st)
+        o.set_state(*st)
+        G.step_time(100)
+        o.step(100)
+        e = _cmp(G.get_state(), o.get_state(), keep)
+        print("C1 fp64 100-step rel errors", e)
+        assert max(e) < 1e-10, e
+        nu_g, nu_o = G.nusselt(), o.nusselt()
+        assert np.abs(nu_g - nu_o).max() < 1e-10
+
+
+def test_c1_fp32_vs_fp64_oracle():
+    print("case: C1 fp32 50 steps", flush=True)
+    st = low_mode_theta(C1["M"], C1["N"], C1["L"], seed=1, amp=1e-1)
+    keep = kept_mask(C1["M"], C1["N"])
+    with _gpu(C1, "f32") as G:
+        o = _orc(C1)
+        G.set_state(*st)
+        o.set_state(*st)
+        G.step_time(50)
+        o.step(50)
+        e = _cmp(G.get_state(), o.get_state(), keep)
+        print("C1 fp32 50-step rel errors", e)
+        assert max(e[:3]) < 1e-4, e
+
+
+@pytest.mark.parametrize("M,N,L", [(64, 32, 4), (32, 64, 6), (128, 128, 16)])
+def test_nonlinear_shapes_fp64(M, N, L):
+    print(f"case: nonlinear fp64 {M}x{N}x{L}", flush=True)
+    cfg = dict(M=M, N=N, L=L, Lx=3.3, Ly=4.1, Ra=2e4, Pr=0.7, dt=2e-4)
+    st = random_state(M, N, L, seed=M + N + L, amp=1e-2, decay=0.05)
+    keep = kept_mask(M, N)
+    with _gpu(cfg) as G:
+        o = _orc(cfg)
+        e = _cmp(G.calculate_nonlinear(*st), o.nonlinear(*st), keep)
+        assert max(e) < 1e-11, e
+
+
+def test_bundled_512_state_fp32_and_fp64():
+    """The reference's bundled 512^2 x 2 state (FP32) through the nonlinear operator."""
+    print("case: bundled 512x512x2 state", flush=True)
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "data_state_kept.npz"))
+    M, N, L = 512, 512, 2
+    keep = kept_mask(M, N)
+    st = []
+    for n in ("theta", "f", "g"):
+        a = np.zeros((L, N, M // 2 + 1), np.complex128)
+        a[:, keep] = d[n]
+        st.append(a)
+    st += [d["F"].astype(np.complex128), d["G"].astype(np.complex128)]
+    cfg = dict(M=M, N=N, L=L, Lx=2 * 2.016 * 2, Ly=2 * 2.016 * 2, Ra=5000.0, Pr=1.0, dt=1e-3)
+    o = _orc(cfg)
+    ref = o.nonlinear(*st)
+    with _gpu(cfg, "f64") as G:
+        e = _cmp(G.calculate_nonlinear(*st), ref, keep)
+        assert max(e) < 1e-11, e
+    with _gpu(cfg, "f32") as G:
+        e = _cmp(G.calculate_nonlinear(*st), ref, keep)
+        print("512^2x2 fp32 nonlinear rel errors", e)
+        assert max(e[:3]) < 2e-5, e
+
+
+def test_c2_fp64_five_steps():
+    print("case: C2 fp64 5 steps", flush=True)
+    cfg = dict(M=128, N=128, L=16, Lx=4.0, Ly=4.0, Ra=1e5, Pr=0.7, dt=2e-4)
+    st = random_theta(128, 128, 16, seed=7, amp=1e-3)
+    keep = kept_mask(128, 128)
+    with _gpu(cfg) as G:
+        o = _orc(cfg, nthreads=16)
+        G.set_state(*st)
+        o.set_state(*st)
+        G.step_time(5)
+        o.step(5)
+        e = _cmp(G.get_state(), o.get_state(), keep)
+        assert max(e[:3]) < 1e-11, e
+
+
+def test_gpu_state_invariants_after_steps():
+    print("case: invariants fp32", flush=True)
+    st = low_mode_theta(C1["M"], C1["N"], C1["L"], seed=3, amp=1e-1)
+    keep = kept_mask(C1["M"], C1["N"])
+    with _gpu(C1, "f32") as G:
+        G.set_state(*st)
+        G.step_time(10)
+        th, f, g, F, G_ = G.get_state()
+        N = C1["N"]
+        for a in (th, f, g):
+            assert np.all(a[:, ~keep] == 0)
+            assert np.all(a[:, 0, 0].imag == 0)
+            for r in range(1, N // 2):
+                assert np.array_equal(a[:, r, 0], np.conj(a[:, N - r, 0]))
+        assert np.all(F.imag == 0) and np.all(G_.imag == 0)
