@@ -1,0 +1,150 @@
+"""Throughput of the Rayleigh-Benard IMEX step on one MI355X.
+
+Workload (one "step" = one IMEX time step of the whole field): 512 x 512 horizontal grid,
+32 vertical Galerkin modes per field, Ra = 1e6, Pr = 0.7, box 4 x 4 x 1, dt = 5e-5, from a
+seeded theta ~ 1e-3 N(0,1) in every kept mode (synthetic input, resident in HBM before
+timing).  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f32|bf16x3|f64]
+                  [--no-cpu] [--cpu-threads T]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "timesteps/sec at 512²×32 (+HBM/MFMA roofline %); closure vs plain Nu err & wall time"
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = dict(M=512, N=512, L=32, Lx=4.0, Ly=4.0, Ra=1e6, Pr=0.7, dt=5e-5)
+
+
+def stage_bytes(st, cs):
+    """Algorithmic HBM bytes per launch of each stage (cs = bytes per complex value).
+    P = kept horizontal modes, H = N * KX hybrid row entries, nq levels, L modes."""
+    P = st.kx_kept * st.ky_kept
+    H = WORKLOAD["N"] * st.kx_kept
+    nq, L = st.nq, WORKLOAD["L"]
+    return {
+        "eval": (3 * L * P + 6 * nq * P) * cs,
+        "fft_y_inv": (6 * nq * P + 6 * nq * H) * cs,
+        "rows_products": (6 * nq * H + 7 * nq * H) * cs,
+        "fft_y_fwd": (7 * nq * H + 7 * nq * P) * cs,
+        "project": (7 * nq * P + 3 * L * P) * cs,
+        # state, N and history in/out; factor records once per |k|^2 group
+        "update": (9 * L * P + 6 * L * P) * cs + st.n_unique_k * (2 * L * L + L * L // 2) * (cs // 2),
+        "update_FG": 8 * L * cs,
+        "make_real": 2 * 3 * st.ky_kept * L * cs,
+    }
+
+
+def cpu_baseline(threads, budget_s=20.0):
+    """Oracle (FP64 C restatement of the reference algorithm) on this host: one step of the
+    same workload with an evenly strided sample of the nonlinear term's (i,j) pair loop,
+    extrapolated to all L^2 pairs."""
+    from oracle import oracle as O
+    from sdc_amd.initial import random_theta
+    W = WORKLOAD
+    O.build()
+    S = O.Stepper(W["M"], W["N"], W["L"], W["Lx"], W["Ly"], W["Ra"], W["Pr"], W["dt"], nthreads=threads)
+    S.set_state(*random_theta(W["M"], W["N"], W["L"], seed=3, amp=1e-3))
+    # calibrate with a coarse sample, then pick the stride that fits the budget
+    L2 = W["L"] ** 2
+    probe = O.time_step_sampled(S, L2 // max(1, threads))
+    per_pair = probe["pairs_sampled"] / max(1, threads)
+    fixed = probe["phys"] + probe["rest"]
+    n_pairs = max(threads, int((budget_s - fixed) / max(per_pair, 1e-9)))
+    stride = max(1, L2 // n_pairs)
+    r = O.time_step_sampled(S, stride)
+    S.close()
+    return {
+        "value": 1.0 / r["s_per_step"],
+        "unit": "timesteps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"one 512x512x32 step (FP64 oracle, OpenMP {threads} threads) with every {stride}-th of the "
+                   f"{L2} (i,j) pairs of the nonlinear term, extrapolated; phases s: physical fields "
+                   f"{r['phys']:.2f}, sampled pairs {r['pairs_sampled']:.2f}, rest {r['rest']:.2f}"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--precision", default="f32", choices=["f32", "bf16x3", "f64"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU layout of this "
+              "path is described in DESIGN.md and not measured here.", file=sys.stderr)
+        sys.exit(2)
+
+    import sdc_amd
+    from sdc_amd.initial import random_theta
+    W = WORKLOAD
+    st = sdc_amd.Stepper(W["M"], W["N"], W["L"], W["Lx"], W["Ly"], W["Ra"], W["Pr"], W["dt"],
+                         precision=a.precision, device=0)
+    st.set_state(*random_theta(W["M"], W["N"], W["L"], seed=3, amp=1e-3))
+    try:
+        import torch
+        tsync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:
+        tsync = lambda: None  # noqa: E731
+    st.step_time(a.warmup)
+    st.synchronize()
+    tsync()
+    t0 = time.perf_counter()
+    stages = st.profile_steps(a.steps)  # K steps, HIP events around every stage on the library's stream
+    st.synchronize()
+    tsync()
+    t1 = time.perf_counter()
+    sec = t1 - t0
+    value = a.steps / sec
+
+    cs = 16 if a.precision == "f64" else 8
+    nbytes = stage_bytes(st, cs)
+    dom = max(stages, key=lambda k: stages[k])
+    achieved = nbytes[dom] / (stages[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+                "stage_GBs": {k: round(nbytes[k] / (stages[k] * 1e-3) / 1e9, 1) for k in stages if stages[k] > 0}}
+    nq, kx, ky, ng = st.nq, st.kx_kept, st.ky_kept, st.n_unique_k
+    st.close()
+
+    cpu = None
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(threads)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "timesteps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1e3 * sec / a.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f32": "f32", "bf16x3": "bf16x3+f32", "f64": "f64"}[a.precision],
+        "data": "synthetic (seeded theta ~ 1e-3 N(0,1) in every kept mode)",
+        "config": {"workload": "RBC IMEX step, 512x512 grid x 32 vertical modes, Ra=1e6, Pr=0.7, 4x4x1 box, "
+                               "dt=5e-5", "M": W["M"], "N": W["N"], "L": W["L"], "nq_levels": nq,
+                   "kept_modes": [kx, ky], "unique_k2_groups": ng, "precision": a.precision,
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -177,3 +177,12 @@ class Stepper:
         out = np.zeros(4 * self.L * self.L)
         assert lib().orc_implicit_factor(self.h, col, row, _p(out)) == 0
         return out.reshape(4, self.L, self.L)
+
+
+def time_step_sampled(stepper, pair_stride):
+    """Sampled CPU timing of one reference step (see orc_time_step_sampled)."""
+    f = lib().orc_time_step_sampled
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    out = np.zeros(4)
+    assert f(stepper.h, int(pair_stride), _p(out)) == 0
+    return {"s_per_step": out[0], "phys": out[1], "pairs_sampled": out[2], "rest": out[3]}

@@ -36,6 +36,14 @@
 
 typedef double complex cplx;
 
+static double omp_wtime_(void) {
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return 0.0;
+#endif
+}
+
 /* ======================================================================================== */
 /* lambda_k (coeff.cu:1232-1257)                                                            */
 /* ======================================================================================== */
@@ -366,6 +374,9 @@ static double delta(int a, int b) { return a == b ? 1.0 : 0.0; }
  * inverse_B_matrix.cu:98-164 do (FP64, PRECISION_DOUBLE build). */
 static int build_factors(orc_ctx* c) {
     const int L = c->L, n2 = 2 * L;
+    int rc_all = 0;
+#pragma omp parallel num_threads(c->nthreads) reduction(| : rc_all)
+    {
     double* ident = malloc(sizeof(double) * n2 * n2);
     double* invB = malloc(sizeof(double) * n2 * n2);
     double* tmpB = malloc(sizeof(double) * n2 * n2);
@@ -374,7 +385,8 @@ static int build_factors(orc_ctx* c) {
     double* work = malloc(sizeof(double) * n2 * n2);
     int* perm = malloc(sizeof(int) * n2);
     int rc = 0;
-    for (int k = 0; k < c->mox && !rc; ++k) {
+#pragma omp for schedule(dynamic, 1)
+    for (int k = 0; k < c->mox; ++k) {
         double mode_x = k * 2.0 * M_PI / c->Lx;
         for (int l = 0; l < c->N && !rc; ++l) {
             double mode_y = 2.0 * M_PI / c->Ly;
@@ -449,7 +461,9 @@ static int build_factors(orc_ctx* c) {
         }
     }
     free(ident); free(invB); free(tmpB); free(Lm); free(C); free(work); free(perm);
-    return rc;
+    rc_all |= rc;
+    }
+    return rc_all;
 }
 
 orc_ctx* orc_create(int M, int N, int L, double Lx, double Ly, double Ra, double Pr, double dt,
@@ -649,12 +663,14 @@ static double coef(const orc_ctx* c, const summand* s, int i, int j, int k) {
     }
 }
 
-int orc_nonlinear(orc_ctx* c, const double* th_, const double* f_, const double* g_, const double* F_,
-                  const double* G_, double* oth, double* of, double* og, double* oF, double* oG) {
+static int nonlinear_impl(orc_ctx* c, const double* th_, const double* f_, const double* g_, const double* F_,
+                          const double* G_, double* oth, double* of, double* og, double* oF, double* oG,
+                          int pair_stride, double* t_phys, double* t_pairs) {
     const int L = c->L, M = c->M, N = c->N, mox = c->mox;
     const size_t plane = c->plane, X = (size_t)M * N;
     const cplx *th = (const cplx*)th_, *f = (const cplx*)f_, *g = (const cplx*)g_;
     const cplx *F = (const cplx*)F_, *G = (const cplx*)G_;
+    const double t_start = omp_wtime_();
     /* physical fields: NPHYS x L planes of M*N reals (part 1, :96-233) */
     double* phys = malloc(sizeof(double) * NPHYS * (size_t)L * X);
     cplx* spec = malloc(sizeof(cplx) * plane);
@@ -684,6 +700,8 @@ int orc_nonlinear(orc_ctx* c, const double* th_, const double* f_, const double*
         orc_fft_c2r(M, N, (double*)spec, phys + ((size_t)DYTH * L + m) * X);
     }
     free(spec);
+    if (t_phys) *t_phys = omp_wtime_() - t_start;
+    double t_pair0 = omp_wtime_();
 
     const int nt = c->nthreads;
     /* per-thread accumulators: theta, f, g (L planes each) and F, G (L) */
@@ -704,7 +722,7 @@ int orc_nonlinear(orc_ctx* c, const double* th_, const double* f_, const double*
         cplx* P = malloc(sizeof(cplx) * 20 * plane);
         cplx* D = malloc(sizeof(cplx) * plane);
 #pragma omp for schedule(dynamic, 1)
-        for (int pair = 0; pair < L * L; ++pair) {
+        for (int pair = 0; pair < L * L; pair += pair_stride) {
             const int j = pair / L, i = pair % L;
             for (int p = 0; p < 20; ++p) {
                 const double* a = phys + ((size_t)PROD[p][0] * L + i) * X;
@@ -741,6 +759,7 @@ int orc_nonlinear(orc_ctx* c, const double* th_, const double* f_, const double*
         free(buf); free(P); free(D);
     }
     free(phys);
+    if (t_pairs) *t_pairs = omp_wtime_() - t_pair0;
     /* reduce in thread order, then scale (:1366-1370) */
     cplx* outs[3] = {(cplx*)oth, (cplx*)of, (cplx*)og};
     const double sc[3] = {-1.0, -1.0 / c->Pr, -1.0 / c->Pr};
@@ -762,6 +781,11 @@ int orc_nonlinear(orc_ctx* c, const double* th_, const double* f_, const double*
     free(acc);
     (void)mox;
     return 0;
+}
+
+int orc_nonlinear(orc_ctx* c, const double* th, const double* f, const double* g, const double* F,
+                  const double* G, double* oth, double* of, double* og, double* oF, double* oG) {
+    return nonlinear_impl(c, th, f, g, F, G, oth, of, og, oF, oG, 1, NULL, NULL);
 }
 
 /* ======================================================================================== */
@@ -930,4 +954,57 @@ int orc_nusselt(orc_ctx* c, double* nu3) {
         }
     nu3[2] = 1.0 + s;
     return 0;
+}
+
+/* ======================================================================================== */
+/* Sampled timing of one reference step (CPU baseline): the whole step runs, except that the */
+/* (i,j) pair loop of the nonlinear term visits every pair_stride-th pair; the pair phase is */
+/* scaled back to all L^2 pairs.  out: [0] extrapolated s/step, [1] physical-field phase,   */
+/* [2] sampled pair phase (unscaled), [3] rest of the step (inverse B, AB2, implicit, masks). */
+/* The state is left advanced by one (approximate, when sampled) step.                      */
+/* ======================================================================================== */
+int orc_time_step_sampled(orc_ctx* c, int pair_stride, double* out) {
+    const int L = c->L;
+    const size_t fs = c->field;
+    if (pair_stride < 1) pair_stride = 1;
+    cplx* N5[5];
+    cplx* R5[5];
+    for (int i = 0; i < 5; ++i) {
+        size_t n = i < 3 ? fs : (size_t)L;
+        N5[i] = calloc(n, sizeof(cplx));
+        R5[i] = calloc(n, sizeof(cplx));
+    }
+    cplx* X[5] = {c->th, c->f, c->g, c->F, c->G};
+    const double dt = c->dt;
+    double t0 = omp_wtime_();
+    for (int i = 0; i < 3; ++i) orc_dealias(c->mox, c->N, L, (double*)X[i]);
+    double tph = 0, tpa = 0;
+    int rc = nonlinear_impl(c, (double*)X[0], (double*)X[1], (double*)X[2], (double*)X[3], (double*)X[4],
+                            (double*)N5[0], (double*)N5[1], (double*)N5[2], (double*)N5[3], (double*)N5[4],
+                            pair_stride, &tph, &tpa);
+    double t1 = omp_wtime_();
+    orc_inverse_B(c, (double*)N5[0], (double*)N5[1], (double*)N5[2], (double*)N5[3], (double*)N5[4],
+                  (double*)R5[0], (double*)R5[1], (double*)R5[2], (double*)R5[3], (double*)R5[4]);
+    for (int i = 0; i < 5; ++i) {
+        size_t n = i < 3 ? fs : (size_t)L;
+        for (size_t q = 0; q < n; ++q) X[i][q] += R5[i][q] * (1.5 * dt);
+        for (size_t q = 0; q < n; ++q) X[i][q] += c->r1[i][q] * (-0.5 * dt);
+        memcpy(c->r1[i], R5[i], sizeof(cplx) * n);
+    }
+    for (int i = 0; i < 3; ++i) orc_dealias(c->mox, c->N, L, (double*)X[i]);
+    orc_implicit(c, (double*)X[0], (double*)X[1], (double*)X[2], (double*)X[3], (double*)X[4],
+                 (double*)N5[0], (double*)N5[1], (double*)N5[2], (double*)N5[3], (double*)N5[4]);
+    for (int i = 0; i < 5; ++i) memcpy(X[i], N5[i], sizeof(cplx) * (i < 3 ? fs : (size_t)L));
+    for (int i = 0; i < 3; ++i) orc_make_real(c->mox, c->N, L, (double*)X[i]);
+    double t2 = omp_wtime_();
+    c->have_hist = 1;
+    const double npairs = (double)L * L;
+    const double nsampled = (double)((L * L + pair_stride - 1) / pair_stride);
+    double rest = (t1 - t0 - tph - tpa) + (t2 - t1);
+    out[0] = tph + tpa * (npairs / nsampled) + rest;
+    out[1] = tph;
+    out[2] = tpa;
+    out[3] = rest;
+    for (int i = 0; i < 5; ++i) { free(N5[i]); free(R5[i]); }
+    return rc;
 }

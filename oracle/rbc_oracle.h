@@ -72,6 +72,10 @@ void orc_reset_history(orc_ctx* c);
  * nu[0] = plate Nu at z=-1/2, nu[1] = plate Nu at z=+1/2, nu[2] = 1 + <w theta>. */
 int orc_nusselt(orc_ctx* c, double* nu3);
 
+/* CPU-baseline timing of one step with every pair_stride-th (i,j) pair of the nonlinear term;
+ * out[4] = {extrapolated seconds per step, physical-field phase, sampled pair phase, rest}. */
+int orc_time_step_sampled(orc_ctx* c, int pair_stride, double* out);
+
 /* Per-wavenumber implicit factor block for (kx col, row): out 4*L*L doubles (TT, Tf, fT, ff). */
 int orc_implicit_factor(orc_ctx* c, int col, int row, double* out);
 

@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -44,7 +45,9 @@ struct rbc_ctx {
     bool have_hist = false;
     std::string err;
     std::vector<unsigned char> stage;  // host staging for one compact field
-    hipEvent_t ev[RBC_NSTAGES + 1]{};
+    static constexpr int kRing = 64;                 // steps of stage events in flight
+    hipEvent_t ev_ring[kRing][RBC_NSTAGES + 1]{};
+    hipEvent_t* ev = ev_ring[0];                     // events of the step being recorded
     bool events = false;
 
     size_t csz() const { return 2 * rsz; }
@@ -58,7 +61,8 @@ struct rbc_ctx {
         if (stream) hipStreamSynchronize(stream);
         for (void* a : allocs) hipFree(a);
         if (events)
-            for (auto& e : ev) hipEventDestroy(e);
+            for (auto& row : ev_ring)
+                for (auto& e : row) hipEventDestroy(e);
         if (stream) hipStreamDestroy(stream);
     }
 };
@@ -274,7 +278,8 @@ int rbc_create(const rbc_params* p, rbc_ctx** out) {
         if (build_factors(g, p->Lx, p->Ly, p->Ra, p->Pr, p->dt, c->fac, err)) return fail(nullptr, RBC_ERR_NUMERIC, err);
 
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
+        for (auto& row : c->ev_ring)
+            for (auto& e : row) HIPCHK(hipEventCreate(&e));
         c->events = true;
 
         const size_t cs = c->csz(), P = d.P, L = d.L;
@@ -334,10 +339,10 @@ int rbc_create(const rbc_params* p, rbc_ctx** out) {
         b.prtab = c->dalloc(pr.size() * c->rsz);
         upload_reals(c.get(), b.evtab, ev);
         upload_reals(c.get(), b.prtab, pr);
-        // twiddles exp(-2 pi i k / n), built in FP64
+        // twiddles exp(-2 pi i k / n), k < n, built in FP64
         auto tw = [&](int n) {
-            std::vector<double> t((size_t)n);  // n/2 complex
-            for (int k = 0; k < n / 2; ++k) {
+            std::vector<double> t((size_t)2 * n);
+            for (int k = 0; k < n; ++k) {
                 t[2 * k] = std::cos(2.0 * M_PI * k / n);
                 t[2 * k + 1] = -std::sin(2.0 * M_PI * k / n);
             }
@@ -550,16 +555,24 @@ int rbc_profile_steps(rbc_ctx* c, int nsteps, double* ms) {
     try {
         hipSetDevice(c->p.device);
         for (int i = 0; i < RBC_NSTAGES; ++i) ms[i] = 0.0;
-        for (int s = 0; s < nsteps; ++s) {
-            if (c->rsz == 8) one_step<double>(c, true);
-            else one_step<float>(c, true);
-            HIPCHK(hipEventSynchronize(c->ev[RBC_NSTAGES]));
-            for (int i = 0; i < RBC_NSTAGES; ++i) {
-                float t = 0.f;
-                HIPCHK(hipEventElapsedTime(&t, c->ev[i], c->ev[i + 1]));
-                ms[i] += t;
+        // record up to kRing steps of stage events, then read them back; the host waits only
+        // once per kRing steps, so the queue stays full as in rbc_step.
+        for (int s0 = 0; s0 < nsteps; s0 += rbc_ctx::kRing) {
+            const int n = std::min(rbc_ctx::kRing, nsteps - s0);
+            for (int s = 0; s < n; ++s) {
+                c->ev = c->ev_ring[s];
+                if (c->rsz == 8) one_step<double>(c, true);
+                else one_step<float>(c, true);
             }
+            HIPCHK(hipEventSynchronize(c->ev_ring[n - 1][RBC_NSTAGES]));
+            for (int s = 0; s < n; ++s)
+                for (int i = 0; i < RBC_NSTAGES; ++i) {
+                    float t = 0.f;
+                    HIPCHK(hipEventElapsedTime(&t, c->ev_ring[s][i], c->ev_ring[s][i + 1]));
+                    ms[i] += t;
+                }
         }
+        c->ev = c->ev_ring[0];
         for (int i = 0; i < RBC_NSTAGES; ++i) ms[i] /= nsteps;
     } catch (const HipError& e) {
         return hip_fail(c, e);

@@ -98,8 +98,8 @@ struct Buffers {
     // tables
     void* evtab;  // nq * 4 * L : C', C, S, S' at level l   [l][t][m]
     void* prtab;  // nq * 5 * L : w*C', w*C'', w*C, w*S, w*S'  [l][t][m]
-    void* twM;    // M/2 complex: exp(-2 pi i k / M)
-    void* twN;    // N/2 complex
+    void* twM;    // M complex: exp(-2 pi i k / M), k < M
+    void* twN;    // N complex
     void* kx;     // P real
     void* ky;     // P real
     void* k2;     // P real

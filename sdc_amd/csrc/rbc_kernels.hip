@@ -9,6 +9,8 @@
 // implicit Euler), make_real.  Everything is templated on the real type T (float/double).
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "rbc_internal.hpp"
 
 namespace rbc {
@@ -43,7 +45,7 @@ __device__ __forceinline__ cpx<T> imul(cpx<T> a, T s) { return mk(-s * a.y, s * 
 
 // ---------------------------------------------------------------------------------------
 // Radix-2 Stockham FFT on nb sequences of length n = 2^logn held in LDS (a), with b as the
-// ping-pong buffer.  tw[k] = exp(-2 pi i k / n), k < n/2.  Unnormalised; inverse uses
+// ping-pong buffer.  tw[k] = exp(-2 pi i k / n) (the first n/2 entries are used).  Unnormalised; inverse uses
 // conjugate twiddles.  All threads of the block take part; returns the buffer holding the
 // result.  Ends with a barrier.
 // ---------------------------------------------------------------------------------------
@@ -557,9 +559,254 @@ __global__ void __launch_bounds__(256) k_nusselt(DevGeom g, const cpx<T>* __rest
     }
 }
 
+// =======================================================================================
+// 512-point fast paths (FP32): one wavefront per transform, radix-8 Stockham in registers.
+// Lane j holds x[j + 64 r], r = 0..7, on entry and X[j + 64 r] on exit.  Two exchanges
+// through a per-wave LDS scratch of 576 complex (one pad slot every 8 to spread banks).
+// =======================================================================================
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int pad8(int i) { return i + (i >> 3); }
+
+// 8-point DFT in registers (decimation in frequency, outputs reordered to natural order).
+template <typename T, bool INV>
+__device__ __forceinline__ void dft8(cpx<T> v[8]) {
+    const T h = T(0.70710678118654752440);
+    // stage 1 (span 4) with twiddles w8^i, w8 = exp(-+ i pi / 4)
+    cpx<T> a[8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        a[i] = v[i] + v[i + 4];
+        a[i + 4] = v[i] - v[i + 4];
+    }
+    // a5 *= w8, a6 *= w8^2, a7 *= w8^3
+    if (!INV) {
+        a[5] = mk(h * (a[5].x + a[5].y), h * (a[5].y - a[5].x));
+        a[6] = mk(a[6].y, -a[6].x);
+        a[7] = mk(h * (a[7].y - a[7].x), -h * (a[7].x + a[7].y));
+    } else {
+        a[5] = mk(h * (a[5].x - a[5].y), h * (a[5].y + a[5].x));
+        a[6] = mk(-a[6].y, a[6].x);
+        a[7] = mk(-h * (a[7].x + a[7].y), h * (a[7].x - a[7].y));
+    }
+    // stage 2 (span 2) with w4 = -+ i on the second element of each pair
+    cpx<T> b[8];
+#pragma unroll
+    for (int hh = 0; hh < 8; hh += 4) {
+        b[hh + 0] = a[hh + 0] + a[hh + 2];
+        b[hh + 2] = a[hh + 0] - a[hh + 2];
+        b[hh + 1] = a[hh + 1] + a[hh + 3];
+        const cpx<T> d = a[hh + 1] - a[hh + 3];
+        b[hh + 3] = INV ? mk(-d.y, d.x) : mk(d.y, -d.x);
+    }
+    // stage 3 (span 1); outputs in bit-reversed order
+    const cpx<T> c0 = b[0] + b[1], c1 = b[0] - b[1], c2 = b[2] + b[3], c3 = b[2] - b[3];
+    const cpx<T> c4 = b[4] + b[5], c5 = b[4] - b[5], c6 = b[6] + b[7], c7 = b[6] - b[7];
+    v[0] = c0;
+    v[1] = c4;
+    v[2] = c2;
+    v[3] = c6;
+    v[4] = c1;
+    v[5] = c5;
+    v[6] = c3;
+    v[7] = c7;
+}
+
+// 512-point FFT of one wavefront; tw = LDS table exp(-2 pi i k / 512), k < 512.
+template <typename T, bool INV>
+__device__ __forceinline__ void fft512_wave(cpx<T> v[8], cpx<T>* sb, const cpx<T>* tw, int lane) {
+    dft8<T, INV>(v);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) sb[pad8(8 * lane + r)] = v[r];
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = sb[pad8(lane + 64 * r)];
+#pragma unroll
+    for (int r = 1; r < 8; ++r) {
+        cpx<T> w = tw[(lane & 7) * r * 8];
+        if (INV) w.y = -w.y;
+        v[r] = cmul(v[r], w);
+    }
+    dft8<T, INV>(v);
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < 8; ++r) sb[pad8((lane >> 3) * 64 + (lane & 7) + 8 * r)] = v[r];
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = sb[pad8(lane + 64 * r)];
+#pragma unroll
+    for (int r = 1; r < 8; ++r) {
+        cpx<T> w = tw[lane * r];
+        if (INV) w.y = -w.y;
+        v[r] = cmul(v[r], w);
+    }
+    dft8<T, INV>(v);
+    wave_sync();  // the scratch may be reused right after
+}
+
+// x-rows at M = 512: one wave per (level, y) row.  Same arithmetic as k_rows.
+__global__ void __launch_bounds__(64) k_rows512(DevGeom g, const cpx<float>* __restrict__ hin,
+                                                const cpx<float>* __restrict__ twM, cpx<float>* __restrict__ hout) {
+    __shared__ cpx<float> tw[512];
+    __shared__ cpx<float> sb[576];
+    const int lane = threadIdx.x;
+    const int KX = g.KX, KXp = g.KXp, nq = g.nq, N = g.N;
+    const int y = blockIdx.x, l = blockIdx.y;
+    const size_t fstride = (size_t)nq * N * KXp;
+    const size_t roff = ((size_t)l * N + y) * KXp;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) tw[lane + 64 * r] = twM[lane + 64 * r];
+    __syncthreads();
+    cpx<float> z[3][8];
+#pragma unroll
+    for (int pr = 0; pr < 3; ++pr) {
+        const cpx<float>* x1 = hin + (size_t)(2 * pr) * fstride + roff;
+        const cpx<float>* x2 = hin + (size_t)(2 * pr + 1) * fstride + roff;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int n = lane + 64 * r;
+            cpx<float> v = mk(0.f, 0.f);
+            if (n < KX) {
+                cpx<float> a = x1[n], b = x2[n];
+                if (n == 0) {
+                    a.y = 0.f;
+                    b.y = 0.f;
+                }
+                v = mk(a.x - b.y, a.y + b.x);  // X1 + i X2
+            } else if (n > 512 - KX) {
+                const cpx<float> a = x1[512 - n], b = x2[512 - n];
+                v = mk(a.x + b.y, -a.y + b.x);  // conj(X1) + i conj(X2)
+            }
+            z[pr][r] = v;
+        }
+        fft512_wave<float, true>(z[pr], sb, tw, lane);
+    }
+    const float s = 1.f / (512.f * (float)N);
+    cpx<float> p[4][8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const float u = z[0][r].x, v = z[0][r].y, w = z[1][r].x, tx = z[1][r].y, ty = z[2][r].x, tz = z[2][r].y;
+        p[0][r] = mk(s * u * u, s * u * v);
+        p[1][r] = mk(s * v * v, s * u * w);
+        p[2][r] = mk(s * v * w, s * w * w);
+        p[3][r] = mk(s * (u * tx + v * ty + w * tz), 0.f);
+    }
+#pragma unroll
+    for (int pr = 0; pr < 4; ++pr) {
+        fft512_wave<float, false>(p[pr], sb, tw, lane);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) sb[pad8(lane + 64 * r)] = p[pr][r];
+        wave_sync();
+        cpx<float>* o1 = hout + (size_t)(2 * pr) * fstride + roff;
+        cpx<float>* o2 = hout + (size_t)(2 * pr + 1) * fstride + roff;
+        for (int k = lane; k < KX; k += 64) {
+            const cpx<float> zk = sb[pad8(k)], zm = sb[pad8((512 - k) & 511)];
+            o1[k] = mk(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+            if (pr < 3) o2[k] = mk(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+        }
+        wave_sync();
+    }
+}
+
+// y inverse at N = 512: block of 4 waves handles 8 kx columns of one (field, level); each
+// wave transforms 2 columns into an LDS tile [y][8+1] which is then written out row-wise.
+constexpr int YT = 8;
+__global__ void __launch_bounds__(256) k_fft_y_inv512(DevGeom g, const cpx<float>* __restrict__ lev,
+                                                      const cpx<float>* __restrict__ twN,
+                                                      cpx<float>* __restrict__ hin) {
+    __shared__ cpx<float> tw[512];
+    __shared__ cpx<float> tile[512 * (YT + 1)];
+    __shared__ cpx<float> sbw[4][576];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int NYc = g.NYc, P = g.P, nq = g.nq, KX = g.KX;
+    const int kx0 = blockIdx.x * YT, l = blockIdx.y, field = blockIdx.z;
+    const int a = (NYc - 1) / 2, shift = 512 - NYc;  // kept rows [0, a] U [512 - a, 511]
+    for (int e = threadIdx.x; e < 512; e += 256) tw[e] = twN[e];
+    __syncthreads();
+    const cpx<float>* src = lev + ((size_t)field * nq + l) * P;
+    for (int cc = 0; cc < 2; ++cc) {
+        const int c = wave * 2 + cc, kx = kx0 + c;
+        cpx<float> v[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int n = lane + 64 * r;
+            v[r] = mk(0.f, 0.f);
+            if (kx < KX) {
+                if (n <= a) v[r] = src[(size_t)kx * NYc + n];
+                else if (n >= 512 - a) v[r] = src[(size_t)kx * NYc + n - shift];
+            }
+        }
+        fft512_wave<float, true>(v, sbw[wave], tw, lane);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) tile[(lane + 64 * r) * (YT + 1) + c] = v[r];
+    }
+    __syncthreads();
+    const int ncol = min(YT, KX - kx0);
+    cpx<float>* dst = hin + ((size_t)field * nq + l) * (size_t)512 * g.KXp + kx0;
+    for (int e = threadIdx.x; e < 512 * YT; e += 256) {
+        const int y = e / YT, c = e - y * YT;
+        if (c < ncol) dst[(size_t)y * g.KXp + c] = tile[y * (YT + 1) + c];
+    }
+}
+
+// y forward at N = 512: the transpose of k_fft_y_inv512 (tile in, kept rows out).
+__global__ void __launch_bounds__(256) k_fft_y_fwd512(DevGeom g, const cpx<float>* __restrict__ hout,
+                                                      const cpx<float>* __restrict__ twN,
+                                                      cpx<float>* __restrict__ prod) {
+    __shared__ cpx<float> tw[512];
+    __shared__ cpx<float> tile[512 * (YT + 1)];
+    __shared__ cpx<float> sbw[4][576];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int NYc = g.NYc, P = g.P, nq = g.nq, KX = g.KX;
+    const int kx0 = blockIdx.x * YT, l = blockIdx.y, q = blockIdx.z;
+    const int a = (NYc - 1) / 2, shift = 512 - NYc;
+    for (int e = threadIdx.x; e < 512; e += 256) tw[e] = twN[e];
+    const int ncol = min(YT, KX - kx0);
+    const cpx<float>* src = hout + ((size_t)q * nq + l) * (size_t)512 * g.KXp + kx0;
+    for (int e = threadIdx.x; e < 512 * YT; e += 256) {
+        const int y = e / YT, c = e - y * YT;
+        tile[y * (YT + 1) + c] = (c < ncol) ? src[(size_t)y * g.KXp + c] : mk(0.f, 0.f);
+    }
+    __syncthreads();
+    cpx<float>* dst = prod + ((size_t)q * nq + l) * P;
+    for (int cc = 0; cc < 2; ++cc) {
+        const int c = wave * 2 + cc, kx = kx0 + c;
+        cpx<float> v[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v[r] = tile[(lane + 64 * r) * (YT + 1) + c];
+        fft512_wave<float, false>(v, sbw[wave], tw, lane);
+        if (kx < KX) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const int n = lane + 64 * r;
+                if (n <= a) dst[(size_t)kx * NYc + n] = v[r];
+                else if (n >= 512 - a) dst[(size_t)kx * NYc + n - shift] = v[r];
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------------------
+// RBC_GENERIC_FFT=1 in the environment selects the generic radix-2 kernels everywhere (used
+// by the tests to pin the 512-point kernels against them).
+static bool generic_fft_forced() {
+    const char* e = getenv("RBC_GENERIC_FFT");
+    return e && e[0] == '1';
+}
+template <typename T>
+static bool fast512_rows(const DevGeom& g) {
+    return sizeof(T) == 4 && g.M == 512 && g.KX <= 192 && !generic_fft_forced();
+}
+template <typename T>
+static bool fast512_y(const DevGeom& g) {
+    return sizeof(T) == 4 && g.N == 512 && !generic_fft_forced();
+}
 static inline int fft_tile(int N, size_t csz) {
     // two LDS buffers of TX columns; stay within 64 KiB
     int tx = (int)(65536 / (2 * (size_t)N * csz));
@@ -580,6 +827,12 @@ void Launch<T>::eval(const DevGeom& g, const Buffers& b, void* const X[3], const
 
 template <typename T>
 void Launch<T>::fft_y_inv(const DevGeom& g, const Buffers& b, hipStream_t s) {
+    if (fast512_y<T>(g)) {
+        dim3 grid((g.KX + YT - 1) / YT, g.nq, 6);
+        hipLaunchKernelGGL(k_fft_y_inv512, grid, dim3(256), 0, s, g, (const cpx<float>*)b.lev,
+                           (const cpx<float>*)b.twN, (cpx<float>*)b.hin);
+        return;
+    }
     const int TX = fft_tile(g.N, sizeof(cpx<T>));
     dim3 grid((g.KX + TX - 1) / TX, g.nq, 6);
     const size_t sm = (size_t)2 * TX * g.N * sizeof(cpx<T>);
@@ -590,6 +843,11 @@ void Launch<T>::fft_y_inv(const DevGeom& g, const Buffers& b, hipStream_t s) {
 template <typename T>
 void Launch<T>::rows(const DevGeom& g, const Buffers& b, hipStream_t s) {
     dim3 grid(g.N, g.nq);
+    if (fast512_rows<T>(g)) {
+        hipLaunchKernelGGL(k_rows512, grid, dim3(64), 0, s, g, (const cpx<float>*)b.hin, (const cpx<float>*)b.twM,
+                           (cpx<float>*)b.hout);
+        return;
+    }
     const size_t sm = (size_t)8 * g.M * sizeof(cpx<T>);
     hipLaunchKernelGGL(k_rows<T>, grid, dim3(256), sm, s, g, (const cpx<T>*)b.hin, (const cpx<T>*)b.twM,
                        (cpx<T>*)b.hout);
@@ -597,6 +855,12 @@ void Launch<T>::rows(const DevGeom& g, const Buffers& b, hipStream_t s) {
 
 template <typename T>
 void Launch<T>::fft_y_fwd(const DevGeom& g, const Buffers& b, hipStream_t s) {
+    if (fast512_y<T>(g)) {
+        dim3 grid((g.KX + YT - 1) / YT, g.nq, 7);
+        hipLaunchKernelGGL(k_fft_y_fwd512, grid, dim3(256), 0, s, g, (const cpx<float>*)b.hout,
+                           (const cpx<float>*)b.twN, (cpx<float>*)b.prod);
+        return;
+    }
     const int TX = fft_tile(g.N, sizeof(cpx<T>));
     dim3 grid((g.KX + TX - 1) / TX, g.nq, 7);
     const size_t sm = (size_t)2 * TX * g.N * sizeof(cpx<T>);
