@@ -43,6 +43,19 @@ __device__ __forceinline__ cpx<T> conjg(cpx<T> a) { return mk(a.x, -a.y); }
 template <typename T>
 __device__ __forceinline__ cpx<T> imul(cpx<T> a, T s) { return mk(-s * a.y, s * a.x); }
 
+// Blocks are dispatched round-robin over the 8 XCDs (each with its own L2).  This maps the
+// hardware block id to a logical id so that logical neighbours (which share cache lines) run
+// on the same XCD: XCD x takes the contiguous logical range [x*per, (x+1)*per).  Grids using
+// it are launched with 8*per blocks; blocks with logical id >= the real count return at once.
+__device__ __forceinline__ int xcd_logical_block(int b, int per) { return (b & 7) * per + (b >> 3); }
+
+// Orders LDS traffic between the lanes of one wavefront (no block-wide barrier).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---------------------------------------------------------------------------------------
 // Radix-2 Stockham FFT on nb sequences of length n = 2^logn held in LDS (a), with b as the
 // ping-pong buffer.  tw[k] = exp(-2 pi i k / n) (the first n/2 entries are used).  Unnormalised; inverse uses
@@ -88,15 +101,20 @@ __device__ cpx<T>* lds_fft(cpx<T>* a, cpx<T>* b, int logn, int nb, const cpx<T>*
 // T_s' = sum S'_m theta_m (calculate_velocity_operator.cu:426-455 for the velocity form).
 // ---------------------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256) k_eval(DevGeom g, const cpx<T>* __restrict__ Xt,
+__global__ void __launch_bounds__(256) k_eval(DevGeom g, int LG, int nblk, int per, const cpx<T>* __restrict__ Xt,
                                               const cpx<T>* __restrict__ Xf, const cpx<T>* __restrict__ Xg,
                                               const cpx<T>* __restrict__ XF, const cpx<T>* __restrict__ XG,
                                               const T* __restrict__ evtab, const T* __restrict__ kxa,
                                               const T* __restrict__ kya, cpx<T>* __restrict__ lev) {
     extern __shared__ unsigned char smem_raw[];
     cpx<T>* sm = reinterpret_cast<cpx<T>*>(smem_raw);  // 3 * L * 64
+    // logical block = (64-mode block, level group); the LG level groups of one mode block are
+    // neighbours in logical order, so they run on one XCD and share its L2 copy of the modes
+    const int lb = xcd_logical_block(blockIdx.x, per);
+    if (lb >= nblk) return;
     const int L = g.L, P = g.P, nq = g.nq;
-    const int p0 = blockIdx.x * 64;
+    const int p0 = (lb / LG) * 64;
+    const int lgq = (nq + LG - 1) / LG, lbeg = (lb % LG) * lgq, lend = min(nq, lbeg + lgq);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // stage: element e of the block's [64][L] slab -> sm[field][m][mode]
     for (int e = threadIdx.x; e < 64 * L; e += blockDim.x) {
@@ -119,39 +137,52 @@ __global__ void __launch_bounds__(256) k_eval(DevGeom g, const cpx<T>* __restric
     const T kx = kxa[p], ky = kya[p];
     const T k2 = kx * kx + ky * ky;
     const size_t plane = (size_t)P;
-    for (int l = wave; l < nq; l += 4) {
-        const T* tb = evtab + (size_t)l * 4 * L;  // C', C, S, S'
-        cpx<T> Fp = mk<T>(0, 0), Fc = Fp, Gs = Fp, Ts = Fp, Tz = Fp;
+    constexpr int LB = 4;  // levels per pass over the mode data
+    for (int l0 = lbeg + wave * LB; l0 < lend; l0 += 4 * LB) {
+        cpx<T> Fp[LB], Fc[LB], Gs[LB], Ts[LB], Tz[LB];
+#pragma unroll
+        for (int b = 0; b < LB; ++b) Fp[b] = Fc[b] = Gs[b] = Ts[b] = Tz[b] = mk<T>(0, 0);
         for (int m = 0; m < L; ++m) {
-            const T cp = tb[m], cv = tb[L + m], sv = tb[2 * L + m], sp = tb[3 * L + m];
             const cpx<T> th = sm[(0 * L + m) * 64 + lane];
             const cpx<T> f = sm[(1 * L + m) * 64 + lane];
             const cpx<T> gg = sm[(2 * L + m) * 64 + lane];
-            Fp = Fp + cscale(f, cp);
-            Fc = Fc + cscale(f, cv);
-            Gs = Gs + cscale(gg, sv);
-            Ts = Ts + cscale(th, sv);
-            Tz = Tz + cscale(th, sp);
-        }
-        cpx<T> u = imul(Fp, kx) + imul(Gs, ky);
-        cpx<T> v = imul(Fp, ky) - imul(Gs, kx);
-        if (p == 0) {  // mean flow enters at k = 0 (add_dc_part, nonlinear...cu:1596-1608)
-            cpx<T> sF = mk<T>(0, 0), sG = sF;
-            for (int m = 0; m < L; ++m) {
-                sF = sF + cscale(XF[m], tb[2 * L + m]);
-                sG = sG + cscale(XG[m], tb[2 * L + m]);
+#pragma unroll
+            for (int b = 0; b < LB; ++b) {
+                const int l = min(l0 + b, lend - 1);
+                const T* tb = evtab + (size_t)l * 4 * L;  // C', C, S, S'
+                const T cp = tb[m], cv = tb[L + m], sv = tb[2 * L + m], sp = tb[3 * L + m];
+                Fp[b] = Fp[b] + cscale(f, cp);
+                Fc[b] = Fc[b] + cscale(f, cv);
+                Gs[b] = Gs[b] + cscale(gg, sv);
+                Ts[b] = Ts[b] + cscale(th, sv);
+                Tz[b] = Tz[b] + cscale(th, sp);
             }
-            u = u + sF;
-            v = v + sG;
         }
-        cpx<T>* o = lev + (size_t)l * plane + p;
-        const size_t fs = (size_t)nq * plane;
-        o[0 * fs] = u;
-        o[1 * fs] = v;
-        o[2 * fs] = cscale(Fc, k2);
-        o[3 * fs] = imul(Ts, kx);
-        o[4 * fs] = imul(Ts, ky);
-        o[5 * fs] = Tz;
+#pragma unroll
+        for (int b = 0; b < LB; ++b) {
+            const int l = l0 + b;
+            if (l >= lend) break;
+            const T* tb = evtab + (size_t)l * 4 * L;
+            cpx<T> u = imul(Fp[b], kx) + imul(Gs[b], ky);
+            cpx<T> v = imul(Fp[b], ky) - imul(Gs[b], kx);
+            if (p == 0) {  // mean flow enters at k = 0 (add_dc_part, nonlinear...cu:1596-1608)
+                cpx<T> sF = mk<T>(0, 0), sG = sF;
+                for (int m = 0; m < L; ++m) {
+                    sF = sF + cscale(XF[m], tb[2 * L + m]);
+                    sG = sG + cscale(XG[m], tb[2 * L + m]);
+                }
+                u = u + sF;
+                v = v + sG;
+            }
+            cpx<T>* o = lev + (size_t)l * plane + p;
+            const size_t fs = (size_t)nq * plane;
+            o[0 * fs] = u;
+            o[1 * fs] = v;
+            o[2 * fs] = cscale(Fc[b], k2);
+            o[3 * fs] = imul(Ts[b], kx);
+            o[4 * fs] = imul(Ts[b], ky);
+            o[5 * fs] = Tz[b];
+        }
     }
 }
 
@@ -286,6 +317,85 @@ __global__ void __launch_bounds__(256) k_fft_y_fwd(DevGeom g, int TX, const cpx<
 // which is the Galerkin sum of nonlinear...cu:479-1067 with each triple integral written
 // as a quadrature over the levels.  Outputs are scaled by -1 (theta) and -1/Pr (:1366-1370).
 // ---------------------------------------------------------------------------------------
+// Variant for L <= 4 * MCH: per round of 4 levels, wave w forms the five level combinations
+// of level l0 + w for the block's 64 modes in LDS once; then every wave accumulates its own
+// MCH vertical modes from LDS (each product is read from memory once).
+template <typename T, int MCH>
+__global__ void __launch_bounds__(256) k_project_lds(DevGeom g, const cpx<T>* __restrict__ prod,
+                                                     const T* __restrict__ prtab, const T* __restrict__ kxa,
+                                                     const T* __restrict__ kya, T Pr, cpx<T>* __restrict__ Nt,
+                                                     cpx<T>* __restrict__ Nf, cpx<T>* __restrict__ Ng,
+                                                     cpx<T>* __restrict__ NF, cpx<T>* __restrict__ NG) {
+    constexpr int LV = 4;
+    __shared__ cpx<T> sD[LV][5][64];
+    const int L = g.L, P = g.P, nq = g.nq;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + lane;
+    const bool valid = p < P;
+    const int pc = valid ? p : 0;
+    const T kx = kxa[pc], ky = kya[pc];
+    const T k2 = kx * kx + ky * ky;
+    const size_t fs = (size_t)nq * P;
+    const T mt = T(-1), mp = T(-1) / Pr;
+    const int mb = wave * MCH;
+    cpx<T> af[MCH], ag[MCH], at[MCH];
+#pragma unroll
+    for (int i = 0; i < MCH; ++i) af[i] = ag[i] = at[i] = mk<T>(0, 0);
+    for (int l0 = 0; l0 < nq; l0 += LV) {
+        const int ls = l0 + wave;
+        if (ls < nq) {
+            const cpx<T>* src = prod + (size_t)ls * P + pc;
+            const cpx<T> Puu = src[0], Puv = src[fs], Pvv = src[2 * fs], Puw = src[3 * fs];
+            const cpx<T> Pvw = src[4 * fs], Pww = src[5 * fs], Pad = src[6 * fs];
+            sD[wave][0][lane] = cscale(Puu, kx * kx) + cscale(Puv, T(2) * kx * ky) + cscale(Pvv, ky * ky) -
+                                cscale(Pww, k2);
+            sD[wave][1][lane] = imul(Puw, kx) + imul(Pvw, ky);
+            sD[wave][2][lane] = cscale(Puu - Pvv, -kx * ky) + cscale(Puv, kx * kx - ky * ky);
+            sD[wave][3][lane] = imul(Pvw, kx) - imul(Puw, ky);
+            sD[wave][4][lane] = Pad;
+        }
+        __syncthreads();
+        for (int lv = 0; lv < LV && l0 + lv < nq; ++lv) {
+            const T* tb = prtab + (size_t)(l0 + lv) * 5 * L;  // wC', wC'', wC, wS, wS'
+            const cpx<T> A = sD[lv][0][lane], Bf = sD[lv][1][lane], Ag = sD[lv][2][lane];
+            const cpx<T> Bg = sD[lv][3][lane], At = sD[lv][4][lane];
+#pragma unroll
+            for (int i = 0; i < MCH; ++i) {
+                const int m = mb + i;
+                if (m < L) {
+                    const T cp = tb[m], cpp = tb[L + m], cv = tb[2 * L + m], sv = tb[3 * L + m], sp = tb[4 * L + m];
+                    af[i] = af[i] + cscale(A, cp) + cscale(Bf, cpp + k2 * cv);
+                    ag[i] = ag[i] + cscale(Ag, sv) + cscale(Bg, sp);
+                    at[i] = at[i] + cscale(At, sv);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (valid) {
+#pragma unroll
+        for (int i = 0; i < MCH; ++i) {
+            const int m = mb + i;
+            if (m < L) {
+                Nt[(size_t)p * L + m] = cscale(at[i], mt);
+                Nf[(size_t)p * L + m] = cscale(af[i], mp);
+                Ng[(size_t)p * L + m] = cscale(ag[i], mp);
+            }
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < L) {
+        const int m = threadIdx.x;
+        cpx<T> sF = mk<T>(0, 0), sG = sF;
+        for (int l = 0; l < nq; ++l) {
+            const T sp = prtab[(size_t)l * 5 * L + 4 * L + m];
+            sF = sF + cscale(prod[3 * fs + (size_t)l * P], sp);
+            sG = sG + cscale(prod[4 * fs + (size_t)l * P], sp);
+        }
+        NF[m] = cscale(sF, -mp);
+        NG[m] = cscale(sG, -mp);
+    }
+}
+
 template <typename T, int MCH>
 __global__ void __launch_bounds__(256) k_project(DevGeom g, const cpx<T>* __restrict__ prod,
                                                  const T* __restrict__ prtab, const T* __restrict__ kxa,
@@ -469,6 +579,90 @@ __global__ void __launch_bounds__(64) k_update(DevGeom g, int L0, int L1, int st
     }
 }
 
+// Group update for L <= 32 with the class matrices held in registers: one wave per group
+// (4 groups per block), lane (class c, row r) keeps row r of C^-1 (n_c <= 32 values) and of
+// B^-1_f (L_c <= 16 values) for every member of the group.  Same arithmetic and rounding
+// order as k_update; mode UPD_STEP only.
+template <typename T>
+__global__ void __launch_bounds__(256) k_update32(DevGeom g, int ngroups, int L0, int L1, int stride, T Pr, T dt,
+                                                  T a_cur, T a_prev, int use_prev, const int* __restrict__ gstart,
+                                                  const int* __restrict__ gmem, const T* __restrict__ gmat,
+                                                  const T* __restrict__ k2a, cpx<T>* __restrict__ Xt,
+                                                  cpx<T>* __restrict__ Xf, cpx<T>* __restrict__ Xg,
+                                                  const cpx<T>* __restrict__ Nt, const cpx<T>* __restrict__ Nf,
+                                                  const cpx<T>* __restrict__ Ng, cpx<T>* __restrict__ Rt,
+                                                  cpx<T>* __restrict__ Rf, cpx<T>* __restrict__ Rg) {
+    __shared__ cpx<T> shm[4][2][2][32];  // [wave][class][0: N_f, 1: rhs][j]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int grp = blockIdx.x * 4 + wave;
+    if (grp >= ngroups) return;  // whole wave leaves; no block-wide barrier below
+    const int L = g.L;
+    const int cls = lane >> 5, r = lane & 31;
+    const int Lc = cls ? L1 : L0, nc = 2 * Lc, n0 = 2 * L0;
+    const T* rec = gmat + (size_t)grp * stride;
+    const T* Cinv = rec + (cls ? (size_t)n0 * n0 : 0);
+    const T* Binv = rec + (size_t)n0 * n0 + (size_t)(2 * L1) * (2 * L1) + (cls ? (size_t)L0 * L0 : 0);
+    T crow[32], brow[16];
+#pragma unroll
+    for (int j = 0; j < 32; ++j) crow[j] = (j < nc && r < nc) ? Cinv[(size_t)j * nc + r] : T(0);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) brow[j] = (j < Lc && r < Lc) ? Binv[(size_t)j * Lc + r] : T(0);
+    cpx<T>* snf = shm[wave][cls][0];
+    cpx<T>* srhs = shm[wave][cls][1];
+    const int b0 = gstart[grp], b1 = gstart[grp + 1];
+    for (int mi = b0; mi < b1; ++mi) {
+        const int p = gmem[mi];
+        const size_t base = (size_t)p * L;
+        const T k2 = k2a[p];
+        const int m = cls + 2 * r;  // theta/f mode of this lane (valid for r < Lc)
+        cpx<T> xt = mk<T>(0, 0), xf = xt;
+        if (r < Lc) {
+            snf[r] = Nf[base + m];
+            xt = Xt[base + m];
+            xf = Xf[base + m];
+        }
+        wave_sync();
+        if (r < Lc) {
+            cpx<T> rf = mk<T>(0, 0);
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (j < Lc) rf = rf + cscale(snf[j], brow[j]);
+            const cpx<T> rt = cscale(Nt[base + m], T(2));
+            xt = xt + cscale(rt, a_cur);
+            xf = xf + cscale(rf, a_cur);
+            if (use_prev) {
+                xt = xt + cscale(Rt[base + m], a_prev);
+                xf = xf + cscale(Rf[base + m], a_prev);
+            }
+            Rt[base + m] = rt;
+            Rf[base + m] = rf;
+            srhs[r] = xt;
+            srhs[Lc + r] = xf;
+        }
+        wave_sync();
+        if (r < nc) {
+            cpx<T> acc = mk<T>(0, 0);
+#pragma unroll
+            for (int j = 0; j < 32; ++j)
+                if (j < nc) acc = acc + cscale(srhs[j], crow[j]);
+            const int mm = cls + 2 * (r < Lc ? r : r - Lc);
+            if (r < Lc) Xt[base + mm] = acc;
+            else Xf[base + mm] = acc;
+        }
+        // g: diagonal factors, one vertical mode per lane
+        if (lane < L) {
+            const int mg = lane;
+            const T mpi = T(mg + 1) * T(M_PI);
+            const cpx<T> rg = (k2 != T(0)) ? cscale(Ng[base + mg], T(-2) * Pr / k2) : mk<T>(0, 0);
+            cpx<T> xg = Xg[base + mg] + cscale(rg, a_cur);
+            if (use_prev) xg = xg + cscale(Rg[base + mg], a_prev);
+            Rg[base + mg] = rg;
+            Xg[base + mg] = (k2 != T(0)) ? cscale(xg, T(1) / (T(1) + dt * Pr * (k2 + mpi * mpi))) : mk<T>(0, 0);
+        }
+        wave_sync();
+    }
+}
+
 // F, G: inverse B = 2 Pr (imag 0), implicit factor 1/(1 + dt Pr (m pi)^2) (imag 0).
 template <typename T>
 __global__ void k_update_FG(int L, T Pr, T dt, T a_cur, T a_prev, int use_prev, int mode, cpx<T>* XF,
@@ -564,12 +758,6 @@ __global__ void __launch_bounds__(256) k_nusselt(DevGeom g, const cpx<T>* __rest
 // Lane j holds x[j + 64 r], r = 0..7, on entry and X[j + 64 r] on exit.  Two exchanges
 // through a per-wave LDS scratch of 576 complex (one pad slot every 8 to spread banks).
 // =======================================================================================
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 __device__ __forceinline__ int pad8(int i) { return i + (i >> 3); }
 
 // 8-point DFT in registers (decimation in frequency, outputs reordered to natural order).
@@ -616,15 +804,17 @@ __device__ __forceinline__ void dft8(cpx<T> v[8]) {
     v[7] = c7;
 }
 
-// 512-point FFT of one wavefront; tw = LDS table exp(-2 pi i k / 512), k < 512.
-template <typename T, bool INV>
+// 512-point FFT of one wavefront.  tw = table exp(-2 pi i k / 512), k < 512 (LDS or global).
+// The scratch element of internal index i is sb[pad8(i) * S] (S = 1 for a private buffer,
+// S = tile row stride when a column of an LDS tile doubles as the scratch).
+template <typename T, bool INV, int S = 1>
 __device__ __forceinline__ void fft512_wave(cpx<T> v[8], cpx<T>* sb, const cpx<T>* tw, int lane) {
     dft8<T, INV>(v);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) sb[pad8(8 * lane + r)] = v[r];
+    for (int r = 0; r < 8; ++r) sb[pad8(8 * lane + r) * S] = v[r];
     wave_sync();
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = sb[pad8(lane + 64 * r)];
+    for (int r = 0; r < 8; ++r) v[r] = sb[pad8(lane + 64 * r) * S];
 #pragma unroll
     for (int r = 1; r < 8; ++r) {
         cpx<T> w = tw[(lane & 7) * r * 8];
@@ -634,10 +824,10 @@ __device__ __forceinline__ void fft512_wave(cpx<T> v[8], cpx<T>* sb, const cpx<T
     dft8<T, INV>(v);
     wave_sync();
 #pragma unroll
-    for (int r = 0; r < 8; ++r) sb[pad8((lane >> 3) * 64 + (lane & 7) + 8 * r)] = v[r];
+    for (int r = 0; r < 8; ++r) sb[pad8((lane >> 3) * 64 + (lane & 7) + 8 * r) * S] = v[r];
     wave_sync();
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = sb[pad8(lane + 64 * r)];
+    for (int r = 0; r < 8; ++r) v[r] = sb[pad8(lane + 64 * r) * S];
 #pragma unroll
     for (int r = 1; r < 8; ++r) {
         cpx<T> w = tw[lane * r];
@@ -686,20 +876,22 @@ __global__ void __launch_bounds__(64) k_rows512(DevGeom g, const cpx<float>* __r
         fft512_wave<float, true>(z[pr], sb, tw, lane);
     }
     const float s = 1.f / (512.f * (float)N);
-    cpx<float> p[4][8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        const float u = z[0][r].x, v = z[0][r].y, w = z[1][r].x, tx = z[1][r].y, ty = z[2][r].x, tz = z[2][r].y;
-        p[0][r] = mk(s * u * u, s * u * v);
-        p[1][r] = mk(s * v * v, s * u * w);
-        p[2][r] = mk(s * v * w, s * w * w);
-        p[3][r] = mk(s * (u * tx + v * ty + w * tz), 0.f);
-    }
 #pragma unroll
     for (int pr = 0; pr < 4; ++pr) {
-        fft512_wave<float, false>(p[pr], sb, tw, lane);
+        // products of this pair only, so that at most one product pair is live at a time
+        cpx<float> p[8];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) sb[pad8(lane + 64 * r)] = p[pr][r];
+        for (int r = 0; r < 8; ++r) {
+            const float u = z[0][r].x, v = z[0][r].y, w = z[1][r].x, tx = z[1][r].y, ty = z[2][r].x,
+                        tz = z[2][r].y;
+            if (pr == 0) p[r] = mk(s * u * u, s * u * v);
+            else if (pr == 1) p[r] = mk(s * v * v, s * u * w);
+            else if (pr == 2) p[r] = mk(s * v * w, s * w * w);
+            else p[r] = mk(s * (u * tx + v * ty + w * tz), 0.f);
+        }
+        fft512_wave<float, false>(p, sb, tw, lane);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) sb[pad8(lane + 64 * r)] = p[r];
         wave_sync();
         cpx<float>* o1 = hout + (size_t)(2 * pr) * fstride + roff;
         cpx<float>* o2 = hout + (size_t)(2 * pr + 1) * fstride + roff;
@@ -712,24 +904,96 @@ __global__ void __launch_bounds__(64) k_rows512(DevGeom g, const cpx<float>* __r
     }
 }
 
-// y inverse at N = 512: block of 4 waves handles 8 kx columns of one (field, level); each
-// wave transforms 2 columns into an LDS tile [y][8+1] which is then written out row-wise.
-constexpr int YT = 8;
-__global__ void __launch_bounds__(256) k_fft_y_inv512(DevGeom g, const cpx<float>* __restrict__ lev,
-                                                      const cpx<float>* __restrict__ twN,
-                                                      cpx<float>* __restrict__ hin) {
+// x-rows at M = 512 with four waves per (level, y) row: waves 0-2 run the three inverse
+// pair transforms into LDS slots, then wave w forms product pair w from the slots and runs its
+// forward transform, reusing slot w as scratch.  Same arithmetic as k_rows512.
+__global__ void __launch_bounds__(256) k_rows512q(DevGeom g, const cpx<float>* __restrict__ hin,
+                                                  const cpx<float>* __restrict__ twM, cpx<float>* __restrict__ hout) {
     __shared__ cpx<float> tw[512];
-    __shared__ cpx<float> tile[512 * (YT + 1)];
-    __shared__ cpx<float> sbw[4][576];
+    __shared__ cpx<float> zs[4][576];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int KX = g.KX, KXp = g.KXp, nq = g.nq, N = g.N;
+    const int y = blockIdx.x, l = blockIdx.y;
+    const size_t fstride = (size_t)nq * N * KXp;
+    const size_t roff = ((size_t)l * N + y) * KXp;
+    for (int e = threadIdx.x; e < 512; e += 256) tw[e] = twM[e];
+    __syncthreads();
+    if (wave < 3) {
+        const int pr = wave;
+        const cpx<float>* x1 = hin + (size_t)(2 * pr) * fstride + roff;
+        const cpx<float>* x2 = hin + (size_t)(2 * pr + 1) * fstride + roff;
+        cpx<float> v[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int n = lane + 64 * r;
+            cpx<float> z = mk(0.f, 0.f);
+            if (n < KX) {
+                cpx<float> a = x1[n], b = x2[n];
+                if (n == 0) {
+                    a.y = 0.f;
+                    b.y = 0.f;
+                }
+                z = mk(a.x - b.y, a.y + b.x);  // X1 + i X2
+            } else if (n > 512 - KX) {
+                const cpx<float> a = x1[512 - n], b = x2[512 - n];
+                z = mk(a.x + b.y, -a.y + b.x);  // conj(X1) + i conj(X2)
+            }
+            v[r] = z;
+        }
+        fft512_wave<float, true>(v, zs[pr], tw, lane);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) zs[pr][pad8(lane + 64 * r)] = v[r];
+    }
+    __syncthreads();
+    const int pr = wave;
+    const float s = 1.f / (512.f * (float)N);
+    cpx<float> p[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int i = pad8(lane + 64 * r);
+        const cpx<float> z0 = zs[0][i], z1 = zs[1][i], z2 = zs[2][i];
+        const float u = z0.x, v = z0.y, w = z1.x, tx = z1.y, ty = z2.x, tz = z2.y;
+        if (pr == 0) p[r] = mk(s * u * u, s * u * v);
+        else if (pr == 1) p[r] = mk(s * v * v, s * u * w);
+        else if (pr == 2) p[r] = mk(s * v * w, s * w * w);
+        else p[r] = mk(s * (u * tx + v * ty + w * tz), 0.f);
+    }
+    __syncthreads();  // every wave has read the slots before they become scratch
+    fft512_wave<float, false>(p, zs[pr], tw, lane);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) zs[pr][pad8(lane + 64 * r)] = p[r];
+    wave_sync();
+    cpx<float>* o1 = hout + (size_t)(2 * pr) * fstride + roff;
+    cpx<float>* o2 = hout + (size_t)(2 * pr + 1) * fstride + roff;
+    for (int k = lane; k < KX; k += 64) {
+        const cpx<float> zk = zs[pr][pad8(k)], zm = zs[pr][pad8((512 - k) & 511)];
+        o1[k] = mk(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+        if (pr < 3) o2[k] = mk(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+    }
+}
+
+// y inverse at N = 512: a block of 8 waves handles YT = 8 kx columns of one (field, level);
+// each wave transforms one column.  The LDS tile [pad8(y)][YT + 1] holds the results and
+// doubles as the FFT scratch of each column; it is written out row-wise (64-byte segments).
+// Launched on a 1-D grid of 8 * per blocks (XCD-aware order, see xcd_logical_block).
+constexpr int YT = 8;
+constexpr int YS = YT + 1;
+__global__ void __launch_bounds__(512) k_fft_y_inv512(DevGeom g, int ntile, int nblk, int per,
+                                                      const cpx<float>* __restrict__ lev,
+                                                      const cpx<float>* __restrict__ tw,
+                                                      cpx<float>* __restrict__ hin) {
+    __shared__ cpx<float> tile[576 * YS];
+    const int lb = xcd_logical_block(blockIdx.x, per);
+    if (lb >= nblk) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int NYc = g.NYc, P = g.P, nq = g.nq, KX = g.KX;
-    const int kx0 = blockIdx.x * YT, l = blockIdx.y, field = blockIdx.z;
+    const int tix = lb % ntile, rest = lb / ntile;
+    const int l = rest % nq, field = rest / nq;
+    const int kx0 = tix * YT;
     const int a = (NYc - 1) / 2, shift = 512 - NYc;  // kept rows [0, a] U [512 - a, 511]
-    for (int e = threadIdx.x; e < 512; e += 256) tw[e] = twN[e];
-    __syncthreads();
     const cpx<float>* src = lev + ((size_t)field * nq + l) * P;
-    for (int cc = 0; cc < 2; ++cc) {
-        const int c = wave * 2 + cc, kx = kx0 + c;
+    {
+        const int c = wave, kx = kx0 + c;  // one column per wave
         cpx<float> v[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
@@ -740,45 +1004,47 @@ __global__ void __launch_bounds__(256) k_fft_y_inv512(DevGeom g, const cpx<float
                 else if (n >= 512 - a) v[r] = src[(size_t)kx * NYc + n - shift];
             }
         }
-        fft512_wave<float, true>(v, sbw[wave], tw, lane);
+        fft512_wave<float, true, YS>(v, tile + c, tw, lane);
 #pragma unroll
-        for (int r = 0; r < 8; ++r) tile[(lane + 64 * r) * (YT + 1) + c] = v[r];
+        for (int r = 0; r < 8; ++r) tile[pad8(lane + 64 * r) * YS + c] = v[r];
     }
     __syncthreads();
     const int ncol = min(YT, KX - kx0);
     cpx<float>* dst = hin + ((size_t)field * nq + l) * (size_t)512 * g.KXp + kx0;
-    for (int e = threadIdx.x; e < 512 * YT; e += 256) {
+    for (int e = threadIdx.x; e < 512 * YT; e += 512) {
         const int y = e / YT, c = e - y * YT;
-        if (c < ncol) dst[(size_t)y * g.KXp + c] = tile[y * (YT + 1) + c];
+        if (c < ncol) dst[(size_t)y * g.KXp + c] = tile[pad8(y) * YS + c];
     }
 }
 
 // y forward at N = 512: the transpose of k_fft_y_inv512 (tile in, kept rows out).
-__global__ void __launch_bounds__(256) k_fft_y_fwd512(DevGeom g, const cpx<float>* __restrict__ hout,
-                                                      const cpx<float>* __restrict__ twN,
+__global__ void __launch_bounds__(512) k_fft_y_fwd512(DevGeom g, int ntile, int nblk, int per,
+                                                      const cpx<float>* __restrict__ hout,
+                                                      const cpx<float>* __restrict__ tw,
                                                       cpx<float>* __restrict__ prod) {
-    __shared__ cpx<float> tw[512];
-    __shared__ cpx<float> tile[512 * (YT + 1)];
-    __shared__ cpx<float> sbw[4][576];
+    __shared__ cpx<float> tile[576 * YS];
+    const int lb = xcd_logical_block(blockIdx.x, per);
+    if (lb >= nblk) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int NYc = g.NYc, P = g.P, nq = g.nq, KX = g.KX;
-    const int kx0 = blockIdx.x * YT, l = blockIdx.y, q = blockIdx.z;
+    const int tix = lb % ntile, rest = lb / ntile;
+    const int l = rest % nq, q = rest / nq;
+    const int kx0 = tix * YT;
     const int a = (NYc - 1) / 2, shift = 512 - NYc;
-    for (int e = threadIdx.x; e < 512; e += 256) tw[e] = twN[e];
     const int ncol = min(YT, KX - kx0);
     const cpx<float>* src = hout + ((size_t)q * nq + l) * (size_t)512 * g.KXp + kx0;
-    for (int e = threadIdx.x; e < 512 * YT; e += 256) {
+    for (int e = threadIdx.x; e < 512 * YT; e += 512) {
         const int y = e / YT, c = e - y * YT;
-        tile[y * (YT + 1) + c] = (c < ncol) ? src[(size_t)y * g.KXp + c] : mk(0.f, 0.f);
+        tile[pad8(y) * YS + c] = (c < ncol) ? src[(size_t)y * g.KXp + c] : mk(0.f, 0.f);
     }
     __syncthreads();
     cpx<float>* dst = prod + ((size_t)q * nq + l) * P;
-    for (int cc = 0; cc < 2; ++cc) {
-        const int c = wave * 2 + cc, kx = kx0 + c;
+    {
+        const int c = wave, kx = kx0 + c;  // one column per wave
         cpx<float> v[8];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) v[r] = tile[(lane + 64 * r) * (YT + 1) + c];
-        fft512_wave<float, false>(v, sbw[wave], tw, lane);
+        for (int r = 0; r < 8; ++r) v[r] = tile[pad8(lane + 64 * r) * YS + c];
+        fft512_wave<float, false, YS>(v, tile + c, tw, lane);
         if (kx < KX) {
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
@@ -819,8 +1085,10 @@ template <typename T>
 void Launch<T>::eval(const DevGeom& g, const Buffers& b, void* const X[3], const void* XF, const void* XG,
                      hipStream_t s) {
     const size_t sm = (size_t)3 * g.L * 64 * sizeof(cpx<T>);
-    dim3 grid((g.P + 63) / 64);
-    hipLaunchKernelGGL(k_eval<T>, grid, dim3(256), sm, s, g, (const cpx<T>*)X[0], (const cpx<T>*)X[1],
+    const int LG = g.nq >= 64 ? g.nq / 32 : 1;  // level groups per 64-mode block
+    const int nblk = ((g.P + 63) / 64) * LG, per = (nblk + 7) / 8;
+    hipLaunchKernelGGL(k_eval<T>, dim3(8 * per), dim3(256), sm, s, g, LG, nblk, per, (const cpx<T>*)X[0],
+                       (const cpx<T>*)X[1],
                        (const cpx<T>*)X[2], (const cpx<T>*)XF, (const cpx<T>*)XG, (const T*)b.evtab,
                        (const T*)b.kx, (const T*)b.ky, (cpx<T>*)b.lev);
 }
@@ -828,9 +1096,9 @@ void Launch<T>::eval(const DevGeom& g, const Buffers& b, void* const X[3], const
 template <typename T>
 void Launch<T>::fft_y_inv(const DevGeom& g, const Buffers& b, hipStream_t s) {
     if (fast512_y<T>(g)) {
-        dim3 grid((g.KX + YT - 1) / YT, g.nq, 6);
-        hipLaunchKernelGGL(k_fft_y_inv512, grid, dim3(256), 0, s, g, (const cpx<float>*)b.lev,
-                           (const cpx<float>*)b.twN, (cpx<float>*)b.hin);
+        const int ntile = (g.KX + YT - 1) / YT, nblk = ntile * g.nq * 6, per = (nblk + 7) / 8;
+        hipLaunchKernelGGL(k_fft_y_inv512, dim3(8 * per), dim3(512), 0, s, g, ntile, nblk, per,
+                           (const cpx<float>*)b.lev, (const cpx<float>*)b.twN, (cpx<float>*)b.hin);
         return;
     }
     const int TX = fft_tile(g.N, sizeof(cpx<T>));
@@ -844,7 +1112,7 @@ template <typename T>
 void Launch<T>::rows(const DevGeom& g, const Buffers& b, hipStream_t s) {
     dim3 grid(g.N, g.nq);
     if (fast512_rows<T>(g)) {
-        hipLaunchKernelGGL(k_rows512, grid, dim3(64), 0, s, g, (const cpx<float>*)b.hin, (const cpx<float>*)b.twM,
+        hipLaunchKernelGGL(k_rows512q, grid, dim3(256), 0, s, g, (const cpx<float>*)b.hin, (const cpx<float>*)b.twM,
                            (cpx<float>*)b.hout);
         return;
     }
@@ -856,9 +1124,9 @@ void Launch<T>::rows(const DevGeom& g, const Buffers& b, hipStream_t s) {
 template <typename T>
 void Launch<T>::fft_y_fwd(const DevGeom& g, const Buffers& b, hipStream_t s) {
     if (fast512_y<T>(g)) {
-        dim3 grid((g.KX + YT - 1) / YT, g.nq, 7);
-        hipLaunchKernelGGL(k_fft_y_fwd512, grid, dim3(256), 0, s, g, (const cpx<float>*)b.hout,
-                           (const cpx<float>*)b.twN, (cpx<float>*)b.prod);
+        const int ntile = (g.KX + YT - 1) / YT, nblk = ntile * g.nq * 7, per = (nblk + 7) / 8;
+        hipLaunchKernelGGL(k_fft_y_fwd512, dim3(8 * per), dim3(512), 0, s, g, ntile, nblk, per,
+                           (const cpx<float>*)b.hout, (const cpx<float>*)b.twN, (cpx<float>*)b.prod);
         return;
     }
     const int TX = fft_tile(g.N, sizeof(cpx<T>));
@@ -873,6 +1141,12 @@ void Launch<T>::project(const DevGeom& g, const Buffers& b, double Pr, void* con
                         hipStream_t s) {
     constexpr int MCH = sizeof(T) == 8 ? 4 : 8;
     dim3 grid((g.P + 63) / 64);
+    if (g.L <= 4 * MCH) {
+        hipLaunchKernelGGL((k_project_lds<T, MCH>), grid, dim3(256), 0, s, g, (const cpx<T>*)b.prod,
+                           (const T*)b.prtab, (const T*)b.kx, (const T*)b.ky, (T)Pr, (cpx<T>*)Nout[0],
+                           (cpx<T>*)Nout[1], (cpx<T>*)Nout[2], (cpx<T>*)NFo, (cpx<T>*)NGo);
+        return;
+    }
     hipLaunchKernelGGL((k_project<T, MCH>), grid, dim3(256), 0, s, g, (const cpx<T>*)b.prod, (const T*)b.prtab,
                        (const T*)b.kx, (const T*)b.ky, (T)Pr, (cpx<T>*)Nout[0], (cpx<T>*)Nout[1],
                        (cpx<T>*)Nout[2], (cpx<T>*)NFo, (cpx<T>*)NGo);
@@ -882,6 +1156,14 @@ template <typename T>
 void Launch<T>::update(const DevGeom& g, const Buffers& b, int ngroups, int L0, int L1, int stride, double Pr,
                        double dt, StepCoef sc, int mode, void* const Xio[3], void* const Nin[3], void* const Rp[3],
                        hipStream_t s) {
+    if (mode == UPD_STEP && g.L <= 32) {
+        hipLaunchKernelGGL(k_update32<T>, dim3((ngroups + 3) / 4), dim3(256), 0, s, g, ngroups, L0, L1, stride, (T)Pr,
+                           (T)dt, (T)sc.a_cur, (T)sc.a_prev, sc.use_prev, b.gstart, b.gmem, (const T*)b.gmat,
+                           (const T*)b.k2, (cpx<T>*)Xio[0], (cpx<T>*)Xio[1], (cpx<T>*)Xio[2], (const cpx<T>*)Nin[0],
+                           (const cpx<T>*)Nin[1], (const cpx<T>*)Nin[2], (cpx<T>*)Rp[0], (cpx<T>*)Rp[1],
+                           (cpx<T>*)Rp[2]);
+        return;
+    }
     const size_t sm = (size_t)3 * g.L * sizeof(cpx<T>);
     hipLaunchKernelGGL(k_update<T>, dim3(ngroups), dim3(64), sm, s, g, L0, L1, stride, (T)Pr, (T)dt,
                        (T)sc.a_cur, (T)sc.a_prev, sc.use_prev, mode, b.gstart, b.gmem, (const T*)b.gmat,
