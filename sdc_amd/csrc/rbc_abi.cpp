@@ -230,6 +230,8 @@ int rbc_create(const rbc_params* p, rbc_ctx** out) {
     if (!(p->Lx > 0) || !(p->Ly > 0) || !(p->Pr > 0) || !(p->dt > 0) || !std::isfinite(p->Ra))
         return fail(nullptr, RBC_ERR_ARG, "Lx, Ly, Pr, dt must be positive and Ra finite");
     if (p->precision < RBC_F64 || p->precision > RBC_BF16X3) return fail(nullptr, RBC_ERR_ARG, "bad precision");
+    if (p->precision == RBC_BF16X3)
+        return fail(nullptr, RBC_ERR_ARG, "RBC_BF16X3 is not available in this build (use RBC_F32 or RBC_F64)");
     if (p->nq < 0 || p->nq > 1024) return fail(nullptr, RBC_ERR_ARG, "nq out of range");
     {
         // LDS budgets of the row kernel (8 M complex) and the y-FFT tiles (2 N complex)
