@@ -32,13 +32,30 @@ def stage_bytes(st, cs):
         "eval": (3 * L * P + 6 * nq * P) * cs,
         "fft_y_inv": (6 * nq * P + 6 * nq * H) * cs,
         "rows_products": (6 * nq * H + 7 * nq * H) * cs,
-        "fft_y_fwd": (7 * nq * H + 7 * nq * P) * cs,
-        "project": (7 * nq * P + 3 * L * P) * cs,
+        # forward y-FFT reads the 7 product rows and writes the 5 level combinations
+        "fft_y_fwd": (7 * nq * H + 5 * nq * P) * cs,
+        "project": (5 * nq * P + 3 * L * P) * cs,
         # state, N and history in/out; factor records once per |k|^2 group
         "update": (9 * L * P + 6 * L * P) * cs + st.n_unique_k * (2 * L * L + L * L // 2) * (cs // 2),
         "update_FG": 8 * L * cs,
         "make_real": 2 * 3 * st.ky_kept * L * cs,
     }
+
+
+def copy_probe(mib):
+    """One device-to-device copy of `mib` MiB by the HIP runtime (its blit kernel reads and writes
+    exactly that many bytes), used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(mib << 20)
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), n) == 0 and hip.hipMalloc(ctypes.byref(dst), n) == 0
+    assert hip.hipMemset(src, 1, n) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(dst, src, n, 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
 
 
 def cpu_baseline(threads, budget_s=20.0):
@@ -79,6 +96,11 @@ def main():
     ap.add_argument("--precision", default="f32", choices=["f32", "bf16x3", "f64"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-torch", action="store_true", help="bracket the timed region with the library's "
+                    "stream synchronisation only (for profiler runs)")
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MB",
+                    help="also run one device-to-device copy of MB mebibytes (a kernel of known traffic "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU layout of this "
@@ -91,11 +113,14 @@ def main():
     st = sdc_amd.Stepper(W["M"], W["N"], W["L"], W["Lx"], W["Ly"], W["Ra"], W["Pr"], W["dt"],
                          precision=a.precision, device=0)
     st.set_state(*random_theta(W["M"], W["N"], W["L"], seed=3, amp=1e-3))
-    try:
-        import torch
-        tsync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:
-        tsync = lambda: None  # noqa: E731
+    tsync = lambda: None  # noqa: E731
+    if not a.no_torch:
+        try:
+            import torch
+            if torch.cuda.is_available():
+                tsync = torch.cuda.synchronize
+        except Exception:
+            pass
     st.step_time(a.warmup)
     st.synchronize()
     tsync()
@@ -117,6 +142,8 @@ def main():
                 "stage_GBs": {k: round(nbytes[k] / (stages[k] * 1e-3) / 1e9, 1) for k in stages if stages[k] > 0}}
     nq, kx, ky, ng = st.nq, st.kx_kept, st.ky_kept, st.n_unique_k
     st.close()
+    if a.copy_calibrate > 0:
+        copy_probe(a.copy_calibrate)
 
     cpu = None
     if not a.no_cpu:

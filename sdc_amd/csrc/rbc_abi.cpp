@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
@@ -230,8 +231,8 @@ int rbc_create(const rbc_params* p, rbc_ctx** out) {
     if (!(p->Lx > 0) || !(p->Ly > 0) || !(p->Pr > 0) || !(p->dt > 0) || !std::isfinite(p->Ra))
         return fail(nullptr, RBC_ERR_ARG, "Lx, Ly, Pr, dt must be positive and Ra finite");
     if (p->precision < RBC_F64 || p->precision > RBC_BF16X3) return fail(nullptr, RBC_ERR_ARG, "bad precision");
-    if (p->precision == RBC_BF16X3)
-        return fail(nullptr, RBC_ERR_ARG, "RBC_BF16X3 is not available in this build (use RBC_F32 or RBC_F64)");
+    if (p->precision == RBC_BF16X3 && p->L > 32)
+        return fail(nullptr, RBC_ERR_ARG, "RBC_BF16X3 supports L <= 32 (one 32-row matrix-core tile)");
     if (p->nq < 0 || p->nq > 1024) return fail(nullptr, RBC_ERR_ARG, "nq out of range");
     {
         // LDS budgets of the row kernel (8 M complex) and the y-FFT tiles (2 N complex)
@@ -307,6 +308,7 @@ int rbc_create(const rbc_params* p, rbc_ctx** out) {
         b.hin = c->dalloc((size_t)6 * nq * d.N * d.KXp * cs);
         b.hout = c->dalloc((size_t)7 * nq * d.N * d.KXp * cs);
         b.prod = c->dalloc((size_t)7 * nq * P * cs);
+        b.fgw = c->dalloc((size_t)2 * nq * cs);
         // zero-initialise state and padding once (not in any step: nothing is memset later)
         for (int i = 0; i < 3; ++i) {
             HIPCHK(hipMemset(b.X[i], 0, P * L * cs));
@@ -341,6 +343,45 @@ int rbc_create(const rbc_params* p, rbc_ctx** out) {
         b.prtab = c->dalloc(pr.size() * c->rsz);
         upload_reals(c.get(), b.evtab, ev);
         upload_reals(c.get(), b.prtab, pr);
+        // matrix-core operand tables in fragment order (see k_eval_mma / k_project_mma)
+        {
+            const char* no = getenv("RBC_NO_MFMA");
+            const bool off = no && no[0] == '1';
+            if (p->precision == RBC_BF16X3) b.mma = 1;
+            else if (p->precision == RBC_F32 && p->L <= 32 && !off) b.mma = 0;
+            if (b.mma >= 0) {
+                const int K = b.mma == 0 ? 2 : 16, KPL = K / 2;
+                b.ev_nlt = (nq + 31) / 32;
+                b.ev_nks = ((int)L + K - 1) / K;
+                b.pr_nks = (nq + K - 1) / K;
+                std::vector<double> eA((size_t)4 * b.ev_nlt * b.ev_nks * 64 * KPL, 0.0);
+                static const int evt[4] = {1, 0, 4, 5};  // C', C, S, S'
+                for (int t = 0; t < 4; ++t)
+                    for (int lt = 0; lt < b.ev_nlt; ++lt)
+                        for (int s = 0; s < b.ev_nks; ++s)
+                            for (int ln = 0; ln < 64; ++ln)
+                                for (int e = 0; e < KPL; ++e) {
+                                    const int l = lt * 32 + (ln & 31), m = s * K + KPL * (ln >> 5) + e;
+                                    if (l < nq && m < (int)L)
+                                        eA[((((size_t)t * b.ev_nlt + lt) * b.ev_nks + s) * 64 + ln) * KPL + e] =
+                                            B(evt[t], m, l);
+                                }
+                std::vector<double> pA((size_t)5 * b.pr_nks * 64 * KPL, 0.0);
+                static const int prt[5] = {1, 2, 0, 4, 5};  // w C', w C'', w C, w S, w S'
+                for (int t = 0; t < 5; ++t)
+                    for (int s = 0; s < b.pr_nks; ++s)
+                        for (int ln = 0; ln < 64; ++ln)
+                            for (int e = 0; e < KPL; ++e) {
+                                const int m = ln & 31, l = s * K + KPL * (ln >> 5) + e;
+                                if (l < nq && m < (int)L)
+                                    pA[(((size_t)t * b.pr_nks + s) * 64 + ln) * KPL + e] = w[l] * B(prt[t], m, l);
+                            }
+                b.evA = c->dalloc(eA.size() * sizeof(float));
+                b.prA = c->dalloc(pA.size() * sizeof(float));
+                upload_real<float>(c.get(), b.evA, eA);
+                upload_real<float>(c.get(), b.prA, pA);
+            }
+        }
         // twiddles exp(-2 pi i k / n), k < n, built in FP64
         auto tw = [&](int n) {
             std::vector<double> t((size_t)2 * n);

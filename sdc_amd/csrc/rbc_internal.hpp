@@ -94,7 +94,10 @@ struct Buffers {
     void* lev;    // 6 * nq * P complex  [field][l][p]
     void* hin;    // 6 * nq * N * KXp complex  [field][l][y][kx]
     void* hout;   // 7 * nq * N * KXp complex  [prod][l][y][kx]
-    void* prod;   // 7 * nq * P complex  [prod][l][p]
+    void* prod;   // 7 * nq * P complex  [prod][l][p] (generic path only)
+    // the projection's five level combinations live in `lev` (free once the inverse y-FFT has
+    // consumed it): [5][l][p] complex; fgw[l] = (P_uw, P_vw) at k = 0 for the mean flow
+    void* fgw;    // nq * 2 complex
     // tables
     void* evtab;  // nq * 4 * L : C', C, S, S' at level l   [l][t][m]
     void* prtab;  // nq * 5 * L : w*C', w*C'', w*C, w*S, w*S'  [l][t][m]
@@ -109,6 +112,11 @@ struct Buffers {
     void* gmat;   // ngroups * stride real
     double* red;  // reduction scratch (doubles)
     int nred;
+    // matrix-core vertical transforms (FP32 storage): -1 = off, 0 = FP32 MFMA, 1 = BF16 x3
+    int mma = -1;
+    void* evA = nullptr;  // [4][ev_nlt][ev_nks][64][KPL] floats (C', C, S, S')
+    void* prA = nullptr;  // [5][pr_nks][64][KPL] floats (wC', wC'', wC, wS, wS')
+    int ev_nlt = 0, ev_nks = 0, pr_nks = 0;
 };
 
 struct StepCoef {

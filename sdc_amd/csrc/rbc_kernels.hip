@@ -307,25 +307,82 @@ __global__ void __launch_bounds__(256) k_fft_y_fwd(DevGeom g, int TX, const cpx<
 }
 
 // ---------------------------------------------------------------------------------------
-// Stage 5: projection.  Block = 64 modes; wave w accumulates vertical modes
-// m = w, w+4, ... in chunks.  Per level the seven product spectra give
-//   A  = kx^2 Puu + 2 kx ky Puv + ky^2 Pvv - k^2 Pww         (tested with C')
-//   Bf = i kx Puw + i ky Pvw                                  (with C'' + k^2 C)
-//   Ag = -kx ky (Puu - Pvv) + (kx^2 - ky^2) Puv               (with S)
-//   Bg = i kx Pvw - i ky Puw                                  (with S')
-//   At = u.grad(theta)                                        (with S)
+// Stage 5: projection.  The seven product spectra of a level combine into five level
+// combinations (formed where (kx, ky) of the output are known: in the forward y-FFT, or by
+// k_combine on the generic path):
+//   D0 = kx^2 Puu + 2 kx ky Puv + ky^2 Pvv - k^2 Pww         (tested with C')
+//   D1 = i kx Puw + i ky Pvw                                  (with C'' + k^2 C)
+//   D2 = -kx ky (Puu - Pvv) + (kx^2 - ky^2) Puv               (with S)
+//   D3 = i kx Pvw - i ky Puw                                  (with S')
+//   D4 = u.grad(theta)                                        (with S)
 // which is the Galerkin sum of nonlinear...cu:479-1067 with each triple integral written
 // as a quadrature over the levels.  Outputs are scaled by -1 (theta) and -1/Pr (:1366-1370).
+// The mean flow needs P_uw, P_vw at k = 0 per level (fgw), which D1, D3 lose (times k).
 // ---------------------------------------------------------------------------------------
-// Variant for L <= 4 * MCH: per round of 4 levels, wave w forms the five level combinations
-// of level l0 + w for the block's 64 modes in LDS once; then every wave accumulates its own
-// MCH vertical modes from LDS (each product is read from memory once).
+template <typename T>
+__device__ __forceinline__ void combine_add(int q, cpx<T> V, T kx, T ky, cpx<T> D[5]) {
+    switch (q) {
+        case 0: D[0] = D[0] + cscale(V, kx * kx); D[2] = D[2] + cscale(V, -kx * ky); break;               // uu
+        case 1: D[0] = D[0] + cscale(V, T(2) * kx * ky); D[2] = D[2] + cscale(V, kx * kx - ky * ky); break;  // uv
+        case 2: D[0] = D[0] + cscale(V, ky * ky); D[2] = D[2] + cscale(V, kx * ky); break;                 // vv
+        case 3: D[1] = D[1] + imul(V, kx); D[3] = D[3] - imul(V, ky); break;                               // uw
+        case 4: D[1] = D[1] + imul(V, ky); D[3] = D[3] + imul(V, kx); break;                               // vw
+        case 5: D[0] = D[0] - cscale(V, kx * kx + ky * ky); break;                                         // ww
+        default: D[4] = D[4] + V; break;                                                                   // u.grad
+    }
+}
+
+// generic path: seven product spectra [q][l][p] -> five combinations [t][l][p] and fgw
+template <typename T>
+__global__ void __launch_bounds__(256) k_combine(DevGeom g, const cpx<T>* __restrict__ prod,
+                                                 const T* __restrict__ kxa, const T* __restrict__ kya,
+                                                 cpx<T>* __restrict__ D, cpx<T>* __restrict__ fgw) {
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t P = g.P, fs = (size_t)g.nq * P;
+    if (e >= fs) return;
+    const size_t p = e % P;
+    const T kx = kxa[p], ky = kya[p];
+    cpx<T> d[5];
+#pragma unroll
+    for (int t = 0; t < 5; ++t) d[t] = mk<T>(0, 0);
+#pragma unroll
+    for (int q = 0; q < 7; ++q) combine_add(q, prod[q * fs + e], kx, ky, d);
+#pragma unroll
+    for (int t = 0; t < 5; ++t) D[t * fs + e] = d[t];
+    if (p == 0) {
+        const size_t l = e / P;
+        fgw[2 * l] = prod[3 * fs + e];
+        fgw[2 * l + 1] = prod[4 * fs + e];
+    }
+}
+
+// mean flow (nonlinear...cu:920-1003): N_F = -<uw, S'>_{k=0}, N_G = -<vw, S'>_{k=0}, times -1/Pr
+template <typename T>
+__device__ __forceinline__ void mean_flow_projection(const DevGeom& g, const cpx<T>* __restrict__ fgw,
+                                                     const T* __restrict__ prtab, T Pr, cpx<T>* NF, cpx<T>* NG) {
+    const int L = g.L, nq = g.nq;
+    if (blockIdx.x == 0 && threadIdx.x < L) {
+        const int m = threadIdx.x;
+        cpx<T> sF = mk<T>(0, 0), sG = sF;
+        for (int l = 0; l < nq; ++l) {
+            const T sp = prtab[(size_t)l * 5 * L + 4 * L + m];
+            sF = sF + cscale(fgw[2 * l], sp);
+            sG = sG + cscale(fgw[2 * l + 1], sp);
+        }
+        NF[m] = cscale(sF, T(1) / Pr);
+        NG[m] = cscale(sG, T(1) / Pr);
+    }
+}
+
+// Variant for L <= 4 * MCH: per round of 4 levels the block stages the combinations of 4 levels
+// of its 64 modes in LDS; every wave then accumulates its own MCH vertical modes.
 template <typename T, int MCH>
-__global__ void __launch_bounds__(256) k_project_lds(DevGeom g, const cpx<T>* __restrict__ prod,
-                                                     const T* __restrict__ prtab, const T* __restrict__ kxa,
-                                                     const T* __restrict__ kya, T Pr, cpx<T>* __restrict__ Nt,
-                                                     cpx<T>* __restrict__ Nf, cpx<T>* __restrict__ Ng,
-                                                     cpx<T>* __restrict__ NF, cpx<T>* __restrict__ NG) {
+__global__ void __launch_bounds__(256) k_project_lds(DevGeom g, const cpx<T>* __restrict__ D,
+                                                     const cpx<T>* __restrict__ fgw, const T* __restrict__ prtab,
+                                                     const T* __restrict__ kxa, const T* __restrict__ kya, T Pr,
+                                                     cpx<T>* __restrict__ Nt, cpx<T>* __restrict__ Nf,
+                                                     cpx<T>* __restrict__ Ng, cpx<T>* __restrict__ NF,
+                                                     cpx<T>* __restrict__ NG) {
     constexpr int LV = 4;
     __shared__ cpx<T> sD[LV][5][64];
     const int L = g.L, P = g.P, nq = g.nq;
@@ -344,15 +401,8 @@ __global__ void __launch_bounds__(256) k_project_lds(DevGeom g, const cpx<T>* __
     for (int l0 = 0; l0 < nq; l0 += LV) {
         const int ls = l0 + wave;
         if (ls < nq) {
-            const cpx<T>* src = prod + (size_t)ls * P + pc;
-            const cpx<T> Puu = src[0], Puv = src[fs], Pvv = src[2 * fs], Puw = src[3 * fs];
-            const cpx<T> Pvw = src[4 * fs], Pww = src[5 * fs], Pad = src[6 * fs];
-            sD[wave][0][lane] = cscale(Puu, kx * kx) + cscale(Puv, T(2) * kx * ky) + cscale(Pvv, ky * ky) -
-                                cscale(Pww, k2);
-            sD[wave][1][lane] = imul(Puw, kx) + imul(Pvw, ky);
-            sD[wave][2][lane] = cscale(Puu - Pvv, -kx * ky) + cscale(Puv, kx * kx - ky * ky);
-            sD[wave][3][lane] = imul(Pvw, kx) - imul(Puw, ky);
-            sD[wave][4][lane] = Pad;
+#pragma unroll
+            for (int t = 0; t < 5; ++t) sD[wave][t][lane] = D[t * fs + (size_t)ls * P + pc];
         }
         __syncthreads();
         for (int lv = 0; lv < LV && l0 + lv < nq; ++lv) {
@@ -383,25 +433,17 @@ __global__ void __launch_bounds__(256) k_project_lds(DevGeom g, const cpx<T>* __
             }
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x < L) {
-        const int m = threadIdx.x;
-        cpx<T> sF = mk<T>(0, 0), sG = sF;
-        for (int l = 0; l < nq; ++l) {
-            const T sp = prtab[(size_t)l * 5 * L + 4 * L + m];
-            sF = sF + cscale(prod[3 * fs + (size_t)l * P], sp);
-            sG = sG + cscale(prod[4 * fs + (size_t)l * P], sp);
-        }
-        NF[m] = cscale(sF, -mp);
-        NG[m] = cscale(sG, -mp);
-    }
+    mean_flow_projection(g, fgw, prtab, Pr, NF, NG);
 }
 
+// General L: block = 64 modes; wave w accumulates vertical modes m = w*MCH + 4*MCH*i.
 template <typename T, int MCH>
-__global__ void __launch_bounds__(256) k_project(DevGeom g, const cpx<T>* __restrict__ prod,
-                                                 const T* __restrict__ prtab, const T* __restrict__ kxa,
-                                                 const T* __restrict__ kya, T Pr, cpx<T>* __restrict__ Nt,
-                                                 cpx<T>* __restrict__ Nf, cpx<T>* __restrict__ Ng,
-                                                 cpx<T>* __restrict__ NF, cpx<T>* __restrict__ NG) {
+__global__ void __launch_bounds__(256) k_project(DevGeom g, const cpx<T>* __restrict__ D,
+                                                 const cpx<T>* __restrict__ fgw, const T* __restrict__ prtab,
+                                                 const T* __restrict__ kxa, const T* __restrict__ kya, T Pr,
+                                                 cpx<T>* __restrict__ Nt, cpx<T>* __restrict__ Nf,
+                                                 cpx<T>* __restrict__ Ng, cpx<T>* __restrict__ NF,
+                                                 cpx<T>* __restrict__ NG) {
     const int L = g.L, P = g.P, nq = g.nq;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int p = blockIdx.x * 64 + lane;
@@ -416,24 +458,17 @@ __global__ void __launch_bounds__(256) k_project(DevGeom g, const cpx<T>* __rest
 #pragma unroll
         for (int i = 0; i < MCH; ++i) af[i] = ag[i] = at[i] = mk<T>(0, 0);
         for (int l = 0; l < nq; ++l) {
-            const cpx<T>* src = prod + (size_t)l * P + pc;
-            const cpx<T> Puu = src[0], Puv = src[fs], Pvv = src[2 * fs], Puw = src[3 * fs];
-            const cpx<T> Pvw = src[4 * fs], Pww = src[5 * fs], Pad = src[6 * fs];
-            const cpx<T> A = cscale(Puu, kx * kx) + cscale(Puv, T(2) * kx * ky) + cscale(Pvv, ky * ky) -
-                             cscale(Pww, k2);
-            const cpx<T> Bf = imul(Puw, kx) + imul(Pvw, ky);
-            const cpx<T> Ag = cscale(Puu - Pvv, -kx * ky) + cscale(Puv, kx * kx - ky * ky);
-            const cpx<T> Bg = imul(Pvw, kx) - imul(Puw, ky);
-            const T* tb = prtab + (size_t)l * 5 * L;  // wC', wC'', wC, wS, wS'
+            const cpx<T>* src = D + (size_t)l * P + pc;
+            const cpx<T> A = src[0], Bf = src[fs], Ag = src[2 * fs], Bg = src[3 * fs], At = src[4 * fs];
+            const T* tb = prtab + (size_t)l * 5 * L;
 #pragma unroll
             for (int i = 0; i < MCH; ++i) {
                 const int m = mb + i;
                 if (m < L) {
-                    const T cp = tb[m], cpp = tb[L + m], cv = tb[2 * L + m], sv = tb[3 * L + m],
-                            sp = tb[4 * L + m];
+                    const T cp = tb[m], cpp = tb[L + m], cv = tb[2 * L + m], sv = tb[3 * L + m], sp = tb[4 * L + m];
                     af[i] = af[i] + cscale(A, cp) + cscale(Bf, cpp + k2 * cv);
                     ag[i] = ag[i] + cscale(Ag, sv) + cscale(Bg, sp);
-                    at[i] = at[i] + cscale(Pad, sv);
+                    at[i] = at[i] + cscale(At, sv);
                 }
             }
         }
@@ -449,18 +484,7 @@ __global__ void __launch_bounds__(256) k_project(DevGeom g, const cpx<T>* __rest
             }
         }
     }
-    // mean flow (nonlinear...cu:920-1003): N_F = -<uw, S'>_{k=0}, N_G = -<vw, S'>_{k=0}
-    if (blockIdx.x == 0 && threadIdx.x < L) {
-        const int m = threadIdx.x;
-        cpx<T> sF = mk<T>(0, 0), sG = sF;
-        for (int l = 0; l < nq; ++l) {
-            const T sp = prtab[(size_t)l * 5 * L + 4 * L + m];
-            sF = sF + cscale(prod[3 * fs + (size_t)l * P], sp);
-            sG = sG + cscale(prod[4 * fs + (size_t)l * P], sp);
-        }
-        NF[m] = cscale(sF, -mp);
-        NG[m] = cscale(sG, -mp);
-    }
+    mean_flow_projection(g, fgw, prtab, Pr, NF, NG);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -610,34 +634,58 @@ __global__ void __launch_bounds__(256) k_update32(DevGeom g, int ngroups, int L0
     cpx<T>* snf = shm[wave][cls][0];
     cpx<T>* srhs = shm[wave][cls][1];
     const int b0 = gstart[grp], b1 = gstart[grp + 1];
-    for (int mi = b0; mi < b1; ++mi) {
+    const int m = cls + 2 * r;           // theta/f mode of this lane (valid for r < Lc)
+    const bool tf = r < Lc, gl = lane < L;
+    const cpx<T> z = mk<T>(0, 0);
+    // all loads of one member are issued together; the next member's are in flight while
+    // the current one is computed
+    cpx<T> nf = z, xt = z, xf = z, nt = z, rt0 = z, rf0 = z, ng = z, xg = z, rg0 = z;
+    T k2 = T(0);
+    size_t base = 0;
+    auto fetch = [&](int mi, cpx<T>& nf_, cpx<T>& xt_, cpx<T>& xf_, cpx<T>& nt_, cpx<T>& rt_, cpx<T>& rf_,
+                     cpx<T>& ng_, cpx<T>& xg_, cpx<T>& rg_, T& k2_, size_t& base_) {
         const int p = gmem[mi];
-        const size_t base = (size_t)p * L;
-        const T k2 = k2a[p];
-        const int m = cls + 2 * r;  // theta/f mode of this lane (valid for r < Lc)
-        cpx<T> xt = mk<T>(0, 0), xf = xt;
-        if (r < Lc) {
-            snf[r] = Nf[base + m];
-            xt = Xt[base + m];
-            xf = Xf[base + m];
+        base_ = (size_t)p * L;
+        k2_ = k2a[p];
+        if (tf) {
+            nf_ = Nf[base_ + m];
+            xt_ = Xt[base_ + m];
+            xf_ = Xf[base_ + m];
+            nt_ = Nt[base_ + m];
+            if (use_prev) {
+                rt_ = Rt[base_ + m];
+                rf_ = Rf[base_ + m];
+            }
         }
+        if (gl) {
+            ng_ = Ng[base_ + lane];
+            xg_ = Xg[base_ + lane];
+            if (use_prev) rg_ = Rg[base_ + lane];
+        }
+    };
+    if (b0 < b1) fetch(b0, nf, xt, xf, nt, rt0, rf0, ng, xg, rg0, k2, base);
+    for (int mi = b0; mi < b1; ++mi) {
+        cpx<T> nnf = z, nxt = z, nxf = z, nnt = z, nrt = z, nrf = z, nng = z, nxg = z, nrg = z;
+        T nk2 = T(0);
+        size_t nbase = 0;
+        if (mi + 1 < b1) fetch(mi + 1, nnf, nxt, nxf, nnt, nrt, nrf, nng, nxg, nrg, nk2, nbase);
+        if (tf) snf[r] = nf;
         wave_sync();
-        if (r < Lc) {
+        if (tf) {
             cpx<T> rf = mk<T>(0, 0);
 #pragma unroll
             for (int j = 0; j < 16; ++j)
                 if (j < Lc) rf = rf + cscale(snf[j], brow[j]);
-            const cpx<T> rt = cscale(Nt[base + m], T(2));
-            xt = xt + cscale(rt, a_cur);
-            xf = xf + cscale(rf, a_cur);
+            const cpx<T> rt = cscale(nt, T(2));
+            cpx<T> x1 = xt + cscale(rt, a_cur), x2 = xf + cscale(rf, a_cur);
             if (use_prev) {
-                xt = xt + cscale(Rt[base + m], a_prev);
-                xf = xf + cscale(Rf[base + m], a_prev);
+                x1 = x1 + cscale(rt0, a_prev);
+                x2 = x2 + cscale(rf0, a_prev);
             }
             Rt[base + m] = rt;
             Rf[base + m] = rf;
-            srhs[r] = xt;
-            srhs[Lc + r] = xf;
+            srhs[r] = x1;
+            srhs[Lc + r] = x2;
         }
         wave_sync();
         if (r < nc) {
@@ -650,16 +698,18 @@ __global__ void __launch_bounds__(256) k_update32(DevGeom g, int ngroups, int L0
             else Xf[base + mm] = acc;
         }
         // g: diagonal factors, one vertical mode per lane
-        if (lane < L) {
-            const int mg = lane;
-            const T mpi = T(mg + 1) * T(M_PI);
-            const cpx<T> rg = (k2 != T(0)) ? cscale(Ng[base + mg], T(-2) * Pr / k2) : mk<T>(0, 0);
-            cpx<T> xg = Xg[base + mg] + cscale(rg, a_cur);
-            if (use_prev) xg = xg + cscale(Rg[base + mg], a_prev);
-            Rg[base + mg] = rg;
-            Xg[base + mg] = (k2 != T(0)) ? cscale(xg, T(1) / (T(1) + dt * Pr * (k2 + mpi * mpi))) : mk<T>(0, 0);
+        if (gl) {
+            const T mpi = T(lane + 1) * T(M_PI);
+            const cpx<T> rg = (k2 != T(0)) ? cscale(ng, T(-2) * Pr / k2) : mk<T>(0, 0);
+            cpx<T> x3 = xg + cscale(rg, a_cur);
+            if (use_prev) x3 = x3 + cscale(rg0, a_prev);
+            Rg[base + lane] = rg;
+            Xg[base + lane] = (k2 != T(0)) ? cscale(x3, T(1) / (T(1) + dt * Pr * (k2 + mpi * mpi))) : mk<T>(0, 0);
         }
         wave_sync();
+        nf = nnf; xt = nxt; xf = nxf; nt = nnt; rt0 = nrt; rf0 = nrf; ng = nng; xg = nxg; rg0 = nrg;
+        k2 = nk2;
+        base = nbase;
     }
 }
 
@@ -1056,6 +1106,356 @@ __global__ void __launch_bounds__(512) k_fft_y_fwd512(DevGeom g, int ntile, int 
     }
 }
 
+// =======================================================================================
+// Vertical transforms on the matrix cores (FP32 storage).  MM = 0: v_mfma_f32_32x32x2f32
+// (K = 2 per instruction); MM = 1: BF16 hi/lo split, v_mfma_f32_32x32x16_bf16 (K = 16),
+// a*b ~ ah*bh + ah*bl + al*bh with FP32 accumulation.
+// Operand layouts (wave64): A (32 x K) lane l holds row l%32, k = KPL*(l/32) + e;
+// B (K x 32) lane l holds column l%32, k = KPL*(l/32) + e; D (32 x 32) lane l, register r
+// holds column l%32, row 8*(r/4) + 4*(l/32) + r%4.  KPL = K/2 elements per lane.
+// The A operands (basis tables) come pre-arranged per (table, tile, k-step, lane) by the host.
+// =======================================================================================
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ __bf16 bf16_rne(float x) {
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return __builtin_bit_cast(__bf16, (unsigned short)(u >> 16));
+}
+__device__ __forceinline__ float bf16_to_f32(__bf16 h) {
+    return __uint_as_float((unsigned)__builtin_bit_cast(unsigned short, h) << 16);
+}
+
+template <int MM>
+struct Mma;
+template <>
+struct Mma<0> {
+    static constexpr int K = 2, KPL = 1;
+    __device__ static __forceinline__ void run(f32x16& c, const float* a, const float* b) {
+        c = __builtin_amdgcn_mfma_f32_32x32x2f32(a[0], b[0], c, 0, 0, 0);
+    }
+};
+template <>
+struct Mma<1> {
+    static constexpr int K = 16, KPL = 8;
+    __device__ static __forceinline__ void run(f32x16& c, const float* a, const float* b) {
+        bf16x8 ah, al, bh, bl;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            ah[e] = bf16_rne(a[e]);
+            al[e] = bf16_rne(a[e] - bf16_to_f32(ah[e]));
+            bh[e] = bf16_rne(b[e]);
+            bl[e] = bf16_rne(b[e] - bf16_to_f32(bh[e]));
+        }
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, c, 0, 0, 0);
+    }
+};
+
+// Evaluation on the matrix cores: D[level][column] = sum_m A_t[level][m] B_f[m][column], with
+// columns = (mode, re/im) of 16 modes per wave and 4 tables C', C, S, S' applied to f, f, g,
+// theta, theta (5 chains).  Block = 64 modes (4 waves); the block's mode data is staged in
+// LDS as [field][m][128 columns].  evA: [4][NLT][NKS][64][KPL] floats.
+template <int MM>
+__global__ void __launch_bounds__(256) k_eval_mma(DevGeom g, int nlt, int nks, int per,
+                                                  const cpx<float>* __restrict__ Xt,
+                                                  const cpx<float>* __restrict__ Xf, const cpx<float>* __restrict__ Xg,
+                                                  const cpx<float>* __restrict__ XF, const cpx<float>* __restrict__ XG,
+                                                  const float* __restrict__ evA, const float* __restrict__ evtab,
+                                                  const float* __restrict__ kxa, const float* __restrict__ kya,
+                                                  cpx<float>* __restrict__ lev) {
+    using MT = Mma<MM>;
+    extern __shared__ float sX[];  // 3 * Lp * 128
+    const int L = g.L, P = g.P, nq = g.nq;
+    const int Lp = nks * MT::K;
+    // logical block = (64-mode block, level tile); the level tiles of a mode block are logical
+    // neighbours and so share one XCD's L2 copy of the staged modes
+    const int lb = xcd_logical_block(blockIdx.x, per);
+    if (lb >= ((P + 63) / 64) * nlt) return;
+    const int p0 = (lb / nlt) * 64, lt_block = lb % nlt;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int e = threadIdx.x; e < 64 * Lp; e += 256) {
+        const int mode = e / Lp, m = e - mode * Lp;
+        const int p = p0 + mode;
+        cpx<float> zt = mk(0.f, 0.f), zf = zt, zg = zt;
+        if (p < P && m < L) {
+            const size_t gi = (size_t)p * L + m;
+            zt = Xt[gi];
+            zf = Xf[gi];
+            zg = Xg[gi];
+        }
+        sX[(0 * Lp + m) * 128 + 2 * mode] = zt.x;
+        sX[(0 * Lp + m) * 128 + 2 * mode + 1] = zt.y;
+        sX[(1 * Lp + m) * 128 + 2 * mode] = zf.x;
+        sX[(1 * Lp + m) * 128 + 2 * mode + 1] = zf.y;
+        sX[(2 * Lp + m) * 128 + 2 * mode] = zg.x;
+        sX[(2 * Lp + m) * 128 + 2 * mode + 1] = zg.y;
+    }
+    __syncthreads();
+    const int j = lane & 31, half = lane >> 5;
+    const int col = wave * 32 + j;
+    const int p = p0 + (col >> 1), comp = col & 1;
+    const bool pvalid = p < P;
+    const int pc = pvalid ? p : 0;
+    const float kx = kxa[pc], ky = kya[pc], k2 = kx * kx + ky * ky;
+    const size_t fs = (size_t)nq * P;
+    {
+        const int lt = lt_block;
+        f32x16 cFp = {}, cFc = {}, cGs = {}, cTs = {}, cTz = {};
+        for (int s = 0; s < nks; ++s) {
+            float a[4][MT::KPL], bt[MT::KPL], bf[MT::KPL], bg[MT::KPL];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const float* src = evA + ((((size_t)t * nlt + lt) * nks + s) * 64 + lane) * MT::KPL;
+#pragma unroll
+                for (int e = 0; e < MT::KPL; ++e) a[t][e] = src[e];
+            }
+#pragma unroll
+            for (int e = 0; e < MT::KPL; ++e) {
+                const int m = s * MT::K + half * MT::KPL + e;
+                bt[e] = sX[(0 * Lp + m) * 128 + col];
+                bf[e] = sX[(1 * Lp + m) * 128 + col];
+                bg[e] = sX[(2 * Lp + m) * 128 + col];
+            }
+            MT::run(cFp, a[0], bf);  // C' . f
+            MT::run(cFc, a[1], bf);  // C  . f
+            MT::run(cGs, a[2], bg);  // S  . g
+            MT::run(cTs, a[2], bt);  // S  . theta
+            MT::run(cTz, a[3], bt);  // S' . theta
+        }
+        // epilogue: partner component of the same mode sits in lane ^ 1
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int l = lt * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
+            const float Fp = cFp[r], Fc = cFc[r], Gs = cGs[r], Ts = cTs[r], Tz = cTz[r];
+            const float oFp = __shfl_xor(Fp, 1), oGs = __shfl_xor(Gs, 1), oTs = __shfl_xor(Ts, 1);
+            float u, v, w, tx, ty, tz;
+            if (comp == 0) {  // real parts; o* hold the imaginary parts
+                u = -kx * oFp - ky * oGs;
+                v = -ky * oFp + kx * oGs;
+                tx = -kx * oTs;
+                ty = -ky * oTs;
+            } else {  // imaginary parts; o* hold the real parts
+                u = kx * oFp + ky * oGs;
+                v = ky * oFp - kx * oGs;
+                tx = kx * oTs;
+                ty = ky * oTs;
+            }
+            w = k2 * Fc;
+            tz = Tz;
+            if (p == 0 && l < nq) {  // mean flow at k = 0 (add_dc_part)
+                const float* tb = evtab + (size_t)l * 4 * L;
+                float sF = 0.f, sG = 0.f;
+                for (int m = 0; m < L; ++m) {
+                    sF += tb[2 * L + m] * (comp ? XF[m].y : XF[m].x);
+                    sG += tb[2 * L + m] * (comp ? XG[m].y : XG[m].x);
+                }
+                u += sF;
+                v += sG;
+            }
+            if (pvalid && l < nq) {
+                float* o = reinterpret_cast<float*>(lev) + ((size_t)l * P + p) * 2 + comp;
+                o[0 * 2 * fs] = u;
+                o[1 * 2 * fs] = v;
+                o[2 * 2 * fs] = w;
+                o[3 * 2 * fs] = tx;
+                o[4 * 2 * fs] = ty;
+                o[5 * 2 * fs] = tz;
+            }
+        }
+    }
+}
+
+// Projection on the matrix cores: Out[m][column] = sum_l T_t[m][l] D_t[l][column]; the B
+// operands are float components of the five level combinations (column = (mode, re/im), so a
+// lane reads one float and the lanes of a wave read 128 contiguous bytes per level).
+// prA: [5][NKS][64][KPL].  Block = 64 modes, wave = 16 modes (32 columns); rows m < L <= 32.
+template <int MM>
+__global__ void __launch_bounds__(256) k_project_mma(DevGeom g, int nks, const float* __restrict__ Dfl,
+                                                     const cpx<float>* __restrict__ fgw, const float* __restrict__ prA,
+                                                     const float* __restrict__ prtab, const float* __restrict__ kxa,
+                                                     const float* __restrict__ kya, float Pr,
+                                                     cpx<float>* __restrict__ Nt, cpx<float>* __restrict__ Nf,
+                                                     cpx<float>* __restrict__ Ng, cpx<float>* __restrict__ NF,
+                                                     cpx<float>* __restrict__ NG) {
+    using MT = Mma<MM>;
+    __shared__ float sO[4][3][16 * 64];  // per wave: [field][mode][2L] staging for the write-out
+    const int L = g.L, P = g.P, nq = g.nq;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j = lane & 31, half = lane >> 5;
+    const int col = wave * 32 + j;
+    const int p = blockIdx.x * 64 + (col >> 1), comp = col & 1;
+    const bool pvalid = p < P;
+    const int pc = pvalid ? p : 0;
+    const float kx = kxa[pc], ky = kya[pc], k2 = kx * kx + ky * ky;
+    const size_t ffs = (size_t)2 * nq * P;  // floats per combination
+    const float* dcol = Dfl + (size_t)2 * pc + comp;
+    f32x16 cF1 = {}, cF2 = {}, cG = {}, cT = {};
+    constexpr int U = MM == 0 ? 4 : 1;  // k-steps whose operand loads are issued together
+    for (int s0 = 0; s0 < nks; s0 += U) {
+        float b[U][5][MT::KPL];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int e = 0; e < MT::KPL; ++e) {
+                const int l = (s0 + u) * MT::K + half * MT::KPL + e;
+#pragma unroll
+                for (int t = 0; t < 5; ++t) b[u][t][e] = (l < nq) ? dcol[t * ffs + (size_t)2 * l * P] : 0.f;
+            }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int s = s0 + u;
+            if (s >= nks) break;
+            float a[5][MT::KPL];
+#pragma unroll
+            for (int t = 0; t < 5; ++t) {
+                const float* src = prA + (((size_t)t * nks + s) * 64 + lane) * MT::KPL;
+#pragma unroll
+                for (int e = 0; e < MT::KPL; ++e) a[t][e] = src[e];
+            }
+            MT::run(cF1, a[0], b[u][0]);  // wC'  . D0
+            MT::run(cF1, a[1], b[u][1]);  // wC'' . D1
+            MT::run(cF2, a[2], b[u][1]);  // wC   . D1   (times k^2 below)
+            MT::run(cG, a[3], b[u][2]);   // wS   . D2
+            MT::run(cG, a[4], b[u][3]);   // wS'  . D3
+            MT::run(cT, a[3], b[u][4]);   // wS   . D4
+        }
+    }
+    const float mt = -1.f, mp = -1.f / Pr;
+    const int lm = j >> 1;  // local mode 0..15 within the wave
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int m = 8 * (r >> 2) + 4 * half + (r & 3);
+        if (m < L) {
+            sO[wave][0][lm * 64 + 2 * m + comp] = mt * cT[r];
+            sO[wave][1][lm * 64 + 2 * m + comp] = mp * (cF1[r] + k2 * cF2[r]);
+            sO[wave][2][lm * 64 + 2 * m + comp] = mp * cG[r];
+        }
+    }
+    wave_sync();
+    // write 16 modes x L complex per field, contiguous in memory ([p][m] layout)
+    const int pw0 = blockIdx.x * 64 + wave * 16;
+    for (int e = lane; e < 16 * 2 * L; e += 64) {
+        const int lmode = e / (2 * L), q = e - lmode * 2 * L;
+        const int pp = pw0 + lmode;
+        if (pp < P) {
+            reinterpret_cast<float*>(Nt)[(size_t)pp * 2 * L + q] = sO[wave][0][lmode * 64 + q];
+            reinterpret_cast<float*>(Nf)[(size_t)pp * 2 * L + q] = sO[wave][1][lmode * 64 + q];
+            reinterpret_cast<float*>(Ng)[(size_t)pp * 2 * L + q] = sO[wave][2][lmode * 64 + q];
+        }
+    }
+    mean_flow_projection(g, fgw, prtab, Pr, NF, NG);
+}
+
+// Forward y-FFT at N = 512 fused with the level combinations.  A block of 8 waves handles 8
+// kx columns of one level and one product group:
+//   group 0: uu, uv, vv, ww -> D0, D2     group 1: uw, vw -> D1, D3     group 2: u.grad -> D4
+// The products of the group are transformed in turn (tile in LDS, one column per wave) and the
+// group's combinations of the kept rows accumulate in registers; the next product's tile is
+// read into registers while the current one is transformed.  Group 1 also writes the k = 0
+// values of P_uw and P_vw (fgw) for the mean flow.
+template <int GRP>
+struct YGroup;
+template <>
+struct YGroup<0> {  // uu, uv, vv, ww -> D0, D2
+    static constexpr int n = 4, t0 = 0, t1 = 2;
+    __device__ static int q(int i) { return i == 3 ? 5 : i; }
+    __device__ static void add(int i, cpx<float> V, float kx, float ky, cpx<float>& d0, cpx<float>& d1) {
+        if (i == 0) { d0 = d0 + cscale(V, kx * kx); d1 = d1 + cscale(V, -kx * ky); }
+        else if (i == 1) { d0 = d0 + cscale(V, 2.f * kx * ky); d1 = d1 + cscale(V, kx * kx - ky * ky); }
+        else if (i == 2) { d0 = d0 + cscale(V, ky * ky); d1 = d1 + cscale(V, kx * ky); }
+        else { d0 = d0 - cscale(V, kx * kx + ky * ky); }
+    }
+};
+template <>
+struct YGroup<1> {  // uw, vw -> D1, D3
+    static constexpr int n = 2, t0 = 1, t1 = 3;
+    __device__ static int q(int i) { return 3 + i; }
+    __device__ static void add(int i, cpx<float> V, float kx, float ky, cpx<float>& d0, cpx<float>& d1) {
+        if (i == 0) { d0 = d0 + imul(V, kx); d1 = d1 - imul(V, ky); }
+        else { d0 = d0 + imul(V, ky); d1 = d1 + imul(V, kx); }
+    }
+};
+template <>
+struct YGroup<2> {  // u.grad(theta) -> D4
+    static constexpr int n = 1, t0 = 4, t1 = -1;
+    __device__ static int q(int) { return 6; }
+    __device__ static void add(int, cpx<float> V, float, float, cpx<float>& d0, cpx<float>&) { d0 = d0 + V; }
+};
+
+template <int GRP>
+__global__ void __launch_bounds__(512) k_fft_y_fwd512c(DevGeom g, int ntile, int nblk, int per,
+                                                       const cpx<float>* __restrict__ hout,
+                                                       const cpx<float>* __restrict__ tw,
+                                                       const float* __restrict__ kxa, const float* __restrict__ kya,
+                                                       cpx<float>* __restrict__ D, cpx<float>* __restrict__ fgw) {
+    using G = YGroup<GRP>;
+    __shared__ cpx<float> tile[576 * YS];
+    const int lb = xcd_logical_block(blockIdx.x, per);
+    if (lb >= nblk) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int NYc = g.NYc, P = g.P, nq = g.nq, KX = g.KX;
+    const int tix = lb % ntile, l = lb / ntile;
+    const int kx0 = tix * YT;
+    const int a = (NYc - 1) / 2, shift = 512 - NYc;
+    const int ncol = min(YT, KX - kx0);
+    const int c = wave, kxc = kx0 + c;
+    const bool cvalid = kxc < KX;
+    int kyc[8];
+    float kyv[8];
+    const float kxv = cvalid ? kxa[(size_t)kxc * NYc] : 0.f;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int n = lane + 64 * r;
+        kyc[r] = (n <= a) ? n : ((n >= 512 - a) ? n - shift : -1);
+        kyv[r] = (cvalid && kyc[r] >= 0) ? kya[(size_t)kxc * NYc + kyc[r]] : 0.f;
+    }
+    cpx<float> d0[8], d1[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) d0[r] = d1[r] = mk(0.f, 0.f);
+    // tile element e = threadIdx.x + 512 i (i < 8): y = e / YT, column e % YT
+    cpx<float> pre[8];
+    auto fetch = [&](int q) {
+        const cpx<float>* src = hout + ((size_t)q * nq + l) * (size_t)512 * g.KXp + kx0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int e = threadIdx.x + 512 * i, y = e / YT, cc = e - y * YT;
+            pre[i] = (cc < ncol) ? src[(size_t)y * g.KXp + cc] : mk(0.f, 0.f);
+        }
+    };
+    fetch(G::q(0));
+#pragma unroll
+    for (int iq = 0; iq < G::n; ++iq) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int e = threadIdx.x + 512 * i, y = e / YT, cc = e - y * YT;
+            tile[pad8(y) * YS + cc] = pre[i];
+        }
+        __syncthreads();
+        if (iq + 1 < G::n) fetch(G::q(iq + 1));
+        cpx<float> v[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v[r] = tile[pad8(lane + 64 * r) * YS + c];
+        fft512_wave<float, false, YS>(v, tile + c, tw, lane);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) G::add(iq, v[r], kxv, kyv[r], d0[r], d1[r]);
+        if (GRP == 1 && kxc == 0 && lane == 0) fgw[2 * l + iq] = v[0];  // P_uw, P_vw at row 0
+        __syncthreads();  // the tile is rewritten for the next product
+    }
+    if (cvalid) {
+        const size_t fs = (size_t)nq * P;
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+            if (kyc[r] >= 0) {
+                cpx<float>* o = D + (size_t)l * P + (size_t)kxc * NYc + kyc[r];
+                o[G::t0 * fs] = d0[r];
+                if (G::t1 >= 0) o[G::t1 * fs] = d1[r];
+            }
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------------------
@@ -1084,6 +1484,23 @@ static inline int fft_tile(int N, size_t csz) {
 template <typename T>
 void Launch<T>::eval(const DevGeom& g, const Buffers& b, void* const X[3], const void* XF, const void* XG,
                      hipStream_t s) {
+    if (sizeof(T) == 4 && b.mma >= 0) {
+        const int K = b.mma == 0 ? 2 : 16;
+        const size_t smm = (size_t)3 * b.ev_nks * K * 128 * sizeof(float);
+        const int nblk = ((g.P + 63) / 64) * b.ev_nlt, per = (nblk + 7) / 8;
+        dim3 grid(8 * per);
+        if (b.mma == 0)
+            hipLaunchKernelGGL(k_eval_mma<0>, grid, dim3(256), smm, s, g, b.ev_nlt, b.ev_nks, per, (const cpx<float>*)X[0],
+                               (const cpx<float>*)X[1], (const cpx<float>*)X[2], (const cpx<float>*)XF,
+                               (const cpx<float>*)XG, (const float*)b.evA, (const float*)b.evtab,
+                               (const float*)b.kx, (const float*)b.ky, (cpx<float>*)b.lev);
+        else
+            hipLaunchKernelGGL(k_eval_mma<1>, grid, dim3(256), smm, s, g, b.ev_nlt, b.ev_nks, per, (const cpx<float>*)X[0],
+                               (const cpx<float>*)X[1], (const cpx<float>*)X[2], (const cpx<float>*)XF,
+                               (const cpx<float>*)XG, (const float*)b.evA, (const float*)b.evtab,
+                               (const float*)b.kx, (const float*)b.ky, (cpx<float>*)b.lev);
+        return;
+    }
     const size_t sm = (size_t)3 * g.L * 64 * sizeof(cpx<T>);
     const int LG = g.nq >= 64 ? g.nq / 32 : 1;  // level groups per 64-mode block
     const int nblk = ((g.P + 63) / 64) * LG, per = (nblk + 7) / 8;
@@ -1124,9 +1541,16 @@ void Launch<T>::rows(const DevGeom& g, const Buffers& b, hipStream_t s) {
 template <typename T>
 void Launch<T>::fft_y_fwd(const DevGeom& g, const Buffers& b, hipStream_t s) {
     if (fast512_y<T>(g)) {
-        const int ntile = (g.KX + YT - 1) / YT, nblk = ntile * g.nq * 7, per = (nblk + 7) / 8;
-        hipLaunchKernelGGL(k_fft_y_fwd512, dim3(8 * per), dim3(512), 0, s, g, ntile, nblk, per,
-                           (const cpx<float>*)b.hout, (const cpx<float>*)b.twN, (cpx<float>*)b.prod);
+        const int ntile = (g.KX + YT - 1) / YT, nblk = ntile * g.nq, per = (nblk + 7) / 8;
+        hipLaunchKernelGGL(k_fft_y_fwd512c<0>, dim3(8 * per), dim3(512), 0, s, g, ntile, nblk, per,
+                           (const cpx<float>*)b.hout, (const cpx<float>*)b.twN, (const float*)b.kx,
+                           (const float*)b.ky, (cpx<float>*)b.lev, (cpx<float>*)b.fgw);
+        hipLaunchKernelGGL(k_fft_y_fwd512c<1>, dim3(8 * per), dim3(512), 0, s, g, ntile, nblk, per,
+                           (const cpx<float>*)b.hout, (const cpx<float>*)b.twN, (const float*)b.kx,
+                           (const float*)b.ky, (cpx<float>*)b.lev, (cpx<float>*)b.fgw);
+        hipLaunchKernelGGL(k_fft_y_fwd512c<2>, dim3(8 * per), dim3(512), 0, s, g, ntile, nblk, per,
+                           (const cpx<float>*)b.hout, (const cpx<float>*)b.twN, (const float*)b.kx,
+                           (const float*)b.ky, (cpx<float>*)b.lev, (cpx<float>*)b.fgw);
         return;
     }
     const int TX = fft_tile(g.N, sizeof(cpx<T>));
@@ -1134,6 +1558,9 @@ void Launch<T>::fft_y_fwd(const DevGeom& g, const Buffers& b, hipStream_t s) {
     const size_t sm = (size_t)2 * TX * g.N * sizeof(cpx<T>);
     hipLaunchKernelGGL(k_fft_y_fwd<T>, grid, dim3(256), sm, s, g, TX, (const cpx<T>*)b.hout, b.rows,
                        (const cpx<T>*)b.twN, (cpx<T>*)b.prod);
+    const size_t n = (size_t)g.nq * g.P;
+    hipLaunchKernelGGL(k_combine<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, (const cpx<T>*)b.prod,
+                       (const T*)b.kx, (const T*)b.ky, (cpx<T>*)b.lev, (cpx<T>*)b.fgw);
 }
 
 template <typename T>
@@ -1141,15 +1568,28 @@ void Launch<T>::project(const DevGeom& g, const Buffers& b, double Pr, void* con
                         hipStream_t s) {
     constexpr int MCH = sizeof(T) == 8 ? 4 : 8;
     dim3 grid((g.P + 63) / 64);
-    if (g.L <= 4 * MCH) {
-        hipLaunchKernelGGL((k_project_lds<T, MCH>), grid, dim3(256), 0, s, g, (const cpx<T>*)b.prod,
-                           (const T*)b.prtab, (const T*)b.kx, (const T*)b.ky, (T)Pr, (cpx<T>*)Nout[0],
-                           (cpx<T>*)Nout[1], (cpx<T>*)Nout[2], (cpx<T>*)NFo, (cpx<T>*)NGo);
+    if (sizeof(T) == 4 && b.mma >= 0) {
+        if (b.mma == 0)
+            hipLaunchKernelGGL(k_project_mma<0>, grid, dim3(256), 0, s, g, b.pr_nks, (const float*)b.lev,
+                               (const cpx<float>*)b.fgw, (const float*)b.prA, (const float*)b.prtab,
+                               (const float*)b.kx, (const float*)b.ky, (float)Pr, (cpx<float>*)Nout[0],
+                               (cpx<float>*)Nout[1], (cpx<float>*)Nout[2], (cpx<float>*)NFo, (cpx<float>*)NGo);
+        else
+            hipLaunchKernelGGL(k_project_mma<1>, grid, dim3(256), 0, s, g, b.pr_nks, (const float*)b.lev,
+                               (const cpx<float>*)b.fgw, (const float*)b.prA, (const float*)b.prtab,
+                               (const float*)b.kx, (const float*)b.ky, (float)Pr, (cpx<float>*)Nout[0],
+                               (cpx<float>*)Nout[1], (cpx<float>*)Nout[2], (cpx<float>*)NFo, (cpx<float>*)NGo);
         return;
     }
-    hipLaunchKernelGGL((k_project<T, MCH>), grid, dim3(256), 0, s, g, (const cpx<T>*)b.prod, (const T*)b.prtab,
-                       (const T*)b.kx, (const T*)b.ky, (T)Pr, (cpx<T>*)Nout[0], (cpx<T>*)Nout[1],
-                       (cpx<T>*)Nout[2], (cpx<T>*)NFo, (cpx<T>*)NGo);
+    if (g.L <= 4 * MCH) {
+        hipLaunchKernelGGL((k_project_lds<T, MCH>), grid, dim3(256), 0, s, g, (const cpx<T>*)b.lev,
+                           (const cpx<T>*)b.fgw, (const T*)b.prtab, (const T*)b.kx, (const T*)b.ky, (T)Pr,
+                           (cpx<T>*)Nout[0], (cpx<T>*)Nout[1], (cpx<T>*)Nout[2], (cpx<T>*)NFo, (cpx<T>*)NGo);
+        return;
+    }
+    hipLaunchKernelGGL((k_project<T, MCH>), grid, dim3(256), 0, s, g, (const cpx<T>*)b.lev, (const cpx<T>*)b.fgw,
+                       (const T*)b.prtab, (const T*)b.kx, (const T*)b.ky, (T)Pr, (cpx<T>*)Nout[0],
+                       (cpx<T>*)Nout[1], (cpx<T>*)Nout[2], (cpx<T>*)NFo, (cpx<T>*)NGo);
 }
 
 template <typename T>

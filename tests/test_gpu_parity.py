@@ -202,3 +202,51 @@ def test_c3_fp32_vs_fp64_gpu_steps():
     e = _cmp(out, ref, keep)
     print("C3 fp32 vs fp64 20-step rel errors", e)
     assert max(e[:3]) < 1e-4, e
+
+
+def test_fp32_matrix_core_vs_valu_and_oracle():
+    """FP32 vertical transforms on the matrix cores against the VALU kernels and the oracle."""
+    print("case: fp32 mfma vs valu, 64x64x16", flush=True)
+    cfg = dict(M=64, N=64, L=16, Lx=3.0, Ly=3.5, Ra=3e4, Pr=0.7, dt=2e-4)
+    st = random_state(64, 64, 16, seed=31, amp=1e-2, decay=0.05)
+    keep = kept_mask(64, 64)
+    ref = _orc(cfg).nonlinear(*st)
+    with _gpu(cfg, "f32") as G:
+        mma = G.calculate_nonlinear(*st)
+    os.environ["RBC_NO_MFMA"] = "1"
+    try:
+        with _gpu(cfg, "f32") as G:
+            valu = G.calculate_nonlinear(*st)
+    finally:
+        del os.environ["RBC_NO_MFMA"]
+    e1, e2, e3 = _cmp(mma, ref, keep), _cmp(valu, ref, keep), _cmp(mma, valu, keep)
+    print("fp32 mfma vs oracle", e1, "valu vs oracle", e2, "mfma vs valu", e3)
+    assert max(e1) < 2e-5 and max(e2) < 2e-5 and max(e3) < 2e-5
+
+
+def test_bf16x3_vertical_transforms():
+    """BF16 hi/lo split matrix-core vertical transforms (FP32 accumulation): nonlinear term vs
+    the oracle and 20 steps of the 512^2 x 32 workload vs the FP64 GPU path."""
+    print("case: bf16x3 nonlinear 64x64x16 and C3 20 steps", flush=True)
+    cfg = dict(M=64, N=64, L=16, Lx=3.0, Ly=3.5, Ra=3e4, Pr=0.7, dt=2e-4)
+    st = random_state(64, 64, 16, seed=31, amp=1e-2, decay=0.05)
+    keep = kept_mask(64, 64)
+    ref = _orc(cfg).nonlinear(*st)
+    with _gpu(cfg, "bf16x3") as G:
+        e = _cmp(G.calculate_nonlinear(*st), ref, keep)
+    print("bf16x3 nonlinear rel errors", e)
+    assert max(e) < 2e-4, e
+    cfg3 = dict(M=512, N=512, L=32, Lx=4.0, Ly=4.0, Ra=1e6, Pr=0.7, dt=5e-5)
+    st3 = random_theta(512, 512, 32, seed=3, amp=1e-3)
+    keep3 = kept_mask(512, 512)
+    with _gpu(cfg3, "f64") as G:
+        G.set_state(*st3)
+        G.step_time(20)
+        r3 = G.get_state()
+    with _gpu(cfg3, "bf16x3") as G:
+        G.set_state(*st3)
+        G.step_time(20)
+        o3 = G.get_state()
+    e3 = _cmp(o3, r3, keep3)
+    print("C3 bf16x3 vs fp64 20-step rel errors", e3)
+    assert max(e3[:3]) < 1e-4, e3
