@@ -409,6 +409,22 @@ int rbc_create(const rbc_params* p, rbc_ctx** out) {
         HIPCHK(hipMemcpy(b.gstart, c->fac.group_start.data(), c->fac.group_start.size() * sizeof(int),
                          hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(b.gmem, c->fac.members.data(), c->fac.members.size() * sizeof(int), hipMemcpyHostToDevice));
+        {  // chunks of at most 8 members per wave of the group update (bounded tail latency)
+            constexpr int kChunk = 8;
+            std::vector<int> cg, cb;
+            const auto& gs = c->fac.group_start;
+            for (int gi = 0; gi + 1 < (int)gs.size(); ++gi)
+                for (int m0 = gs[gi]; m0 < gs[gi + 1]; m0 += kChunk) {
+                    cg.push_back(gi);
+                    cb.push_back(m0);
+                }
+            cb.push_back(gs.back());
+            b.nchunk = (int)cg.size();
+            b.cgrp = (int*)c->dalloc(cg.size() * sizeof(int));
+            b.cbeg = (int*)c->dalloc(cb.size() * sizeof(int));
+            HIPCHK(hipMemcpy(b.cgrp, cg.data(), cg.size() * sizeof(int), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(b.cbeg, cb.data(), cb.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
         b.gmat = c->dalloc(c->fac.mats.size() * c->rsz);
         upload_reals(c.get(), b.gmat, c->fac.mats);
         b.nred = (int)((P + 255) / 256) + 2;

@@ -109,6 +109,11 @@ struct Buffers {
     int* rows;    // NYc kept rows
     int* gstart;  // ngroups + 1
     int* gmem;    // P
+    // work chunks of at most kChunk members of one group (a group's chunks are consecutive):
+    // chunk c covers members [cbeg[c], cbeg[c+1]) of group cgrp[c]
+    int* cgrp;
+    int* cbeg;
+    int nchunk;
     void* gmat;   // ngroups * stride real
     double* red;  // reduction scratch (doubles)
     int nred;

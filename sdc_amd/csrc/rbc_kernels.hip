@@ -608,8 +608,9 @@ __global__ void __launch_bounds__(64) k_update(DevGeom g, int L0, int L1, int st
 // B^-1_f (L_c <= 16 values) for every member of the group.  Same arithmetic and rounding
 // order as k_update; mode UPD_STEP only.
 template <typename T>
-__global__ void __launch_bounds__(256) k_update32(DevGeom g, int ngroups, int L0, int L1, int stride, T Pr, T dt,
-                                                  T a_cur, T a_prev, int use_prev, const int* __restrict__ gstart,
+__global__ void __launch_bounds__(256) k_update32(DevGeom g, int nchunk, int L0, int L1, int stride, T Pr, T dt,
+                                                  T a_cur, T a_prev, int use_prev, const int* __restrict__ cgrp,
+                                                  const int* __restrict__ cbeg,
                                                   const int* __restrict__ gmem, const T* __restrict__ gmat,
                                                   const T* __restrict__ k2a, cpx<T>* __restrict__ Xt,
                                                   cpx<T>* __restrict__ Xf, cpx<T>* __restrict__ Xg,
@@ -618,8 +619,9 @@ __global__ void __launch_bounds__(256) k_update32(DevGeom g, int ngroups, int L0
                                                   cpx<T>* __restrict__ Rf, cpx<T>* __restrict__ Rg) {
     __shared__ cpx<T> shm[4][2][2][32];  // [wave][class][0: N_f, 1: rhs][j]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int grp = blockIdx.x * 4 + wave;
-    if (grp >= ngroups) return;  // whole wave leaves; no block-wide barrier below
+    const int ch = blockIdx.x * 4 + wave;  // one chunk (<= kChunk members of one group) per wave
+    if (ch >= nchunk) return;               // whole wave leaves; no block-wide barrier below
+    const int grp = cgrp[ch];
     const int L = g.L;
     const int cls = lane >> 5, r = lane & 31;
     const int Lc = cls ? L1 : L0, nc = 2 * Lc, n0 = 2 * L0;
@@ -633,7 +635,7 @@ __global__ void __launch_bounds__(256) k_update32(DevGeom g, int ngroups, int L0
     for (int j = 0; j < 16; ++j) brow[j] = (j < Lc && r < Lc) ? Binv[(size_t)j * Lc + r] : T(0);
     cpx<T>* snf = shm[wave][cls][0];
     cpx<T>* srhs = shm[wave][cls][1];
-    const int b0 = gstart[grp], b1 = gstart[grp + 1];
+    const int b0 = cbeg[ch], b1 = cbeg[ch + 1];
     const int m = cls + 2 * r;           // theta/f mode of this lane (valid for r < Lc)
     const bool tf = r < Lc, gl = lane < L;
     const cpx<T> z = mk<T>(0, 0);
@@ -1167,7 +1169,7 @@ __global__ void __launch_bounds__(256) k_eval_mma(DevGeom g, int nlt, int nks, i
                                                   const float* __restrict__ kxa, const float* __restrict__ kya,
                                                   cpx<float>* __restrict__ lev) {
     using MT = Mma<MM>;
-    extern __shared__ float sX[];  // 3 * Lp * 128
+    extern __shared__ float sX[];  // 3 * 128 * (Lp + 1)
     const int L = g.L, P = g.P, nq = g.nq;
     const int Lp = nks * MT::K;
     // logical block = (64-mode block, level tile); the level tiles of a mode block are logical
@@ -1176,6 +1178,10 @@ __global__ void __launch_bounds__(256) k_eval_mma(DevGeom g, int nlt, int nks, i
     if (lb >= ((P + 63) / 64) * nlt) return;
     const int p0 = (lb / nlt) * 64, lt_block = lb % nlt;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // LDS layout [field][column][LP] with column = 2 mode + re/im and an odd row length LP, so
+    // that both the staging writes (m fastest) and the operand reads (column fastest) spread
+    // over the banks
+    const int LP = Lp + 1;
     for (int e = threadIdx.x; e < 64 * Lp; e += 256) {
         const int mode = e / Lp, m = e - mode * Lp;
         const int p = p0 + mode;
@@ -1186,12 +1192,12 @@ __global__ void __launch_bounds__(256) k_eval_mma(DevGeom g, int nlt, int nks, i
             zf = Xf[gi];
             zg = Xg[gi];
         }
-        sX[(0 * Lp + m) * 128 + 2 * mode] = zt.x;
-        sX[(0 * Lp + m) * 128 + 2 * mode + 1] = zt.y;
-        sX[(1 * Lp + m) * 128 + 2 * mode] = zf.x;
-        sX[(1 * Lp + m) * 128 + 2 * mode + 1] = zf.y;
-        sX[(2 * Lp + m) * 128 + 2 * mode] = zg.x;
-        sX[(2 * Lp + m) * 128 + 2 * mode + 1] = zg.y;
+        sX[(0 * 128 + 2 * mode) * LP + m] = zt.x;
+        sX[(0 * 128 + 2 * mode + 1) * LP + m] = zt.y;
+        sX[(1 * 128 + 2 * mode) * LP + m] = zf.x;
+        sX[(1 * 128 + 2 * mode + 1) * LP + m] = zf.y;
+        sX[(2 * 128 + 2 * mode) * LP + m] = zg.x;
+        sX[(2 * 128 + 2 * mode + 1) * LP + m] = zg.y;
     }
     __syncthreads();
     const int j = lane & 31, half = lane >> 5;
@@ -1215,9 +1221,9 @@ __global__ void __launch_bounds__(256) k_eval_mma(DevGeom g, int nlt, int nks, i
 #pragma unroll
             for (int e = 0; e < MT::KPL; ++e) {
                 const int m = s * MT::K + half * MT::KPL + e;
-                bt[e] = sX[(0 * Lp + m) * 128 + col];
-                bf[e] = sX[(1 * Lp + m) * 128 + col];
-                bg[e] = sX[(2 * Lp + m) * 128 + col];
+                bt[e] = sX[(0 * 128 + col) * LP + m];
+                bf[e] = sX[(1 * 128 + col) * LP + m];
+                bg[e] = sX[(2 * 128 + col) * LP + m];
             }
             MT::run(cFp, a[0], bf);  // C' . f
             MT::run(cFc, a[1], bf);  // C  . f
@@ -1281,7 +1287,8 @@ __global__ void __launch_bounds__(256) k_project_mma(DevGeom g, int nks, const f
                                                      cpx<float>* __restrict__ Ng, cpx<float>* __restrict__ NF,
                                                      cpx<float>* __restrict__ NG) {
     using MT = Mma<MM>;
-    __shared__ float sO[4][3][16 * 64];  // per wave: [field][mode][2L] staging for the write-out
+    constexpr int OS = 66;               // staging row (2L <= 64 floats plus 2 to spread banks)
+    __shared__ float sO[4][3][16 * OS];  // per wave: [field][mode][OS] staging for the write-out
     const int L = g.L, P = g.P, nq = g.nq;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j = lane & 31, half = lane >> 5;
@@ -1329,9 +1336,9 @@ __global__ void __launch_bounds__(256) k_project_mma(DevGeom g, int nks, const f
     for (int r = 0; r < 16; ++r) {
         const int m = 8 * (r >> 2) + 4 * half + (r & 3);
         if (m < L) {
-            sO[wave][0][lm * 64 + 2 * m + comp] = mt * cT[r];
-            sO[wave][1][lm * 64 + 2 * m + comp] = mp * (cF1[r] + k2 * cF2[r]);
-            sO[wave][2][lm * 64 + 2 * m + comp] = mp * cG[r];
+            sO[wave][0][lm * OS + 2 * m + comp] = mt * cT[r];
+            sO[wave][1][lm * OS + 2 * m + comp] = mp * (cF1[r] + k2 * cF2[r]);
+            sO[wave][2][lm * OS + 2 * m + comp] = mp * cG[r];
         }
     }
     wave_sync();
@@ -1341,9 +1348,9 @@ __global__ void __launch_bounds__(256) k_project_mma(DevGeom g, int nks, const f
         const int lmode = e / (2 * L), q = e - lmode * 2 * L;
         const int pp = pw0 + lmode;
         if (pp < P) {
-            reinterpret_cast<float*>(Nt)[(size_t)pp * 2 * L + q] = sO[wave][0][lmode * 64 + q];
-            reinterpret_cast<float*>(Nf)[(size_t)pp * 2 * L + q] = sO[wave][1][lmode * 64 + q];
-            reinterpret_cast<float*>(Ng)[(size_t)pp * 2 * L + q] = sO[wave][2][lmode * 64 + q];
+            reinterpret_cast<float*>(Nt)[(size_t)pp * 2 * L + q] = sO[wave][0][lmode * OS + q];
+            reinterpret_cast<float*>(Nf)[(size_t)pp * 2 * L + q] = sO[wave][1][lmode * OS + q];
+            reinterpret_cast<float*>(Ng)[(size_t)pp * 2 * L + q] = sO[wave][2][lmode * OS + q];
         }
     }
     mean_flow_projection(g, fgw, prtab, Pr, NF, NG);
@@ -1486,7 +1493,7 @@ void Launch<T>::eval(const DevGeom& g, const Buffers& b, void* const X[3], const
                      hipStream_t s) {
     if (sizeof(T) == 4 && b.mma >= 0) {
         const int K = b.mma == 0 ? 2 : 16;
-        const size_t smm = (size_t)3 * b.ev_nks * K * 128 * sizeof(float);
+        const size_t smm = (size_t)3 * 128 * (b.ev_nks * K + 1) * sizeof(float);
         const int nblk = ((g.P + 63) / 64) * b.ev_nlt, per = (nblk + 7) / 8;
         dim3 grid(8 * per);
         if (b.mma == 0)
@@ -1597,8 +1604,8 @@ void Launch<T>::update(const DevGeom& g, const Buffers& b, int ngroups, int L0, 
                        double dt, StepCoef sc, int mode, void* const Xio[3], void* const Nin[3], void* const Rp[3],
                        hipStream_t s) {
     if (mode == UPD_STEP && g.L <= 32) {
-        hipLaunchKernelGGL(k_update32<T>, dim3((ngroups + 3) / 4), dim3(256), 0, s, g, ngroups, L0, L1, stride, (T)Pr,
-                           (T)dt, (T)sc.a_cur, (T)sc.a_prev, sc.use_prev, b.gstart, b.gmem, (const T*)b.gmat,
+        hipLaunchKernelGGL(k_update32<T>, dim3((b.nchunk + 3) / 4), dim3(256), 0, s, g, b.nchunk, L0, L1, stride,
+                           (T)Pr, (T)dt, (T)sc.a_cur, (T)sc.a_prev, sc.use_prev, b.cgrp, b.cbeg, b.gmem, (const T*)b.gmat,
                            (const T*)b.k2, (cpx<T>*)Xio[0], (cpx<T>*)Xio[1], (cpx<T>*)Xio[2], (const cpx<T>*)Nin[0],
                            (const cpx<T>*)Nin[1], (const cpx<T>*)Nin[2], (cpx<T>*)Rp[0], (cpx<T>*)Rp[1],
                            (cpx<T>*)Rp[2]);
