@@ -107,6 +107,88 @@ const char* rbc_stage_name(int i);
 /* Blocks until all work queued on the context's stream has finished. */
 int rbc_synchronize(rbc_ctx* c);
 
+/* ---- State files in the reference's binary format ---------------------------------------
+ * ≙ matrix_folder_writer::read_binary_file / write_binary_file (output/matrix_folder_writer.cu:
+ * 3-103) for the five files theta, f, g, F, G (main.cu:359-381 read, :700-713 write).  Format:
+ * int32 1, int32 3, int32 dims {M/2+1, N, L} (F, G: {1, 1, L}), then complex values
+ * (complex64 from an FP32 build, complex128 from a PRECISION_DOUBLE build; the reader infers
+ * which from the file size).
+ *   which = 0: the state; reading it applies make_real as main.cu:396 does.
+ *   which = 1: the AB2 history (the previous step's B^-1 N, implicit_explicit_stepping.h
+ *              res_1), which the reference never writes; restoring it makes a restart continue
+ *              with AB2 instead of an Euler step.
+ * elem_bytes: 8 (complex64), 16 (complex128) or 0 (the context's precision: 16 for RBC_F64). */
+int rbc_read_state_files(rbc_ctx* c, const char* const paths[5], int which);
+int rbc_write_state_files(rbc_ctx* c, const char* const paths[5], int which, int elem_bytes);
+/* AB2 history in the reference layout (see rbc_set_state).  get fails with RBC_ERR_STATE
+ * before the first step; set makes the next step an AB2 step. */
+int rbc_get_history(rbc_ctx* c, double* theta, double* f, double* g, double* F, double* G);
+int rbc_set_history(rbc_ctx* c, const double* theta, const double* f, const double* g,
+                    const double* F, const double* G);
+
+/* ---- Velocity operator ------------------------------------------------------------------
+ * ≙ calculate_velocity_operator::calculate_operator_at (calculate_velocity_operator.cu:
+ * 106-158): u_x, u_y, u_z of the current state on the M x N grid at nz heights z (clamped to
+ * [-1/2, 1/2]), FP32, layout [l][y][x] per component.  Allocates scratch for the call. */
+int rbc_velocity_grid(rbc_ctx* c, int nz, const double* z, float* ux, float* uy, float* uz);
+/* The velocity of the current state at n arbitrary points, summed exactly over the kept
+ * modes (no counterpart in the reference; the spectral tracer's evaluation).  L <= 32. */
+int rbc_velocity_at(rbc_ctx* c, int n, const float* x, const float* y, const float* z,
+                    float* u, float* v, float* w);
+
+/* ---- Passive tracers (≙ particle_tracer, particle/particle_tracer.h:21-89) ----------------
+ * Positions are FP32, stored as a ring of buffer_size time slots (the reference's layout
+ * [slot][particle]).  diff_coeff ≙ the constructor's pDiffCoeff (the Lewis-number argument,
+ * main.cu:497); random numbers come from SplitMix64 counters on `seed` (the reference's
+ * cuRAND streams cannot be reproduced).
+ *   RBC_TRACER_RK4_TRILINEAR : the reference: RK4 in the frozen velocity on a 128-level grid
+ *                              with trilinear, periodic interpolation (particle_tracer.cu:
+ *                              141-212, 508-687) and reflection at z = +-(1/2 - 1e-3).
+ *   RBC_TRACER_SPECTRAL_RK2  : midpoint RK2 with the velocity summed exactly at each
+ *                              particle (rbc_velocity_at); L <= 32. */
+#define RBC_TRACER_RK4_TRILINEAR 0
+#define RBC_TRACER_SPECTRAL_RK2 1
+/* init types in the order of particle_tracer::initType (particle_tracer.h:50) */
+#define RBC_PART_RANDOM 0
+#define RBC_PART_CENTER 1
+#define RBC_PART_CENTER_SIGMA 2
+#define RBC_PART_DIAGONAL 3
+#define RBC_PART_PAIRS 4
+#define RBC_PART_COORD_SIGMA 5
+typedef struct rbc_tracer rbc_tracer;
+/* ≙ particle_tracer(M, N, L, Np, buffer, cube_length, Le) (particle_tracer.cu:7-44): all
+ * particles start at the centre. */
+int rbc_tracer_create(rbc_ctx* c, int num_particles, int buffer_size, int mode, double diff_coeff,
+                      unsigned long long seed, rbc_tracer** out);
+void rbc_tracer_destroy(rbc_tracer* t);
+const char* rbc_tracer_last_error(const rbc_tracer* t);
+/* ≙ init_positions (particle_tracer.cu:60-135); (cx, cy, cz, sigma) for the coord/sigma types */
+int rbc_tracer_init(rbc_tracer* t, int type, double cx, double cy, double cz, double sigma);
+/* overwrite the current slot */
+int rbc_tracer_set_positions(rbc_tracer* t, const float* x, const float* y, const float* z);
+/* ≙ step_particles (particle_tracer.cu:141-212) in the context's current velocity; fails with
+ * RBC_ERR_STATE when the ring is full (is_buffer_full). */
+int rbc_tracer_step(rbc_tracer* t, double dt);
+/* ≙ get_particle_positions (particle_tracer.cu:272-311): copies num_timesteps slots
+ * ([slot][particle], at most buffer_size - 1) and, with clear, moves the current slot to 0. */
+int rbc_tracer_get_positions(rbc_tracer* t, int clear, float* x, float* y, float* z, int* num_timesteps);
+int rbc_tracer_current(rbc_tracer* t, float* x, float* y, float* z);
+int rbc_tracer_info(const rbc_tracer* t, int* iteration_index, int* particle_time);
+/* FP64 moments of the current slot: {<x>, <y>, calc_secondmoment, calc_pair_secondmoment}
+ * (particle_tracer.cu:217-258, 771-866, PERIODIC minimum image for pairs). */
+int rbc_tracer_moments(rbc_tracer* t, double* out4);
+
+/* ---- Kinetic-energy spectra (FP64) -------------------------------------------------------
+ * rbc_energy_spectrum: E_h(k) = shell sums over |k| in bins of width min(2 pi/Lx, 2 pi/Ly)
+ * (bin s holds |k| in [(s - 1/2) dk, (s + 1/2) dk)) of (1/2)|u_h,k|^2 integrated over z by the
+ * context's Gauss-Legendre rule; totals = {E_h, E_z} (volume integrals of (u^2+v^2)/2 and w^2/2
+ * over unit area).  No counterpart in the reference.
+ * rbc_energy_spectrum_ref: the reference's binning (energy_spectrum_2d.cu:53-129) of |c_k|^2
+ * for each velocity component c at height z: E3[c * nbins + b].
+ * Both return the bin count in *nbins; pass NULL output to query it. */
+int rbc_energy_spectrum(rbc_ctx* c, int cap, double* E, int* nbins, double* totals);
+int rbc_energy_spectrum_ref(rbc_ctx* c, double z, int cap, double* E3, int* nbins);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,62 +11,13 @@
 #include <string>
 #include <vector>
 
-#include "rbc_internal.hpp"
+#include "rbc_ctx.hpp"
 
 using namespace rbc;
 
 namespace {
-
 thread_local std::string g_create_error;
-
-struct HipError {
-    hipError_t e;
-    const char* what;
-};
-
-#define HIPCHK(x)                                        \
-    do {                                                 \
-        hipError_t _e = (x);                             \
-        if (_e != hipSuccess) throw HipError{_e, #x};    \
-    } while (0)
-
 }  // namespace
-
-struct rbc_ctx {
-    rbc_params p{};
-    Geometry geo;
-    DevGeom dg{};
-    Factors fac;
-    int prec = RBC_F64;
-    size_t rsz = 8;  // bytes per real on the device
-    hipStream_t stream = nullptr;
-    Buffers b{};
-    void* ia1 = nullptr;  // L*L doubles (Nusselt)
-    std::vector<void*> allocs;
-    bool have_hist = false;
-    std::string err;
-    std::vector<unsigned char> stage;  // host staging for one compact field
-    static constexpr int kRing = 64;                 // steps of stage events in flight
-    hipEvent_t ev_ring[kRing][RBC_NSTAGES + 1]{};
-    hipEvent_t* ev = ev_ring[0];                     // events of the step being recorded
-    bool events = false;
-
-    size_t csz() const { return 2 * rsz; }
-    void* dalloc(size_t bytes) {
-        void* ptr = nullptr;
-        HIPCHK(hipMalloc(&ptr, bytes > 0 ? bytes : 16));
-        allocs.push_back(ptr);
-        return ptr;
-    }
-    ~rbc_ctx() {
-        if (stream) hipStreamSynchronize(stream);
-        for (void* a : allocs) hipFree(a);
-        if (events)
-            for (auto& row : ev_ring)
-                for (auto& e : row) hipEventDestroy(e);
-        if (stream) hipStreamDestroy(stream);
-    }
-};
 
 namespace {
 
@@ -116,6 +67,10 @@ void unpack_field(const rbc_ctx* c, const T* src, double* dst) {
         }
 }
 
+}  // namespace
+
+namespace rbc {
+
 void put_field(rbc_ctx* c, const double* host, void* dev) {
     const size_t n = (size_t)c->geo.P() * c->geo.L * c->csz();
     c->stage.resize(n);
@@ -152,6 +107,10 @@ void get_vec(rbc_ctx* c, const void* dev, double* host) {
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int i = 0; i < 2 * L; ++i) host[i] = (c->rsz == 8) ? ((double*)tmp.data())[i] : ((float*)tmp.data())[i];
 }
+
+}  // namespace rbc
+
+namespace {
 
 // ---- pipeline, dispatched on precision -------------------------------------------------
 template <typename T>
@@ -198,6 +157,10 @@ void one_step(rbc_ctx* c, bool timed) {
     c->have_hist = true;
 }
 
+}  // namespace
+
+namespace rbc {
+
 void run_steps(rbc_ctx* c, int n) {
     for (int i = 0; i < n; ++i) {
         if (c->rsz == 8) one_step<double>(c, false);
@@ -214,6 +177,10 @@ int fail(rbc_ctx* c, int code, const std::string& msg) {
 int hip_fail(rbc_ctx* c, const HipError& e) {
     return fail(c, RBC_ERR_HIP, std::string(e.what) + ": " + hipGetErrorString(e.e));
 }
+
+}  // namespace rbc
+
+namespace {
 
 const char* kStageNames[RBC_NSTAGES] = {"eval", "fft_y_inv", "rows_products", "fft_y_fwd",
                                         "project", "update", "update_FG", "make_real"};

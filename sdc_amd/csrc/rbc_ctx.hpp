@@ -1,0 +1,87 @@
+// rbc_ctx.hpp -- the context object behind the C-ABI and the host helpers shared by the
+// translation units of libsdc_rbc.so (stepper, state files, diagnostics, tracers,
+// ensembles, learned closure).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "rbc_abi.h"
+#include "rbc_internal.hpp"
+
+namespace rbc {
+
+struct HipError {
+    hipError_t e;
+    const char* what;
+};
+
+#define HIPCHK(x)                                        \
+    do {                                                 \
+        hipError_t _e = (x);                             \
+        if (_e != hipSuccess) throw rbc::HipError{_e, #x}; \
+    } while (0)
+
+}  // namespace rbc
+
+struct rbc_ctx {
+    rbc_params p{};
+    rbc::Geometry geo;
+    rbc::DevGeom dg{};
+    rbc::Factors fac;
+    int prec = RBC_F64;
+    size_t rsz = 8;  // bytes per real on the device
+    hipStream_t stream = nullptr;
+    bool own_stream = true;
+    rbc::Buffers b{};
+    void* ia1 = nullptr;  // L*L doubles (Nusselt)
+    std::vector<void*> allocs;
+    bool have_hist = false;
+    std::string err;
+    std::vector<unsigned char> stage;  // host staging for one compact field
+    static constexpr int kRing = 64;                 // steps of stage events in flight
+    hipEvent_t ev_ring[kRing][RBC_NSTAGES + 1]{};
+    hipEvent_t* ev = ev_ring[0];                     // events of the step being recorded
+    bool events = false;
+    // diagnostics tables (built on first use): wavenumber shell per compact mode
+    int* shell = nullptr;
+    int nshell = 0;
+    double* quadw = nullptr;  // nq Gauss-Legendre weights (FP64)
+    void* diagtab = nullptr;  // [l][3][L] C', C, S at the levels (FP64)
+
+    size_t csz() const { return 2 * rsz; }
+    void* dalloc(size_t bytes) {
+        void* ptr = nullptr;
+        HIPCHK(hipMalloc(&ptr, bytes > 0 ? bytes : 16));
+        allocs.push_back(ptr);
+        return ptr;
+    }
+    ~rbc_ctx() {
+        if (stream) hipStreamSynchronize(stream);
+        for (void* a : allocs) hipFree(a);
+        if (events)
+            for (auto& row : ev_ring)
+                for (auto& e : row) hipEventDestroy(e);
+        if (stream && own_stream) hipStreamDestroy(stream);
+    }
+};
+
+namespace rbc {
+
+// error text into the context (or the thread's create error when c is null); returns code
+int fail(rbc_ctx* c, int code, const std::string& msg);
+int hip_fail(rbc_ctx* c, const HipError& e);
+
+// host <-> device exchange of one field in the reference layout (complex doubles,
+// (M/2+1) x N x L, x fastest) and of an L-vector
+void put_field(rbc_ctx* c, const double* host, void* dev);
+void get_field(rbc_ctx* c, const void* dev, double* host);
+void put_vec(rbc_ctx* c, const double* host, void* dev);
+void get_vec(rbc_ctx* c, const void* dev, double* host);
+
+// one IMEX step on the context's stream (no host synchronisation)
+void run_steps(rbc_ctx* c, int n);
+
+}  // namespace rbc

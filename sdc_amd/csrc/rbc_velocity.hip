@@ -1,0 +1,1077 @@
+// rbc_velocity.hip -- the velocity operator and what is built on it: the velocity on a
+// grid of heights (calculate_velocity_operator), passive tracers in the reference's mode
+// (RK4 with trilinear interpolation on a 128-level grid, particle_tracer.cu) and in a
+// spectral mode (midpoint RK2 with the velocity summed exactly at each particle), and the
+// kinetic-energy spectra.
+//
+// Velocity from the state (calculate_velocity_operator.cu:172-245, 426-505):
+//   u = sum_m C'_m(z) d_x f_m + S_m(z) (d_y g_m + F_m at k = 0)
+//   v = sum_m C'_m(z) d_y f_m + S_m(z) (-d_x g_m + G_m at k = 0)
+//   w = sum_m C_m(z) (-Laplace_h f_m)
+// with the unnormalised C2R as the horizontal synthesis.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "rbc_ctx.hpp"
+#include "rbc_device.hpp"
+
+using namespace rbc;
+
+namespace {
+
+constexpr int kZNum = 128;                    // PARTICLE_Z_NUM (cuda_defines.h:44)
+constexpr double kZMax = 0.5f - 1e-3;         // PARTICLE_Z_MAX (cuda_defines.h:45), a double
+constexpr int kTile = 64;                     // modes per LDS tile of the spectral evaluation
+
+// ---- counter-based random numbers (SplitMix64, Steele, Lea & Flood 2014) ----------------
+__host__ __device__ inline uint64_t splitmix64(uint64_t x) {
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+// uniform in (0, 1] from (seed, counter), 24 bits
+__device__ inline float rng_u01(uint64_t seed, uint64_t ctr) {
+    const uint64_t h = splitmix64(seed * 4294967296ull + ctr);
+    return (float)((h >> 40) + 1) * (1.0f / 16777216.0f);
+}
+// standard normal by Box-Muller from the uniforms at 2 ctr and 2 ctr + 1
+__device__ inline float rng_normal(uint64_t seed, uint64_t ctr) {
+    const float u1 = rng_u01(seed, 2 * ctr), u2 = rng_u01(seed, 2 * ctr + 1);
+    return sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2);
+}
+
+// ---- basis values at one height, FP64 (coeff.h:17-36; the clamp of get_z_coeff_at,
+//      calculate_velocity_operator.cu:462-466) --------------------------------------------
+__host__ __device__ inline void basis_at(int m, double lam, double z, double& C, double& Cp, double& S) {
+    if (z < -0.5) z = -0.5;
+    else if (z > 0.5) z = 0.5;
+    const int k = m + 1;
+    const double lz = lam * z;
+    if (k % 2) {
+        const double ch = cosh(0.5 * lam), c = cos(0.5 * lam);
+        C = cosh(lz) / ch - cos(lz) / c;
+        Cp = lam * (sinh(lz) / ch + sin(lz) / c);
+    } else {
+        const double sh = sinh(0.5 * lam), s = sin(0.5 * lam);
+        C = sinh(lz) / sh - sin(lz) / s;
+        Cp = lam * (cosh(lz) / sh - cos(lz) / s);
+    }
+    S = sin(M_PI * k * (z + 0.5));
+}
+
+// ---------------------------------------------------------------------------------------
+// Spectral velocity at nz heights: spec[c][l][p] for c = u, v, w (complex), thread per mode.
+// zb[l][3][L] = C'_m, C_m, S_m at height l.
+// ---------------------------------------------------------------------------------------
+template <typename T, typename O>
+__global__ void __launch_bounds__(256) k_vel_levels(DevGeom g, int nz, const double* __restrict__ zb,
+                                                    const cpx<T>* __restrict__ Xf, const cpx<T>* __restrict__ Xg,
+                                                    const cpx<T>* __restrict__ XF, const cpx<T>* __restrict__ XG,
+                                                    const T* __restrict__ kxa, const T* __restrict__ kya,
+                                                    cpx<O>* __restrict__ spec) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x, l = blockIdx.y;
+    if (p >= g.P) return;
+    const int L = g.L;
+    const double* tb = zb + (size_t)l * 3 * L;
+    cpx<double> A = mk<double>(0, 0), B = A, Cc = A;
+    for (int m = 0; m < L; ++m) {
+        const cpx<T> f = Xf[(size_t)p * L + m], gg = Xg[(size_t)p * L + m];
+        const cpx<double> fd = mk<double>(f.x, f.y), gd = mk<double>(gg.x, gg.y);
+        A = A + cscale(fd, tb[m]);
+        Cc = Cc + cscale(fd, tb[L + m]);
+        B = B + cscale(gd, tb[2 * L + m]);
+    }
+    const double kx = kxa[p], ky = kya[p];
+    cpx<double> u = imul(A, kx) + imul(B, ky);
+    cpx<double> v = imul(A, ky) - imul(B, kx);
+    const cpx<double> w = cscale(Cc, kx * kx + ky * ky);
+    if (p == 0) {  // add_dc_part (calculate_velocity_operator.cu:479-494)
+        for (int m = 0; m < L; ++m) {
+            u = u + cscale(mk<double>(XF[m].x, XF[m].y), tb[2 * L + m]);
+            v = v + cscale(mk<double>(XG[m].x, XG[m].y), tb[2 * L + m]);
+        }
+    }
+    const size_t plane = (size_t)nz * g.P;
+    cpx<O>* o = spec + (size_t)l * g.P + p;
+    o[0] = mk<O>((O)u.x, (O)u.y);
+    o[plane] = mk<O>((O)v.x, (O)v.y);
+    o[2 * plane] = mk<O>((O)w.x, (O)w.y);
+}
+
+// ---------------------------------------------------------------------------------------
+// x C2R of the hybrid rows [c][l][y][kx] of the three components into the real grid
+// [c][l][y][x] (the reference's u_i[x + y M + l M N] layout).  Block per (y, l): u + i v and
+// w go through two complex inverse FFTs of length M.
+// ---------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_vel_rows(DevGeom g, const cpx<float>* __restrict__ hyb,
+                                                  const cpx<float>* __restrict__ twM, float* __restrict__ grid) {
+    extern __shared__ unsigned char smem_raw[];
+    const int M = g.M, N = g.N, KX = g.KX, KXp = g.KXp, nz = g.nq;
+    cpx<float>* a = reinterpret_cast<cpx<float>*>(smem_raw);  // 2 M
+    cpx<float>* b = a + 2 * M;
+    const int y = blockIdx.x, l = blockIdx.y;
+    const size_t fstride = (size_t)nz * N * KXp;
+    const size_t roff = ((size_t)l * N + y) * KXp;
+    for (int e = threadIdx.x; e < 2 * M; e += blockDim.x) a[e] = mk<float>(0, 0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < 2 * KX; e += blockDim.x) {
+        const int pr = e / KX, k = e - pr * KX;
+        cpx<float> x1 = hyb[(size_t)(2 * pr) * fstride + roff + k];
+        cpx<float> x2 = pr == 0 ? hyb[fstride + roff + k] : mk<float>(0, 0);
+        if (k == 0) {  // a C2R ignores the imaginary part at kx = 0
+            x1.y = 0;
+            x2.y = 0;
+        }
+        cpx<float>* z = a + pr * M;
+        z[k] = mk(x1.x - x2.y, x1.y + x2.x);
+        if (k > 0) z[M - k] = mk(x1.x + x2.y, -x1.y + x2.x);
+    }
+    __syncthreads();
+    const cpx<float>* r = lds_fft(a, b, g.logM, 2, twM, true);
+    const size_t cs = (size_t)nz * N * M;
+    float* o = grid + ((size_t)l * N + y) * M;
+    for (int x = threadIdx.x; x < M; x += blockDim.x) {
+        o[x] = r[x].x;
+        o[cs + x] = r[x].y;
+        o[2 * cs + x] = r[M + x].x;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Reference tracer step: RK4 in the frozen velocity grid with trilinear interpolation and
+// periodic lookup (particle_tracer.cu:508-687, PERIODIC build), optional diffusion
+// (:692-717) and reflection at the lids (:722-764).  Mixed float/double expressions are
+// kept in the reference's types.
+// ---------------------------------------------------------------------------------------
+__device__ inline void vel_trilinear(const float* __restrict__ ux, const float* __restrict__ uy,
+                                     const float* __restrict__ uz, int xs, int ys, float clx, float cly, float x,
+                                     float y, float z, float& ru, float& rv, float& rw) {
+    const int xy = xs * ys;
+    float xg = x * xs / clx;
+    float yg = y * ys / cly;
+    float zg = (kZMax + z) / (2 * kZMax) * (kZNum - 1);
+    xg -= floorf(xg / xs) * xs;
+    yg -= floorf(yg / ys) * ys;
+    int ix0 = (int)floorf(xg), iy0 = (int)floorf(yg), iz0 = (int)floorf(zg);
+    if (ix0 < 0 || ix0 >= xs) ix0 = 0;
+    if (iy0 < 0 || iy0 >= ys) iy0 = 0;
+    if (iz0 < 0) iz0 = 0;
+    else if (iz0 >= kZNum) iz0 = kZNum - 1;
+    const int ix1 = (ix0 + 1 >= xs) ? 0 : ix0 + 1;
+    const int iy1 = (iy0 + 1 >= ys) ? 0 : iy0 + 1;
+    const int iz1 = (iz0 + 1 >= kZNum) ? iz0 : iz0 + 1;
+    float wR = xg - float(ix0), wL = 1 - wR;
+    auto lerpx = [&](int iy, int iz, float& a, float& b, float& c) {
+        const int i0 = ix0 + iy * xs + iz * xy, i1 = ix1 + iy * xs + iz * xy;
+        a = wL * ux[i0] + wR * ux[i1];
+        b = wL * uy[i0] + wR * uy[i1];
+        c = wL * uz[i0] + wR * uz[i1];
+    };
+    float x00, y00, z00, x10, y10, z10, x01, y01, z01, x11, y11, z11;
+    lerpx(iy0, iz0, x00, y00, z00);
+    lerpx(iy1, iz0, x10, y10, z10);
+    lerpx(iy0, iz1, x01, y01, z01);
+    lerpx(iy1, iz1, x11, y11, z11);
+    wR = yg - float(iy0);
+    wL = 1 - wR;
+    const float cx0 = wL * x00 + wR * x10, cy0 = wL * y00 + wR * y10, cz0 = wL * z00 + wR * z10;
+    const float cx1 = wL * x01 + wR * x11, cy1 = wL * y01 + wR * y11, cz1 = wL * z01 + wR * z11;
+    wR = zg - float(iz0);
+    wL = 1 - wR;
+    ru = wL * cx0 + wR * cx1;
+    rv = wL * cy0 + wR * cy1;
+    rw = wL * cz0 + wR * cz1;
+}
+
+// diffusion with spread sigma (counter-based normals) and the lid reflection
+__device__ inline void finish_particle(float& x, float& y, float& z, int i, int np, float sigma, uint64_t seed,
+                                       uint64_t ctr0) {
+    if (sigma > 0.f) {
+        x += sigma * rng_normal(seed, ctr0 + i);
+        y += sigma * rng_normal(seed, ctr0 + (uint64_t)np + i);
+        z += sigma * rng_normal(seed, ctr0 + 2 * (uint64_t)np + i);
+    }
+    if (!(z <= kZMax)) {  // NaN goes to the top lid (particle_tracer.cu:741-749)
+        z = (z < 2 * kZMax) ? (float)(2 * kZMax - z) : (float)kZMax;
+    } else if (z < -kZMax) {
+        z = (z > -2 * kZMax) ? (float)(-2 * kZMax - z) : (float)(-kZMax);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_tracer_rk4(int np, const float* __restrict__ grid, int M, int N, float Lx,
+                                                    float Ly, const float* __restrict__ x0, const float* __restrict__ y0,
+                                                    const float* __restrict__ z0, float dt, float* __restrict__ x1,
+                                                    float* __restrict__ y1, float* __restrict__ z1, float sigma,
+                                                    uint64_t seed, uint64_t ctr0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const size_t cs = (size_t)kZNum * N * M;
+    const float *ux = grid, *uy = grid + cs, *uz = grid + 2 * cs;
+    float x = x0[i], y = y0[i], z = z0[i];
+    float vx, vy, vz;
+    vel_trilinear(ux, uy, uz, M, N, Lx, Ly, x, y, z, vx, vy, vz);
+    const float dx1 = dt * vx, dy1 = dt * vy, dz1 = dt * vz;
+    float xt = x + 0.5 * dx1, yt = y + 0.5 * dy1, zt = z + 0.5 * dz1;
+    vel_trilinear(ux, uy, uz, M, N, Lx, Ly, xt, yt, zt, vx, vy, vz);
+    const float dx2 = dt * vx, dy2 = dt * vy, dz2 = dt * vz;
+    xt = x + 0.5 * dx2;
+    yt = y + 0.5 * dy2;
+    zt = z + 0.5 * dz2;
+    vel_trilinear(ux, uy, uz, M, N, Lx, Ly, xt, yt, zt, vx, vy, vz);
+    const float dx3 = dt * vx, dy3 = dt * vy, dz3 = dt * vz;
+    xt = x + dx3;
+    yt = y + dy3;
+    zt = z + dz3;
+    vel_trilinear(ux, uy, uz, M, N, Lx, Ly, xt, yt, zt, vx, vy, vz);
+    const float dx4 = dt * vx, dy4 = dt * vy, dz4 = dt * vz;
+    x += (dx1 + 2 * (dx2 + dx3) + dx4) / 6.;
+    y += (dy1 + 2 * (dy2 + dy3) + dy4) / 6.;
+    z += (dz1 + 2 * (dz2 + dz3) + dz4) / 6.;
+    finish_particle(x, y, z, i, np, sigma, seed, ctr0);
+    x1[i] = x;
+    y1[i] = y;
+    z1[i] = z;
+}
+
+// ---------------------------------------------------------------------------------------
+// Spectral velocity at particle positions: u(x) = sum over kept modes of
+// w_k Re(exp(i k.x) u_k(z)), w_k = 1 at kx = 0 (both +-ky rows are stored) and 2 beyond.
+// Block = 256 particles; the modes stream through LDS in tiles of kTile; blockIdx.y takes a
+// slice of the tiles (partial sums part[slice][3][np], summed in order by k_sum_slices).
+// Basis values at the particle's height are held in registers (LMAX >= L, zero-padded).
+// ---------------------------------------------------------------------------------------
+template <typename T, int LMAX>
+__global__ void __launch_bounds__(256) k_vel_points(DevGeom g, int np, const float* __restrict__ px,
+                                                    const float* __restrict__ py, const float* __restrict__ pz,
+                                                    const cpx<T>* __restrict__ Xf, const cpx<T>* __restrict__ Xg,
+                                                    const cpx<T>* __restrict__ XF, const cpx<T>* __restrict__ XG,
+                                                    const double* __restrict__ lam, const float4* __restrict__ modes,
+                                                    int tiles_per_slice, float* __restrict__ part) {
+    __shared__ float4 sc[kTile][LMAX];  // (f.re, f.im, g.re, g.im)
+    __shared__ float4 sm[kTile];        // (kx index, ky index, kx, ky)
+    const int L = g.L, P = g.P;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int slice = blockIdx.y;
+    const bool live = i < np;
+    const float x = live ? px[i] : 0.f, y = live ? py[i] : 0.f, z = live ? pz[i] : 0.f;
+    float cp[LMAX], cc[LMAX], ss[LMAX];
+#pragma unroll
+    for (int m = 0; m < LMAX; ++m) {
+        double C = 0, Cp = 0, S = 0;
+        if (m < L) basis_at(m, lam[m], (double)z, C, Cp, S);
+        cp[m] = (float)Cp;
+        cc[m] = (float)C;
+        ss[m] = (float)S;
+    }
+    // periodic coordinates in units of the box
+    const float xr = x / (float)g.Lx - floorf(x / (float)g.Lx);
+    const float yr = y / (float)g.Ly - floorf(y / (float)g.Ly);
+    float u = 0.f, v = 0.f, w = 0.f;
+    if (slice == 0) {  // mean flow (k = 0): u += sum S_m Re F_m, v += sum S_m Re G_m
+#pragma unroll
+        for (int m = 0; m < LMAX; ++m)
+            if (m < L) {
+                u += ss[m] * (float)XF[m].x;
+                v += ss[m] * (float)XG[m].x;
+            }
+    }
+    const int ntile = (P + kTile - 1) / kTile;
+    const int t0 = slice * tiles_per_slice, t1 = min(ntile, t0 + tiles_per_slice);
+    for (int t = t0; t < t1; ++t) {
+        const int pbase = t * kTile, nm = min(kTile, P - pbase);
+        __syncthreads();
+        for (int e = threadIdx.x; e < kTile * LMAX; e += blockDim.x) {
+            const int j = e / LMAX, m = e - j * LMAX;
+            float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (j < nm && m < L) {
+                const cpx<T> f = Xf[(size_t)(pbase + j) * L + m], gg = Xg[(size_t)(pbase + j) * L + m];
+                val = make_float4((float)f.x, (float)f.y, (float)gg.x, (float)gg.y);
+            }
+            sc[j][m] = val;
+        }
+        for (int j = threadIdx.x; j < kTile; j += blockDim.x)
+            sm[j] = j < nm ? modes[pbase + j] : make_float4(0.f, 0.f, 0.f, 0.f);
+        __syncthreads();
+        for (int j = 0; j < nm; ++j) {
+            float ar = 0.f, ai = 0.f, br = 0.f, bi = 0.f, cr = 0.f, ci = 0.f;
+#pragma unroll
+            for (int m = 0; m < LMAX; ++m) {
+                const float4 c = sc[j][m];
+                ar += c.x * cp[m];
+                ai += c.y * cp[m];
+                cr += c.x * cc[m];
+                ci += c.y * cc[m];
+                br += c.z * ss[m];
+                bi += c.w * ss[m];
+            }
+            const float4 md = sm[j];
+            float ph = md.x * xr + md.y * yr;
+            ph -= rintf(ph);
+            float sn, cs;
+            sincospif(2.0f * ph, &sn, &cs);
+            const float wk = md.x == 0.f ? 1.f : 2.f;
+            const float kx = md.z, ky = md.w;
+            // u_k = i kx A + i ky B, v_k = i ky A - i kx B, w_k = k^2 Cc
+            const float ur = -kx * ai - ky * bi, ui = kx * ar + ky * br;
+            const float vr = -ky * ai + kx * bi, vi = ky * ar - kx * br;
+            const float k2 = kx * kx + ky * ky;
+            u += wk * (cs * ur - sn * ui);
+            v += wk * (cs * vr - sn * vi);
+            w += wk * k2 * (cs * cr - sn * ci);
+        }
+    }
+    if (live) {
+        float* o = part + (size_t)slice * 3 * np;
+        o[i] = u;
+        o[np + i] = v;
+        o[2 * (size_t)np + i] = w;
+    }
+}
+
+__global__ void k_sum_slices(int np, int nslice, const float* __restrict__ part, float* __restrict__ u,
+                             float* __restrict__ v, float* __restrict__ w) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    float a = 0.f, b = 0.f, c = 0.f;
+    for (int s = 0; s < nslice; ++s) {
+        const float* o = part + (size_t)s * 3 * np;
+        a += o[i];
+        b += o[np + i];
+        c += o[2 * (size_t)np + i];
+    }
+    u[i] = a;
+    v[i] = b;
+    w[i] = c;
+}
+
+// midpoint position x + (dt/2) u (the spectral RK2 half step)
+__global__ void k_tracer_half(int np, const float* __restrict__ x0, const float* __restrict__ y0,
+                              const float* __restrict__ z0, const float* __restrict__ u, const float* __restrict__ v,
+                              const float* __restrict__ w, float hdt, float* __restrict__ xm, float* __restrict__ ym,
+                              float* __restrict__ zm) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    xm[i] = x0[i] + hdt * u[i];
+    ym[i] = y0[i] + hdt * v[i];
+    zm[i] = z0[i] + hdt * w[i];
+}
+
+// x1 = x0 + dt u(midpoint), then diffusion and the lid reflection
+__global__ void k_tracer_full(int np, const float* __restrict__ x0, const float* __restrict__ y0,
+                              const float* __restrict__ z0, const float* __restrict__ u, const float* __restrict__ v,
+                              const float* __restrict__ w, float dt, float* __restrict__ x1, float* __restrict__ y1,
+                              float* __restrict__ z1, float sigma, uint64_t seed, uint64_t ctr0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    float x = x0[i] + dt * u[i], y = y0[i] + dt * v[i], z = z0[i] + dt * w[i];
+    finish_particle(x, y, z, i, np, sigma, seed, ctr0);
+    x1[i] = x;
+    y1[i] = y;
+    z1[i] = z;
+}
+
+// initial positions (particle_tracer.cu:81-135, 390-453) with counter-based random numbers
+__global__ void k_tracer_init(int type, int np, float Lx, float Ly, float cx, float cy, float cz, float sigma,
+                              uint64_t seed, float* __restrict__ x, float* __restrict__ y, float* __restrict__ z) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    float a, b, c;
+    switch (type) {
+        case RBC_PART_RANDOM:
+            a = Lx * rng_u01(seed, 3 * (uint64_t)i);
+            b = Ly * rng_u01(seed, 3 * (uint64_t)i + 1);
+            c = rng_u01(seed, 3 * (uint64_t)i + 2) - 0.5f;
+            break;
+        case RBC_PART_DIAGONAL:
+            a = (Lx * i) / float(np);
+            b = (Ly * i) / float(np);
+            c = 0.f;
+            break;
+        case RBC_PART_PAIRS: {
+            const uint64_t pi = (uint64_t)(i / 2);
+            a = Lx * rng_u01(seed, 2 * pi);
+            b = Ly * rng_u01(seed, 2 * pi + 1);
+            c = 0.f;
+            break;
+        }
+        default:  // center, center_sigma, coord_sigma
+            a = cx;
+            b = cy;
+            c = cz;
+            if (sigma > 0.f) {
+                a += sigma * rng_normal(seed, (uint64_t)i);
+                b += sigma * rng_normal(seed, (uint64_t)np + i);
+                c += sigma * rng_normal(seed, 2 * (uint64_t)np + i);
+            }
+    }
+    x[i] = a;
+    y[i] = b;
+    z[i] = c;
+}
+
+// moments of the current positions in FP64, one block, fixed order (particle_tracer.cu:771-821
+// computes the same quantities in the device real type):
+// out = {<x>, <y>, <x^2+y^2> - <x>^2 - <y>^2, <|x_2i - x_2i+1|^2> (minimum image)}
+__global__ void __launch_bounds__(1024) k_tracer_moments(int np, const float* __restrict__ x,
+                                                         const float* __restrict__ y, float Lx, float Ly,
+                                                         double* __restrict__ out) {
+    __shared__ double s[4][1024];
+    double a = 0, b = 0, c = 0, d = 0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) {
+        const double xi = x[i], yi = y[i];
+        a += xi;
+        b += yi;
+        c += xi * xi + yi * yi;
+    }
+    for (int j = threadIdx.x; 2 * j + 1 < np; j += blockDim.x) {
+        float dx = fabsf(x[2 * j + 1] - x[2 * j]), dy = fabsf(y[2 * j + 1] - y[2 * j]);
+        if (dx > 0.5f * Lx) dx = Lx - dx;
+        if (dy > 0.5f * Ly) dy = Ly - dy;
+        d += (double)dx * dx + (double)dy * dy;
+    }
+    s[0][threadIdx.x] = a;
+    s[1][threadIdx.x] = b;
+    s[2][threadIdx.x] = c;
+    s[3][threadIdx.x] = d;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int k = 0; k < 4; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double mx = s[0][0] / np, my = s[1][0] / np;
+        out[0] = mx;
+        out[1] = my;
+        out[2] = s[2][0] / np - mx * mx - my * my;
+        out[3] = np >= 2 ? s[3][0] / (np / 2) : 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Kinetic energy per mode, FP64: out[c][p] = sum_l wz_l |c_k(z_l)|^2 for c = u, v, w.
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_energy_modes(DevGeom g, int nz, const double* __restrict__ zb,
+                                                      const double* __restrict__ wz, const cpx<T>* __restrict__ Xf,
+                                                      const cpx<T>* __restrict__ Xg, const cpx<T>* __restrict__ XF,
+                                                      const cpx<T>* __restrict__ XG, const T* __restrict__ kxa,
+                                                      const T* __restrict__ kya, double* __restrict__ out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.P) return;
+    const int L = g.L;
+    const double kx = kxa[p], ky = kya[p], k2 = kx * kx + ky * ky;
+    double eu = 0, ev = 0, ew = 0;
+    for (int l = 0; l < nz; ++l) {
+        const double* tb = zb + (size_t)l * 3 * L;
+        cpx<double> A = mk<double>(0, 0), B = A, Cc = A;
+        for (int m = 0; m < L; ++m) {
+            const cpx<T> f = Xf[(size_t)p * L + m], gg = Xg[(size_t)p * L + m];
+            A = A + cscale(mk<double>(f.x, f.y), tb[m]);
+            Cc = Cc + cscale(mk<double>(f.x, f.y), tb[L + m]);
+            B = B + cscale(mk<double>(gg.x, gg.y), tb[2 * L + m]);
+        }
+        cpx<double> u = imul(A, kx) + imul(B, ky), v = imul(A, ky) - imul(B, kx);
+        if (p == 0)
+            for (int m = 0; m < L; ++m) {
+                u = u + cscale(mk<double>(XF[m].x, XF[m].y), tb[2 * L + m]);
+                v = v + cscale(mk<double>(XG[m].x, XG[m].y), tb[2 * L + m]);
+            }
+        const cpx<double> w = cscale(Cc, k2);
+        eu += wz[l] * (u.x * u.x + u.y * u.y);
+        ev += wz[l] * (v.x * v.x + v.y * v.y);
+        ew += wz[l] * (w.x * w.x + w.y * w.y);
+    }
+    out[p] = eu;
+    out[g.P + p] = ev;
+    out[2 * (size_t)g.P + p] = ew;
+}
+
+// ---- host helpers ------------------------------------------------------------------------
+
+// basis table zb[l][3][L] = C'_m, C_m, S_m at heights z (FP64)
+std::vector<double> basis_table(int L, const std::vector<double>& z) {
+    std::vector<double> t((size_t)z.size() * 3 * L);
+    for (size_t l = 0; l < z.size(); ++l)
+        for (int m = 0; m < L; ++m) {
+            double C, Cp, S;
+            basis_at(m, lambda_k(m + 1), z[l], C, Cp, S);
+            t[(l * 3 + 0) * L + m] = Cp;
+            t[(l * 3 + 1) * L + m] = C;
+            t[(l * 3 + 2) * L + m] = S;
+        }
+    return t;
+}
+
+std::vector<float> twiddles_f(int n) {
+    std::vector<float> t(2 * (size_t)n);
+    for (int k = 0; k < n; ++k) {
+        const double a = -2.0 * M_PI * k / n;
+        t[2 * k] = (float)std::cos(a);
+        t[2 * k + 1] = (float)std::sin(a);
+    }
+    return t;
+}
+
+// mode table (kx index, ky index, kx, ky) per compact mode
+std::vector<float> mode_table(const rbc_ctx* c) {
+    const Geometry& g = c->geo;
+    const int NYc = g.NYc();
+    std::vector<float> t(4 * (size_t)g.P());
+    for (int p = 0; p < g.P(); ++p) {
+        const int kx = p / NYc, row = g.rows[p % NYc];
+        const int ky = row <= g.N / 2 ? row : row - g.N;
+        t[4 * p] = (float)kx;
+        t[4 * p + 1] = (float)ky;
+        t[4 * p + 2] = (float)c->fac.kx[p];
+        t[4 * p + 3] = (float)c->fac.ky[p];
+    }
+    return t;
+}
+
+template <typename T>
+void launch_vel_levels(rbc_ctx* c, const DevGeom& gt, int nz, const double* zb, void* spec) {
+    const Buffers& b = c->b;
+    dim3 grid((c->dg.P + 255) / 256, nz);
+    hipLaunchKernelGGL((k_vel_levels<T, float>), grid, dim3(256), 0, c->stream, gt, nz, zb, (const cpx<T>*)b.X[1],
+                       (const cpx<T>*)b.X[2], (const cpx<T>*)b.XF, (const cpx<T>*)b.XG, (const T*)b.kx,
+                       (const T*)b.ky, (cpx<float>*)spec);
+}
+
+// scratch of one velocity-grid evaluation at nz heights
+struct GridWork {
+    double* zb = nullptr;       // [nz][3][L]
+    cpx<float>* spec = nullptr; // [3][nz][P]
+    cpx<float>* hyb = nullptr;  // [3][nz][N][KXp]
+    cpx<float>* twM = nullptr;
+    cpx<float>* twN = nullptr;
+    float* grid = nullptr;      // [3][nz][N][M]
+    int nz = 0;
+};
+
+void alloc_grid_work(rbc_ctx* c, int nz, GridWork& w, std::vector<void*>& owned) {
+    const DevGeom& g = c->dg;
+    auto al = [&](size_t bytes) {
+        void* p = nullptr;
+        HIPCHK(hipMalloc(&p, bytes));
+        owned.push_back(p);
+        return p;
+    };
+    w.nz = nz;
+    w.zb = (double*)al((size_t)nz * 3 * g.L * sizeof(double));
+    w.spec = (cpx<float>*)al((size_t)3 * nz * g.P * sizeof(cpx<float>));
+    w.hyb = (cpx<float>*)al((size_t)3 * nz * g.N * g.KXp * sizeof(cpx<float>));
+    w.grid = (float*)al((size_t)3 * nz * g.N * g.M * sizeof(float));
+    w.twM = (cpx<float>*)al((size_t)g.M * sizeof(cpx<float>));
+    w.twN = (cpx<float>*)al((size_t)g.N * sizeof(cpx<float>));
+    const std::vector<float> tm = twiddles_f(g.M), tn = twiddles_f(g.N);
+    HIPCHK(hipMemcpy(w.twM, tm.data(), tm.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(w.twN, tn.data(), tn.size() * sizeof(float), hipMemcpyHostToDevice));
+}
+
+void set_heights(rbc_ctx* c, GridWork& w, const std::vector<double>& z) {
+    const std::vector<double> t = basis_table(c->dg.L, z);
+    HIPCHK(hipMemcpy(w.zb, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+}
+
+// velocity grid at the heights of w (spectral -> y-FFT -> x C2R), on the context's stream
+void velocity_grid(rbc_ctx* c, const GridWork& w) {
+    DevGeom gt = c->dg;
+    gt.nq = w.nz;
+    if (c->rsz == 8) launch_vel_levels<double>(c, gt, w.nz, w.zb, w.spec);
+    else launch_vel_levels<float>(c, gt, w.nz, w.zb, w.spec);
+    int TX = (int)(65536 / (2 * (size_t)gt.N * sizeof(cpx<float>)));
+    TX = std::max(1, std::min(16, TX));
+    dim3 gy((gt.KX + TX - 1) / TX, w.nz, 3);
+    hipLaunchKernelGGL(k_fft_y_inv<float>, gy, dim3(256), (size_t)2 * TX * gt.N * sizeof(cpx<float>), c->stream, gt,
+                       TX, (const cpx<float>*)w.spec, c->b.rows, (const cpx<float>*)w.twN, w.hyb);
+    hipLaunchKernelGGL(k_vel_rows, dim3(gt.N, w.nz), dim3(256), (size_t)4 * gt.M * sizeof(cpx<float>), c->stream, gt,
+                       (const cpx<float>*)w.hyb, (const cpx<float>*)w.twM, w.grid);
+    HIPCHK(hipGetLastError());
+}
+
+// spectral velocity at np points (device arrays); part holds nslice*3*np floats
+struct PointWork {
+    double* lam = nullptr;
+    float4* modes = nullptr;
+    float* part = nullptr;
+    int nslice = 1, tiles_per_slice = 1;
+};
+
+int lmax_for(int L) { return L <= 8 ? 8 : L <= 16 ? 16 : L <= 24 ? 24 : L <= 32 ? 32 : 0; }
+
+void plan_points(rbc_ctx* c, int np, PointWork& pw) {
+    const int ntile = (c->dg.P + kTile - 1) / kTile;
+    const int nblk = (np + 255) / 256;
+    // enough blocks to give every SIMD of the chip several waves
+    int ns = (8192 + 4 * nblk - 1) / (4 * nblk);
+    ns = std::max(1, std::min(std::min(16, ns), ntile));
+    pw.tiles_per_slice = (ntile + ns - 1) / ns;
+    pw.nslice = (ntile + pw.tiles_per_slice - 1) / pw.tiles_per_slice;
+}
+
+void alloc_point_work(rbc_ctx* c, int np, PointWork& pw, std::vector<void*>& owned) {
+    auto al = [&](size_t bytes) {
+        void* p = nullptr;
+        HIPCHK(hipMalloc(&p, bytes > 0 ? bytes : 16));
+        owned.push_back(p);
+        return p;
+    };
+    plan_points(c, np, pw);
+    const int L = c->dg.L;
+    pw.lam = (double*)al((size_t)L * sizeof(double));
+    pw.modes = (float4*)al((size_t)c->dg.P * sizeof(float4));
+    pw.part = (float*)al((size_t)pw.nslice * 3 * np * sizeof(float));
+    std::vector<double> lam(L);
+    for (int m = 0; m < L; ++m) lam[m] = lambda_k(m + 1);
+    const std::vector<float> mt = mode_table(c);
+    HIPCHK(hipMemcpy(pw.lam, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(pw.modes, mt.data(), mt.size() * sizeof(float), hipMemcpyHostToDevice));
+}
+
+template <typename T, int LM>
+void launch_points_t(rbc_ctx* c, int np, const float* x, const float* y, const float* z, const PointWork& pw,
+                     float* u, float* v, float* w) {
+    const Buffers& b = c->b;
+    dim3 grid((np + 255) / 256, pw.nslice);
+    hipLaunchKernelGGL((k_vel_points<T, LM>), grid, dim3(256), 0, c->stream, c->dg, np, x, y, z,
+                       (const cpx<T>*)b.X[1], (const cpx<T>*)b.X[2], (const cpx<T>*)b.XF, (const cpx<T>*)b.XG,
+                       (const double*)pw.lam, (const float4*)pw.modes, pw.tiles_per_slice, pw.part);
+    hipLaunchKernelGGL(k_sum_slices, dim3((np + 255) / 256), dim3(256), 0, c->stream, np, pw.nslice,
+                       (const float*)pw.part, u, v, w);
+}
+
+template <typename T>
+void launch_points(rbc_ctx* c, int np, const float* x, const float* y, const float* z, const PointWork& pw, float* u,
+                   float* v, float* w) {
+    switch (lmax_for(c->dg.L)) {
+        case 8: launch_points_t<T, 8>(c, np, x, y, z, pw, u, v, w); break;
+        case 16: launch_points_t<T, 16>(c, np, x, y, z, pw, u, v, w); break;
+        case 24: launch_points_t<T, 24>(c, np, x, y, z, pw, u, v, w); break;
+        default: launch_points_t<T, 32>(c, np, x, y, z, pw, u, v, w); break;
+    }
+}
+
+void velocity_points(rbc_ctx* c, int np, const float* x, const float* y, const float* z, const PointWork& pw, float* u,
+                     float* v, float* w) {
+    if (np <= 0) return;
+    if (c->rsz == 8) launch_points<double>(c, np, x, y, z, pw, u, v, w);
+    else launch_points<float>(c, np, x, y, z, pw, u, v, w);
+    HIPCHK(hipGetLastError());
+}
+
+}  // namespace
+
+// ---- tracer object -----------------------------------------------------------------------
+struct rbc_tracer {
+    rbc_ctx* c = nullptr;
+    int np = 0, buf = 0, mode = 0;
+    double diff = 0.0;
+    uint64_t seed = 0;
+    float *x = nullptr, *y = nullptr, *z = nullptr;  // [buf][np]
+    int iter = 0, ptime = 0;
+    GridWork gw;               // reference mode
+    PointWork pw;              // spectral mode
+    float *vel = nullptr;      // 3 * np (spectral mode)
+    float *mid = nullptr;      // 3 * np
+    double* red = nullptr;     // 4 doubles
+    std::vector<void*> owned;
+    std::string err;
+    ~rbc_tracer() {
+        if (c && c->stream) hipStreamSynchronize(c->stream);
+        for (void* p : owned) hipFree(p);
+    }
+};
+
+namespace {
+
+int tfail(rbc_tracer* t, int code, const std::string& msg) {
+    t->err = msg;
+    t->c->err = msg;
+    return code;
+}
+
+int check_ctx_for_velocity(rbc_ctx* c) {
+    if (c->dg.M > 4096 || c->dg.N > 4096) return fail(c, RBC_ERR_ARG, "velocity grid supports M, N <= 4096");
+    return RBC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rbc_tracer_create(rbc_ctx* c, int num_particles, int buffer_size, int mode, double diff_coeff,
+                      unsigned long long seed, rbc_tracer** out) {
+    if (!c || !out) return RBC_ERR_ARG;
+    *out = nullptr;
+    if (num_particles < 1 || buffer_size < 2) return fail(c, RBC_ERR_ARG, "need num_particles >= 1 and buffer_size >= 2");
+    if (mode != RBC_TRACER_RK4_TRILINEAR && mode != RBC_TRACER_SPECTRAL_RK2)
+        return fail(c, RBC_ERR_ARG, "unknown tracer mode");
+    if (mode == RBC_TRACER_SPECTRAL_RK2 && lmax_for(c->dg.L) == 0)
+        return fail(c, RBC_ERR_ARG, "the spectral tracer supports L <= 32");
+    if (diff_coeff < 0) return fail(c, RBC_ERR_ARG, "diffusion coefficient must be >= 0");
+    if (int rc = check_ctx_for_velocity(c)) return rc;
+    rbc_tracer* t = new rbc_tracer;
+    t->c = c;
+    t->np = num_particles;
+    t->buf = buffer_size;
+    t->mode = mode;
+    t->diff = diff_coeff;
+    t->seed = seed;
+    try {
+        hipSetDevice(c->p.device);
+        auto al = [&](size_t bytes) {
+            void* p = nullptr;
+            HIPCHK(hipMalloc(&p, bytes));
+            t->owned.push_back(p);
+            return p;
+        };
+        const size_t pos = (size_t)num_particles * buffer_size * sizeof(float);
+        t->x = (float*)al(pos);
+        t->y = (float*)al(pos);
+        t->z = (float*)al(pos);
+        t->red = (double*)al(4 * sizeof(double));
+        if (mode == RBC_TRACER_RK4_TRILINEAR) {
+            alloc_grid_work(c, kZNum, t->gw, t->owned);
+            // linspace(-PARTICLE_Z_MAX, PARTICLE_Z_MAX) in the device real type (particle_tracer.cu:494-501)
+            const float zmin = (float)(-kZMax), zmax = (float)kZMax;
+            std::vector<double> zl(kZNum);
+            for (int l = 0; l < kZNum; ++l) zl[l] = (double)(zmin + l * (zmax - zmin) / (kZNum - 1));
+            set_heights(c, t->gw, zl);
+        } else {
+            alloc_point_work(c, num_particles, t->pw, t->owned);
+            t->vel = (float*)al((size_t)3 * num_particles * sizeof(float));
+            t->mid = (float*)al((size_t)3 * num_particles * sizeof(float));
+        }
+        // the constructor places every particle at the centre (particle_tracer.cu:43)
+        hipLaunchKernelGGL(k_tracer_init, dim3((num_particles + 255) / 256), dim3(256), 0, c->stream,
+                           (int)RBC_PART_CENTER, num_particles, (float)c->p.Lx, (float)c->p.Ly,
+                           (float)(0.5 * c->p.Lx), (float)(0.5 * c->p.Ly), 0.f, 0.f, (uint64_t)seed, t->x, t->y, t->z);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const HipError& e) {
+        const int rc = hip_fail(c, e);
+        delete t;
+        return rc;
+    }
+    *out = t;
+    return RBC_OK;
+}
+
+void rbc_tracer_destroy(rbc_tracer* t) { delete t; }
+
+const char* rbc_tracer_last_error(const rbc_tracer* t) { return t ? t->err.c_str() : ""; }
+
+int rbc_tracer_init(rbc_tracer* t, int type, double cx, double cy, double cz, double sigma) {
+    if (!t) return RBC_ERR_ARG;
+    if (type < RBC_PART_RANDOM || type > RBC_PART_COORD_SIGMA) return tfail(t, RBC_ERR_ARG, "unknown init type");
+    if (type == RBC_PART_PAIRS && (t->np & 1))
+        return tfail(t, RBC_ERR_ARG, "Position pairs only possible with even number of particles.");
+    rbc_ctx* c = t->c;
+    float ax = (float)cx, ay = (float)cy, az = (float)cz, sg = (float)sigma;
+    if (type == RBC_PART_CENTER || type == RBC_PART_CENTER_SIGMA) {
+        ax = (float)(0.5 * c->p.Lx);
+        ay = (float)(0.5 * c->p.Ly);
+        az = 0.f;
+        if (type == RBC_PART_CENTER) sg = 0.f;
+    }
+    try {
+        hipSetDevice(c->p.device);
+        const size_t off = (size_t)t->iter * t->np;
+        hipLaunchKernelGGL(k_tracer_init, dim3((t->np + 255) / 256), dim3(256), 0, c->stream, type, t->np,
+                           (float)c->p.Lx, (float)c->p.Ly, ax, ay, az, sg, t->seed, t->x + off, t->y + off, t->z + off);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+int rbc_tracer_set_positions(rbc_tracer* t, const float* x, const float* y, const float* z) {
+    if (!t || !x || !y || !z) return RBC_ERR_ARG;
+    rbc_ctx* c = t->c;
+    try {
+        hipSetDevice(c->p.device);
+        const size_t off = (size_t)t->iter * t->np, n = (size_t)t->np * sizeof(float);
+        HIPCHK(hipMemcpyAsync(t->x + off, x, n, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(t->y + off, y, n, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(t->z + off, z, n, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+int rbc_tracer_step(rbc_tracer* t, double dt) {
+    if (!t) return RBC_ERR_ARG;
+    if (t->iter >= t->buf - 1)  // particle_tracer.cu:145-147, is_buffer_full :263-265
+        return tfail(t, RBC_ERR_STATE, "buffer of particle tracer is filled! read out the positions first");
+    rbc_ctx* c = t->c;
+    const int np = t->np;
+    const size_t in = (size_t)t->iter * np, outo = in + np;
+    const float sigma = t->diff > 0 ? (float)std::sqrt(2.0 * t->diff * dt) : 0.f;
+    const uint64_t ctr0 = (uint64_t)(t->ptime + 1) * 3 * (uint64_t)np;
+    const dim3 pg((np + 255) / 256);
+    try {
+        hipSetDevice(c->p.device);
+        if (t->mode == RBC_TRACER_RK4_TRILINEAR) {
+            velocity_grid(c, t->gw);
+            hipLaunchKernelGGL(k_tracer_rk4, pg, dim3(256), 0, c->stream, np, (const float*)t->gw.grid, c->dg.M,
+                               c->dg.N, (float)c->p.Lx, (float)c->p.Ly, (const float*)t->x + in,
+                               (const float*)t->y + in, (const float*)t->z + in, (float)dt, t->x + outo, t->y + outo,
+                               t->z + outo, sigma, t->seed, ctr0);
+        } else {
+            float *u = t->vel, *v = t->vel + np, *w = t->vel + 2 * (size_t)np;
+            float *xm = t->mid, *ym = t->mid + np, *zm = t->mid + 2 * (size_t)np;
+            velocity_points(c, np, t->x + in, t->y + in, t->z + in, t->pw, u, v, w);
+            hipLaunchKernelGGL(k_tracer_half, pg, dim3(256), 0, c->stream, np, (const float*)t->x + in,
+                               (const float*)t->y + in, (const float*)t->z + in, (const float*)u, (const float*)v,
+                               (const float*)w, (float)(0.5 * dt), xm, ym, zm);
+            velocity_points(c, np, xm, ym, zm, t->pw, u, v, w);
+            hipLaunchKernelGGL(k_tracer_full, pg, dim3(256), 0, c->stream, np, (const float*)t->x + in,
+                               (const float*)t->y + in, (const float*)t->z + in, (const float*)u, (const float*)v,
+                               (const float*)w, (float)dt, t->x + outo, t->y + outo, t->z + outo, sigma, t->seed,
+                               ctr0);
+        }
+        HIPCHK(hipGetLastError());
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    t->iter++;
+    t->ptime++;
+    return RBC_OK;
+}
+
+int rbc_tracer_get_positions(rbc_tracer* t, int clear, float* x, float* y, float* z, int* num_timesteps) {
+    if (!t || !x || !y || !z) return RBC_ERR_ARG;
+    rbc_ctx* c = t->c;
+    // particle_tracer.cu:272-311: the current slot is left out unless nothing was stepped
+    int nt = t->iter;
+    if (t->iter >= t->buf) nt = t->buf - 1;
+    else if (t->iter < 1) nt = 1;
+    try {
+        hipSetDevice(c->p.device);
+        const size_t n = (size_t)t->np * nt * sizeof(float);
+        HIPCHK(hipMemcpyAsync(x, t->x, n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(y, t->y, n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(z, t->z, n, hipMemcpyDeviceToHost, c->stream));
+        if (clear && t->iter > 0) {  // reset_particle_positions: slot nt becomes slot 0
+            const size_t off = (size_t)nt * t->np, m = (size_t)t->np * sizeof(float);
+            HIPCHK(hipMemcpyAsync(t->x, t->x + off, m, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(t->y, t->y + off, m, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(t->z, t->z + off, m, hipMemcpyDeviceToDevice, c->stream));
+            t->iter = 0;
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    if (num_timesteps) *num_timesteps = nt;
+    return RBC_OK;
+}
+
+int rbc_tracer_current(rbc_tracer* t, float* x, float* y, float* z) {
+    if (!t || !x || !y || !z) return RBC_ERR_ARG;
+    rbc_ctx* c = t->c;
+    try {
+        hipSetDevice(c->p.device);
+        const size_t off = (size_t)t->iter * t->np, n = (size_t)t->np * sizeof(float);
+        HIPCHK(hipMemcpyAsync(x, t->x + off, n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(y, t->y + off, n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(z, t->z + off, n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+int rbc_tracer_info(const rbc_tracer* t, int* iteration_index, int* particle_time) {
+    if (!t) return RBC_ERR_ARG;
+    if (iteration_index) *iteration_index = t->iter;
+    if (particle_time) *particle_time = t->ptime;
+    return RBC_OK;
+}
+
+int rbc_tracer_moments(rbc_tracer* t, double* out4) {
+    if (!t || !out4) return RBC_ERR_ARG;
+    rbc_ctx* c = t->c;
+    try {
+        hipSetDevice(c->p.device);
+        const size_t off = (size_t)t->iter * t->np;
+        hipLaunchKernelGGL(k_tracer_moments, dim3(1), dim3(1024), 0, c->stream, t->np, (const float*)t->x + off,
+                           (const float*)t->y + off, (float)c->p.Lx, (float)c->p.Ly, t->red);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out4, t->red, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+int rbc_velocity_grid(rbc_ctx* c, int nz, const double* z, float* ux, float* uy, float* uz) {
+    if (!c || nz < 1 || !z || !ux || !uy || !uz) return RBC_ERR_ARG;
+    if (int rc = check_ctx_for_velocity(c)) return rc;
+    std::vector<void*> owned;
+    int rc = RBC_OK;
+    try {
+        hipSetDevice(c->p.device);
+        GridWork w;
+        alloc_grid_work(c, nz, w, owned);
+        set_heights(c, w, std::vector<double>(z, z + nz));
+        velocity_grid(c, w);
+        const size_t n = (size_t)nz * c->dg.N * c->dg.M;
+        HIPCHK(hipMemcpyAsync(ux, w.grid, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(uy, w.grid + n, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(uz, w.grid + 2 * n, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const HipError& e) {
+        rc = hip_fail(c, e);
+    }
+    for (void* p : owned) hipFree(p);
+    return rc;
+}
+
+int rbc_velocity_at(rbc_ctx* c, int n, const float* x, const float* y, const float* z, float* u, float* v, float* w) {
+    if (!c || n < 0 || (n > 0 && (!x || !y || !z || !u || !v || !w))) return RBC_ERR_ARG;
+    if (lmax_for(c->dg.L) == 0) return fail(c, RBC_ERR_ARG, "spectral evaluation supports L <= 32");
+    if (n == 0) return RBC_OK;
+    std::vector<void*> owned;
+    int rc = RBC_OK;
+    try {
+        hipSetDevice(c->p.device);
+        PointWork pw;
+        alloc_point_work(c, n, pw, owned);
+        float* d = nullptr;
+        HIPCHK(hipMalloc(&d, (size_t)6 * n * sizeof(float)));
+        owned.push_back(d);
+        const size_t b = (size_t)n * sizeof(float);
+        HIPCHK(hipMemcpyAsync(d, x, b, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(d + n, y, b, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(d + 2 * (size_t)n, z, b, hipMemcpyHostToDevice, c->stream));
+        float* o = d + 3 * (size_t)n;
+        velocity_points(c, n, d, d + n, d + 2 * (size_t)n, pw, o, o + n, o + 2 * (size_t)n);
+        HIPCHK(hipMemcpyAsync(u, o, b, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(v, o + n, b, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(w, o + 2 * (size_t)n, b, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const HipError& e) {
+        rc = hip_fail(c, e);
+    }
+    for (void* p : owned) hipFree(p);
+    return rc;
+}
+
+// per-mode energies at heights z with weights wz (FP64, host result [3][P])
+static int mode_energies(rbc_ctx* c, const std::vector<double>& z, const std::vector<double>& wz,
+                         std::vector<double>& e) {
+    std::vector<void*> owned;
+    int rc = RBC_OK;
+    try {
+        hipSetDevice(c->p.device);
+        const DevGeom& g = c->dg;
+        const std::vector<double> tb = basis_table(g.L, z);
+        double *dzb = nullptr, *dw = nullptr, *de = nullptr;
+        HIPCHK(hipMalloc(&dzb, tb.size() * sizeof(double)));
+        owned.push_back(dzb);
+        HIPCHK(hipMalloc(&dw, wz.size() * sizeof(double)));
+        owned.push_back(dw);
+        HIPCHK(hipMalloc(&de, (size_t)3 * g.P * sizeof(double)));
+        owned.push_back(de);
+        HIPCHK(hipMemcpy(dzb, tb.data(), tb.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dw, wz.data(), wz.size() * sizeof(double), hipMemcpyHostToDevice));
+        const Buffers& b = c->b;
+        const int nz = (int)z.size();
+        if (c->rsz == 8)
+            hipLaunchKernelGGL(k_energy_modes<double>, dim3((g.P + 255) / 256), dim3(256), 0, c->stream, g, nz, dzb, dw,
+                               (const cpx<double>*)b.X[1], (const cpx<double>*)b.X[2], (const cpx<double>*)b.XF,
+                               (const cpx<double>*)b.XG, (const double*)b.kx, (const double*)b.ky, de);
+        else
+            hipLaunchKernelGGL(k_energy_modes<float>, dim3((g.P + 255) / 256), dim3(256), 0, c->stream, g, nz, dzb, dw,
+                               (const cpx<float>*)b.X[1], (const cpx<float>*)b.X[2], (const cpx<float>*)b.XF,
+                               (const cpx<float>*)b.XG, (const float*)b.kx, (const float*)b.ky, de);
+        HIPCHK(hipGetLastError());
+        e.resize((size_t)3 * g.P);
+        HIPCHK(hipMemcpyAsync(e.data(), de, e.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const HipError& err) {
+        rc = hip_fail(c, err);
+    }
+    for (void* p : owned) hipFree(p);
+    return rc;
+}
+
+int rbc_energy_spectrum(rbc_ctx* c, int cap, double* E, int* nbins, double* totals) {
+    if (!c || !nbins) return RBC_ERR_ARG;
+    const Geometry& g = c->geo;
+    const double dk = std::min(2 * M_PI / c->p.Lx, 2 * M_PI / c->p.Ly);
+    std::vector<double> zq, wq;
+    gauss_legendre(c->dg.nq, zq, wq);
+    std::vector<double> e;
+    if (int rc = mode_energies(c, zq, wq, e)) return rc;
+    const int P = g.P(), NYc = g.NYc();
+    std::vector<double> spec;
+    double eh = 0, ez = 0;
+    for (int p = 0; p < P; ++p) {
+        const double wk = (p / NYc == 0) ? 1.0 : 2.0;
+        const double kk = std::sqrt(c->fac.kx[p] * c->fac.kx[p] + c->fac.ky[p] * c->fac.ky[p]);
+        const int s = (int)std::floor(kk / dk + 0.5);
+        if ((int)spec.size() <= s) spec.resize(s + 1, 0.0);
+        const double h = 0.5 * wk * (e[p] + e[P + p]);
+        spec[s] += h;
+        eh += h;
+        ez += 0.5 * wk * e[2 * (size_t)P + p];
+    }
+    *nbins = (int)spec.size();
+    if (E) {
+        if (cap < (int)spec.size()) return fail(c, RBC_ERR_ARG, "spectrum buffer too small");
+        std::copy(spec.begin(), spec.end(), E);
+    }
+    if (totals) {
+        totals[0] = eh;
+        totals[1] = ez;
+    }
+    return RBC_OK;
+}
+
+int rbc_energy_spectrum_ref(rbc_ctx* c, double z, int cap, double* E3, int* nbins) {
+    if (!c || !nbins) return RBC_ERR_ARG;
+    const Geometry& g = c->geo;
+    std::vector<double> e;
+    if (int rc = mode_energies(c, std::vector<double>{z}, std::vector<double>{1.0}, e)) return rc;
+    // energy_spectrum_2d.cu:53-56, 85-129: size = max(M/2+1, N/2+1) bins of width
+    // |k_max| / size, |k_max| from (2 pi (M/2) / Lx, 2 pi (N/2) / Ly); each bin sums |c_k|^2
+    // over the stored half spectrum (no factor 2 for kx > 0).
+    const int mox = g.mox, size = std::max(mox, g.N / 2 + 1);
+    const double kmx = 2.0 * M_PI * (mox - 1) / c->p.Lx, kmy = 2.0 * M_PI * (g.N / 2) / c->p.Ly;
+    const double inc = std::sqrt(kmx * kmx + kmy * kmy) / size;
+    *nbins = size;
+    if (!E3) return RBC_OK;
+    if (cap < size) return fail(c, RBC_ERR_ARG, "spectrum buffer too small");
+    for (int i = 0; i < 3 * size; ++i) E3[i] = 0.0;
+    const int P = g.P(), NYc = g.NYc();
+    for (int p = 0; p < P; ++p) {
+        const int col = p / NYc, row = g.rows[p % NYc];
+        const double mx = 2.0 * M_PI * col / c->p.Lx;
+        const double my = 2.0 * M_PI * (row <= g.N / 2 ? row : g.N - row) / c->p.Ly;
+        const double a = std::sqrt(mx * mx + my * my);
+        const int b0 = std::max(0, (int)std::floor(a / inc) - 1);
+        for (int b = b0; b < std::min(size, b0 + 3); ++b)
+            if (b * inc <= a && (b + 1) * inc > a) {
+                for (int k = 0; k < 3; ++k) E3[k * size + b] += e[(size_t)k * P + p];
+                break;
+            }
+    }
+    return RBC_OK;
+}
+
+}  // extern "C"

@@ -40,7 +40,15 @@ _lib = None
 
 EXPORTS = ["rbc_create", "rbc_destroy", "rbc_last_error", "rbc_get_info", "rbc_set_state",
            "rbc_get_state", "rbc_step", "rbc_reset_history", "rbc_nonlinear", "rbc_inverse_B",
-           "rbc_implicit", "rbc_nusselt", "rbc_profile_steps", "rbc_stage_name", "rbc_synchronize"]
+           "rbc_implicit", "rbc_nusselt", "rbc_profile_steps", "rbc_stage_name", "rbc_synchronize",
+           "rbc_read_state_files", "rbc_write_state_files", "rbc_get_history", "rbc_set_history",
+           "rbc_velocity_grid", "rbc_velocity_at", "rbc_tracer_create", "rbc_tracer_destroy",
+           "rbc_tracer_last_error", "rbc_tracer_init", "rbc_tracer_set_positions", "rbc_tracer_step",
+           "rbc_tracer_get_positions", "rbc_tracer_current", "rbc_tracer_info", "rbc_tracer_moments",
+           "rbc_energy_spectrum", "rbc_energy_spectrum_ref"]
+
+TRACER_MODES = {"rk4_trilinear": 0, "spectral_rk2": 1}
+PART_INIT = {"random": 0, "center": 1, "center_sigma": 2, "diagonal": 3, "pairs": 4, "coord_sigma": 5}
 
 
 def load_library(path=LIB_PATH):
@@ -72,12 +80,45 @@ def load_library(path=LIB_PATH):
     L.rbc_stage_name.argtypes = [ctypes.c_int]
     L.rbc_stage_name.restype = ctypes.c_char_p
     L.rbc_synchronize.argtypes = [vp]
+    cpp = ctypes.POINTER(ctypes.c_char_p)
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.rbc_read_state_files.argtypes = [vp, cpp, ctypes.c_int]
+    L.rbc_write_state_files.argtypes = [vp, cpp, ctypes.c_int, ctypes.c_int]
+    L.rbc_get_history.argtypes = [vp] + [dp] * 5
+    L.rbc_set_history.argtypes = [vp] + [dp] * 5
+    L.rbc_velocity_grid.argtypes = [vp, ctypes.c_int, dp, fp, fp, fp]
+    L.rbc_velocity_at.argtypes = [vp, ctypes.c_int, fp, fp, fp, fp, fp, fp]
+    L.rbc_tracer_create.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_ulonglong, ctypes.POINTER(vp)]
+    L.rbc_tracer_destroy.argtypes = [vp]
+    L.rbc_tracer_destroy.restype = None
+    L.rbc_tracer_last_error.argtypes = [vp]
+    L.rbc_tracer_last_error.restype = ctypes.c_char_p
+    L.rbc_tracer_init.argtypes = [vp, ctypes.c_int] + [ctypes.c_double] * 4
+    L.rbc_tracer_set_positions.argtypes = [vp, fp, fp, fp]
+    L.rbc_tracer_step.argtypes = [vp, ctypes.c_double]
+    L.rbc_tracer_get_positions.argtypes = [vp, ctypes.c_int, fp, fp, fp, ip]
+    L.rbc_tracer_current.argtypes = [vp, fp, fp, fp]
+    L.rbc_tracer_info.argtypes = [vp, ip, ip]
+    L.rbc_tracer_moments.argtypes = [vp, dp]
+    L.rbc_energy_spectrum.argtypes = [vp, ctypes.c_int, dp, ip, dp]
+    L.rbc_energy_spectrum_ref.argtypes = [vp, ctypes.c_double, ctypes.c_int, dp, ip]
     _lib = L
     return L
 
 
 def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _paths(paths):
+    if len(paths) != 5:
+        raise ValueError("five paths are needed: theta, f, g, F, G")
+    return (ctypes.c_char_p * 5)(*[os.fsencode(p) for p in paths])
 
 
 class Stepper:
@@ -167,3 +208,126 @@ class Stepper:
         self._check(self.lib.rbc_profile_steps(self.h, int(nsteps), _dp(ms)), "rbc_profile_steps")
         names = [self.lib.rbc_stage_name(i).decode() for i in range(NSTAGES)]
         return dict(zip(names, ms.tolist()))
+
+    # ---- state files and AB2 history -------------------------------------------------------
+    def read_state_files(self, paths, which=0):
+        """Reads theta, f, g, F, G files in the reference format (which=1: the AB2 history)."""
+        self._check(self.lib.rbc_read_state_files(self.h, _paths(paths), int(which)), "rbc_read_state_files")
+
+    def write_state_files(self, paths, which=0, elem_bytes=0):
+        self._check(self.lib.rbc_write_state_files(self.h, _paths(paths), int(which), int(elem_bytes)),
+                    "rbc_write_state_files")
+
+    def get_history(self):
+        out = self.zeros()
+        self._check(self.lib.rbc_get_history(self.h, *[_dp(a.view(np.float64)) for a in out]), "rbc_get_history")
+        return out
+
+    def set_history(self, theta, f, g, F, G):
+        arrs = [np.ascontiguousarray(a, np.complex128) for a in (theta, f, g, F, G)]
+        self._check(self.lib.rbc_set_history(self.h, *[_dp(a.view(np.float64)) for a in arrs]), "rbc_set_history")
+
+    # ---- velocity and spectra ----------------------------------------------------------------
+    def velocity_grid(self, z):
+        """(u_x, u_y, u_z), each float32 (nz, N, M), of the current state at heights z."""
+        z = np.ascontiguousarray(z, np.float64)
+        out = [np.zeros((len(z), self.N, self.M), np.float32) for _ in range(3)]
+        self._check(self.lib.rbc_velocity_grid(self.h, len(z), _dp(z), *[_fp(a) for a in out]), "rbc_velocity_grid")
+        return out
+
+    def velocity_at(self, x, y, z):
+        x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+        out = [np.zeros(len(x), np.float32) for _ in range(3)]
+        self._check(self.lib.rbc_velocity_at(self.h, len(x), _fp(x), _fp(y), _fp(z), *[_fp(a) for a in out]),
+                    "rbc_velocity_at")
+        return out
+
+    def energy_spectrum(self):
+        """(E_h per |k| shell, (E_h total, E_z total)), FP64."""
+        n = ctypes.c_int()
+        self._check(self.lib.rbc_energy_spectrum(self.h, 0, None, ctypes.byref(n), None), "rbc_energy_spectrum")
+        E = np.zeros(n.value)
+        tot = np.zeros(2)
+        self._check(self.lib.rbc_energy_spectrum(self.h, n.value, _dp(E), ctypes.byref(n), _dp(tot)),
+                    "rbc_energy_spectrum")
+        return E, tot
+
+    def energy_spectrum_ref(self, z):
+        """The reference's binning of |c_k|^2 at height z: array (3, nbins) for u_x, u_y, u_z."""
+        n = ctypes.c_int()
+        self._check(self.lib.rbc_energy_spectrum_ref(self.h, float(z), 0, None, ctypes.byref(n)),
+                    "rbc_energy_spectrum_ref")
+        E = np.zeros(3 * n.value)
+        self._check(self.lib.rbc_energy_spectrum_ref(self.h, float(z), n.value, _dp(E), ctypes.byref(n)),
+                    "rbc_energy_spectrum_ref")
+        return E.reshape(3, n.value)
+
+
+class Tracer:
+    """Passive tracers advected by a Stepper's velocity (≙ particle_tracer,
+    particle/particle_tracer.h:21-89).  mode "rk4_trilinear" is the reference's scheme,
+    "spectral_rk2" evaluates the velocity exactly at each particle."""
+
+    def __init__(self, stepper, num_particles, buffer_size=256, mode="rk4_trilinear", diff_coeff=0.0, seed=0):
+        self.lib = stepper.lib
+        self.stepper = stepper
+        self.np = int(num_particles)
+        self.buffer_size = int(buffer_size)
+        m = TRACER_MODES[mode] if isinstance(mode, str) else int(mode)
+        h = ctypes.c_void_p()
+        rc = self.lib.rbc_tracer_create(stepper.h, self.np, self.buffer_size, m, float(diff_coeff), int(seed),
+                                        ctypes.byref(h))
+        if rc != 0:
+            raise RbcError(f"rbc_tracer_create failed ({rc}): {self.lib.rbc_last_error(stepper.h).decode()}")
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RbcError(f"{what} failed ({rc}): {self.lib.rbc_tracer_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rbc_tracer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_positions(self, kind="random", coord=(0.0, 0.0, 0.0), sigma=0.0):
+        t = PART_INIT[kind] if isinstance(kind, str) else int(kind)
+        self._check(self.lib.rbc_tracer_init(self.h, t, float(coord[0]), float(coord[1]), float(coord[2]),
+                                             float(sigma)), "rbc_tracer_init")
+
+    def set_positions(self, x, y, z):
+        x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+        self._check(self.lib.rbc_tracer_set_positions(self.h, _fp(x), _fp(y), _fp(z)), "rbc_tracer_set_positions")
+
+    def step_particles(self, dt):
+        self._check(self.lib.rbc_tracer_step(self.h, float(dt)), "rbc_tracer_step")
+
+    def positions(self):
+        out = [np.zeros(self.np, np.float32) for _ in range(3)]
+        self._check(self.lib.rbc_tracer_current(self.h, *[_fp(a) for a in out]), "rbc_tracer_current")
+        return out
+
+    def get_particle_positions(self, clear=True):
+        """(x, y, z) arrays shaped (num_timesteps, num_particles)."""
+        cap = (self.buffer_size - 1) * self.np
+        out = [np.zeros(cap, np.float32) for _ in range(3)]
+        nt = ctypes.c_int()
+        self._check(self.lib.rbc_tracer_get_positions(self.h, int(bool(clear)), *[_fp(a) for a in out],
+                                                      ctypes.byref(nt)), "rbc_tracer_get_positions")
+        return [a[:nt.value * self.np].reshape(nt.value, self.np) for a in out]
+
+    def info(self):
+        it, pt = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.rbc_tracer_info(self.h, ctypes.byref(it), ctypes.byref(pt)), "rbc_tracer_info")
+        return it.value, pt.value
+
+    def moments(self):
+        out = np.zeros(4)
+        self._check(self.lib.rbc_tracer_moments(self.h, _dp(out)), "rbc_tracer_moments")
+        return out
