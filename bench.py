@@ -7,6 +7,7 @@ timing).  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f32|bf16x3|f64]
                   [--no-cpu] [--cpu-threads T]
+  python bench.py --ensemble [--members 64] [--steps K] [--ens-k 10]   (batched workload)
 """
 import argparse
 import json
@@ -88,10 +89,74 @@ def cpu_baseline(threads, budget_s=20.0):
     }
 
 
+def ensemble_configs(n=64, seed=4):
+    """The batched workload: n runs with the grid drawn from {64^2, 128^2, 256^2}, L from
+    {8, 16, 24}, Ra log-uniform in [1e4, 1e6], Pr = 0.7, box 4 x 4.  dt = 0.25 dx / U with the
+    free-fall velocity U = (Ra Pr)^1/2 (in thermal-diffusion units) keeps the explicit AB2
+    advection stable through the onset transient of every member."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    cfgs = []
+    for i in range(n):
+        M = int(rng.choice([64, 128, 256]))
+        L = int(rng.choice([8, 16, 24]))
+        Ra = float(10 ** rng.uniform(4, 6))
+        dt = 0.25 * (4.0 / M) / (Ra * 0.7) ** 0.5
+        cfgs.append(dict(M=M, N=M, L=L, Lx=4.0, Ly=4.0, Ra=Ra, Pr=0.7, dt=dt, precision="f32"))
+    return cfgs
+
+
+def run_ensemble(a):
+    """Member-steps per second of the batched workload: graph replay vs eager launches."""
+    import numpy as np
+    from sdc_amd import Ensemble
+    from sdc_amd.initial import random_theta
+    cfgs = ensemble_configs(a.members)
+    t0 = time.perf_counter()
+    E = Ensemble(cfgs)
+    t_create = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rep = E.tune()
+    t_tune = time.perf_counter() - t0
+    for i, (m, c) in enumerate(zip(E.members, cfgs)):
+        m.set_state(*random_theta(c["M"], c["N"], c["L"], seed=1000 + i, amp=1e-3))
+    E.step(1 + a.warmup, steps_per_launch=a.ens_k)  # Euler start, capture, warm-up
+    res = {}
+    for mode, k in (("graph", a.ens_k), ("eager", 0)):
+        t0 = time.perf_counter()
+        E.step(a.steps, steps_per_launch=k)
+        res[mode] = time.perf_counter() - t0
+    nu = np.array([m.nusselt()[2] for m in E.members])
+    finite = int(np.isfinite(nu).sum())
+    E.close()
+    n = len(cfgs)
+    classes = sorted({(c["M"], c["L"]) for c in cfgs})
+    out = {
+        "metric": "ensemble member-steps/s (64 uneven RBC runs, one captured graph per launch)",
+        "value": round(n * a.steps / res["graph"], 1),
+        "unit": "member-steps/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(1e3 * res["graph"] / a.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded theta ~ 1e-3 N(0,1) per member, seed 1000 + member)",
+        "config": {"workload": f"ensemble of {n} runs, grid in {{64^2,128^2,256^2}}, L in {{8,16,24}}, "
+                               "Ra log-uniform [1e4,1e6], Pr=0.7, 4x4x1 box",
+                   "steps_per_launch": a.ens_k, "size_classes": [list(c) for c in classes]},
+        "eager_member_steps_per_s": round(n * a.steps / res["eager"], 1),
+        "graph_speedup_vs_eager": round(res["eager"] / res["graph"], 3),
+        "create_s": round(t_create, 2), "tune_s": round(t_tune, 2),
+        "tuned_shapes": {f"{c['M']}^2xL{c['L']}": [int(r[0]), int(r[1]), round(float(r[2]), 4)]
+                         for c, r in zip(cfgs, rep)},
+        "members_finite": finite,
+        "nu_vol_range": [round(float(np.nanmin(nu)), 6), round(float(np.nanmax(nu)), 6)],
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 200; 500 with --ensemble)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="f32", choices=["f32", "bf16x3", "f64"])
     ap.add_argument("--no-cpu", action="store_true")
@@ -101,11 +166,20 @@ def main():
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MB",
                     help="also run one device-to-device copy of MB mebibytes (a kernel of known traffic "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--ensemble", action="store_true", help="run the batched-ensemble workload instead")
+    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--ens-k", type=int, default=10, help="steps captured per graph launch")
     a = ap.parse_args()
+    if a.steps is None:
+        a.steps = 500 if a.ensemble else 200
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU layout of this "
               "path is described in DESIGN.md and not measured here.", file=sys.stderr)
         sys.exit(2)
+
+    if a.ensemble:
+        run_ensemble(a)
+        return
 
     import sdc_amd
     from sdc_amd.initial import random_theta

@@ -189,6 +189,27 @@ int rbc_tracer_moments(rbc_tracer* t, double* out4);
 int rbc_energy_spectrum(rbc_ctx* c, int cap, double* E, int* nbins, double* totals);
 int rbc_energy_spectrum_ref(rbc_ctx* c, double z, int cap, double* E3, int* nbins);
 
+/* ---- Ensembles of independent runs --------------------------------------------------------
+ * No counterpart in the reference, which runs one simulation per process
+ * (run/script_onset_sdc.sh:8).  Members may differ in M, N, L, Ra, Pr, dt and precision but
+ * share one device; each member is a full context (rbc_ensemble_member gives it for
+ * rbc_set_state, rbc_get_state, rbc_nusselt, ...; it stays owned by the ensemble).
+ * rbc_ensemble_step advances every member by nsteps: a member without AB2 history takes its
+ * Euler step eagerly, then steps_per_launch steps of all members are captured once into one
+ * HIP graph and replayed (steps_per_launch = 0: eager launches).  Results equal stepping each
+ * member alone.  rbc_ensemble_tune times launch shapes of the generic FFT kernels per size
+ * class (M, N, L, precision) on the first member of the class (its state is restored) and
+ * applies the fastest to the class; report (optional, 3 doubles per member) receives
+ * {y-FFT columns per block, row-kernel threads, ms per step}. */
+typedef struct rbc_ensemble rbc_ensemble;
+int rbc_ensemble_create(int n, const rbc_params* params, rbc_ensemble** out);
+void rbc_ensemble_destroy(rbc_ensemble* e);
+const char* rbc_ensemble_last_error(const rbc_ensemble* e);
+int rbc_ensemble_size(const rbc_ensemble* e);
+rbc_ctx* rbc_ensemble_member(rbc_ensemble* e, int i);
+int rbc_ensemble_tune(rbc_ensemble* e, double* report);
+int rbc_ensemble_step(rbc_ensemble* e, int nsteps, int steps_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,277 @@
+// rbc_ensemble.cpp -- batches of independent runs of uneven size, stepped together through
+// one captured HIP graph, with a launch-shape autotuner per size class.
+//
+// The reference runs one simulation per process (run/script_onset_sdc.sh:8 loops over
+// processes).  Here every member is a full context (its own factors, buffers and stream);
+// K steps of every member are captured once into a graph whose branches (one per member)
+// are independent, and each rbc_ensemble_step replays it.  The first step of a member with
+// no AB2 history is an Euler step with other coefficients, so it runs eagerly before the
+// graph is used.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "rbc_ctx.hpp"
+
+using namespace rbc;
+
+struct rbc_ensemble {
+    std::vector<rbc_ctx*> m;
+    int device = 0;
+    hipStream_t s0 = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_steps = 0;
+    std::string err;
+    std::vector<double> tune_report;  // per member: ytx, rows threads, ms per step
+    ~rbc_ensemble() {
+        if (s0) hipStreamSynchronize(s0);
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        for (hipEvent_t e : join) hipEventDestroy(e);
+        if (fork) hipEventDestroy(fork);
+        for (rbc_ctx* c : m) rbc_destroy(c);
+        if (s0) hipStreamDestroy(s0);
+    }
+};
+
+namespace {
+
+int efail(rbc_ensemble* e, int code, const std::string& msg) {
+    e->err = msg;
+    return code;
+}
+
+void drop_graph(rbc_ensemble* e) {
+    if (e->exec) hipGraphExecDestroy(e->exec);
+    if (e->graph) hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+    e->graph_steps = 0;
+}
+
+// members without AB2 history take their Euler step eagerly (on their own streams)
+void euler_starts(rbc_ensemble* e) {
+    for (rbc_ctx* c : e->m)
+        if (!c->have_hist) run_steps(c, 1);
+}
+
+void capture(rbc_ensemble* e, int K) {
+    drop_graph(e);
+    HIPCHK(hipStreamBeginCapture(e->s0, hipStreamCaptureModeThreadLocal));
+    HIPCHK(hipEventRecord(e->fork, e->s0));
+    for (size_t i = 0; i < e->m.size(); ++i) {
+        rbc_ctx* c = e->m[i];
+        HIPCHK(hipStreamWaitEvent(c->stream, e->fork, 0));
+        run_steps(c, K);
+        HIPCHK(hipEventRecord(e->join[i], c->stream));
+        HIPCHK(hipStreamWaitEvent(e->s0, e->join[i], 0));
+    }
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamEndCapture(e->s0, &g));
+    e->graph = g;
+    HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    e->graph_steps = K;
+}
+
+// device copies of the state and history of one member (autotuning steps it and restores)
+struct Snapshot {
+    std::vector<void*> bufs;
+    bool have_hist = false;
+};
+
+std::vector<std::pair<void*, size_t>> state_spans(rbc_ctx* c) {
+    const size_t field = (size_t)c->dg.P * c->dg.L * c->csz(), vec = (size_t)c->dg.L * c->csz();
+    Buffers& b = c->b;
+    return {{b.X[0], field}, {b.X[1], field}, {b.X[2], field}, {b.XF, vec}, {b.XG, vec},
+            {b.Rp[0], field}, {b.Rp[1], field}, {b.Rp[2], field}, {b.RpF, vec}, {b.RpG, vec}};
+}
+
+void take(rbc_ctx* c, Snapshot& s) {
+    s.have_hist = c->have_hist;
+    for (auto& sp : state_spans(c)) {
+        void* p = nullptr;
+        HIPCHK(hipMalloc(&p, sp.second));
+        s.bufs.push_back(p);
+        HIPCHK(hipMemcpyAsync(p, sp.first, sp.second, hipMemcpyDeviceToDevice, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+void restore(rbc_ctx* c, Snapshot& s) {
+    auto spans = state_spans(c);
+    for (size_t i = 0; i < spans.size(); ++i)
+        HIPCHK(hipMemcpyAsync(spans[i].first, s.bufs[i], spans[i].second, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (void* p : s.bufs) hipFree(p);
+    s.bufs.clear();
+    c->have_hist = s.have_hist;
+}
+
+double time_steps(rbc_ctx* c, int reps) {
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    run_steps(c, 1);  // warm-up (and the Euler start)
+    HIPCHK(hipEventRecord(a, c->stream));
+    run_steps(c, reps);
+    HIPCHK(hipEventRecord(b, c->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    return ms / reps;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rbc_ensemble_create(int n, const rbc_params* params, rbc_ensemble** out) {
+    if (!out || n < 1 || !params) return RBC_ERR_ARG;
+    *out = nullptr;
+    rbc_ensemble* e = new rbc_ensemble;
+    e->device = params[0].device;
+    for (int i = 0; i < n; ++i) {
+        if (params[i].device != e->device) {
+            delete e;
+            return fail(nullptr, RBC_ERR_ARG, "all members of an ensemble must use one device");
+        }
+        rbc_ctx* c = nullptr;
+        const int rc = rbc_create(&params[i], &c);
+        if (rc != RBC_OK) {
+            const std::string msg = std::string("member ") + std::to_string(i) + ": " + rbc_last_error(nullptr);
+            delete e;
+            return fail(nullptr, rc, msg);
+        }
+        e->m.push_back(c);
+    }
+    try {
+        hipSetDevice(e->device);
+        HIPCHK(hipStreamCreateWithFlags(&e->s0, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming));
+        e->join.resize(n);
+        for (auto& j : e->join) HIPCHK(hipEventCreateWithFlags(&j, hipEventDisableTiming));
+    } catch (const HipError& err) {
+        const std::string msg = std::string(err.what) + ": " + hipGetErrorString(err.e);
+        delete e;
+        return fail(nullptr, RBC_ERR_HIP, msg);
+    }
+    *out = e;
+    return RBC_OK;
+}
+
+void rbc_ensemble_destroy(rbc_ensemble* e) { delete e; }
+
+const char* rbc_ensemble_last_error(const rbc_ensemble* e) { return e ? e->err.c_str() : ""; }
+
+int rbc_ensemble_size(const rbc_ensemble* e) { return e ? (int)e->m.size() : 0; }
+
+rbc_ctx* rbc_ensemble_member(rbc_ensemble* e, int i) {
+    if (!e || i < 0 || i >= (int)e->m.size()) return nullptr;
+    return e->m[i];
+}
+
+int rbc_ensemble_tune(rbc_ensemble* e, double* report) {
+    if (!e) return RBC_ERR_ARG;
+    // one search per size class (M, N, L, precision); the first member of a class is timed
+    // with every candidate (its state is restored afterwards) and the winner applies to all
+    std::map<std::tuple<int, int, int, int>, std::pair<int, int>> best;
+    std::map<std::tuple<int, int, int, int>, double> best_ms;
+    try {
+        hipSetDevice(e->device);
+        drop_graph(e);
+        for (rbc_ctx* c : e->m) {
+            const auto key = std::make_tuple(c->dg.M, c->dg.N, c->dg.L, c->prec);
+            if (best.count(key)) continue;
+            Snapshot snap;
+            take(c, snap);
+            const size_t cs = c->csz();
+            std::vector<int> txs;
+            for (int tx = 1; tx <= 16; tx *= 2)
+                if ((size_t)2 * tx * c->dg.N * cs <= 65536) txs.push_back(tx);
+            // coordinate search: y-FFT tile first (rows at 256 threads), then the row kernel
+            int btx = txs.back(), bth = 256;
+            double bms = 1e30;
+            for (int tx : txs) {
+                c->b.tune_ytx = tx;
+                c->b.tune_rows_threads = 256;
+                const double ms = time_steps(c, 3);
+                if (ms < bms) {
+                    bms = ms;
+                    btx = tx;
+                }
+            }
+            for (int th : {64, 128}) {
+                c->b.tune_ytx = btx;
+                c->b.tune_rows_threads = th;
+                const double ms = time_steps(c, 3);
+                if (ms < bms) {
+                    bms = ms;
+                    bth = th;
+                }
+            }
+            restore(c, snap);
+            best[key] = {btx, bth};
+            best_ms[key] = bms;
+        }
+        for (size_t i = 0; i < e->m.size(); ++i) {
+            rbc_ctx* c = e->m[i];
+            const auto key = std::make_tuple(c->dg.M, c->dg.N, c->dg.L, c->prec);
+            c->b.tune_ytx = best[key].first;
+            c->b.tune_rows_threads = best[key].second;
+            if (report) {
+                report[3 * i] = best[key].first;
+                report[3 * i + 1] = best[key].second;
+                report[3 * i + 2] = best_ms[key];
+            }
+        }
+    } catch (const HipError& err) {
+        return efail(e, RBC_ERR_HIP, std::string(err.what) + ": " + hipGetErrorString(err.e));
+    }
+    return RBC_OK;
+}
+
+int rbc_ensemble_step(rbc_ensemble* e, int nsteps, int steps_per_launch) {
+    if (!e || nsteps < 0 || steps_per_launch < 0) return RBC_ERR_ARG;
+    try {
+        hipSetDevice(e->device);
+        int left = nsteps;
+        if (left > 0) {
+            bool any_euler = false;
+            for (rbc_ctx* c : e->m) any_euler = any_euler || !c->have_hist;
+            if (any_euler) {
+                euler_starts(e);
+                left -= 1;
+            }
+        }
+        if (steps_per_launch == 0) {  // eager: every member's kernels launched one by one
+            for (int s = 0; s < left; ++s)
+                for (rbc_ctx* c : e->m) run_steps(c, 1);
+        } else {
+            if (left >= steps_per_launch) {
+                // the members' own streams must be idle before their work moves into the graph
+                for (rbc_ctx* c : e->m) HIPCHK(hipStreamSynchronize(c->stream));
+                if (!e->exec || e->graph_steps != steps_per_launch) capture(e, steps_per_launch);
+                for (; left >= steps_per_launch; left -= steps_per_launch) HIPCHK(hipGraphLaunch(e->exec, e->s0));
+                HIPCHK(hipStreamSynchronize(e->s0));
+            }
+            for (int s = 0; s < left; ++s)
+                for (rbc_ctx* c : e->m) run_steps(c, 1);
+        }
+        for (rbc_ctx* c : e->m) HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipStreamSynchronize(e->s0));
+    } catch (const HipError& err) {
+        return efail(e, RBC_ERR_HIP, std::string(err.what) + ": " + hipGetErrorString(err.e));
+    }
+    return RBC_OK;
+}
+
+}  // extern "C"

@@ -122,6 +122,10 @@ struct Buffers {
     void* evA = nullptr;  // [4][ev_nlt][ev_nks][64][KPL] floats (C', C, S, S')
     void* prA = nullptr;  // [5][pr_nks][64][KPL] floats (wC', wC'', wC, wS, wS')
     int ev_nlt = 0, ev_nks = 0, pr_nks = 0;
+    // launch shapes of the generic (non-512) FFT kernels; 0 = built-in choice.  Set by the
+    // ensemble autotuner; the arithmetic per element does not depend on them.
+    int tune_ytx = 0;           // kx columns per block of the y-FFTs
+    int tune_rows_threads = 0;  // threads per block of the row kernel (64, 128 or 256)
 };
 
 struct StepCoef {

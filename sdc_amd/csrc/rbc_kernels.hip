@@ -1415,7 +1415,7 @@ void Launch<T>::fft_y_inv(const DevGeom& g, const Buffers& b, hipStream_t s) {
                            (const cpx<float>*)b.lev, (const cpx<float>*)b.twN, (cpx<float>*)b.hin);
         return;
     }
-    const int TX = fft_tile(g.N, sizeof(cpx<T>));
+    const int TX = b.tune_ytx > 0 ? b.tune_ytx : fft_tile(g.N, sizeof(cpx<T>));
     dim3 grid((g.KX + TX - 1) / TX, g.nq, 6);
     const size_t sm = (size_t)2 * TX * g.N * sizeof(cpx<T>);
     hipLaunchKernelGGL(k_fft_y_inv<T>, grid, dim3(256), sm, s, g, TX, (const cpx<T>*)b.lev, b.rows,
@@ -1431,7 +1431,8 @@ void Launch<T>::rows(const DevGeom& g, const Buffers& b, hipStream_t s) {
         return;
     }
     const size_t sm = (size_t)8 * g.M * sizeof(cpx<T>);
-    hipLaunchKernelGGL(k_rows<T>, grid, dim3(256), sm, s, g, (const cpx<T>*)b.hin, (const cpx<T>*)b.twM,
+    const int nth = b.tune_rows_threads > 0 ? b.tune_rows_threads : 256;
+    hipLaunchKernelGGL(k_rows<T>, grid, dim3(nth), sm, s, g, (const cpx<T>*)b.hin, (const cpx<T>*)b.twM,
                        (cpx<T>*)b.hout);
 }
 
@@ -1450,7 +1451,7 @@ void Launch<T>::fft_y_fwd(const DevGeom& g, const Buffers& b, hipStream_t s) {
                            (const float*)b.ky, (cpx<float>*)b.lev, (cpx<float>*)b.fgw);
         return;
     }
-    const int TX = fft_tile(g.N, sizeof(cpx<T>));
+    const int TX = b.tune_ytx > 0 ? b.tune_ytx : fft_tile(g.N, sizeof(cpx<T>));
     dim3 grid((g.KX + TX - 1) / TX, g.nq, 7);
     const size_t sm = (size_t)2 * TX * g.N * sizeof(cpx<T>);
     hipLaunchKernelGGL(k_fft_y_fwd<T>, grid, dim3(256), sm, s, g, TX, (const cpx<T>*)b.hout, b.rows,

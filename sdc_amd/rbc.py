@@ -45,7 +45,9 @@ EXPORTS = ["rbc_create", "rbc_destroy", "rbc_last_error", "rbc_get_info", "rbc_s
            "rbc_velocity_grid", "rbc_velocity_at", "rbc_tracer_create", "rbc_tracer_destroy",
            "rbc_tracer_last_error", "rbc_tracer_init", "rbc_tracer_set_positions", "rbc_tracer_step",
            "rbc_tracer_get_positions", "rbc_tracer_current", "rbc_tracer_info", "rbc_tracer_moments",
-           "rbc_energy_spectrum", "rbc_energy_spectrum_ref"]
+           "rbc_energy_spectrum", "rbc_energy_spectrum_ref", "rbc_ensemble_create", "rbc_ensemble_destroy",
+           "rbc_ensemble_last_error", "rbc_ensemble_size", "rbc_ensemble_member", "rbc_ensemble_tune",
+           "rbc_ensemble_step"]
 
 TRACER_MODES = {"rk4_trilinear": 0, "spectral_rk2": 1}
 PART_INIT = {"random": 0, "center": 1, "center_sigma": 2, "diagonal": 3, "pairs": 4, "coord_sigma": 5}
@@ -103,6 +105,16 @@ def load_library(path=LIB_PATH):
     L.rbc_tracer_moments.argtypes = [vp, dp]
     L.rbc_energy_spectrum.argtypes = [vp, ctypes.c_int, dp, ip, dp]
     L.rbc_energy_spectrum_ref.argtypes = [vp, ctypes.c_double, ctypes.c_int, dp, ip]
+    L.rbc_ensemble_create.argtypes = [ctypes.c_int, ctypes.POINTER(rbc_params), ctypes.POINTER(vp)]
+    L.rbc_ensemble_destroy.argtypes = [vp]
+    L.rbc_ensemble_destroy.restype = None
+    L.rbc_ensemble_last_error.argtypes = [vp]
+    L.rbc_ensemble_last_error.restype = ctypes.c_char_p
+    L.rbc_ensemble_size.argtypes = [vp]
+    L.rbc_ensemble_member.argtypes = [vp, ctypes.c_int]
+    L.rbc_ensemble_member.restype = vp
+    L.rbc_ensemble_tune.argtypes = [vp, dp]
+    L.rbc_ensemble_step.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     _lib = L
     return L
 
@@ -124,17 +136,21 @@ def _paths(paths):
 class Stepper:
     """One RBC context on one GPU (≙ the operator set main.cu:431-454 builds)."""
 
-    def __init__(self, M, N, L, Lx, Ly, Ra, Pr, dt, precision="f64", nq=0, device=0):
+    def __init__(self, M, N, L, Lx, Ly, Ra, Pr, dt, precision="f64", nq=0, device=0, _handle=None):
         self.lib = load_library()
         self.M, self.N, self.L = M, N, L
         self.mox = M // 2 + 1
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         self.params = rbc_params(M, N, L, Lx, Ly, Ra, Pr, dt, prec, nq, device)
-        h = ctypes.c_void_p()
-        rc = self.lib.rbc_create(ctypes.byref(self.params), ctypes.byref(h))
-        if rc != 0:
-            raise RbcError(f"rbc_create failed ({rc}): {self.lib.rbc_last_error(None).decode()}")
-        self.h = h
+        self._owned = _handle is None
+        if _handle is None:
+            h = ctypes.c_void_p()
+            rc = self.lib.rbc_create(ctypes.byref(self.params), ctypes.byref(h))
+            if rc != 0:
+                raise RbcError(f"rbc_create failed ({rc}): {self.lib.rbc_last_error(None).decode()}")
+            self.h = h
+        else:
+            self.h = ctypes.c_void_p(_handle)
         nq_, kx, ky, ng = (ctypes.c_int() for _ in range(4))
         self.lib.rbc_get_info(self.h, ctypes.byref(nq_), ctypes.byref(kx), ctypes.byref(ky), ctypes.byref(ng))
         self.nq, self.kx_kept, self.ky_kept, self.n_unique_k = nq_.value, kx.value, ky.value, ng.value
@@ -145,7 +161,8 @@ class Stepper:
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.rbc_destroy(self.h)
+            if self._owned:
+                self.lib.rbc_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -331,3 +348,60 @@ class Tracer:
         out = np.zeros(4)
         self._check(self.lib.rbc_tracer_moments(self.h, _dp(out)), "rbc_tracer_moments")
         return out
+
+
+class Ensemble:
+    """Independent runs of uneven size on one device, stepped together through a captured HIP
+    graph (rbc_ensemble_*).  members[i] is a Stepper view of member i (owned by the ensemble)."""
+
+    def __init__(self, configs, device=0):
+        """configs: list of dicts with M, N, L, Lx, Ly, Ra, Pr, dt and optional precision, nq."""
+        self.lib = load_library()
+        n = len(configs)
+        arr = (rbc_params * n)()
+        for i, c in enumerate(configs):
+            prec = c.get("precision", "f32")
+            prec = PRECISIONS[prec] if isinstance(prec, str) else int(prec)
+            arr[i] = rbc_params(c["M"], c["N"], c["L"], c["Lx"], c["Ly"], c["Ra"], c["Pr"], c["dt"], prec,
+                                c.get("nq", 0), device)
+        h = ctypes.c_void_p()
+        rc = self.lib.rbc_ensemble_create(n, arr, ctypes.byref(h))
+        if rc != 0:
+            raise RbcError(f"rbc_ensemble_create failed ({rc}): {self.lib.rbc_last_error(None).decode()}")
+        self.h = h
+        self.members = []
+        for i, c in enumerate(configs):
+            mh = self.lib.rbc_ensemble_member(self.h, i)
+            self.members.append(Stepper(c["M"], c["N"], c["L"], c["Lx"], c["Ly"], c["Ra"], c["Pr"], c["dt"],
+                                        precision=c.get("precision", "f32"), device=device, _handle=mh))
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RbcError(f"{what} failed ({rc}): {self.lib.rbc_ensemble_last_error(self.h).decode()}")
+
+    def tune(self):
+        rep = np.zeros(3 * len(self.members))
+        self._check(self.lib.rbc_ensemble_tune(self.h, _dp(rep)), "rbc_ensemble_tune")
+        return rep.reshape(-1, 3)
+
+    def step(self, nsteps, steps_per_launch=10):
+        self._check(self.lib.rbc_ensemble_step(self.h, int(nsteps), int(steps_per_launch)), "rbc_ensemble_step")
+
+    def close(self):
+        if getattr(self, "h", None):
+            for m in self.members:
+                m.h = None
+            self.lib.rbc_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
