@@ -8,6 +8,7 @@ timing).  Prints one JSON line.
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f32|bf16x3|f64]
                   [--no-cpu] [--cpu-threads T]
   python bench.py --ensemble [--members 64] [--steps K] [--ens-k 10]   (batched workload)
+  python bench.py --closure [--closure-T 1.0] [--precision f32|bf16x3]   (learned closure)
 """
 import argparse
 import json
@@ -153,6 +154,35 @@ def run_ensemble(a):
     print(json.dumps(out))
 
 
+CLOSURE_FINE = dict(M=512, N=512, L=32, Lx=4.0, Ly=4.0, Ra=1e6, Pr=0.7, dt=5e-5)
+CLOSURE_COARSE = dict(M=128, N=128, L=16, Lx=4.0, Ly=4.0, Ra=1e6, Pr=0.7, dt=2e-4)
+
+
+def run_closure(a):
+    """The learned-closure experiment: plain 512^2 x 32 run (judge and data source) vs a
+    128^2 x 16 run with and without the trained closure over one time unit."""
+    from sdc_amd import closure
+    r = closure.run(CLOSURE_FINE, CLOSURE_COARSE, T=a.closure_T, spinup_steps=a.closure_spinup,
+                    samples=a.closure_samples, train_steps=a.closure_train_steps, batch=65536, lr=1e-3,
+                    fine_precision=a.precision, log=lambda *m: print("#", *m, file=sys.stderr, flush=True))
+    cc = r["closure_coarse"]
+    out = {
+        "metric": "closure vs plain Nu err & wall time (128^2x16 with closure vs plain 512^2x32, Ra=1e6)",
+        "value": round(cc["nu_rel_err"], 5), "unit": "relative Nu error", "higher_is_better": False,
+        "n_gpus": 1, "steps": r["fine"]["steps"], "warmup": a.closure_spinup,
+        "ms_per_step": round(1e3 * cc["wall_s"] / cc["steps"], 4), "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 weights+f32", "data": "synthetic (seeded theta ~ 1e-3 N(0,1) in every kept mode)",
+        "config": {"workload": f"RBC Ra=1e6 Pr=0.7 4x4x1, {a.closure_T} time unit(s) after "
+                               f"{a.closure_spinup} fine spin-up steps",
+                   "fine": CLOSURE_FINE, "coarse": CLOSURE_COARSE, "fine_precision": a.precision},
+        "speedup_wall_per_time_unit": r["speedup_wall_per_time_unit"],
+        "nu_fine": r["nu_fine"], "plain_coarse": r["plain_coarse"], "closure_coarse": cc,
+        "training": r["training"], "fallback_to_plain": r["fallback_to_plain"], "tolerance": r["tolerance"],
+        "fine": {k: v for k, v in r["fine"].items() if k != "nu_series"},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +197,11 @@ def main():
                     help="also run one device-to-device copy of MB mebibytes (a kernel of known traffic "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--ensemble", action="store_true", help="run the batched-ensemble workload instead")
+    ap.add_argument("--closure", action="store_true", help="run the learned-closure experiment instead")
+    ap.add_argument("--closure-T", type=float, default=1.0)
+    ap.add_argument("--closure-spinup", type=int, default=2000)
+    ap.add_argument("--closure-samples", type=int, default=10)
+    ap.add_argument("--closure-train-steps", type=int, default=2000)
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--ens-k", type=int, default=10, help="steps captured per graph launch")
     a = ap.parse_args()
@@ -179,6 +214,9 @@ def main():
 
     if a.ensemble:
         run_ensemble(a)
+        return
+    if a.closure:
+        run_closure(a)
         return
 
     import sdc_amd

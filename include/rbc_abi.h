@@ -155,7 +155,7 @@ int rbc_velocity_at(rbc_ctx* c, int n, const float* x, const float* y, const flo
 #define RBC_PART_DIAGONAL 3
 #define RBC_PART_PAIRS 4
 #define RBC_PART_COORD_SIGMA 5
-typedef struct rbc_tracer rbc_tracer;
+typedef struct rbc_tracer rbc_tracer;  /* destroy before its context */
 /* ≙ particle_tracer(M, N, L, Np, buffer, cube_length, Le) (particle_tracer.cu:7-44): all
  * particles start at the centre. */
 int rbc_tracer_create(rbc_ctx* c, int num_particles, int buffer_size, int mode, double diff_coeff,
@@ -209,6 +209,55 @@ int rbc_ensemble_size(const rbc_ensemble* e);
 rbc_ctx* rbc_ensemble_member(rbc_ensemble* e, int i);
 int rbc_ensemble_tune(rbc_ensemble* e, double* report);
 int rbc_ensemble_step(rbc_ensemble* e, int nsteps, int steps_per_launch);
+
+/* ---- Learned closure for coarse runs -------------------------------------------------------
+ * No counterpart in the reference.  A pointwise MLP (inputs u, v, w, theta, d_x theta,
+ * d_y theta, d_z theta, z; two tanh layers of 64; outputs r_theta, r_f, r_g; 4931 parameters,
+ * BF16 weights, FP32 arithmetic) attached to a coarse context.  The points are the coarse
+ * M x N grid at nz Gauss-Legendre levels (0: 2 L).
+ *   rbc_closure_sample        appends to the dataset (capacity_points) the inputs and the
+ *                             targets r of R = P_c N_fine(X_fine) - N_coarse(P_c X_fine) at the
+ *                             points, from a finer context's state (P_c: truncation to the
+ *                             coarse modes; the coarse context's state is not touched)
+ *   rbc_closure_restrict_state  coarse state <- P_c (fine state); AB2 history cleared
+ *   rbc_closure_fit_normalization  input mean / 1/std and target rms from the first n points
+ *   rbc_closure_train         Adam steps on random minibatches of dataset points
+ *                             [first, first + n) with the GPU forward/backward kernels;
+ *                             loss[steps] receives the batch losses (normalised MSE)
+ *   rbc_closure_gradient      gradient (and loss) of the normalised MSE over n given points
+ *   rbc_closure_predict       r at n given input points
+ *   rbc_closure_eval          normalised MSE of the model and of predicting 0 on dataset points
+ *   rbc_closure_enable        attach (on = 1) / detach the correction to the coarse context's
+ *                             steps (rbc_step then runs the corrected method)
+ *   rbc_closure_step          corrected steps in chunks of check_every; a chunk whose last
+ *                             correction has |dN| / |N| above max_ratio (or not finite) is redone
+ *                             with the plain method, which then continues (fallback)
+ *   rbc_closure_status        attached?, fallback taken?, last |dN| / |N| */
+typedef struct rbc_closure rbc_closure;  /* destroy before its coarse context */
+int rbc_closure_create(rbc_ctx* coarse, int nz, int capacity_points, rbc_closure** out);
+void rbc_closure_destroy(rbc_closure* cl);
+const char* rbc_closure_last_error(const rbc_closure* cl);
+int rbc_closure_info(const rbc_closure* cl, int* npts, int* nweights, int* count, int* nz);
+int rbc_closure_restrict_state(rbc_closure* cl, rbc_ctx* fine);
+int rbc_closure_sample(rbc_closure* cl, rbc_ctx* fine, int* count);
+int rbc_closure_points(rbc_closure* cl, rbc_ctx* fine, float* X, float* Y);
+int rbc_closure_dataset(rbc_closure* cl, int first, int n, float* X, float* Y);
+int rbc_closure_load_dataset(rbc_closure* cl, int n, const float* X, const float* Y); /* replaces it */
+int rbc_closure_clear_dataset(rbc_closure* cl);
+int rbc_closure_fit_normalization(rbc_closure* cl, int n);
+int rbc_closure_get_normalization(const rbc_closure* cl, float* in_mean, float* in_inv, float* out_scale);
+int rbc_closure_set_normalization(rbc_closure* cl, const float* in_mean, const float* in_inv,
+                                  const float* out_scale);
+int rbc_closure_get_weights(rbc_closure* cl, float* w);
+int rbc_closure_set_weights(rbc_closure* cl, const float* w);
+int rbc_closure_gradient(rbc_closure* cl, int n, const float* X, const float* Y, float* grad, double* loss);
+int rbc_closure_train(rbc_closure* cl, int steps, int batch, double lr, unsigned long long seed, int first,
+                      int n, double* loss);
+int rbc_closure_predict(rbc_closure* cl, int n, const float* X, float* Y);
+int rbc_closure_eval(rbc_closure* cl, int first, int n, double* mse_model, double* mse_zero);
+int rbc_closure_enable(rbc_closure* cl, int on, double max_ratio);
+int rbc_closure_step(rbc_closure* cl, int nsteps, int check_every);
+int rbc_closure_status(const rbc_closure* cl, int* active, int* fallback, double* last_ratio);
 
 #ifdef __cplusplus
 }

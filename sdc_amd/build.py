@@ -14,7 +14,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsdc_rbc.so")
 
 SOURCES = ["rbc_setup.cpp", "rbc_abi.cpp", "rbc_io.cpp", "rbc_ensemble.cpp", "rbc_kernels.hip",
-           "rbc_velocity.hip"]
+           "rbc_velocity.hip", "rbc_closure.hip"]
 ARCH = os.environ.get("SDC_AMD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -1,0 +1,146 @@
+"""Learned-closure workflow on one GPU: a plain fine run supplies training data and is the
+judge; a coarse run with the trained closure is compared with it and with the plain coarse
+run, and the closure is dropped (fallback to the plain coarse method) when its Nusselt
+number misses the fine run's by more than a tolerance.
+
+All compute runs in libsdc_rbc.so (rbc_step, rbc_closure_*); this module only sequences it.
+"""
+import time
+
+import numpy as np
+
+from .initial import random_theta
+from .rbc import Closure, Stepper
+
+
+def _mk(cfg, precision):
+    return Stepper(cfg["M"], cfg["N"], cfg["L"], cfg["Lx"], cfg["Ly"], cfg["Ra"], cfg["Pr"], cfg["dt"],
+                   precision=precision)
+
+
+def _record(st, nsteps, every, advance):
+    """Advance nsteps in chunks of `every` with advance(k); returns (times of the samples,
+    volume Nu, plate Nu (mean of both plates), summed E_h spectra, number of spectra, wall s)."""
+    t_nu, nu_v, nu_p = [], [], []
+    spec, nspec = None, 0
+    wall = 0.0
+    done = 0
+    while done < nsteps:
+        k = min(every, nsteps - done)
+        st.synchronize()
+        t0 = time.perf_counter()
+        advance(k)
+        st.synchronize()
+        wall += time.perf_counter() - t0
+        done += k
+        nu = st.nusselt()
+        t_nu.append(done)
+        nu_v.append(nu[2])
+        nu_p.append(0.5 * (nu[0] + nu[1]))
+        E, _ = st.energy_spectrum()
+        if spec is None:
+            spec = E.copy()
+        else:
+            n = min(len(spec), len(E))
+            spec = spec[:n] + E[:n]
+        nspec += 1
+    return np.array(t_nu), np.array(nu_v), np.array(nu_p), spec, nspec, wall
+
+
+def run(fine, coarse, *, T=1.0, spinup_steps=2000, samples=10, train_steps=2000, batch=65536, lr=1e-3,
+        nz=0, seed=3, tol=0.02, every=200, fine_precision="f32", coarse_precision="f32", max_ratio=10.0,
+        check_every=200, log=None):
+    """The whole experiment.  fine / coarse: dicts with M, N, L, Lx, Ly, Ra, Pr, dt (same box,
+    Ra, Pr).  Returns a dict of the measurements."""
+    say = log or (lambda *a: None)
+    out = {}
+    F = _mk(fine, fine_precision)
+    F.set_state(*random_theta(fine["M"], fine["N"], fine["L"], seed=seed, amp=1e-3))
+    t0 = time.perf_counter()
+    F.step_time(spinup_steps)
+    F.synchronize()
+    out["fine_spinup_s"] = time.perf_counter() - t0
+    say("spin-up done", F.nusselt())
+    Cc = _mk(coarse, coarse_precision)
+    n_f = int(round(T / fine["dt"]))
+    n_c = int(round(T / coarse["dt"]))
+    cl = Closure(Cc, nz=nz, capacity_points=0)  # probe the point count
+    npts = cl.npts
+    cl.close()
+    cl = Closure(Cc, nz=nz, capacity_points=samples * npts)
+    cl.restrict_state(F)
+    c0 = Cc.get_state()
+    # fine run over T, sampling training data at evenly spaced times of its first half
+    gap = max(1, (n_f // 2) // samples)
+    t_sample = 0.0
+    done_f = 0
+
+    def fine_advance(k):
+        # samples fall on multiples of `gap` inside the first half of the run
+        nonlocal t_sample, done_f
+        start = done_f
+        F.step_time(k)
+        done_f += k
+        for s in range(start // gap + 1, done_f // gap + 1):
+            if s <= samples and cl.count + npts <= samples * npts:
+                F.synchronize()
+                ts = time.perf_counter()
+                cl.sample(F)
+                t_sample += time.perf_counter() - ts
+    tf, nuf, nupf, specf, nsf, wall_f = _record(F, n_f, every, fine_advance)
+    wall_f -= t_sample
+    out["fine"] = dict(steps=n_f, wall_s=wall_f, s_per_time_unit=wall_f / T, nu_series=nuf.tolist())
+    say("fine run done", wall_f, "s; samples", cl.count // npts)
+    # training on the GPU
+    t0 = time.perf_counter()
+    cl.fit_normalization()
+    n_train = cl.count - npts if cl.count > npts else cl.count
+    loss = cl.train(train_steps, batch=batch, lr=lr, seed=seed, first=0, n=n_train)
+    train_s = time.perf_counter() - t0
+    mse, zero = cl.evaluate(n_train, cl.count - n_train) if cl.count > n_train else (float("nan"), float("nan"))
+    out["training"] = dict(points=int(cl.count), train_points=int(n_train), steps=train_steps, batch=batch,
+                           wall_s=train_s, loss_first=float(loss[0]), loss_last=float(np.mean(loss[-20:])),
+                           heldout_mse=mse, heldout_mse_no_closure=zero)
+    say("training done", out["training"])
+
+    def coarse_run(with_closure):
+        Cc.set_state(*c0)
+        Cc.reset_history()
+        if with_closure:
+            cl.enable(True, max_ratio=max_ratio)
+            adv = lambda k: cl.step(k, check_every=check_every)  # noqa: E731
+        else:
+            cl.enable(False)
+            adv = Cc.step_time
+        # record at the same simulated times as the fine run
+        r = _record(Cc, n_c, max(1, int(round(every * fine["dt"] / coarse["dt"]))), adv)
+        status = cl.status()
+        cl.enable(False)
+        return r, status
+
+    (tp, nup, nupp, specp, nsp, wall_p), _ = coarse_run(False)
+    (tc, nuc, nucp, specc, nsc, wall_c), status = coarse_run(True)
+    half = lambda a: a[len(a) // 2:]  # noqa: E731
+    nu_f, nu_p, nu_c = float(np.mean(half(nuf))), float(np.mean(half(nup))), float(np.mean(half(nuc)))
+    ef, ep, ec = specf / nsf, specp / nsp, specc / nsc
+    nb = min(len(ef), len(ep), len(ec))
+    kcut = max(2, nb - 1)
+
+    def spec_err(e):
+        return float(np.linalg.norm(e[1:kcut] - ef[1:kcut]) / np.linalg.norm(ef[1:kcut]))
+
+    out["plain_coarse"] = dict(steps=n_c, wall_s=wall_p, nu=nu_p, nu_rel_err=abs(nu_p - nu_f) / nu_f,
+                               spectrum_rel_err=spec_err(ep))
+    out["closure_coarse"] = dict(steps=n_c, wall_s=wall_c, nu=nu_c, nu_rel_err=abs(nu_c - nu_f) / nu_f,
+                                 spectrum_rel_err=spec_err(ec), runtime_fallback=status[1],
+                                 last_correction_ratio=status[2])
+    out["nu_fine"] = nu_f
+    out["speedup_wall_per_time_unit"] = wall_f / wall_c if wall_c > 0 else None
+    use_closure = (not status[1]) and np.isfinite(nu_c) and abs(nu_c - nu_f) / nu_f <= tol
+    out["tolerance"] = tol
+    out["fallback_to_plain"] = not use_closure
+    out["result_nu"] = nu_c if use_closure else nu_p
+    cl.close()
+    Cc.close()
+    F.close()
+    return out

@@ -145,6 +145,7 @@ void one_step(rbc_ctx* c, bool timed) {
     }
     if (timed) HIPCHK(hipEventRecord(c->ev[0], c->stream));
     nonlinear_into<T>(c, c->b.X, c->b.XF, c->b.XG, c->b.Nn, c->b.NF, c->b.NG, timed ? 0 : -1);
+    if (c->post_nonlinear) c->post_nonlinear(c, c->post_user);
     Launch<T>::update(c->dg, c->b, c->fac.ngroups(), c->fac.L0, c->fac.L1, c->fac.stride, c->p.Pr, c->p.dt, sc,
                       UPD_STEP, c->b.X, c->b.Nn, c->b.Rp, c->stream);
     if (timed) HIPCHK(hipEventRecord(c->ev[6], c->stream));
@@ -160,6 +161,12 @@ void one_step(rbc_ctx* c, bool timed) {
 }  // namespace
 
 namespace rbc {
+
+void nonlinear_op(rbc_ctx* c, void* const X[3], const void* XF, const void* XG, void* const Nout[3], void* NF,
+                  void* NG) {
+    if (c->rsz == 8) nonlinear_into<double>(c, X, XF, XG, Nout, NF, NG, -1);
+    else nonlinear_into<float>(c, X, XF, XG, Nout, NF, NG, -1);
+}
 
 void run_steps(rbc_ctx* c, int n) {
     for (int i = 0; i < n; ++i) {

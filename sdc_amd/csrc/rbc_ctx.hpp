@@ -45,11 +45,10 @@ struct rbc_ctx {
     hipEvent_t ev_ring[kRing][RBC_NSTAGES + 1]{};
     hipEvent_t* ev = ev_ring[0];                     // events of the step being recorded
     bool events = false;
-    // diagnostics tables (built on first use): wavenumber shell per compact mode
-    int* shell = nullptr;
-    int nshell = 0;
-    double* quadw = nullptr;  // nq Gauss-Legendre weights (FP64)
-    void* diagtab = nullptr;  // [l][3][L] C', C, S at the levels (FP64)
+    // called inside every time step right after the nonlinear term (b.Nn, b.NF, b.NG) is
+    // formed; the learned closure adds its correction there.  Null = plain method.
+    void (*post_nonlinear)(rbc_ctx*, void*) = nullptr;
+    void* post_user = nullptr;
 
     size_t csz() const { return 2 * rsz; }
     void* dalloc(size_t bytes) {
@@ -81,7 +80,11 @@ void get_field(rbc_ctx* c, const void* dev, double* host);
 void put_vec(rbc_ctx* c, const double* host, void* dev);
 void get_vec(rbc_ctx* c, const void* dev, double* host);
 
-// one IMEX step on the context's stream (no host synchronisation)
+// n IMEX steps on the context's stream (no host synchronisation)
 void run_steps(rbc_ctx* c, int n);
+
+// the nonlinear term of a state held in the context's device layout, on its stream
+void nonlinear_op(rbc_ctx* c, void* const X[3], const void* XF, const void* XG, void* const Nout[3], void* NF,
+                  void* NG);
 
 }  // namespace rbc

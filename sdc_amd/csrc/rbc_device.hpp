@@ -119,4 +119,32 @@ __global__ void __launch_bounds__(256) k_fft_y_inv(DevGeom g, int TX, const cpx<
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Stage 4: forward y-FFT of the seven product fields, keeping the kept rows only.
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_fft_y_fwd(DevGeom g, int TX, const cpx<T>* __restrict__ hout,
+                                                   const int* __restrict__ rows, const cpx<T>* __restrict__ twN,
+                                                   cpx<T>* __restrict__ prod) {
+    extern __shared__ unsigned char smem_raw[];
+    cpx<T>* a = reinterpret_cast<cpx<T>*>(smem_raw);
+    const int N = g.N, NYc = g.NYc, P = g.P, nq = g.nq;
+    cpx<T>* b = a + (size_t)TX * N;
+    const int kx0 = blockIdx.x * TX;
+    const int l = blockIdx.y, q = blockIdx.z;
+    const int ncol = min(TX, g.KX - kx0);
+    const cpx<T>* src = hout + ((size_t)q * nq + l) * (size_t)N * g.KXp;
+    for (int e = threadIdx.x; e < N * TX; e += blockDim.x) {
+        const int y = e / TX, c = e - y * TX;
+        a[c * N + y] = (c < ncol) ? src[(size_t)y * g.KXp + kx0 + c] : mk<T>(0, 0);
+    }
+    __syncthreads();
+    cpx<T>* r = lds_fft(a, b, g.logN, TX, twN, false);
+    cpx<T>* dst = prod + ((size_t)q * nq + l) * P;
+    for (int e = threadIdx.x; e < ncol * NYc; e += blockDim.x) {
+        const int c = e / NYc, kyc = e - c * NYc;
+        dst[(size_t)(kx0 + c) * NYc + kyc] = r[c * N + rows[kyc]];
+    }
+}
+
 }  // namespace rbc

@@ -14,6 +14,7 @@ There is no CPU fallback: if the library is missing or no GPU is present the cal
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -47,7 +48,15 @@ EXPORTS = ["rbc_create", "rbc_destroy", "rbc_last_error", "rbc_get_info", "rbc_s
            "rbc_tracer_get_positions", "rbc_tracer_current", "rbc_tracer_info", "rbc_tracer_moments",
            "rbc_energy_spectrum", "rbc_energy_spectrum_ref", "rbc_ensemble_create", "rbc_ensemble_destroy",
            "rbc_ensemble_last_error", "rbc_ensemble_size", "rbc_ensemble_member", "rbc_ensemble_tune",
-           "rbc_ensemble_step"]
+           "rbc_ensemble_step", "rbc_closure_create", "rbc_closure_destroy", "rbc_closure_last_error",
+           "rbc_closure_info", "rbc_closure_restrict_state", "rbc_closure_sample", "rbc_closure_points",
+           "rbc_closure_dataset", "rbc_closure_load_dataset", "rbc_closure_clear_dataset",
+           "rbc_closure_fit_normalization", "rbc_closure_get_normalization", "rbc_closure_set_normalization",
+           "rbc_closure_get_weights", "rbc_closure_set_weights", "rbc_closure_gradient", "rbc_closure_train",
+           "rbc_closure_predict", "rbc_closure_eval", "rbc_closure_enable", "rbc_closure_step",
+           "rbc_closure_status"]
+
+CLOSURE_NIN, CLOSURE_NOUT = 8, 3
 
 TRACER_MODES = {"rk4_trilinear": 0, "spectral_rk2": 1}
 PART_INIT = {"random": 0, "center": 1, "center_sigma": 2, "diagonal": 3, "pairs": 4, "coord_sigma": 5}
@@ -115,6 +124,31 @@ def load_library(path=LIB_PATH):
     L.rbc_ensemble_member.restype = vp
     L.rbc_ensemble_tune.argtypes = [vp, dp]
     L.rbc_ensemble_step.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    i_ = ctypes.c_int
+    L.rbc_closure_create.argtypes = [vp, i_, i_, ctypes.POINTER(vp)]
+    L.rbc_closure_destroy.argtypes = [vp]
+    L.rbc_closure_destroy.restype = None
+    L.rbc_closure_last_error.argtypes = [vp]
+    L.rbc_closure_last_error.restype = ctypes.c_char_p
+    L.rbc_closure_info.argtypes = [vp, ip, ip, ip, ip]
+    L.rbc_closure_restrict_state.argtypes = [vp, vp]
+    L.rbc_closure_sample.argtypes = [vp, vp, ip]
+    L.rbc_closure_points.argtypes = [vp, vp, fp, fp]
+    L.rbc_closure_dataset.argtypes = [vp, i_, i_, fp, fp]
+    L.rbc_closure_load_dataset.argtypes = [vp, i_, fp, fp]
+    L.rbc_closure_clear_dataset.argtypes = [vp]
+    L.rbc_closure_fit_normalization.argtypes = [vp, i_]
+    L.rbc_closure_get_normalization.argtypes = [vp, fp, fp, fp]
+    L.rbc_closure_set_normalization.argtypes = [vp, fp, fp, fp]
+    L.rbc_closure_get_weights.argtypes = [vp, fp]
+    L.rbc_closure_set_weights.argtypes = [vp, fp]
+    L.rbc_closure_gradient.argtypes = [vp, i_, fp, fp, fp, dp]
+    L.rbc_closure_train.argtypes = [vp, i_, i_, ctypes.c_double, ctypes.c_ulonglong, i_, i_, dp]
+    L.rbc_closure_predict.argtypes = [vp, i_, fp, fp]
+    L.rbc_closure_eval.argtypes = [vp, i_, i_, dp, dp]
+    L.rbc_closure_enable.argtypes = [vp, i_, ctypes.c_double]
+    L.rbc_closure_step.argtypes = [vp, i_, i_]
+    L.rbc_closure_status.argtypes = [vp, ip, ip, dp]
     _lib = L
     return L
 
@@ -143,6 +177,7 @@ class Stepper:
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         self.params = rbc_params(M, N, L, Lx, Ly, Ra, Pr, dt, prec, nq, device)
         self._owned = _handle is None
+        self._dependents = []  # tracers and closures, closed before this context
         if _handle is None:
             h = ctypes.c_void_p()
             rc = self.lib.rbc_create(ctypes.byref(self.params), ctypes.byref(h))
@@ -160,6 +195,11 @@ class Stepper:
             raise RbcError(f"{what} failed ({rc}): {self.lib.rbc_last_error(self.h).decode()}")
 
     def close(self):
+        for ref in getattr(self, "_dependents", []):
+            dep = ref()
+            if dep is not None:
+                dep.close()
+        self._dependents = []
         if getattr(self, "h", None):
             if self._owned:
                 self.lib.rbc_destroy(self.h)
@@ -297,6 +337,7 @@ class Tracer:
         if rc != 0:
             raise RbcError(f"rbc_tracer_create failed ({rc}): {self.lib.rbc_last_error(stepper.h).decode()}")
         self.h = h
+        stepper._dependents.append(weakref.ref(self))
 
     def _check(self, rc, what):
         if rc != 0:
@@ -390,7 +431,7 @@ class Ensemble:
     def close(self):
         if getattr(self, "h", None):
             for m in self.members:
-                m.h = None
+                m.close()  # closes its tracers / closures; the context itself belongs to the ensemble
             self.lib.rbc_ensemble_destroy(self.h)
             self.h = None
 
@@ -405,3 +446,136 @@ class Ensemble:
 
     def __exit__(self, *a):
         self.close()
+
+
+class Closure:
+    """Learned closure attached to a coarse Stepper (rbc_closure_*): dataset generation from a
+    finer Stepper, GPU training, and corrected coarse steps with fallback to the plain method."""
+
+    def __init__(self, coarse, nz=0, capacity_points=0):
+        self.lib = coarse.lib
+        self.coarse = coarse
+        h = ctypes.c_void_p()
+        rc = self.lib.rbc_closure_create(coarse.h, int(nz), int(capacity_points), ctypes.byref(h))
+        if rc != 0:
+            raise RbcError(f"rbc_closure_create failed ({rc}): {self.lib.rbc_last_error(coarse.h).decode()}")
+        self.h = h
+        coarse._dependents.append(weakref.ref(self))
+        a, b, c, d = (ctypes.c_int() for _ in range(4))
+        self.lib.rbc_closure_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d))
+        self.npts, self.nweights, self.nz = a.value, b.value, d.value
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RbcError(f"{what} failed ({rc}): {self.lib.rbc_closure_last_error(self.h).decode()}")
+
+    @property
+    def count(self):
+        c = ctypes.c_int()
+        self.lib.rbc_closure_info(self.h, None, None, ctypes.byref(c), None)
+        return c.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rbc_closure_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def restrict_state(self, fine):
+        self._check(self.lib.rbc_closure_restrict_state(self.h, fine.h), "rbc_closure_restrict_state")
+
+    def sample(self, fine):
+        n = ctypes.c_int()
+        self._check(self.lib.rbc_closure_sample(self.h, fine.h, ctypes.byref(n)), "rbc_closure_sample")
+        return n.value
+
+    def points(self, fine):
+        X = np.zeros((self.npts, CLOSURE_NIN), np.float32)
+        Y = np.zeros((self.npts, CLOSURE_NOUT), np.float32)
+        self._check(self.lib.rbc_closure_points(self.h, fine.h, _fp(X), _fp(Y)), "rbc_closure_points")
+        return X, Y
+
+    def dataset(self, first=0, n=None):
+        n = self.count - first if n is None else n
+        X = np.zeros((n, CLOSURE_NIN), np.float32)
+        Y = np.zeros((n, CLOSURE_NOUT), np.float32)
+        self._check(self.lib.rbc_closure_dataset(self.h, int(first), int(n), _fp(X), _fp(Y)), "rbc_closure_dataset")
+        return X, Y
+
+    def load_dataset(self, X, Y):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        self._check(self.lib.rbc_closure_load_dataset(self.h, len(X), _fp(X), _fp(Y)), "rbc_closure_load_dataset")
+
+    def clear_dataset(self):
+        self._check(self.lib.rbc_closure_clear_dataset(self.h), "rbc_closure_clear_dataset")
+
+    def fit_normalization(self, n=0):
+        self._check(self.lib.rbc_closure_fit_normalization(self.h, int(n)), "rbc_closure_fit_normalization")
+
+    def normalization(self):
+        m = np.zeros(CLOSURE_NIN, np.float32)
+        i = np.zeros(CLOSURE_NIN, np.float32)
+        s = np.zeros(CLOSURE_NOUT, np.float32)
+        self._check(self.lib.rbc_closure_get_normalization(self.h, _fp(m), _fp(i), _fp(s)),
+                    "rbc_closure_get_normalization")
+        return m, i, s
+
+    def set_normalization(self, mean, inv, scale):
+        m, i, s = (np.ascontiguousarray(a, np.float32) for a in (mean, inv, scale))
+        self._check(self.lib.rbc_closure_set_normalization(self.h, _fp(m), _fp(i), _fp(s)),
+                    "rbc_closure_set_normalization")
+
+    def weights(self):
+        w = np.zeros(self.nweights, np.float32)
+        self._check(self.lib.rbc_closure_get_weights(self.h, _fp(w)), "rbc_closure_get_weights")
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == self.nweights
+        self._check(self.lib.rbc_closure_set_weights(self.h, _fp(w)), "rbc_closure_set_weights")
+
+    def gradient(self, X, Y):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        g = np.zeros(self.nweights, np.float32)
+        loss = ctypes.c_double()
+        self._check(self.lib.rbc_closure_gradient(self.h, len(X), _fp(X), _fp(Y), _fp(g), ctypes.byref(loss)),
+                    "rbc_closure_gradient")
+        return g, loss.value
+
+    def train(self, steps, batch=65536, lr=1e-3, seed=0, first=0, n=0):
+        loss = np.zeros(max(1, steps))
+        self._check(self.lib.rbc_closure_train(self.h, int(steps), int(batch), float(lr), int(seed), int(first),
+                                               int(n), _dp(loss)), "rbc_closure_train")
+        return loss[:steps]
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((len(X), CLOSURE_NOUT), np.float32)
+        self._check(self.lib.rbc_closure_predict(self.h, len(X), _fp(X), _fp(Y)), "rbc_closure_predict")
+        return Y
+
+    def evaluate(self, first, n):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.rbc_closure_eval(self.h, int(first), int(n), ctypes.byref(a), ctypes.byref(b)),
+                    "rbc_closure_eval")
+        return a.value, b.value
+
+    def enable(self, on=True, max_ratio=1.0):
+        self._check(self.lib.rbc_closure_enable(self.h, int(bool(on)), float(max_ratio)), "rbc_closure_enable")
+
+    def step(self, nsteps, check_every=100):
+        self._check(self.lib.rbc_closure_step(self.h, int(nsteps), int(check_every)), "rbc_closure_step")
+
+    def status(self):
+        a, f, r = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        self._check(self.lib.rbc_closure_status(self.h, ctypes.byref(a), ctypes.byref(f), ctypes.byref(r)),
+                    "rbc_closure_status")
+        return bool(a.value), bool(f.value), r.value
