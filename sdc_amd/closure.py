@@ -18,7 +18,7 @@ def _mk(cfg, precision):
                    precision=precision)
 
 
-def _record(st, nsteps, every, advance):
+def _record(st, nsteps, every, advance, log=None, tag=""):
     """Advance nsteps in chunks of `every` with advance(k); returns (times of the samples,
     volume Nu, plate Nu (mean of both plates), summed E_h spectra, number of spectra, wall s)."""
     t_nu, nu_v, nu_p = [], [], []
@@ -44,6 +44,8 @@ def _record(st, nsteps, every, advance):
             n = min(len(spec), len(E))
             spec = spec[:n] + E[:n]
         nspec += 1
+        if log and nspec % 10 == 0:
+            log(tag, "step", done, "of", nsteps, "Nu", nu.tolist(), "wall", round(wall, 2))
     return np.array(t_nu), np.array(nu_v), np.array(nu_p), spec, nspec, wall
 
 
@@ -87,7 +89,7 @@ def run(fine, coarse, *, T=1.0, spinup_steps=2000, samples=10, train_steps=2000,
                 ts = time.perf_counter()
                 cl.sample(F)
                 t_sample += time.perf_counter() - ts
-    tf, nuf, nupf, specf, nsf, wall_f = _record(F, n_f, every, fine_advance)
+    tf, nuf, nupf, specf, nsf, wall_f = _record(F, n_f, every, fine_advance, say, "fine")
     wall_f -= t_sample
     out["fine"] = dict(steps=n_f, wall_s=wall_f, s_per_time_unit=wall_f / T, nu_series=nuf.tolist())
     say("fine run done", wall_f, "s; samples", cl.count // npts)
@@ -113,7 +115,8 @@ def run(fine, coarse, *, T=1.0, spinup_steps=2000, samples=10, train_steps=2000,
             cl.enable(False)
             adv = Cc.step_time
         # record at the same simulated times as the fine run
-        r = _record(Cc, n_c, max(1, int(round(every * fine["dt"] / coarse["dt"]))), adv)
+        r = _record(Cc, n_c, max(1, int(round(every * fine["dt"] / coarse["dt"]))), adv, say,
+                    "coarse+closure" if with_closure else "coarse")
         status = cl.status()
         cl.enable(False)
         return r, status

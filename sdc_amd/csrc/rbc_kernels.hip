@@ -752,75 +752,9 @@ __device__ __forceinline__ void fft512_wave(cpx<T> v[8], cpx<T>* sb, const cpx<T
     wave_sync();  // the scratch may be reused right after
 }
 
-// x-rows at M = 512: one wave per (level, y) row.  Same arithmetic as k_rows.
-__global__ void __launch_bounds__(64) k_rows512(DevGeom g, const cpx<float>* __restrict__ hin,
-                                                const cpx<float>* __restrict__ twM, cpx<float>* __restrict__ hout) {
-    __shared__ cpx<float> tw[512];
-    __shared__ cpx<float> sb[576];
-    const int lane = threadIdx.x;
-    const int KX = g.KX, KXp = g.KXp, nq = g.nq, N = g.N;
-    const int y = blockIdx.x, l = blockIdx.y;
-    const size_t fstride = (size_t)nq * N * KXp;
-    const size_t roff = ((size_t)l * N + y) * KXp;
-#pragma unroll
-    for (int r = 0; r < 8; ++r) tw[lane + 64 * r] = twM[lane + 64 * r];
-    __syncthreads();
-    cpx<float> z[3][8];
-#pragma unroll
-    for (int pr = 0; pr < 3; ++pr) {
-        const cpx<float>* x1 = hin + (size_t)(2 * pr) * fstride + roff;
-        const cpx<float>* x2 = hin + (size_t)(2 * pr + 1) * fstride + roff;
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            const int n = lane + 64 * r;
-            cpx<float> v = mk(0.f, 0.f);
-            if (n < KX) {
-                cpx<float> a = x1[n], b = x2[n];
-                if (n == 0) {
-                    a.y = 0.f;
-                    b.y = 0.f;
-                }
-                v = mk(a.x - b.y, a.y + b.x);  // X1 + i X2
-            } else if (n > 512 - KX) {
-                const cpx<float> a = x1[512 - n], b = x2[512 - n];
-                v = mk(a.x + b.y, -a.y + b.x);  // conj(X1) + i conj(X2)
-            }
-            z[pr][r] = v;
-        }
-        fft512_wave<float, true>(z[pr], sb, tw, lane);
-    }
-    const float s = 1.f / (512.f * (float)N);
-#pragma unroll
-    for (int pr = 0; pr < 4; ++pr) {
-        // products of this pair only, so that at most one product pair is live at a time
-        cpx<float> p[8];
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            const float u = z[0][r].x, v = z[0][r].y, w = z[1][r].x, tx = z[1][r].y, ty = z[2][r].x,
-                        tz = z[2][r].y;
-            if (pr == 0) p[r] = mk(s * u * u, s * u * v);
-            else if (pr == 1) p[r] = mk(s * v * v, s * u * w);
-            else if (pr == 2) p[r] = mk(s * v * w, s * w * w);
-            else p[r] = mk(s * (u * tx + v * ty + w * tz), 0.f);
-        }
-        fft512_wave<float, false>(p, sb, tw, lane);
-#pragma unroll
-        for (int r = 0; r < 8; ++r) sb[pad8(lane + 64 * r)] = p[r];
-        wave_sync();
-        cpx<float>* o1 = hout + (size_t)(2 * pr) * fstride + roff;
-        cpx<float>* o2 = hout + (size_t)(2 * pr + 1) * fstride + roff;
-        for (int k = lane; k < KX; k += 64) {
-            const cpx<float> zk = sb[pad8(k)], zm = sb[pad8((512 - k) & 511)];
-            o1[k] = mk(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-            if (pr < 3) o2[k] = mk(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
-        }
-        wave_sync();
-    }
-}
-
 // x-rows at M = 512 with four waves per (level, y) row: waves 0-2 run the three inverse
 // pair transforms into LDS slots, then wave w forms product pair w from the slots and runs its
-// forward transform, reusing slot w as scratch.  Same arithmetic as k_rows512.
+// forward transform, reusing slot w as scratch.  Same arithmetic as k_rows.
 __global__ void __launch_bounds__(256) k_rows512q(DevGeom g, const cpx<float>* __restrict__ hin,
                                                   const cpx<float>* __restrict__ twM, cpx<float>* __restrict__ hout) {
     __shared__ cpx<float> tw[512];
@@ -886,6 +820,125 @@ __global__ void __launch_bounds__(256) k_rows512q(DevGeom g, const cpx<float>* _
     }
 }
 
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+// power of two that brings the row maximum mb of the second field of a pair to that of the
+// first (ma), so that one complex transform carries both with the rounding of each set by its
+// own size; scaling by a power of two is exact
+__device__ __forceinline__ float pair_scale(float ma, float mb) {
+    if (!(mb > 0.f) || !(ma > 0.f)) return 1.f;
+    int ea, eb;
+    frexpf(ma, &ea);
+    frexpf(mb, &eb);
+    return ldexpf(1.f, max(-100, min(100, ea - eb)));
+}
+
+// x-rows at M = N = 512 for the fused forward y-FFT: three waves per (level, y) row.  Phase 1:
+// wave pr inverse-transforms the pair (2 pr, 2 pr + 1) of level fields.  Phase 2: the six
+// real product fields h = uu + vv - 2 ww, d = uu - vv, uv, uw, vw, u.grad(theta) go back in
+// pairs, one pair per wave, scaled by 1/(M N).  They are all the level combinations need:
+// kx^2 Puu + ky^2 Pvv - k^2 Pww = (k^2 / 2) h + ((kx^2 - ky^2) / 2) d and Puu - Pvv = d, with
+// d formed before any rounding of the larger ww (which would swamp uu - vv).  The second
+// field of every pair is scaled to the size of the first for the transform (pair_scale).
+__global__ void __launch_bounds__(192) k_rows512t(DevGeom g, const cpx<float>* __restrict__ hin,
+                                                  const cpx<float>* __restrict__ twM, cpx<float>* __restrict__ hout) {
+    __shared__ cpx<float> tw[512];
+    __shared__ cpx<float> zs[3][576];
+    const int lane = threadIdx.x & 63, pr = threadIdx.x >> 6;
+    const int KX = g.KX, KXp = g.KXp, nq = g.nq, N = g.N;
+    const int y = blockIdx.x, l = blockIdx.y;
+    const size_t fstride = (size_t)nq * N * KXp;
+    const size_t roff = ((size_t)l * N + y) * KXp;
+    for (int e = threadIdx.x; e < 512; e += 192) tw[e] = twM[e];
+    __syncthreads();
+    {
+        const cpx<float>* x1 = hin + (size_t)(2 * pr) * fstride + roff;
+        const cpx<float>* x2 = hin + (size_t)(2 * pr + 1) * fstride + roff;
+        cpx<float> av[4], bv[4];  // n = lane + 64 r < KX <= 192 for r < 3
+        float ma = 0.f, mb = 0.f;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int n = lane + 64 * r;
+            av[r] = bv[r] = mk(0.f, 0.f);
+            if (n < KX) {
+                av[r] = x1[n];
+                bv[r] = x2[n];
+                if (n == 0) {
+                    av[r].y = 0.f;
+                    bv[r].y = 0.f;
+                }
+            }
+            ma = fmaxf(ma, fmaxf(fabsf(av[r].x), fabsf(av[r].y)));
+            mb = fmaxf(mb, fmaxf(fabsf(bv[r].x), fabsf(bv[r].y)));
+        }
+        const float sg = pair_scale(wave_max(ma), wave_max(mb)), isg = 1.f / sg;
+        // spectrum n < KX from (av, bv) at r = n / 64; n > 512 - KX mirrors 512 - n < KX,
+        // which sits in lane (512 - n) % 64 of slot (512 - n) / 64 of another lane: read it
+        // from LDS after a staging pass
+        cpx<float>* st = zs[pr];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int n = lane + 64 * r;
+            if (n < KX) {
+                st[2 * n] = av[r];
+                st[2 * n + 1] = mk(sg * bv[r].x, sg * bv[r].y);
+            }
+        }
+        wave_sync();
+        cpx<float> v[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int n = lane + 64 * r;
+            cpx<float> z = mk(0.f, 0.f);
+            if (n < KX) {
+                const cpx<float> a = st[2 * n], b = st[2 * n + 1];
+                z = mk(a.x - b.y, a.y + b.x);  // X1 + i s X2
+            } else if (n > 512 - KX) {
+                const cpx<float> a = st[2 * (512 - n)], b = st[2 * (512 - n) + 1];
+                z = mk(a.x + b.y, -a.y + b.x);  // conj(X1) + i s conj(X2)
+            }
+            v[r] = z;
+        }
+        wave_sync();  // the staging area becomes the transform's scratch
+        fft512_wave<float, true>(v, zs[pr], tw, lane);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) zs[pr][pad8(lane + 64 * r)] = mk(v[r].x, isg * v[r].y);
+    }
+    __syncthreads();
+    const float s = 1.f / (512.f * (float)N);
+    cpx<float> p[8];
+    float ma = 0.f, mb = 0.f;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int i = pad8(lane + 64 * r);
+        const cpx<float> z0 = zs[0][i], z1 = zs[1][i], z2 = zs[2][i];
+        const float u = z0.x, v = z0.y, w = z1.x, tx = z1.y, ty = z2.x, tz = z2.y;
+        if (pr == 0) p[r] = mk(s * (u * u + v * v - 2.f * w * w), s * (u * u - v * v));
+        else if (pr == 1) p[r] = mk(s * u * v, s * u * w);
+        else p[r] = mk(s * v * w, s * (u * tx + v * ty + w * tz));
+        ma = fmaxf(ma, fabsf(p[r].x));
+        mb = fmaxf(mb, fabsf(p[r].y));
+    }
+    const float sg = pair_scale(wave_max(ma), wave_max(mb)), isg = 1.f / sg;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) p[r].y *= sg;
+    __syncthreads();  // every wave has read the slots before they become scratch
+    fft512_wave<float, false>(p, zs[pr], tw, lane);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) zs[pr][pad8(lane + 64 * r)] = p[r];
+    wave_sync();
+    cpx<float>* o1 = hout + (size_t)(2 * pr) * fstride + roff;
+    cpx<float>* o2 = hout + (size_t)(2 * pr + 1) * fstride + roff;
+    for (int k = lane; k < KX; k += 64) {
+        const cpx<float> zk = zs[pr][pad8(k)], zm = zs[pr][pad8((512 - k) & 511)];
+        o1[k] = mk(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+        o2[k] = mk(0.5f * isg * (zk.y + zm.y), -0.5f * isg * (zk.x - zm.x));
+    }
+}
+
 // y inverse at N = 512: a block of 8 waves handles YT = 8 kx columns of one (field, level);
 // each wave transforms one column.  The LDS tile [pad8(y)][YT + 1] holds the results and
 // doubles as the FFT scratch of each column; it is written out row-wise (64-byte segments).
@@ -928,45 +981,6 @@ __global__ void __launch_bounds__(512) k_fft_y_inv512(DevGeom g, int ntile, int 
     for (int e = threadIdx.x; e < 512 * YT; e += 512) {
         const int y = e / YT, c = e - y * YT;
         if (c < ncol) dst[(size_t)y * g.KXp + c] = tile[pad8(y) * YS + c];
-    }
-}
-
-// y forward at N = 512: the transpose of k_fft_y_inv512 (tile in, kept rows out).
-__global__ void __launch_bounds__(512) k_fft_y_fwd512(DevGeom g, int ntile, int nblk, int per,
-                                                      const cpx<float>* __restrict__ hout,
-                                                      const cpx<float>* __restrict__ tw,
-                                                      cpx<float>* __restrict__ prod) {
-    __shared__ cpx<float> tile[576 * YS];
-    const int lb = xcd_logical_block(blockIdx.x, per);
-    if (lb >= nblk) return;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int NYc = g.NYc, P = g.P, nq = g.nq, KX = g.KX;
-    const int tix = lb % ntile, rest = lb / ntile;
-    const int l = rest % nq, q = rest / nq;
-    const int kx0 = tix * YT;
-    const int a = (NYc - 1) / 2, shift = 512 - NYc;
-    const int ncol = min(YT, KX - kx0);
-    const cpx<float>* src = hout + ((size_t)q * nq + l) * (size_t)512 * g.KXp + kx0;
-    for (int e = threadIdx.x; e < 512 * YT; e += 512) {
-        const int y = e / YT, c = e - y * YT;
-        tile[pad8(y) * YS + c] = (c < ncol) ? src[(size_t)y * g.KXp + c] : mk(0.f, 0.f);
-    }
-    __syncthreads();
-    cpx<float>* dst = prod + ((size_t)q * nq + l) * P;
-    {
-        const int c = wave, kx = kx0 + c;  // one column per wave
-        cpx<float> v[8];
-#pragma unroll
-        for (int r = 0; r < 8; ++r) v[r] = tile[pad8(lane + 64 * r) * YS + c];
-        fft512_wave<float, false, YS>(v, tile + c, tw, lane);
-        if (kx < KX) {
-#pragma unroll
-            for (int r = 0; r < 8; ++r) {
-                const int n = lane + 64 * r;
-                if (n <= a) dst[(size_t)kx * NYc + n] = v[r];
-                else if (n >= 512 - a) dst[(size_t)kx * NYc + n - shift] = v[r];
-            }
-        }
     }
 }
 
@@ -1228,14 +1242,15 @@ __global__ void __launch_bounds__(256) k_project_mma(DevGeom g, int nks, const f
 template <int GRP>
 struct YGroup;
 template <>
-struct YGroup<0> {  // uu, uv, vv, ww -> D0, D2
-    static constexpr int n = 4, t0 = 0, t1 = 2;
-    __device__ static int q(int i) { return i == 3 ? 5 : i; }
+// product fields of k_rows512t: 0 h = uu + vv - 2 ww, 1 d = uu - vv, 2 uv, 3 uw, 4 vw, 5 u.grad(theta)
+// h, d, uv -> D0 = (k^2 / 2) h + ((kx^2 - ky^2) / 2) d + 2 kx ky uv, D2 = -kx ky d + (kx^2 - ky^2) uv
+struct YGroup<0> {
+    static constexpr int n = 3, t0 = 0, t1 = 2;
+    __device__ static int q(int i) { return i; }
     __device__ static void add(int i, cpx<float> V, float kx, float ky, cpx<float>& d0, cpx<float>& d1) {
-        if (i == 0) { d0 = d0 + cscale(V, kx * kx); d1 = d1 + cscale(V, -kx * ky); }
-        else if (i == 1) { d0 = d0 + cscale(V, 2.f * kx * ky); d1 = d1 + cscale(V, kx * kx - ky * ky); }
-        else if (i == 2) { d0 = d0 + cscale(V, ky * ky); d1 = d1 + cscale(V, kx * ky); }
-        else { d0 = d0 - cscale(V, kx * kx + ky * ky); }
+        if (i == 0) { d0 = d0 + cscale(V, 0.5f * (kx * kx + ky * ky)); }
+        else if (i == 1) { d0 = d0 + cscale(V, 0.5f * (kx * kx - ky * ky)); d1 = d1 + cscale(V, -kx * ky); }
+        else { d0 = d0 + cscale(V, 2.f * kx * ky); d1 = d1 + cscale(V, kx * kx - ky * ky); }
     }
 };
 template <>
@@ -1250,7 +1265,7 @@ struct YGroup<1> {  // uw, vw -> D1, D3
 template <>
 struct YGroup<2> {  // u.grad(theta) -> D4
     static constexpr int n = 1, t0 = 4, t1 = -1;
-    __device__ static int q(int) { return 6; }
+    __device__ static int q(int) { return 5; }
     __device__ static void add(int, cpx<float> V, float, float, cpx<float>& d0, cpx<float>&) { d0 = d0 + V; }
 };
 
@@ -1342,6 +1357,11 @@ template <typename T>
 static bool fast512_y(const DevGeom& g) {
     return sizeof(T) == 4 && g.N == 512 && !generic_fft_forced();
 }
+// x-rows and forward y-FFT at M = N = 512 share the six-field product layout of k_rows512t
+template <typename T>
+static bool fast512_pipe(const DevGeom& g) {
+    return fast512_rows<T>(g) && fast512_y<T>(g);
+}
 static inline int fft_tile(int N, size_t csz) {
     // two LDS buffers of TX columns; stay within 64 KiB
     int tx = (int)(65536 / (2 * (size_t)N * csz));
@@ -1397,6 +1417,11 @@ void Launch<T>::fft_y_inv(const DevGeom& g, const Buffers& b, hipStream_t s) {
 template <typename T>
 void Launch<T>::rows(const DevGeom& g, const Buffers& b, hipStream_t s) {
     dim3 grid(g.N, g.nq);
+    if (fast512_pipe<T>(g)) {
+        hipLaunchKernelGGL(k_rows512t, grid, dim3(192), 0, s, g, (const cpx<float>*)b.hin, (const cpx<float>*)b.twM,
+                           (cpx<float>*)b.hout);
+        return;
+    }
     if (fast512_rows<T>(g)) {
         hipLaunchKernelGGL(k_rows512q, grid, dim3(256), 0, s, g, (const cpx<float>*)b.hin, (const cpx<float>*)b.twM,
                            (cpx<float>*)b.hout);
@@ -1410,7 +1435,7 @@ void Launch<T>::rows(const DevGeom& g, const Buffers& b, hipStream_t s) {
 
 template <typename T>
 void Launch<T>::fft_y_fwd(const DevGeom& g, const Buffers& b, hipStream_t s) {
-    if (fast512_y<T>(g)) {
+    if (fast512_pipe<T>(g)) {
         const int ntile = (g.KX + YT - 1) / YT, nblk = ntile * g.nq, per = (nblk + 7) / 8;
         hipLaunchKernelGGL(k_fft_y_fwd512c<0>, dim3(8 * per), dim3(512), 0, s, g, ntile, nblk, per,
                            (const cpx<float>*)b.hout, (const cpx<float>*)b.twN, (const float*)b.kx,
