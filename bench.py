@@ -24,18 +24,19 @@ HBM_PEAK_GBS = 8000.0
 WORKLOAD = dict(M=512, N=512, L=32, Lx=4.0, Ly=4.0, Ra=1e6, Pr=0.7, dt=5e-5)
 
 
-def stage_bytes(st, cs):
-    """Algorithmic HBM bytes per launch of each stage (cs = bytes per complex value).
-    P = kept horizontal modes, H = N * KX hybrid row entries, nq levels, L modes."""
+def stage_bytes(st, cs, nprod):
+    """Algorithmic HBM bytes per step of each stage (cs = bytes per complex value).
+    P = kept horizontal modes, H = N * KX hybrid row entries, nq levels, L modes, nprod product
+    fields between the row and forward y stages (6 on the 512 x 512 FP32 kernels, 7 elsewhere)."""
     P = st.kx_kept * st.ky_kept
     H = WORKLOAD["N"] * st.kx_kept
     nq, L = st.nq, WORKLOAD["L"]
     return {
         "eval": (3 * L * P + 6 * nq * P) * cs,
         "fft_y_inv": (6 * nq * P + 6 * nq * H) * cs,
-        "rows_products": (6 * nq * H + 7 * nq * H) * cs,
-        # forward y-FFT reads the 7 product rows and writes the 5 level combinations
-        "fft_y_fwd": (7 * nq * H + 5 * nq * P) * cs,
+        "rows_products": (6 * nq * H + nprod * nq * H) * cs,
+        # forward y-FFT reads the product rows and writes the 5 level combinations
+        "fft_y_fwd": (nprod * nq * H + 5 * nq * P) * cs,
         "project": (5 * nq * P + 3 * L * P) * cs,
         # state, N and history in/out; factor records once per |k|^2 group
         "update": (9 * L * P + 6 * L * P) * cs + st.n_unique_k * (2 * L * L + L * L // 2) * (cs // 2),
@@ -154,8 +155,12 @@ def run_ensemble(a):
     print(json.dumps(out))
 
 
-CLOSURE_FINE = dict(M=512, N=512, L=32, Lx=4.0, Ly=4.0, Ra=1e6, Pr=0.7, dt=5e-5)
-CLOSURE_COARSE = dict(M=128, N=128, L=16, Lx=4.0, Ly=4.0, Ra=1e6, Pr=0.7, dt=2e-4)
+# The explicit AB2 advection limits the step: from a random theta at Ra = 1e6 the onset plumes
+# reach |u| ~ 500, and u dt k_max must stay well below 1 (k_max = 2 pi 171 / 4 at 512^2, 43
+# columns at 128^2).  512^2 x 32 is unstable at dt = 5e-5 and 1e-5 and stable at 2.5e-6; the
+# 128^2 x 16 grid is stable at 1e-5.
+CLOSURE_FINE = dict(M=512, N=512, L=32, Lx=4.0, Ly=4.0, Ra=1e6, Pr=0.7, dt=2.5e-6)
+CLOSURE_COARSE = dict(M=128, N=128, L=16, Lx=4.0, Ly=4.0, Ra=1e6, Pr=0.7, dt=1e-5)
 
 
 def run_closure(a):
@@ -163,7 +168,7 @@ def run_closure(a):
     128^2 x 16 run with and without the trained closure over one time unit."""
     from sdc_amd import closure
     r = closure.run(CLOSURE_FINE, CLOSURE_COARSE, T=a.closure_T, spinup_steps=a.closure_spinup,
-                    samples=a.closure_samples, train_steps=a.closure_train_steps, batch=65536, lr=1e-3,
+                    samples=a.closure_samples, train_steps=a.closure_train_steps, batch=65536, lr=1e-3, every=2000,
                     fine_precision=a.precision, log=lambda *m: print("#", *m, file=sys.stderr, flush=True))
     cc = r["closure_coarse"]
     out = {
@@ -198,8 +203,8 @@ def main():
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--ensemble", action="store_true", help="run the batched-ensemble workload instead")
     ap.add_argument("--closure", action="store_true", help="run the learned-closure experiment instead")
-    ap.add_argument("--closure-T", type=float, default=1.0)
-    ap.add_argument("--closure-spinup", type=int, default=2000)
+    ap.add_argument("--closure-T", type=float, default=0.2)
+    ap.add_argument("--closure-spinup", type=int, default=8000)
     ap.add_argument("--closure-samples", type=int, default=10)
     ap.add_argument("--closure-train-steps", type=int, default=2000)
     ap.add_argument("--members", type=int, default=64)
@@ -245,7 +250,8 @@ def main():
     value = a.steps / sec
 
     cs = 16 if a.precision == "f64" else 8
-    nbytes = stage_bytes(st, cs)
+    nprod = 7 if a.precision == "f64" else 6
+    nbytes = stage_bytes(st, cs, nprod)
     dom = max(stages, key=lambda k: stages[k])
     achieved = nbytes[dom] / (stages[dom] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
