@@ -45,6 +45,25 @@ def stage_bytes(st, cs, nprod):
     }
 
 
+# kernels of each stage on the 512 x 512 FP32 path (BF16X3 runs the <1> variants of the two
+# matrix-core kernels, with the same memory traffic)
+STAGE_KERNELS = {"eval": ["k_eval_mma<0>"], "fft_y_inv": ["k_fft_y_inv512"], "rows_products": ["k_rows512t"],
+                 "fft_y_fwd": ["k_fft_y_fwd512c<0>", "k_fft_y_fwd512c<1>", "k_fft_y_fwd512c<2>"],
+                 "project": ["k_project_mma<0>"], "update": ["k_update32<float>"]}
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_512x512x32_f32.md")
+
+
+def measured_traffic(stage):
+    """HBM bytes per step of a stage from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (calibrated against a device copy); None when the stage's kernels are not in it."""
+    try:
+        txt = open(TRAFFIC_PROFILE).read()
+        table = json.loads(txt.split("<!-- traffic-bytes ", 1)[1].split(" -->", 1)[0])
+        return sum(table[k] for k in STAGE_KERNELS[stage])
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
 def copy_probe(mib):
     """One device-to-device copy of `mib` MiB by the HIP runtime (its blit kernel reads and writes
     exactly that many bytes), used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
@@ -254,8 +273,10 @@ def main():
     nbytes = stage_bytes(st, cs, nprod)
     dom = max(stages, key=lambda k: stages[k])
     achieved = nbytes[dom] / (stages[dom] * 1e-3) / 1e9
+    traffic = measured_traffic(dom) if a.precision != "f64" else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
+                "algorithmic_bytes": nbytes[dom],
                 "stage_ms": {k: round(v, 4) for k, v in stages.items()},
                 "stage_GBs": {k: round(nbytes[k] / (stages[k] * 1e-3) / 1e9, 1) for k in stages if stages[k] > 0}}
     nq, kx, ky, ng = st.nq, st.kx_kept, st.ky_kept, st.n_unique_k
