@@ -63,7 +63,7 @@ __host__ __device__ inline uint64_t mix64(uint64_t x) {
 // features at the levels: spec[f][l][p], f = u, v, w, theta, theta_x, theta_y, theta_z;
 // zb[l][4][L] = C', C, S, S' (FP64)
 template <typename T>
-__global__ void __launch_bounds__(256) k_feat_levels(DevGeom g, int nz, const double* __restrict__ zb,
+__global__ void __launch_bounds__(256) k_feat_levels(DevGeom g, int nz, const T* __restrict__ zb,
                                                      const cpx<T>* __restrict__ Xt, const cpx<T>* __restrict__ Xf,
                                                      const cpx<T>* __restrict__ Xg, const cpx<T>* __restrict__ XF,
                                                      const cpx<T>* __restrict__ XG, const T* __restrict__ kxa,
@@ -71,26 +71,25 @@ __global__ void __launch_bounds__(256) k_feat_levels(DevGeom g, int nz, const do
     const int p = blockIdx.x * blockDim.x + threadIdx.x, l = blockIdx.y;
     if (p >= g.P) return;
     const int L = g.L;
-    const double* tb = zb + (size_t)l * 4 * L;
-    cpx<double> A = mk<double>(0, 0), B = A, Cc = A, Ts = A, Tz = A;
+    const T* tb = zb + (size_t)l * 4 * L;
+    cpx<T> A = mk<T>(0, 0), B = A, Cc = A, Ts = A, Tz = A;
     for (int m = 0; m < L; ++m) {
         const size_t i = (size_t)p * L + m;
-        const cpx<double> f = mk<double>(Xf[i].x, Xf[i].y), gg = mk<double>(Xg[i].x, Xg[i].y);
-        const cpx<double> t = mk<double>(Xt[i].x, Xt[i].y);
+        const cpx<T> f = Xf[i], gg = Xg[i], t = Xt[i];
         A = A + cscale(f, tb[m]);
         Cc = Cc + cscale(f, tb[L + m]);
         B = B + cscale(gg, tb[2 * L + m]);
         Ts = Ts + cscale(t, tb[2 * L + m]);
         Tz = Tz + cscale(t, tb[3 * L + m]);
     }
-    const double kx = kxa[p], ky = kya[p];
-    cpx<double> u = imul(A, kx) + imul(B, ky), v = imul(A, ky) - imul(B, kx);
+    const T kx = kxa[p], ky = kya[p];
+    cpx<T> u = imul(A, kx) + imul(B, ky), v = imul(A, ky) - imul(B, kx);
     if (p == 0)
         for (int m = 0; m < L; ++m) {
-            u = u + cscale(mk<double>(XF[m].x, XF[m].y), tb[2 * L + m]);
-            v = v + cscale(mk<double>(XG[m].x, XG[m].y), tb[2 * L + m]);
+            u = u + cscale(XF[m], tb[2 * L + m]);
+            v = v + cscale(XG[m], tb[2 * L + m]);
         }
-    const cpx<double> vals[NFEAT] = {u, v, cscale(Cc, kx * kx + ky * ky), Ts, imul(Ts, kx), imul(Ts, ky), Tz};
+    const cpx<T> vals[NFEAT] = {u, v, cscale(Cc, kx * kx + ky * ky), Ts, imul(Ts, kx), imul(Ts, ky), Tz};
     const size_t plane = (size_t)nz * g.P;
 #pragma unroll
     for (int f = 0; f < NFEAT; ++f)
@@ -99,7 +98,7 @@ __global__ void __launch_bounds__(256) k_feat_levels(DevGeom g, int nz, const do
 
 // residual fields at the levels: spec[f][l][p] for (Nt - No) of theta (S), f (C), g (S)
 template <typename T>
-__global__ void __launch_bounds__(256) k_res_levels(DevGeom g, int nz, const double* __restrict__ zb,
+__global__ void __launch_bounds__(256) k_res_levels(DevGeom g, int nz, const T* __restrict__ zb,
                                                     const cpx<T>* __restrict__ Nt0, const cpx<T>* __restrict__ Nt1,
                                                     const cpx<T>* __restrict__ Nt2, const cpx<T>* __restrict__ No0,
                                                     const cpx<T>* __restrict__ No1, const cpx<T>* __restrict__ No2,
@@ -107,13 +106,13 @@ __global__ void __launch_bounds__(256) k_res_levels(DevGeom g, int nz, const dou
     const int p = blockIdx.x * blockDim.x + threadIdx.x, l = blockIdx.y;
     if (p >= g.P) return;
     const int L = g.L;
-    const double* tb = zb + (size_t)l * 4 * L;
-    cpx<double> a = mk<double>(0, 0), b = a, c = a;
+    const T* tb = zb + (size_t)l * 4 * L;
+    cpx<T> a = mk<T>(0, 0), b = a, c = a;
     for (int m = 0; m < L; ++m) {
         const size_t i = (size_t)p * L + m;
-        a = a + cscale(mk<double>((double)Nt0[i].x - No0[i].x, (double)Nt0[i].y - No0[i].y), tb[2 * L + m]);
-        b = b + cscale(mk<double>((double)Nt1[i].x - No1[i].x, (double)Nt1[i].y - No1[i].y), tb[L + m]);
-        c = c + cscale(mk<double>((double)Nt2[i].x - No2[i].x, (double)Nt2[i].y - No2[i].y), tb[2 * L + m]);
+        a = a + cscale(Nt0[i] - No0[i], tb[2 * L + m]);
+        b = b + cscale(Nt1[i] - No1[i], tb[L + m]);
+        c = c + cscale(Nt2[i] - No2[i], tb[2 * L + m]);
     }
     const size_t plane = (size_t)nz * g.P;
     spec[(size_t)l * g.P + p] = mk<float>((float)a.x, (float)a.y);
@@ -183,7 +182,7 @@ __global__ void __launch_bounds__(256) k_anal_rows(DevGeom g, const float* __res
 // N: N_t += sum_l pw[0][m][l] s0_l, N_f += sum_l pw[1][m][l] s1_l, N_g += ...; pw holds
 // w_l psi_m(z_l) / <psi_m, psi_m>.  Per-block sums of |dN|^2 and |N|^2 go to guard.
 template <typename T>
-__global__ void __launch_bounds__(256) k_closure_project(DevGeom g, int nz, const double* __restrict__ pw,
+__global__ void __launch_bounds__(256) k_closure_project(DevGeom g, int nz, const T* __restrict__ pw,
                                                          const cpx<float>* __restrict__ spec, cpx<T>* __restrict__ N0,
                                                          cpx<T>* __restrict__ N1, cpx<T>* __restrict__ N2,
                                                          double* __restrict__ guard) {
@@ -197,18 +196,18 @@ __global__ void __launch_bounds__(256) k_closure_project(DevGeom g, int nz, cons
         cpx<T>* outs[3] = {N0, N1, N2};
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
-            const double* w = pw + ((size_t)f * L + m) * nz;
-            double ax = 0.0, ay = 0.0;
+            const T* w = pw + ((size_t)f * L + m) * nz;
+            T ax = 0, ay = 0;
             for (int l = 0; l < nz; ++l) {
                 const cpx<float> v = spec[f * plane + (size_t)l * P + p];
-                ax += w[l] * v.x;
-                ay += w[l] * v.y;
+                ax += w[l] * (T)v.x;
+                ay += w[l] * (T)v.y;
             }
             cpx<T> n = outs[f][e];
             nn += (double)n.x * n.x + (double)n.y * n.y;
-            dd += ax * ax + ay * ay;
-            n.x += (T)ax;
-            n.y += (T)ay;
+            dd += (double)ax * ax + (double)ay * ay;
+            n.x += ax;
+            n.y += ay;
             outs[f][e] = n;
         }
     }
@@ -240,38 +239,130 @@ __global__ void k_restrict(int Pc, int Lc, int Lf, const int* __restrict__ pmap,
 }
 
 // ---- the network ---------------------------------------------------------------------------
-// inference, one point per lane; weights (BF16) expanded to FP32 in LDS
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// tanh(x) = 1 - 2 / (e^{2x} + 1) with the hardware exponential and reciprocal (absolute error
+// ~1e-7; saturates to +-1 and propagates NaN)
+__device__ __forceinline__ float tanh_fast(float x) {
+    const float e = __expf(2.f * x);
+    return 1.f - 2.f * __builtin_amdgcn_rcpf(e + 1.f);
+}
+__device__ __forceinline__ __bf16 bf16_of(float x) {
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return __builtin_bit_cast(__bf16, (unsigned short)(u >> 16));
+}
+__device__ __forceinline__ float f32_of(__bf16 h) {
+    return __uint_as_float((unsigned)__builtin_bit_cast(unsigned short, h) << 16);
+}
+__device__ __forceinline__ __bf16 bf16_bits(uint16_t u) { return __builtin_bit_cast(__bf16, u); }
+
+// Inference on the matrix cores, 32 points per wave.  Both hidden layers are computed
+// transposed, H = W X^T (rows = hidden units, columns = points), with v_mfma_f32_32x32x16_bf16:
+// the weights are BF16 already and each FP32 activation goes in as a BF16 hi + lo pair (two
+// MFMAs), so the products keep FP32 accuracy.  The 32x32 accumulator of a lane holds column
+// l % 32 (its point) and rows 8 (r / 4) + 4 (l / 32) + r % 4, r < 16; those registers are
+// directly the next layer's B operand (lane half h supplies K elements 8 h .. 8 h + 7) if the
+// K order of the next A operand is permuted to match (the a2 fragments below).  The 3 outputs
+// are dot products over each lane's 32 hidden values plus one exchange between lane halves.
 __global__ void __launch_bounds__(256) k_mlp_infer(int npts, const uint16_t* __restrict__ wbf, Norm nm,
                                                    const float* __restrict__ X, float* __restrict__ Y) {
-    __shared__ float w[NW];
-    for (int i = threadIdx.x; i < NW; i += blockDim.x) w[i] = bf16_bits_to_f32(wbf[i]);
+    __shared__ bf16x8 a1s[2][64];
+    __shared__ bf16x8 a2s[2][4][64];
+    __shared__ float b1s[NH], b2s[NH], w3s[NOUT * NH], b3s[NOUT];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 2 * 64; i += blockDim.x) {
+        const int t = i >> 6, l = i & 63, h = l >> 5, row = 32 * t + (l & 31);
+        bf16x8 v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int k = 8 * h + e;
+            v[e] = k < NIN ? bf16_bits(wbf[OFF_W1 + row * NIN + k]) : bf16_of(0.f);
+        }
+        a1s[t][l] = v;
+    }
+    for (int i = tid; i < 2 * 4 * 64; i += blockDim.x) {
+        const int t2 = i >> 8, sblk = (i >> 6) & 3, l = i & 63, h = l >> 5, row = 32 * t2 + (l & 31);
+        bf16x8 v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int r = 8 * (sblk & 1) + e;
+            const int feat = 32 * (sblk >> 1) + 8 * (r >> 2) + 4 * h + (r & 3);
+            v[e] = bf16_bits(wbf[OFF_W2 + row * NH + feat]);
+        }
+        a2s[t2][sblk][l] = v;
+    }
+    for (int i = tid; i < NH; i += blockDim.x) {
+        b1s[i] = bf16_bits_to_f32(wbf[OFF_B1 + i]);
+        b2s[i] = bf16_bits_to_f32(wbf[OFF_B2 + i]);
+    }
+    for (int i = tid; i < NOUT * NH; i += blockDim.x) w3s[i] = bf16_bits_to_f32(wbf[OFF_W3 + i]);
+    if (tid < NOUT) b3s[tid] = bf16_bits_to_f32(wbf[OFF_B3 + tid]);
     __syncthreads();
-    const int pt = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pt >= npts) return;
-    float x[NIN];
+    const int lane = tid & 63, wave = tid >> 6, h = lane >> 5;
+    const int ngroups = (npts + 31) / 32;
+    for (int gi = blockIdx.x * 4 + wave; gi < ngroups; gi += gridDim.x * 4) {  // wave-uniform loop
+        const int pt = gi * 32 + (lane & 31);
+        const bool valid = pt < npts;
+        bf16x8 xh, xl;
 #pragma unroll
-    for (int i = 0; i < NIN; ++i) x[i] = (X[(size_t)pt * NIN + i] - nm.in_mean[i]) * nm.in_inv[i];
-    float h1[NH];
+        for (int e = 0; e < 8; ++e) {
+            const float v = (h == 0 && valid) ? (X[(size_t)pt * NIN + e] - nm.in_mean[e]) * nm.in_inv[e] : 0.f;
+            xh[e] = bf16_of(v);
+            xl[e] = bf16_of(v - f32_of(xh[e]));
+        }
+        f32x16 d1[2];
 #pragma unroll
-    for (int j = 0; j < NH; ++j) {
-        float a = w[OFF_B1 + j];
+        for (int t = 0; t < 2; ++t) {
+            f32x16 c = {};
+            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1s[t][lane], xh, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1s[t][lane], xl, c, 0, 0, 0);
 #pragma unroll
-        for (int i = 0; i < NIN; ++i) a += w[OFF_W1 + j * NIN + i] * x[i];
-        h1[j] = tanhf(a);
+            for (int r = 0; r < 16; ++r) c[r] = tanh_fast(c[r] + b1s[32 * t + 8 * (r >> 2) + 4 * h + (r & 3)]);
+            d1[t] = c;
+        }
+        f32x16 d2[2] = {};
+#pragma unroll
+        for (int sblk = 0; sblk < 4; ++sblk) {
+            bf16x8 hh, hl;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float v = d1[sblk >> 1][8 * (sblk & 1) + e];
+                hh[e] = bf16_of(v);
+                hl[e] = bf16_of(v - f32_of(hh[e]));
+            }
+#pragma unroll
+            for (int t2 = 0; t2 < 2; ++t2) {
+                d2[t2] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2s[t2][sblk][lane], hh, d2[t2], 0, 0, 0);
+                d2[t2] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2s[t2][sblk][lane], hl, d2[t2], 0, 0, 0);
+            }
+        }
+        float y0 = 0.f, y1 = 0.f, y2 = 0.f;
+#pragma unroll
+        for (int t2 = 0; t2 < 2; ++t2)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int feat = 32 * t2 + 8 * (r >> 2) + 4 * h + (r & 3);
+                const float hv = tanh_fast(d2[t2][r] + b2s[feat]);
+                y0 += w3s[feat] * hv;
+                y1 += w3s[NH + feat] * hv;
+                y2 += w3s[2 * NH + feat] * hv;
+            }
+        y0 += __shfl_xor(y0, 32);
+        y1 += __shfl_xor(y1, 32);
+        y2 += __shfl_xor(y2, 32);
+        if (h == 0 && valid) {
+            Y[(size_t)pt * NOUT] = (y0 + b3s[0]) * nm.out_scale[0];
+            Y[(size_t)pt * NOUT + 1] = (y1 + b3s[1]) * nm.out_scale[1];
+            Y[(size_t)pt * NOUT + 2] = (y2 + b3s[2]) * nm.out_scale[2];
+        }
     }
-    float y0 = w[OFF_B3], y1 = w[OFF_B3 + 1], y2 = w[OFF_B3 + 2];
-    for (int j = 0; j < NH; ++j) {
-        float a = w[OFF_B2 + j];
-#pragma unroll
-        for (int i = 0; i < NH; ++i) a += w[OFF_W2 + j * NH + i] * h1[i];
-        const float h = tanhf(a);
-        y0 += w[OFF_W3 + j] * h;
-        y1 += w[OFF_W3 + NH + j] * h;
-        y2 += w[OFF_W3 + 2 * NH + j] * h;
-    }
-    Y[(size_t)pt * NOUT] = y0 * nm.out_scale[0];
-    Y[(size_t)pt * NOUT + 1] = y1 * nm.out_scale[1];
-    Y[(size_t)pt * NOUT + 2] = y2 * nm.out_scale[2];
+}
+
+int infer_blocks(int npts) {
+    const int groups = (npts + 31) / 32;
+    return std::max(1, std::min(2048, (groups + 3) / 4));
 }
 
 __device__ __forceinline__ float wave_sum(float v) {
@@ -319,13 +410,13 @@ __global__ void __launch_bounds__(64) k_mlp_train(int count, int batch, int per,
         float a1 = b1;
 #pragma unroll
         for (int i = 0; i < NIN; ++i) a1 += w1[i] * x[i];
-        const float h1 = tanhf(a1);
+        const float h1 = tanh_fast(a1);
         sh1[j] = h1;
         wave_sync();
         float a2 = b2;
 #pragma unroll
         for (int i = 0; i < NH; ++i) a2 += w2[i] * sh1[i];
-        const float h2 = tanhf(a2);
+        const float h2 = tanh_fast(a2);
         float dy[NOUT];
         float d2s = 0.f;
 #pragma unroll
@@ -443,9 +534,9 @@ struct rbc_closure {
     std::vector<void*> owned;
     std::string err;
     // tables
-    double* zb = nullptr;             // [nz][4][L]
+    void* zb = nullptr;               // [nz][4][L] in the context's precision
     float* zl = nullptr;              // [nz]
-    double* pw = nullptr;             // [3][L][nz]
+    void* pw = nullptr;               // [3][L][nz] in the context's precision
     cpx<float>* twM = nullptr;
     cpx<float>* twN = nullptr;
     // scratch
@@ -512,7 +603,7 @@ template <typename T>
 void launch_feat_levels(rbc_closure* cl, void* const X[3], const void* XF, const void* XG) {
     rbc_ctx* c = cl->c;
     dim3 grid((c->dg.P + 255) / 256, cl->nz);
-    hipLaunchKernelGGL(k_feat_levels<T>, grid, dim3(256), 0, c->stream, cl->gz, cl->nz, (const double*)cl->zb,
+    hipLaunchKernelGGL(k_feat_levels<T>, grid, dim3(256), 0, c->stream, cl->gz, cl->nz, (const T*)cl->zb,
                        (const cpx<T>*)X[0], (const cpx<T>*)X[1], (const cpx<T>*)X[2], (const cpx<T>*)XF,
                        (const cpx<T>*)XG, (const T*)c->b.kx, (const T*)c->b.ky, cl->spec);
 }
@@ -546,7 +637,7 @@ void launch_project(rbc_closure* cl) {
     rbc_ctx* c = cl->c;
     const DevGeom& g = cl->gz;
     hipLaunchKernelGGL(k_closure_project<T>, dim3(cl->nguard), dim3(256), 0, c->stream, g, cl->nz,
-                       (const double*)cl->pw, (const cpx<float>*)cl->spec, (cpx<T>*)c->b.Nn[0], (cpx<T>*)c->b.Nn[1],
+                       (const T*)cl->pw, (const cpx<float>*)cl->spec, (cpx<T>*)c->b.Nn[0], (cpx<T>*)c->b.Nn[1],
                        (cpx<T>*)c->b.Nn[2], cl->guard);
 }
 
@@ -555,7 +646,7 @@ void apply_closure(rbc_ctx* c, void* user) {
     rbc_closure* cl = (rbc_closure*)user;
     if (!cl->enabled) return;
     features_of(cl, c->b.X, c->b.XF, c->b.XG, cl->feat);
-    hipLaunchKernelGGL(k_mlp_infer, dim3((cl->npts + 255) / 256), dim3(256), 0, c->stream, cl->npts,
+    hipLaunchKernelGGL(k_mlp_infer, dim3(infer_blocks(cl->npts)), dim3(256), 0, c->stream, cl->npts,
                        (const uint16_t*)cl->wbf, cl->nm, (const float*)cl->feat, cl->pred);
     const DevGeom& g = cl->gz;
     hipLaunchKernelGGL(k_anal_rows, dim3(g.N, cl->nz), dim3(256), (size_t)2 * NOUT * g.M * sizeof(cpx<float>), c->stream, g,
@@ -641,7 +732,7 @@ template <typename T>
 void launch_res_levels(rbc_closure* cl) {
     rbc_ctx* c = cl->c;
     dim3 grid((c->dg.P + 255) / 256, cl->nz);
-    hipLaunchKernelGGL(k_res_levels<T>, grid, dim3(256), 0, c->stream, cl->gz, cl->nz, (const double*)cl->zb,
+    hipLaunchKernelGGL(k_res_levels<T>, grid, dim3(256), 0, c->stream, cl->gz, cl->nz, (const T*)cl->zb,
                        (const cpx<T>*)cl->Nt[0], (const cpx<T>*)cl->Nt[1], (const cpx<T>*)cl->Nt[2],
                        (const cpx<T>*)c->b.Oo[0], (const cpx<T>*)c->b.Oo[1], (const cpx<T>*)c->b.Oo[2], cl->spec);
 }
@@ -735,11 +826,19 @@ int rbc_closure_create(rbc_ctx* c, int nz, int capacity_points, rbc_closure** ou
                 for (int l = 0; l < nz; ++l) pw[((size_t)f * L + m) * nz + l] = w[l] * B(kinds[f], m, l) * inv;
             }
         std::vector<float> zlf(z.begin(), z.end());
-        cl->zb = (double*)cl->al(zb.size() * sizeof(double));
-        cl->pw = (double*)cl->al(pw.size() * sizeof(double));
+        auto upload = [&](const std::vector<double>& v) {
+            void* d = cl->al(v.size() * c->rsz);
+            if (c->rsz == 8) {
+                HIPCHK(hipMemcpy(d, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+            } else {
+                std::vector<float> f(v.begin(), v.end());
+                HIPCHK(hipMemcpy(d, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+            }
+            return d;
+        };
+        cl->zb = upload(zb);
+        cl->pw = upload(pw);
         cl->zl = (float*)cl->al(zlf.size() * sizeof(float));
-        HIPCHK(hipMemcpy(cl->zb, zb.data(), zb.size() * sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(cl->pw, pw.data(), pw.size() * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(cl->zl, zlf.data(), zlf.size() * sizeof(float), hipMemcpyHostToDevice));
         const std::vector<float> tm = twiddles(g.M), tn = twiddles(g.N);
         cl->twM = (cpx<float>*)cl->al(tm.size() * sizeof(float));
@@ -1061,7 +1160,7 @@ int rbc_closure_predict(rbc_closure* cl, int n, const float* X, float* Y) {
         HIPCHK(hipMalloc(&dx, (size_t)n * NIN * sizeof(float)));
         HIPCHK(hipMalloc(&dy, (size_t)n * NOUT * sizeof(float)));
         HIPCHK(hipMemcpy(dx, X, (size_t)n * NIN * sizeof(float), hipMemcpyHostToDevice));
-        hipLaunchKernelGGL(k_mlp_infer, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, (const uint16_t*)cl->wbf,
+        hipLaunchKernelGGL(k_mlp_infer, dim3(infer_blocks(n)), dim3(256), 0, c->stream, n, (const uint16_t*)cl->wbf,
                            cl->nm, (const float*)dx, dy);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(Y, dy, (size_t)n * NOUT * sizeof(float), hipMemcpyDeviceToHost, c->stream));

@@ -1154,8 +1154,8 @@ __global__ void __launch_bounds__(256) k_eval_mma(DevGeom g, int nlt, int nks, i
 // operands are float components of the five level combinations (column = (mode, re/im), so a
 // lane reads one float and the lanes of a wave read 128 contiguous bytes per level).
 // prA: [5][NKS][64][KPL].  Block = 64 modes, wave = 16 modes (32 columns); rows m < L <= 32.
-template <int MM>
-__global__ void __launch_bounds__(256) k_project_mma(DevGeom g, int nks, const float* __restrict__ Dfl,
+template <int MM, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_project_mma(DevGeom g, int nks, const float* __restrict__ Dfl,
                                                      const cpx<float>* __restrict__ fgw, const float* __restrict__ prA,
                                                      const float* __restrict__ prtab, const float* __restrict__ kxa,
                                                      const float* __restrict__ kya, float Pr,
@@ -1164,12 +1164,12 @@ __global__ void __launch_bounds__(256) k_project_mma(DevGeom g, int nks, const f
                                                      cpx<float>* __restrict__ NG) {
     using MT = Mma<MM>;
     constexpr int OS = 66;               // staging row (2L <= 64 floats plus 2 to spread banks)
-    __shared__ float sO[4][3][16 * OS];  // per wave: [field][mode][OS] staging for the write-out
+    __shared__ float sO[WPB][3][16 * OS];  // per wave: [field][mode][OS] staging for the write-out
     const int L = g.L, P = g.P, nq = g.nq;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j = lane & 31, half = lane >> 5;
     const int col = wave * 32 + j;
-    const int p = blockIdx.x * 64 + (col >> 1), comp = col & 1;
+    const int p = blockIdx.x * 16 * WPB + (col >> 1), comp = col & 1;
     const bool pvalid = p < P;
     const int pc = pvalid ? p : 0;
     const float kx = kxa[pc], ky = kya[pc], k2 = kx * kx + ky * ky;
@@ -1219,7 +1219,7 @@ __global__ void __launch_bounds__(256) k_project_mma(DevGeom g, int nks, const f
     }
     wave_sync();
     // write 16 modes x L complex per field, contiguous in memory ([p][m] layout)
-    const int pw0 = blockIdx.x * 64 + wave * 16;
+    const int pw0 = blockIdx.x * 16 * WPB + wave * 16;
     for (int e = lane; e < 16 * 2 * L; e += 64) {
         const int lmode = e / (2 * L), q = e - lmode * 2 * L;
         const int pp = pw0 + lmode;
@@ -1464,16 +1464,22 @@ void Launch<T>::project(const DevGeom& g, const Buffers& b, double Pr, void* con
     constexpr int MCH = sizeof(T) == 8 ? 4 : 8;
     dim3 grid((g.P + 63) / 64);
     if (sizeof(T) == 4 && b.mma >= 0) {
-        if (b.mma == 0)
-            hipLaunchKernelGGL(k_project_mma<0>, grid, dim3(256), 0, s, g, b.pr_nks, (const float*)b.lev,
-                               (const cpx<float>*)b.fgw, (const float*)b.prA, (const float*)b.prtab,
-                               (const float*)b.kx, (const float*)b.ky, (float)Pr, (cpx<float>*)Nout[0],
-                               (cpx<float>*)Nout[1], (cpx<float>*)Nout[2], (cpx<float>*)NFo, (cpx<float>*)NGo);
-        else
-            hipLaunchKernelGGL(k_project_mma<1>, grid, dim3(256), 0, s, g, b.pr_nks, (const float*)b.lev,
-                               (const cpx<float>*)b.fgw, (const float*)b.prA, (const float*)b.prtab,
-                               (const float*)b.kx, (const float*)b.ky, (float)Pr, (cpx<float>*)Nout[0],
-                               (cpx<float>*)Nout[1], (cpx<float>*)Nout[2], (cpx<float>*)NFo, (cpx<float>*)NGo);
+        // one-wave blocks when 4-wave blocks would leave most of the 256 CUs idle
+        const bool small = (g.P + 63) / 64 < 512;
+        const dim3 gr = small ? dim3((g.P + 15) / 16) : grid, bl = small ? dim3(64) : dim3(256);
+#define RBC_PROJECT_MMA(MMV, WPBV)                                                                              \
+    hipLaunchKernelGGL((k_project_mma<MMV, WPBV>), gr, bl, 0, s, g, b.pr_nks, (const float*)b.lev,                \
+                       (const cpx<float>*)b.fgw, (const float*)b.prA, (const float*)b.prtab, (const float*)b.kx,   \
+                       (const float*)b.ky, (float)Pr, (cpx<float>*)Nout[0], (cpx<float>*)Nout[1],                \
+                       (cpx<float>*)Nout[2], (cpx<float>*)NFo, (cpx<float>*)NGo)
+        if (b.mma == 0) {
+            if (small) RBC_PROJECT_MMA(0, 1);
+            else RBC_PROJECT_MMA(0, 4);
+        } else {
+            if (small) RBC_PROJECT_MMA(1, 1);
+            else RBC_PROJECT_MMA(1, 4);
+        }
+#undef RBC_PROJECT_MMA
         return;
     }
     if (g.L <= 4 * MCH) {
