@@ -27,7 +27,7 @@ WORKLOAD = dict(M=512, N=512, L=32, Lx=4.0, Ly=4.0, Ra=1e6, Pr=0.7, dt=5e-5)
 def stage_bytes(st, cs, nprod):
     """Algorithmic HBM bytes per step of each stage (cs = bytes per complex value).
     P = kept horizontal modes, H = N * KX hybrid row entries, nq levels, L modes, nprod product
-    fields between the row and forward y stages (6 on the 512 x 512 FP32 kernels, 7 elsewhere)."""
+    fields between the row and forward y stages (h, d, uv, uw, vw, u.grad(theta))."""
     P = st.kx_kept * st.ky_kept
     H = WORKLOAD["N"] * st.kx_kept
     nq, L = st.nq, WORKLOAD["L"]
@@ -269,7 +269,7 @@ def main():
     value = a.steps / sec
 
     cs = 16 if a.precision == "f64" else 8
-    nprod = 7 if a.precision == "f64" else 6
+    nprod = 6
     nbytes = stage_bytes(st, cs, nprod)
     dom = max(stages, key=lambda k: stages[k])
     achieved = nbytes[dom] / (stages[dom] * 1e-3) / 1e9

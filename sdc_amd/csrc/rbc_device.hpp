@@ -51,32 +51,69 @@ inline __device__ __forceinline__ void wave_sync() {
 }
 
 // ---------------------------------------------------------------------------------------
-// Radix-2 Stockham FFT on nb sequences of length n = 2^logn held in LDS (a), with b as the
-// ping-pong buffer.  tw[k] = exp(-2 pi i k / n) (the first n/2 entries are used).  Unnormalised; inverse uses
-// conjugate twiddles.  All threads of the block take part; returns the buffer holding the
-// result.  Ends with a barrier.
+// Stockham FFT on nb sequences of length n = 2^logn held in LDS (a), with b as the ping-pong
+// buffer: radix-4 passes while two or more radix-2 stages remain, then one radix-2 pass if
+// logn is odd.  Pass with sub-length Ls and radix R: y[R j - (R-1) k + p Ls] =
+// sum_q x[j + q n/R] w^{q k} e^{-+2 pi i p q / R}, k = j mod Ls, w = exp(-+2 pi i / (R Ls)).
+// tw[k] = exp(-2 pi i k / n), k < n.  Unnormalised; inverse uses conjugate twiddles.  All
+// threads of the block take part; returns the buffer holding the result.  Ends with a barrier.
 // ---------------------------------------------------------------------------------------
 template <typename T>
 __device__ cpx<T>* lds_fft(cpx<T>* a, cpx<T>* b, int logn, int nb, const cpx<T>* __restrict__ tw,
                            bool inverse) {
-    const int n = 1 << logn, half = n >> 1;
-    const int total = nb * half;
-    for (int ls = 0; ls < logn; ++ls) {
-        const int Ls = 1 << ls;
-        const int tws = logn - 1 - ls;  // twiddle stride = n / (2 Ls)
-        for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-            const int s = idx >> (logn - 1);
-            const int j = idx & (half - 1);
-            const int k = j & (Ls - 1);
-            const cpx<T>* x = a + (s << logn);
-            cpx<T>* y = b + (s << logn);
-            cpx<T> u = x[j], v = x[j + half];
-            cpx<T> w = tw[k << tws];
-            if (inverse) w.y = -w.y;
-            cpx<T> t = cmul(v, w);
-            const int i0 = 2 * j - k;
-            y[i0] = u + t;
-            y[i0 + Ls] = u - t;
+    const int n = 1 << logn;
+    int Ls = 1;
+    for (int ls = 0; ls < logn;) {
+        if (logn - ls >= 2) {
+            const int quarter = n >> 2, total = nb * quarter;
+            const int tsh = logn - 2 - ls;  // twiddle stride n / (4 Ls) = 2^tsh
+            for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+                const int s = idx >> (logn - 2);
+                const int j = idx & (quarter - 1);
+                const int k = j & (Ls - 1);
+                const cpx<T>* x = a + (s << logn);
+                cpx<T>* y = b + (s << logn);
+                cpx<T> a0 = x[j], a1 = x[j + quarter], a2 = x[j + 2 * quarter], a3 = x[j + 3 * quarter];
+                if (k) {
+                    cpx<T> w1 = tw[k << tsh], w2 = tw[(2 * k) << tsh], w3 = tw[(3 * k) << tsh];
+                    if (inverse) {
+                        w1.y = -w1.y;
+                        w2.y = -w2.y;
+                        w3.y = -w3.y;
+                    }
+                    a1 = cmul(a1, w1);
+                    a2 = cmul(a2, w2);
+                    a3 = cmul(a3, w3);
+                }
+                const cpx<T> t0 = a0 + a2, t1 = a0 - a2, t2 = a1 + a3, t3 = a1 - a3;
+                const cpx<T> it3 = inverse ? mk(-t3.y, t3.x) : mk(t3.y, -t3.x);  // -+ i t3
+                const int base = 4 * j - 3 * k;
+                y[base] = t0 + t2;
+                y[base + Ls] = t1 + it3;
+                y[base + 2 * Ls] = t0 - t2;
+                y[base + 3 * Ls] = t1 - it3;
+            }
+            ls += 2;
+            Ls <<= 2;
+        } else {
+            const int half = n >> 1, total = nb * half;
+            const int tsh = logn - 1 - ls;
+            for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+                const int s = idx >> (logn - 1);
+                const int j = idx & (half - 1);
+                const int k = j & (Ls - 1);
+                const cpx<T>* x = a + (s << logn);
+                cpx<T>* y = b + (s << logn);
+                cpx<T> u = x[j], v = x[j + half];
+                cpx<T> w = tw[k << tsh];
+                if (inverse) w.y = -w.y;
+                const cpx<T> t = cmul(v, w);
+                const int i0 = 2 * j - k;
+                y[i0] = u + t;
+                y[i0 + Ls] = u - t;
+            }
+            ls += 1;
+            Ls <<= 1;
         }
         __syncthreads();
         cpx<T>* tmp = a;

@@ -111,18 +111,32 @@ __global__ void __launch_bounds__(256) k_eval(DevGeom g, int LG, int nblk, int p
 
 // ---------------------------------------------------------------------------------------
 // Stage 3: one (level, y) row per block: three complex inverse FFTs of length M carry the
-// six real fields in pairs (u+iv, w+i theta_x, theta_y+i theta_z); the seven products
-// uu uv vv uw vw ww and u.grad(theta), scaled by 1/(M N) as the reference scales before its
-// R2C (nonlinear...cu:252), go back through four forward FFTs (pairs) and are unpacked to
-// kx = 0..KX-1.  The imaginary parts of the kx = 0 inputs are dropped, as a C2R does.
+// six real fields in pairs (u+iv, w+i theta_x, theta_y+i theta_z); the six product fields
+// h = uu + vv - 2 ww, d = uu - vv, uv, uw, vw and u.grad(theta), scaled by 1/(M N) as the
+// reference scales before its R2C (nonlinear...cu:252), go back through three forward FFTs
+// (pairs) and are unpacked to kx = 0..KX-1.  They carry everything the level combinations need
+// (kx^2 Puu + ky^2 Pvv - k^2 Pww = (k^2 / 2) h + ((kx^2 - ky^2) / 2) d, Puu - Pvv = d).  The
+// imaginary parts of the kx = 0 inputs are dropped, as a C2R does.  In every product pair the
+// second field is scaled by a power of two to the size of the first (block-wide maxima), so each
+// keeps the rounding of its own size.
 // ---------------------------------------------------------------------------------------
+// power of two that brings a second pair member of row maximum mb to the first's ma (exact)
+__device__ __forceinline__ double pair_scale_d(double ma, double mb) {
+    if (!(mb > 0.0) || !(ma > 0.0)) return 1.0;
+    int ea, eb;
+    frexp(ma, &ea);
+    frexp(mb, &eb);
+    return ldexp(1.0, max(-200, min(200, ea - eb)));
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_rows(DevGeom g, const cpx<T>* __restrict__ hin,
                                               const cpx<T>* __restrict__ twM, cpx<T>* __restrict__ hout) {
     extern __shared__ unsigned char smem_raw[];
+    __shared__ int smaxb[6];
     const int M = g.M, N = g.N, KX = g.KX, KXp = g.KXp, nq = g.nq;
-    cpx<T>* a = reinterpret_cast<cpx<T>*>(smem_raw);  // 4 * M
-    cpx<T>* b = a + 4 * M;                            // 4 * M
+    cpx<T>* a = reinterpret_cast<cpx<T>*>(smem_raw);  // 3 * M
+    cpx<T>* b = a + 3 * M;                            // 3 * M
     const int y = blockIdx.x, l = blockIdx.y;
     const size_t fstride = (size_t)nq * N * KXp;
     const size_t roff = ((size_t)l * N + y) * KXp;
@@ -145,33 +159,63 @@ __global__ void __launch_bounds__(256) k_rows(DevGeom g, const cpx<T>* __restric
     cpx<T>* r = lds_fft(a, b, g.logM, 3, twM, true);
     cpx<T>* o = (r == a) ? b : a;
     const T s = T(1) / (T(M) * T(N));
+    // pass 1: row maxima of the six products (non-negative floats order like their bits)
+    if (threadIdx.x < 6) smaxb[threadIdx.x] = 0;
+    __syncthreads();
+    {
+        float pm[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        for (int x = threadIdx.x; x < M; x += blockDim.x) {
+            const cpx<T> z0 = r[x], z1 = r[M + x], z2 = r[2 * M + x];
+            const T u = z0.x, v = z0.y, w = z1.x, tx = z1.y, ty = z2.x, tz = z2.y;
+            pm[0] = fmaxf(pm[0], fabsf((float)(u * u + v * v - T(2) * w * w)));
+            pm[1] = fmaxf(pm[1], fabsf((float)(u * u - v * v)));
+            pm[2] = fmaxf(pm[2], fabsf((float)(u * v)));
+            pm[3] = fmaxf(pm[3], fabsf((float)(u * w)));
+            pm[4] = fmaxf(pm[4], fabsf((float)(v * w)));
+            pm[5] = fmaxf(pm[5], fabsf((float)(u * tx + v * ty + w * tz)));
+        }
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            float x = pm[i];
+#pragma unroll
+            for (int o2 = 32; o2 > 0; o2 >>= 1) x = fmaxf(x, __shfl_xor(x, o2));
+            if ((threadIdx.x & 63) == 0) atomicMax(&smaxb[i], __float_as_int(x));
+        }
+    }
+    __syncthreads();
+    T sg[3], isg[3];
+#pragma unroll
+    for (int pr = 0; pr < 3; ++pr) {
+        sg[pr] = (T)pair_scale_d(__int_as_float(smaxb[2 * pr]), __int_as_float(smaxb[2 * pr + 1]));
+        isg[pr] = T(1) / sg[pr];
+    }
+    // pass 2: the scaled product pairs
     for (int x = threadIdx.x; x < M; x += blockDim.x) {
         const cpx<T> z0 = r[x], z1 = r[M + x], z2 = r[2 * M + x];
         const T u = z0.x, v = z0.y, w = z1.x, tx = z1.y, ty = z2.x, tz = z2.y;
-        o[x] = mk(s * u * u, s * u * v);
-        o[M + x] = mk(s * v * v, s * u * w);
-        o[2 * M + x] = mk(s * v * w, s * w * w);
-        o[3 * M + x] = mk(s * (u * tx + v * ty + w * tz), T(0));
+        o[x] = mk(s * (u * u + v * v - T(2) * w * w), sg[0] * s * (u * u - v * v));
+        o[M + x] = mk(s * u * v, sg[1] * s * u * w);
+        o[2 * M + x] = mk(s * v * w, sg[2] * s * (u * tx + v * ty + w * tz));
     }
     __syncthreads();
     cpx<T>* spare = (o == a) ? b : a;
-    cpx<T>* R = lds_fft(o, spare, g.logM, 4, twM, false);
-    // unpack pair spectra: X1_k = (Z_k + conj Z_{M-k}) / 2, X2_k = -i (Z_k - conj Z_{M-k}) / 2
-    for (int e = threadIdx.x; e < 4 * KX; e += blockDim.x) {
+    cpx<T>* R = lds_fft(o, spare, g.logM, 3, twM, false);
+    // unpack pair spectra: X1_k = (Z_k + conj Z_{M-k}) / 2, X2_k = -i (Z_k - conj Z_{M-k}) / (2 s)
+    for (int e = threadIdx.x; e < 3 * KX; e += blockDim.x) {
         const int pr = e / KX, k = e - pr * KX;
         const cpx<T>* Z = R + pr * M;
         const cpx<T> zk = Z[k], zm = Z[(M - k) & (M - 1)];
         const cpx<T> x1 = mk(T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y));
-        const cpx<T> x2 = mk(T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x));
+        const cpx<T> x2 = mk(T(0.5) * isg[pr] * (zk.y + zm.y), T(-0.5) * isg[pr] * (zk.x - zm.x));
         hout[(size_t)(2 * pr) * fstride + roff + k] = x1;
-        if (pr < 3) hout[(size_t)(2 * pr + 1) * fstride + roff + k] = x2;
+        hout[(size_t)(2 * pr + 1) * fstride + roff + k] = x2;
     }
 }
 
 // ---------------------------------------------------------------------------------------
-// Stage 5: projection.  The seven product spectra of a level combine into five level
-// combinations (formed where (kx, ky) of the output are known: in the forward y-FFT, or by
-// k_combine on the generic path):
+// Stage 5: projection.  The six product spectra of a level (h, d, uv, uw, vw, u.grad(theta))
+// combine into five level combinations (formed where (kx, ky) of the output are known: in the
+// forward y-FFT, or by k_combine on the generic path):
 //   D0 = kx^2 Puu + 2 kx ky Puv + ky^2 Pvv - k^2 Pww         (tested with C')
 //   D1 = i kx Puw + i ky Pvw                                  (with C'' + k^2 C)
 //   D2 = -kx ky (Puu - Pvv) + (kx^2 - ky^2) Puv               (with S)
@@ -181,20 +225,20 @@ __global__ void __launch_bounds__(256) k_rows(DevGeom g, const cpx<T>* __restric
 // as a quadrature over the levels.  Outputs are scaled by -1 (theta) and -1/Pr (:1366-1370).
 // The mean flow needs P_uw, P_vw at k = 0 per level (fgw), which D1, D3 lose (times k).
 // ---------------------------------------------------------------------------------------
+// product fields: 0 h = uu + vv - 2 ww, 1 d = uu - vv, 2 uv, 3 uw, 4 vw, 5 u.grad(theta)
 template <typename T>
 __device__ __forceinline__ void combine_add(int q, cpx<T> V, T kx, T ky, cpx<T> D[5]) {
     switch (q) {
-        case 0: D[0] = D[0] + cscale(V, kx * kx); D[2] = D[2] + cscale(V, -kx * ky); break;               // uu
-        case 1: D[0] = D[0] + cscale(V, T(2) * kx * ky); D[2] = D[2] + cscale(V, kx * kx - ky * ky); break;  // uv
-        case 2: D[0] = D[0] + cscale(V, ky * ky); D[2] = D[2] + cscale(V, kx * ky); break;                 // vv
+        case 0: D[0] = D[0] + cscale(V, T(0.5) * (kx * kx + ky * ky)); break;                             // h
+        case 1: D[0] = D[0] + cscale(V, T(0.5) * (kx * kx - ky * ky)); D[2] = D[2] + cscale(V, -kx * ky); break;  // d
+        case 2: D[0] = D[0] + cscale(V, T(2) * kx * ky); D[2] = D[2] + cscale(V, kx * kx - ky * ky); break;  // uv
         case 3: D[1] = D[1] + imul(V, kx); D[3] = D[3] - imul(V, ky); break;                               // uw
         case 4: D[1] = D[1] + imul(V, ky); D[3] = D[3] + imul(V, kx); break;                               // vw
-        case 5: D[0] = D[0] - cscale(V, kx * kx + ky * ky); break;                                         // ww
         default: D[4] = D[4] + V; break;                                                                   // u.grad
     }
 }
 
-// generic path: seven product spectra [q][l][p] -> five combinations [t][l][p] and fgw
+// generic path: six product spectra [q][l][p] -> five combinations [t][l][p] and fgw
 template <typename T>
 __global__ void __launch_bounds__(256) k_combine(DevGeom g, const cpx<T>* __restrict__ prod,
                                                  const T* __restrict__ kxa, const T* __restrict__ kya,
@@ -208,7 +252,7 @@ __global__ void __launch_bounds__(256) k_combine(DevGeom g, const cpx<T>* __rest
 #pragma unroll
     for (int t = 0; t < 5; ++t) d[t] = mk<T>(0, 0);
 #pragma unroll
-    for (int q = 0; q < 7; ++q) combine_add(q, prod[q * fs + e], kx, ky, d);
+    for (int q = 0; q < 6; ++q) combine_add(q, prod[q * fs + e], kx, ky, d);
 #pragma unroll
     for (int t = 0; t < 5; ++t) D[t * fs + e] = d[t];
     if (p == 0) {
@@ -752,74 +796,6 @@ __device__ __forceinline__ void fft512_wave(cpx<T> v[8], cpx<T>* sb, const cpx<T
     wave_sync();  // the scratch may be reused right after
 }
 
-// x-rows at M = 512 with four waves per (level, y) row: waves 0-2 run the three inverse
-// pair transforms into LDS slots, then wave w forms product pair w from the slots and runs its
-// forward transform, reusing slot w as scratch.  Same arithmetic as k_rows.
-__global__ void __launch_bounds__(256) k_rows512q(DevGeom g, const cpx<float>* __restrict__ hin,
-                                                  const cpx<float>* __restrict__ twM, cpx<float>* __restrict__ hout) {
-    __shared__ cpx<float> tw[512];
-    __shared__ cpx<float> zs[4][576];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int KX = g.KX, KXp = g.KXp, nq = g.nq, N = g.N;
-    const int y = blockIdx.x, l = blockIdx.y;
-    const size_t fstride = (size_t)nq * N * KXp;
-    const size_t roff = ((size_t)l * N + y) * KXp;
-    for (int e = threadIdx.x; e < 512; e += 256) tw[e] = twM[e];
-    __syncthreads();
-    if (wave < 3) {
-        const int pr = wave;
-        const cpx<float>* x1 = hin + (size_t)(2 * pr) * fstride + roff;
-        const cpx<float>* x2 = hin + (size_t)(2 * pr + 1) * fstride + roff;
-        cpx<float> v[8];
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            const int n = lane + 64 * r;
-            cpx<float> z = mk(0.f, 0.f);
-            if (n < KX) {
-                cpx<float> a = x1[n], b = x2[n];
-                if (n == 0) {
-                    a.y = 0.f;
-                    b.y = 0.f;
-                }
-                z = mk(a.x - b.y, a.y + b.x);  // X1 + i X2
-            } else if (n > 512 - KX) {
-                const cpx<float> a = x1[512 - n], b = x2[512 - n];
-                z = mk(a.x + b.y, -a.y + b.x);  // conj(X1) + i conj(X2)
-            }
-            v[r] = z;
-        }
-        fft512_wave<float, true>(v, zs[pr], tw, lane);
-#pragma unroll
-        for (int r = 0; r < 8; ++r) zs[pr][pad8(lane + 64 * r)] = v[r];
-    }
-    __syncthreads();
-    const int pr = wave;
-    const float s = 1.f / (512.f * (float)N);
-    cpx<float> p[8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        const int i = pad8(lane + 64 * r);
-        const cpx<float> z0 = zs[0][i], z1 = zs[1][i], z2 = zs[2][i];
-        const float u = z0.x, v = z0.y, w = z1.x, tx = z1.y, ty = z2.x, tz = z2.y;
-        if (pr == 0) p[r] = mk(s * u * u, s * u * v);
-        else if (pr == 1) p[r] = mk(s * v * v, s * u * w);
-        else if (pr == 2) p[r] = mk(s * v * w, s * w * w);
-        else p[r] = mk(s * (u * tx + v * ty + w * tz), 0.f);
-    }
-    __syncthreads();  // every wave has read the slots before they become scratch
-    fft512_wave<float, false>(p, zs[pr], tw, lane);
-#pragma unroll
-    for (int r = 0; r < 8; ++r) zs[pr][pad8(lane + 64 * r)] = p[r];
-    wave_sync();
-    cpx<float>* o1 = hout + (size_t)(2 * pr) * fstride + roff;
-    cpx<float>* o2 = hout + (size_t)(2 * pr + 1) * fstride + roff;
-    for (int k = lane; k < KX; k += 64) {
-        const cpx<float> zk = zs[pr][pad8(k)], zm = zs[pr][pad8((512 - k) & 511)];
-        o1[k] = mk(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-        if (pr < 3) o2[k] = mk(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
-    }
-}
-
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
@@ -1357,11 +1333,7 @@ template <typename T>
 static bool fast512_y(const DevGeom& g) {
     return sizeof(T) == 4 && g.N == 512 && !generic_fft_forced();
 }
-// x-rows and forward y-FFT at M = N = 512 share the six-field product layout of k_rows512t
-template <typename T>
-static bool fast512_pipe(const DevGeom& g) {
-    return fast512_rows<T>(g) && fast512_y<T>(g);
-}
+
 static inline int fft_tile(int N, size_t csz) {
     // two LDS buffers of TX columns; stay within 64 KiB
     int tx = (int)(65536 / (2 * (size_t)N * csz));
@@ -1417,17 +1389,12 @@ void Launch<T>::fft_y_inv(const DevGeom& g, const Buffers& b, hipStream_t s) {
 template <typename T>
 void Launch<T>::rows(const DevGeom& g, const Buffers& b, hipStream_t s) {
     dim3 grid(g.N, g.nq);
-    if (fast512_pipe<T>(g)) {
+    if (fast512_rows<T>(g)) {
         hipLaunchKernelGGL(k_rows512t, grid, dim3(192), 0, s, g, (const cpx<float>*)b.hin, (const cpx<float>*)b.twM,
                            (cpx<float>*)b.hout);
         return;
     }
-    if (fast512_rows<T>(g)) {
-        hipLaunchKernelGGL(k_rows512q, grid, dim3(256), 0, s, g, (const cpx<float>*)b.hin, (const cpx<float>*)b.twM,
-                           (cpx<float>*)b.hout);
-        return;
-    }
-    const size_t sm = (size_t)8 * g.M * sizeof(cpx<T>);
+    const size_t sm = (size_t)6 * g.M * sizeof(cpx<T>);
     const int nth = b.tune_rows_threads > 0 ? b.tune_rows_threads : 256;
     hipLaunchKernelGGL(k_rows<T>, grid, dim3(nth), sm, s, g, (const cpx<T>*)b.hin, (const cpx<T>*)b.twM,
                        (cpx<T>*)b.hout);
@@ -1435,7 +1402,7 @@ void Launch<T>::rows(const DevGeom& g, const Buffers& b, hipStream_t s) {
 
 template <typename T>
 void Launch<T>::fft_y_fwd(const DevGeom& g, const Buffers& b, hipStream_t s) {
-    if (fast512_pipe<T>(g)) {
+    if (fast512_y<T>(g)) {
         const int ntile = (g.KX + YT - 1) / YT, nblk = ntile * g.nq, per = (nblk + 7) / 8;
         hipLaunchKernelGGL(k_fft_y_fwd512c<0>, dim3(8 * per), dim3(512), 0, s, g, ntile, nblk, per,
                            (const cpx<float>*)b.hout, (const cpx<float>*)b.twN, (const float*)b.kx,
@@ -1449,7 +1416,7 @@ void Launch<T>::fft_y_fwd(const DevGeom& g, const Buffers& b, hipStream_t s) {
         return;
     }
     const int TX = b.tune_ytx > 0 ? b.tune_ytx : fft_tile(g.N, sizeof(cpx<T>));
-    dim3 grid((g.KX + TX - 1) / TX, g.nq, 7);
+    dim3 grid((g.KX + TX - 1) / TX, g.nq, 6);
     const size_t sm = (size_t)2 * TX * g.N * sizeof(cpx<T>);
     hipLaunchKernelGGL(k_fft_y_fwd<T>, grid, dim3(256), sm, s, g, TX, (const cpx<T>*)b.hout, b.rows,
                        (const cpx<T>*)b.twN, (cpx<T>*)b.prod);
