@@ -176,10 +176,11 @@ def run_ensemble(a):
 
 # The explicit AB2 advection limits the step: from a random theta at Ra = 1e6 the onset plumes
 # reach |u| ~ 500, and u dt k_max must stay well below 1 (k_max = 2 pi 171 / 4 at 512^2, 43
-# columns at 128^2).  512^2 x 32 is unstable at dt = 5e-5 and 1e-5 and stable at 2.5e-6; the
-# 128^2 x 16 grid is stable at 1e-5.
-CLOSURE_FINE = dict(M=512, N=512, L=32, Lx=4.0, Ly=4.0, Ra=1e6, Pr=0.7, dt=2.5e-6)
-CLOSURE_COARSE = dict(M=128, N=128, L=16, Lx=4.0, Ly=4.0, Ra=1e6, Pr=0.7, dt=1e-5)
+# columns at 128^2).  Each grid runs at the largest step of a doubling search that stays stable
+# over 0.2 time units from the developed state: 512^2 x 32 is stable at 5e-6 and blows up at
+# 1e-5; 128^2 x 16 is stable at 2e-5 and blows up at 4e-5.
+CLOSURE_FINE = dict(M=512, N=512, L=32, Lx=4.0, Ly=4.0, Ra=1e6, Pr=0.7, dt=5e-6)
+CLOSURE_COARSE = dict(M=128, N=128, L=16, Lx=4.0, Ly=4.0, Ra=1e6, Pr=0.7, dt=2e-5)
 
 
 def run_closure(a):
@@ -188,7 +189,8 @@ def run_closure(a):
     from sdc_amd import closure
     r = closure.run(CLOSURE_FINE, CLOSURE_COARSE, T=a.closure_T, spinup_steps=a.closure_spinup,
                     samples=a.closure_samples, train_steps=a.closure_train_steps, batch=65536, lr=1e-3, every=2000,
-                    fine_precision=a.precision, log=lambda *m: print("#", *m, file=sys.stderr, flush=True))
+                    fine_precision=a.precision, closure_every=[int(k) for k in a.closure_every.split(",")],
+                    log=lambda *m: print("#", *m, file=sys.stderr, flush=True))
     cc = r["closure_coarse"]
     out = {
         "metric": "closure vs plain Nu err & wall time (128^2x16 with closure vs plain 512^2x32, Ra=1e6)",
@@ -201,6 +203,7 @@ def run_closure(a):
                    "fine": CLOSURE_FINE, "coarse": CLOSURE_COARSE, "fine_precision": a.precision},
         "speedup_wall_per_time_unit": r["speedup_wall_per_time_unit"],
         "nu_fine": r["nu_fine"], "plain_coarse": r["plain_coarse"], "closure_coarse": cc,
+        "closure_variants": r["closure_variants"],
         "training": r["training"], "fallback_to_plain": r["fallback_to_plain"], "tolerance": r["tolerance"],
         "fine": {k: v for k, v in r["fine"].items() if k != "nu_series"},
     }
@@ -222,7 +225,9 @@ def main():
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--ensemble", action="store_true", help="run the batched-ensemble workload instead")
     ap.add_argument("--closure", action="store_true", help="run the learned-closure experiment instead")
-    ap.add_argument("--closure-T", type=float, default=0.2)
+    ap.add_argument("--closure-T", type=float, default=1.0)
+    ap.add_argument("--closure-every", default="1", help="network intervals of the corrected runs, e.g. 1,2,4 "
+                    "(the first is reported)")
     ap.add_argument("--closure-spinup", type=int, default=8000)
     ap.add_argument("--closure-samples", type=int, default=10)
     ap.add_argument("--closure-train-steps", type=int, default=2000)

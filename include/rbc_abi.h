@@ -214,7 +214,7 @@ int rbc_ensemble_step(rbc_ensemble* e, int nsteps, int steps_per_launch);
  * No counterpart in the reference.  A pointwise MLP (inputs u, v, w, theta, d_x theta,
  * d_y theta, d_z theta, z; two tanh layers of 64; outputs r_theta, r_f, r_g; 4931 parameters,
  * BF16 weights, FP32 arithmetic) attached to a coarse context.  The points are the coarse
- * M x N grid at nz Gauss-Legendre levels (0: 2 L).
+ * M x N grid at nz Gauss-Legendre levels (0: 2 L; at most 512).
  *   rbc_closure_sample        appends to the dataset (capacity_points) the inputs and the
  *                             targets r of R = P_c N_fine(X_fine) - N_coarse(P_c X_fine) at the
  *                             points, from a finer context's state (P_c: truncation to the
@@ -229,6 +229,8 @@ int rbc_ensemble_step(rbc_ensemble* e, int nsteps, int steps_per_launch);
  *   rbc_closure_eval          normalised MSE of the model and of predicting 0 on dataset points
  *   rbc_closure_enable        attach (on = 1) / detach the correction to the coarse context's
  *                             steps (rbc_step then runs the corrected method)
+ *   rbc_closure_interval      evaluate the network on every `every`-th corrected step and add
+ *                             the last evaluated correction on the steps between (default 1)
  *   rbc_closure_step          corrected steps in chunks of check_every; a chunk whose last
  *                             correction has |dN| / |N| above max_ratio (or not finite) is redone
  *                             with the plain method, which then continues (fallback)
@@ -256,6 +258,7 @@ int rbc_closure_train(rbc_closure* cl, int steps, int batch, double lr, unsigned
 int rbc_closure_predict(rbc_closure* cl, int n, const float* X, float* Y);
 int rbc_closure_eval(rbc_closure* cl, int first, int n, double* mse_model, double* mse_zero);
 int rbc_closure_enable(rbc_closure* cl, int on, double max_ratio);
+int rbc_closure_interval(rbc_closure* cl, int every);
 int rbc_closure_step(rbc_closure* cl, int nsteps, int check_every);
 int rbc_closure_status(const rbc_closure* cl, int* active, int* fallback, double* last_ratio);
 

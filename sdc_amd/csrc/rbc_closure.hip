@@ -61,39 +61,72 @@ __host__ __device__ inline uint64_t mix64(uint64_t x) {
 
 // ---- point synthesis ---------------------------------------------------------------------
 // features at the levels: spec[f][l][p], f = u, v, w, theta, theta_x, theta_y, theta_z;
-// zb[l][4][L] = C', C, S, S' (FP64)
+// zb[l][4][L] = C', C, S, S' (FP64).  Block = MB modes (MB = 64, 32 or 16, so that LDS holds
+// them): their [MB][L] slabs of theta, f, g are staged in LDS as [field][m][mode] (coalesced
+// reads, each mode read once for all levels), the mean flow at every level is staged by the
+// block holding p = 0, and lane (mode, sub-level) of wave w then walks levels
+// (w 64 / MB + sub) + 256 / MB i.
 template <typename T>
-__global__ void __launch_bounds__(256) k_feat_levels(DevGeom g, int nz, const T* __restrict__ zb,
+__global__ void __launch_bounds__(256) k_feat_levels(DevGeom g, int nz, int MB, const T* __restrict__ zb,
                                                      const cpx<T>* __restrict__ Xt, const cpx<T>* __restrict__ Xf,
                                                      const cpx<T>* __restrict__ Xg, const cpx<T>* __restrict__ XF,
                                                      const cpx<T>* __restrict__ XG, const T* __restrict__ kxa,
                                                      const T* __restrict__ kya, cpx<float>* __restrict__ spec) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x, l = blockIdx.y;
-    if (p >= g.P) return;
-    const int L = g.L;
-    const T* tb = zb + (size_t)l * 4 * L;
-    cpx<T> A = mk<T>(0, 0), B = A, Cc = A, Ts = A, Tz = A;
-    for (int m = 0; m < L; ++m) {
-        const size_t i = (size_t)p * L + m;
-        const cpx<T> f = Xf[i], gg = Xg[i], t = Xt[i];
-        A = A + cscale(f, tb[m]);
-        Cc = Cc + cscale(f, tb[L + m]);
-        B = B + cscale(gg, tb[2 * L + m]);
-        Ts = Ts + cscale(t, tb[2 * L + m]);
-        Tz = Tz + cscale(t, tb[3 * L + m]);
-    }
-    const T kx = kxa[p], ky = kya[p];
-    cpx<T> u = imul(A, kx) + imul(B, ky), v = imul(A, ky) - imul(B, kx);
-    if (p == 0)
-        for (int m = 0; m < L; ++m) {
-            u = u + cscale(XF[m], tb[2 * L + m]);
-            v = v + cscale(XG[m], tb[2 * L + m]);
+    extern __shared__ unsigned char smem_raw[];
+    const int L = g.L, P = g.P;
+    cpx<T>* sm = reinterpret_cast<cpx<T>*>(smem_raw);  // [3][L][MB]
+    cpx<T>* smf = sm + 3 * L * MB;                      // [2][nz] mean flow (block 0)
+    const int p0 = blockIdx.x * MB, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int e = threadIdx.x; e < MB * L; e += blockDim.x) {
+        const int mode = e / L, m = e - mode * L, pp = p0 + mode;
+        cpx<T> zt = mk<T>(0, 0), zf = zt, zg = zt;
+        if (pp < P) {
+            const size_t gi = (size_t)pp * L + m;
+            zt = Xt[gi];
+            zf = Xf[gi];
+            zg = Xg[gi];
         }
-    const cpx<T> vals[NFEAT] = {u, v, cscale(Cc, kx * kx + ky * ky), Ts, imul(Ts, kx), imul(Ts, ky), Tz};
-    const size_t plane = (size_t)nz * g.P;
+        sm[(0 * L + m) * MB + mode] = zt;
+        sm[(1 * L + m) * MB + mode] = zf;
+        sm[(2 * L + m) * MB + mode] = zg;
+    }
+    if (p0 == 0)
+        for (int t = threadIdx.x; t < 2 * nz; t += blockDim.x) {
+            const int which = t / nz, l = t - which * nz;
+            const T* tb = zb + (size_t)l * 4 * L + 2 * L;
+            const cpx<T>* Xm = which ? XG : XF;
+            cpx<T> sum = mk<T>(0, 0);
+            for (int m = 0; m < L; ++m) sum = sum + cscale(Xm[m], tb[m]);
+            smf[t] = sum;
+        }
+    __syncthreads();
+    const int mode = lane % MB, sub = lane / MB, nsub = 64 / MB;
+    const int p = p0 + mode;
+    if (p >= P) return;
+    const T kx = kxa[p], ky = kya[p], k2 = kx * kx + ky * ky;
+    const size_t plane = (size_t)nz * P;
+    for (int l = wave * nsub + sub; l < nz; l += 4 * nsub) {
+        const T* tb = zb + (size_t)l * 4 * L;
+        cpx<T> A = mk<T>(0, 0), B = A, Cc = A, Ts = A, Tz = A;
+        for (int m = 0; m < L; ++m) {
+            const cpx<T> t = sm[(0 * L + m) * MB + mode], f = sm[(1 * L + m) * MB + mode],
+                         gg = sm[(2 * L + m) * MB + mode];
+            A = A + cscale(f, tb[m]);
+            Cc = Cc + cscale(f, tb[L + m]);
+            B = B + cscale(gg, tb[2 * L + m]);
+            Ts = Ts + cscale(t, tb[2 * L + m]);
+            Tz = Tz + cscale(t, tb[3 * L + m]);
+        }
+        cpx<T> u = imul(A, kx) + imul(B, ky), v = imul(A, ky) - imul(B, kx);
+        if (p == 0) {
+            u = u + smf[l];
+            v = v + smf[nz + l];
+        }
+        const cpx<T> vals[NFEAT] = {u, v, cscale(Cc, k2), Ts, imul(Ts, kx), imul(Ts, ky), Tz};
 #pragma unroll
-    for (int f = 0; f < NFEAT; ++f)
-        spec[f * plane + (size_t)l * g.P + p] = mk<float>((float)vals[f].x, (float)vals[f].y);
+        for (int f = 0; f < NFEAT; ++f)
+            spec[f * plane + (size_t)l * P + p] = mk<float>((float)vals[f].x, (float)vals[f].y);
+    }
 }
 
 // residual fields at the levels: spec[f][l][p] for (Nt - No) of theta (S), f (C), g (S)
@@ -180,36 +213,51 @@ __global__ void __launch_bounds__(256) k_anal_rows(DevGeom g, const float* __res
 
 // Projection of the level spectra of the three predicted fields onto the modes, added to
 // N: N_t += sum_l pw[0][m][l] s0_l, N_f += sum_l pw[1][m][l] s1_l, N_g += ...; pw holds
-// w_l psi_m(z_l) / <psi_m, psi_m>.  Per-block sums of |dN|^2 and |N|^2 go to guard.
+// w_l psi_m(z_l) / <psi_m, psi_m>.  One thread per (field, mode, m); the correction is also
+// kept in dN (reused by steps that do not re-evaluate the network).  Per-block sums of
+// |dN|^2 and |N|^2 go to guard.
 template <typename T>
 __global__ void __launch_bounds__(256) k_closure_project(DevGeom g, int nz, const T* __restrict__ pw,
                                                          const cpx<float>* __restrict__ spec, cpx<T>* __restrict__ N0,
                                                          cpx<T>* __restrict__ N1, cpx<T>* __restrict__ N2,
-                                                         double* __restrict__ guard) {
+                                                         cpx<T>* __restrict__ dN, double* __restrict__ guard) {
     __shared__ double sd[256], sn[256];
     const int L = g.L, P = g.P;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t PL = (size_t)P * L;
+    const size_t e3 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     double dd = 0.0, nn = 0.0;
-    if (e < P * L) {
-        const int p = e / L, m = e - p * L;
+    if (e3 < 3 * PL) {
+        const int f = (int)(e3 / PL);
+        const size_t e = e3 - f * PL;
+        const int p = (int)(e / L), m = (int)(e - (size_t)p * L);
         const size_t plane = (size_t)nz * P;
-        cpx<T>* outs[3] = {N0, N1, N2};
+        cpx<T>* out = f == 0 ? N0 : (f == 1 ? N1 : N2);
+        const T* w = pw + ((size_t)f * L + m) * nz;
+        T ax = 0, ay = 0;
+        // eight levels' loads in flight at a time; the sum runs in level order
+        for (int l0 = 0; l0 < nz; l0 += 8) {
+            cpx<float> v[8];
+            T wl[8];
 #pragma unroll
-        for (int f = 0; f < 3; ++f) {
-            const T* w = pw + ((size_t)f * L + m) * nz;
-            T ax = 0, ay = 0;
-            for (int l = 0; l < nz; ++l) {
-                const cpx<float> v = spec[f * plane + (size_t)l * P + p];
-                ax += w[l] * (T)v.x;
-                ay += w[l] * (T)v.y;
+            for (int u = 0; u < 8; ++u) {
+                const int l = l0 + u;
+                v[u] = l < nz ? spec[f * plane + (size_t)l * P + p] : mk<float>(0, 0);
+                wl[u] = l < nz ? w[l] : T(0);
             }
-            cpx<T> n = outs[f][e];
-            nn += (double)n.x * n.x + (double)n.y * n.y;
-            dd += (double)ax * ax + (double)ay * ay;
-            n.x += ax;
-            n.y += ay;
-            outs[f][e] = n;
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (l0 + u < nz) {
+                    ax += wl[u] * (T)v[u].x;
+                    ay += wl[u] * (T)v[u].y;
+                }
         }
+        cpx<T> n = out[e];
+        nn = (double)n.x * n.x + (double)n.y * n.y;
+        dd = (double)ax * ax + (double)ay * ay;
+        dN[e3] = mk<T>(ax, ay);
+        n.x += ax;
+        n.y += ay;
+        out[e] = n;
     }
     sd[threadIdx.x] = dd;
     sn[threadIdx.x] = nn;
@@ -225,6 +273,19 @@ __global__ void __launch_bounds__(256) k_closure_project(DevGeom g, int nz, cons
         guard[2 * blockIdx.x] = sd[0];
         guard[2 * blockIdx.x + 1] = sn[0];
     }
+}
+
+// N += dN (the last evaluated correction) on the steps between evaluations
+template <typename T>
+__global__ void __launch_bounds__(256) k_closure_reuse(size_t PL, const cpx<T>* __restrict__ dN,
+                                                       cpx<T>* __restrict__ N0, cpx<T>* __restrict__ N1,
+                                                       cpx<T>* __restrict__ N2) {
+    const size_t e3 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e3 >= 3 * PL) return;
+    const int f = (int)(e3 / PL);
+    const size_t e = e3 - f * PL;
+    cpx<T>* out = f == 0 ? N0 : (f == 1 ? N1 : N2);
+    out[e] = out[e] + dN[e3];
 }
 
 // truncation of one field from a finer context: b[pc][m] = a[pmap[pc]][m], m < Lc
@@ -248,11 +309,8 @@ __device__ __forceinline__ float tanh_fast(float x) {
     const float e = __expf(2.f * x);
     return 1.f - 2.f * __builtin_amdgcn_rcpf(e + 1.f);
 }
-__device__ __forceinline__ __bf16 bf16_of(float x) {
-    unsigned u = __float_as_uint(x);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return __builtin_bit_cast(__bf16, (unsigned short)(u >> 16));
-}
+// FP32 -> BF16, round to nearest even (one v_cvt_pk_bf16_f32 per pair on gfx950)
+__device__ __forceinline__ __bf16 bf16_of(float x) { return (__bf16)x; }
 __device__ __forceinline__ float f32_of(__bf16 h) {
     return __uint_as_float((unsigned)__builtin_bit_cast(unsigned short, h) << 16);
 }
@@ -546,6 +604,9 @@ struct rbc_closure {
     float* pred = nullptr;            // [npts][3]
     double* guard = nullptr;          // [nblk][2]
     int nguard = 0;
+    void* dN = nullptr;               // [3][P][L] last correction, the context's precision
+    int every = 1;                    // network evaluated on every `every`-th corrected step
+    int phase = 0;                    // corrected steps since enable
     // sampling
     int* pmap = nullptr;              // coarse compact mode -> fine compact mode (for the fine geometry below)
     int pmap_M = 0, pmap_N = 0, pmap_L = 0;
@@ -602,8 +663,12 @@ int chip(rbc_closure* cl, const HipError& e) {
 template <typename T>
 void launch_feat_levels(rbc_closure* cl, void* const X[3], const void* XF, const void* XG) {
     rbc_ctx* c = cl->c;
-    dim3 grid((c->dg.P + 255) / 256, cl->nz);
-    hipLaunchKernelGGL(k_feat_levels<T>, grid, dim3(256), 0, c->stream, cl->gz, cl->nz, (const T*)cl->zb,
+    const DevGeom& g = cl->gz;
+    int MB = 64;
+    while (MB > 16 && ((size_t)3 * g.L * MB + 2 * (size_t)cl->nz) * sizeof(cpx<T>) > 65536) MB /= 2;
+    const size_t sm = ((size_t)3 * g.L * MB + 2 * (size_t)cl->nz) * sizeof(cpx<T>);
+    hipLaunchKernelGGL(k_feat_levels<T>, dim3((g.P + MB - 1) / MB), dim3(256), sm, c->stream, g, cl->nz, MB,
+                       (const T*)cl->zb,
                        (const cpx<T>*)X[0], (const cpx<T>*)X[1], (const cpx<T>*)X[2], (const cpx<T>*)XF,
                        (const cpx<T>*)XG, (const T*)c->b.kx, (const T*)c->b.ky, cl->spec);
 }
@@ -638,13 +703,26 @@ void launch_project(rbc_closure* cl) {
     const DevGeom& g = cl->gz;
     hipLaunchKernelGGL(k_closure_project<T>, dim3(cl->nguard), dim3(256), 0, c->stream, g, cl->nz,
                        (const T*)cl->pw, (const cpx<float>*)cl->spec, (cpx<T>*)c->b.Nn[0], (cpx<T>*)c->b.Nn[1],
-                       (cpx<T>*)c->b.Nn[2], cl->guard);
+                       (cpx<T>*)c->b.Nn[2], (cpx<T>*)cl->dN, cl->guard);
+}
+
+template <typename T>
+void launch_reuse(rbc_closure* cl) {
+    rbc_ctx* c = cl->c;
+    const size_t PL = (size_t)cl->gz.P * cl->gz.L;
+    hipLaunchKernelGGL(k_closure_reuse<T>, dim3((unsigned)((3 * PL + 255) / 256)), dim3(256), 0, c->stream, PL,
+                       (const cpx<T>*)cl->dN, (cpx<T>*)c->b.Nn[0], (cpx<T>*)c->b.Nn[1], (cpx<T>*)c->b.Nn[2]);
 }
 
 // the correction inside a coarse step (registered as the context's post-nonlinear hook)
 void apply_closure(rbc_ctx* c, void* user) {
     rbc_closure* cl = (rbc_closure*)user;
     if (!cl->enabled) return;
+    if (cl->phase++ % cl->every != 0) {  // between evaluations: the last correction again
+        if (c->rsz == 8) launch_reuse<double>(cl);
+        else launch_reuse<float>(cl);
+        return;
+    }
     features_of(cl, c->b.X, c->b.XF, c->b.XG, cl->feat);
     hipLaunchKernelGGL(k_mlp_infer, dim3(infer_blocks(cl->npts)), dim3(256), 0, c->stream, cl->npts,
                        (const uint16_t*)cl->wbf, cl->nm, (const float*)cl->feat, cl->pred);
@@ -777,6 +855,7 @@ int rbc_closure_create(rbc_ctx* c, int nz, int capacity_points, rbc_closure** ou
     if (!c || !out) return RBC_ERR_ARG;
     *out = nullptr;
     if (nz <= 0) nz = 2 * c->dg.L;
+    if (nz > 512) return fail(c, RBC_ERR_ARG, "nz must be <= 512");
     if (capacity_points < 0) return fail(c, RBC_ERR_ARG, "capacity_points must be >= 0");
     // one LDS transform per input field and row: 2 * 7 * M complex floats within 64 KiB
     if ((size_t)2 * NFEAT * c->dg.M * sizeof(cpx<float>) > 65536)
@@ -849,8 +928,9 @@ int rbc_closure_create(rbc_ctx* c, int nz, int capacity_points, rbc_closure** ou
         cl->hyb = (cpx<float>*)cl->al((size_t)NFEAT * nz * g.N * g.KXp * sizeof(cpx<float>));
         cl->feat = (float*)cl->al(npts * NIN * sizeof(float));
         cl->pred = (float*)cl->al(npts * NOUT * sizeof(float));
-        cl->nguard = (g.P * L + 255) / 256;
+        cl->nguard = (int)((3 * (size_t)g.P * L + 255) / 256);
         cl->guard = (double*)cl->al((size_t)2 * cl->nguard * sizeof(double));
+        cl->dN = cl->al((size_t)3 * g.P * L * c->csz());
         for (int i = 0; i < 3; ++i) cl->Nt[i] = cl->al((size_t)g.P * L * c->csz());
         cl->DX = (float*)cl->al((size_t)capacity_points * NIN * sizeof(float));
         cl->DY = (float*)cl->al((size_t)capacity_points * NOUT * sizeof(float));
@@ -1204,6 +1284,7 @@ int rbc_closure_enable(rbc_closure* cl, int on, double max_ratio) {
         cl->enabled = true;
         cl->fallback = false;
         cl->max_ratio = max_ratio > 0 ? max_ratio : 1.0;
+        cl->phase = 0;
     } else {
         if (c->post_user == cl) {
             c->post_nonlinear = nullptr;
@@ -1211,6 +1292,14 @@ int rbc_closure_enable(rbc_closure* cl, int on, double max_ratio) {
         }
         cl->enabled = false;
     }
+    return RBC_OK;
+}
+
+int rbc_closure_interval(rbc_closure* cl, int every) {
+    if (!cl) return RBC_ERR_ARG;
+    if (every < 1) return cfail(cl, RBC_ERR_ARG, "every must be >= 1");
+    cl->every = every;
+    cl->phase = 0;
     return RBC_OK;
 }
 

@@ -129,43 +129,55 @@ __device__ __forceinline__ double pair_scale_d(double ma, double mb) {
     return ldexp(1.0, max(-200, min(200, ea - eb)));
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_rows(DevGeom g, const cpx<T>* __restrict__ hin,
+// A block handles RB consecutive y-rows of one level (RB M <= 512 complex per field pair, so that
+// the passes of the LDS transform carry enough butterflies to keep every thread busy); the
+// blockDim / RB threads of a row take its maxima together.  TWL: the twiddles are staged in LDS.
+template <typename T, bool TWL>
+__global__ void __launch_bounds__(256) k_rows(DevGeom g, int RB, const cpx<T>* __restrict__ hin,
                                               const cpx<T>* __restrict__ twM, cpx<T>* __restrict__ hout) {
     extern __shared__ unsigned char smem_raw[];
-    __shared__ int smaxb[6];
+    __shared__ int smaxb[8][6];
+    __shared__ T sisg[8][3];  // inverse pair scales per (row, pair)
     const int M = g.M, N = g.N, KX = g.KX, KXp = g.KXp, nq = g.nq;
-    cpx<T>* a = reinterpret_cast<cpx<T>*>(smem_raw);  // 3 * M
-    cpx<T>* b = a + 3 * M;                            // 3 * M
-    const int y = blockIdx.x, l = blockIdx.y;
+    const int FM = 3 * RB * M;                        // complex values per buffer
+    cpx<T>* a = reinterpret_cast<cpx<T>*>(smem_raw);  // [RB][3][M]
+    cpx<T>* b = a + FM;                               // [RB][3][M]
+    cpx<T>* tws = b + FM;                             // [M] when TWL
+    const int y0 = blockIdx.x * RB, l = blockIdx.y;
     const size_t fstride = (size_t)nq * N * KXp;
-    const size_t roff = ((size_t)l * N + y) * KXp;
-    // build the three Hermitian-extended pair spectra Z = X1 + i X2
-    for (int e = threadIdx.x; e < 3 * M; e += blockDim.x) a[e] = mk<T>(0, 0);
+    const size_t roff0 = ((size_t)l * N + y0) * KXp;
+    // build the three Hermitian-extended pair spectra Z = X1 + i X2 of every row
+    for (int e = threadIdx.x; e < FM; e += blockDim.x) a[e] = mk<T>(0, 0);
+    if (TWL)
+        for (int e = threadIdx.x; e < M; e += blockDim.x) tws[e] = twM[e];
+    if (threadIdx.x < 6 * RB) smaxb[threadIdx.x / 6][threadIdx.x % 6] = 0;
     __syncthreads();
-    for (int e = threadIdx.x; e < 3 * KX; e += blockDim.x) {
-        const int pr = e / KX, k = e - pr * KX;
-        cpx<T> x1 = hin[(size_t)(2 * pr) * fstride + roff + k];
-        cpx<T> x2 = hin[(size_t)(2 * pr + 1) * fstride + roff + k];
+    const cpx<T>* tw = TWL ? tws : twM;
+    for (int e = threadIdx.x; e < 3 * RB * KX; e += blockDim.x) {
+        const int rp = e / KX, k = e - rp * KX;  // rp = row * 3 + pair
+        const int rr = rp / 3, pr = rp - 3 * rr;
+        const size_t off = roff0 + (size_t)rr * KXp + k;
+        cpx<T> x1 = hin[(size_t)(2 * pr) * fstride + off];
+        cpx<T> x2 = hin[(size_t)(2 * pr + 1) * fstride + off];
         if (k == 0) {
             x1.y = 0;
             x2.y = 0;
         }
-        cpx<T>* z = a + pr * M;
+        cpx<T>* z = a + rp * M;
         z[k] = mk(x1.x - x2.y, x1.y + x2.x);  // X1 + i X2
         if (k > 0) z[M - k] = mk(x1.x + x2.y, -x1.y + x2.x);  // conj(X1) + i conj(X2)
     }
     __syncthreads();
-    cpx<T>* r = lds_fft(a, b, g.logM, 3, twM, true);
+    cpx<T>* r = lds_fft(a, b, g.logM, 3 * RB, tw, true);
     cpx<T>* o = (r == a) ? b : a;
     const T s = T(1) / (T(M) * T(N));
     // pass 1: row maxima of the six products (non-negative floats order like their bits)
-    if (threadIdx.x < 6) smaxb[threadIdx.x] = 0;
-    __syncthreads();
+    const int tpr = blockDim.x / RB, rr = threadIdx.x / tpr, tx0 = threadIdx.x - rr * tpr;
+    const cpx<T>* rz = r + (size_t)rr * 3 * M;
     {
         float pm[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        for (int x = threadIdx.x; x < M; x += blockDim.x) {
-            const cpx<T> z0 = r[x], z1 = r[M + x], z2 = r[2 * M + x];
+        for (int x = tx0; x < M; x += tpr) {
+            const cpx<T> z0 = rz[x], z1 = rz[M + x], z2 = rz[2 * M + x];
             const T u = z0.x, v = z0.y, w = z1.x, tx = z1.y, ty = z2.x, tz = z2.y;
             pm[0] = fmaxf(pm[0], fabsf((float)(u * u + v * v - T(2) * w * w)));
             pm[1] = fmaxf(pm[1], fabsf((float)(u * u - v * v)));
@@ -174,41 +186,50 @@ __global__ void __launch_bounds__(256) k_rows(DevGeom g, const cpx<T>* __restric
             pm[4] = fmaxf(pm[4], fabsf((float)(v * w)));
             pm[5] = fmaxf(pm[5], fabsf((float)(u * tx + v * ty + w * tz)));
         }
+        // the tpr threads of a row are whole waves or aligned groups of 32, 16, ... lanes
+        const int span = tpr < 64 ? tpr : 64;
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
             float x = pm[i];
-#pragma unroll
-            for (int o2 = 32; o2 > 0; o2 >>= 1) x = fmaxf(x, __shfl_xor(x, o2));
-            if ((threadIdx.x & 63) == 0) atomicMax(&smaxb[i], __float_as_int(x));
+            for (int o2 = span >> 1; o2 > 0; o2 >>= 1) x = fmaxf(x, __shfl_xor(x, o2));
+            if ((threadIdx.x & (span - 1)) == 0) atomicMax(&smaxb[rr][i], __float_as_int(x));
         }
     }
     __syncthreads();
-    T sg[3], isg[3];
+    T sg[3];
 #pragma unroll
-    for (int pr = 0; pr < 3; ++pr) {
-        sg[pr] = (T)pair_scale_d(__int_as_float(smaxb[2 * pr]), __int_as_float(smaxb[2 * pr + 1]));
-        isg[pr] = T(1) / sg[pr];
+    for (int pr = 0; pr < 3; ++pr)
+        sg[pr] = (T)pair_scale_d(__int_as_float(smaxb[rr][2 * pr]), __int_as_float(smaxb[rr][2 * pr + 1]));
+    if (threadIdx.x < 3 * RB) {
+        const int q = threadIdx.x / 3, pr = threadIdx.x - 3 * q;
+        sisg[q][pr] = T(1) / (T)pair_scale_d(__int_as_float(smaxb[q][2 * pr]), __int_as_float(smaxb[q][2 * pr + 1]));
     }
     // pass 2: the scaled product pairs
-    for (int x = threadIdx.x; x < M; x += blockDim.x) {
-        const cpx<T> z0 = r[x], z1 = r[M + x], z2 = r[2 * M + x];
-        const T u = z0.x, v = z0.y, w = z1.x, tx = z1.y, ty = z2.x, tz = z2.y;
-        o[x] = mk(s * (u * u + v * v - T(2) * w * w), sg[0] * s * (u * u - v * v));
-        o[M + x] = mk(s * u * v, sg[1] * s * u * w);
-        o[2 * M + x] = mk(s * v * w, sg[2] * s * (u * tx + v * ty + w * tz));
+    {
+        cpx<T>* oz = o + (size_t)rr * 3 * M;
+        for (int x = tx0; x < M; x += tpr) {
+            const cpx<T> z0 = rz[x], z1 = rz[M + x], z2 = rz[2 * M + x];
+            const T u = z0.x, v = z0.y, w = z1.x, tx = z1.y, ty = z2.x, tz = z2.y;
+            oz[x] = mk(s * (u * u + v * v - T(2) * w * w), sg[0] * s * (u * u - v * v));
+            oz[M + x] = mk(s * u * v, sg[1] * s * u * w);
+            oz[2 * M + x] = mk(s * v * w, sg[2] * s * (u * tx + v * ty + w * tz));
+        }
     }
     __syncthreads();
     cpx<T>* spare = (o == a) ? b : a;
-    cpx<T>* R = lds_fft(o, spare, g.logM, 3, twM, false);
+    cpx<T>* R = lds_fft(o, spare, g.logM, 3 * RB, tw, false);
     // unpack pair spectra: X1_k = (Z_k + conj Z_{M-k}) / 2, X2_k = -i (Z_k - conj Z_{M-k}) / (2 s)
-    for (int e = threadIdx.x; e < 3 * KX; e += blockDim.x) {
-        const int pr = e / KX, k = e - pr * KX;
-        const cpx<T>* Z = R + pr * M;
+    for (int e = threadIdx.x; e < 3 * RB * KX; e += blockDim.x) {
+        const int rp = e / KX, k = e - rp * KX;
+        const int rq = rp / 3, pr = rp - 3 * rq;
+        const T is = sisg[rq][pr];
+        const cpx<T>* Z = R + rp * M;
         const cpx<T> zk = Z[k], zm = Z[(M - k) & (M - 1)];
         const cpx<T> x1 = mk(T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y));
-        const cpx<T> x2 = mk(T(0.5) * isg[pr] * (zk.y + zm.y), T(-0.5) * isg[pr] * (zk.x - zm.x));
-        hout[(size_t)(2 * pr) * fstride + roff + k] = x1;
-        hout[(size_t)(2 * pr + 1) * fstride + roff + k] = x2;
+        const cpx<T> x2 = mk(T(0.5) * is * (zk.y + zm.y), T(-0.5) * is * (zk.x - zm.x));
+        const size_t off = roff0 + (size_t)rq * KXp + k;
+        hout[(size_t)(2 * pr) * fstride + off] = x1;
+        hout[(size_t)(2 * pr + 1) * fstride + off] = x2;
     }
 }
 
@@ -266,15 +287,27 @@ __global__ void __launch_bounds__(256) k_combine(DevGeom g, const cpx<T>* __rest
 template <typename T>
 __device__ __forceinline__ void mean_flow_projection(const DevGeom& g, const cpx<T>* __restrict__ fgw,
                                                      const T* __restrict__ prtab, T Pr, cpx<T>* NF, cpx<T>* NG) {
+    // first wave of block 0: lane = (level parity, m) for L <= 32 (the two halves of the wave
+    // take alternate levels and are added at the end), lane = m for L <= 64
     const int L = g.L, nq = g.nq;
-    if (blockIdx.x == 0 && threadIdx.x < L) {
-        const int m = threadIdx.x;
-        cpx<T> sF = mk<T>(0, 0), sG = sF;
-        for (int l = 0; l < nq; ++l) {
+    if (blockIdx.x != 0 || threadIdx.x >= 64) return;
+    const int G = L <= 32 ? 2 : 1, m = threadIdx.x % (64 / G), par = threadIdx.x / (64 / G);
+    cpx<T> sF = mk<T>(0, 0), sG = sF;
+    if (m < L) {
+#pragma unroll 4
+        for (int l = par; l < nq; l += G) {
             const T sp = prtab[(size_t)l * 5 * L + 4 * L + m];
             sF = sF + cscale(fgw[2 * l], sp);
             sG = sG + cscale(fgw[2 * l + 1], sp);
         }
+    }
+    if (G == 2) {
+        sF.x += __shfl_xor(sF.x, 32);
+        sF.y += __shfl_xor(sF.y, 32);
+        sG.x += __shfl_xor(sG.x, 32);
+        sG.y += __shfl_xor(sG.y, 32);
+    }
+    if (par == 0 && m < L) {
         NF[m] = cscale(sF, T(1) / Pr);
         NG[m] = cscale(sG, T(1) / Pr);
     }
@@ -972,11 +1005,8 @@ __global__ void __launch_bounds__(512) k_fft_y_inv512(DevGeom g, int ntile, int 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ __bf16 bf16_rne(float x) {
-    unsigned u = __float_as_uint(x);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return __builtin_bit_cast(__bf16, (unsigned short)(u >> 16));
-}
+// FP32 -> BF16, round to nearest even (one v_cvt_pk_bf16_f32 per pair on gfx950)
+__device__ __forceinline__ __bf16 bf16_rne(float x) { return (__bf16)x; }
 __device__ __forceinline__ float bf16_to_f32(__bf16 h) {
     return __uint_as_float((unsigned)__builtin_bit_cast(unsigned short, h) << 16);
 }
@@ -1011,8 +1041,9 @@ struct Mma<1> {
 // Evaluation on the matrix cores: D[level][column] = sum_m A_t[level][m] B_f[m][column], with
 // columns = (mode, re/im) of 16 modes per wave and 4 tables C', C, S, S' applied to f, f, g,
 // theta, theta (5 chains).  Block = 64 modes (4 waves); the block's mode data is staged in
-// LDS as [field][m][128 columns].  evA: [4][NLT][NKS][64][KPL] floats.
-template <int MM>
+// LDS as [field][m][128 columns].  evA: [4][NLT][NKS][64][KPL] floats; ALDS: the block's A
+// operands are copied to LDS with the modes (when both fit in 64 KiB).
+template <int MM, bool ALDS>
 __global__ void __launch_bounds__(256) k_eval_mma(DevGeom g, int nlt, int nks, int per,
                                                   const cpx<float>* __restrict__ Xt,
                                                   const cpx<float>* __restrict__ Xf, const cpx<float>* __restrict__ Xg,
@@ -1021,7 +1052,7 @@ __global__ void __launch_bounds__(256) k_eval_mma(DevGeom g, int nlt, int nks, i
                                                   const float* __restrict__ kxa, const float* __restrict__ kya,
                                                   cpx<float>* __restrict__ lev) {
     using MT = Mma<MM>;
-    extern __shared__ float sX[];  // 3 * 128 * (Lp + 1)
+    extern __shared__ float sX[];  // 3 * 128 * (Lp + 1) modes, then sA: the block's A operands
     const int L = g.L, P = g.P, nq = g.nq;
     const int Lp = nks * MT::K;
     // logical block = (64-mode block, level tile); the level tiles of a mode block are logical
@@ -1034,22 +1065,57 @@ __global__ void __launch_bounds__(256) k_eval_mma(DevGeom g, int nlt, int nks, i
     // that both the staging writes (m fastest) and the operand reads (column fastest) spread
     // over the banks
     const int LP = Lp + 1;
-    for (int e = threadIdx.x; e < 64 * Lp; e += 256) {
-        const int mode = e / Lp, m = e - mode * Lp;
-        const int p = p0 + mode;
-        cpx<float> zt = mk(0.f, 0.f), zf = zt, zg = zt;
-        if (p < P && m < L) {
-            const size_t gi = (size_t)p * L + m;
-            zt = Xt[gi];
-            zf = Xf[gi];
-            zg = Xg[gi];
+    // A operands of the block's level tile, [4][nks][64][KPL], copied next to the modes
+    float* sA = sX + 3 * 128 * LP;
+    if (ALDS) {
+        const int na = nks * 64 * MT::KPL / 4;  // float4 per table
+        for (int e = threadIdx.x; e < 4 * na; e += 256) {
+            const int t = e / na, i = e - t * na;
+            reinterpret_cast<float4*>(sA)[e] =
+                reinterpret_cast<const float4*>(evA + ((size_t)t * nlt + lt_block) * nks * 64 * MT::KPL)[i];
         }
-        sX[(0 * 128 + 2 * mode) * LP + m] = zt.x;
-        sX[(0 * 128 + 2 * mode + 1) * LP + m] = zt.y;
-        sX[(1 * 128 + 2 * mode) * LP + m] = zf.x;
-        sX[(1 * 128 + 2 * mode + 1) * LP + m] = zf.y;
-        sX[(2 * 128 + 2 * mode) * LP + m] = zg.x;
-        sX[(2 * 128 + 2 * mode + 1) * LP + m] = zg.y;
+    }
+    // mean flow u, v (sum_m C_m F_m, sum_m C_m G_m) at the block's 32 levels, for mode p = 0
+    __shared__ cpx<float> smf[2][32];
+    if (p0 == 0 && threadIdx.x < 64) {
+        const int ll = threadIdx.x & 31, which = threadIdx.x >> 5, l = lt_block * 32 + ll;
+        const cpx<float>* Xm = which ? XG : XF;
+        cpx<float> sum = mk(0.f, 0.f);
+        if (l < nq) {
+            const float* tb = evtab + (size_t)l * 4 * L + 2 * L;
+#pragma unroll 4
+            for (int m = 0; m < L; ++m) sum = sum + cscale(Xm[m], tb[m]);
+        }
+        smf[which][ll] = sum;
+    }
+    // four elements per thread in flight: the loads of a round are all issued before its stores
+    constexpr int SU = 4;
+    for (int e0 = threadIdx.x; e0 < 64 * Lp; e0 += 256 * SU) {
+        cpx<float> zt[SU], zf[SU], zg[SU];
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+            const int e = e0 + 256 * u, mode = e / Lp, m = e - mode * Lp;
+            const int p = p0 + mode;
+            zt[u] = zf[u] = zg[u] = mk(0.f, 0.f);
+            if (e < 64 * Lp && p < P && m < L) {
+                const size_t gi = (size_t)p * L + m;
+                zt[u] = Xt[gi];
+                zf[u] = Xf[gi];
+                zg[u] = Xg[gi];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+            const int e = e0 + 256 * u, mode = e / Lp, m = e - mode * Lp;
+            if (e < 64 * Lp) {
+                sX[(0 * 128 + 2 * mode) * LP + m] = zt[u].x;
+                sX[(0 * 128 + 2 * mode + 1) * LP + m] = zt[u].y;
+                sX[(1 * 128 + 2 * mode) * LP + m] = zf[u].x;
+                sX[(1 * 128 + 2 * mode + 1) * LP + m] = zf[u].y;
+                sX[(2 * 128 + 2 * mode) * LP + m] = zg[u].x;
+                sX[(2 * 128 + 2 * mode + 1) * LP + m] = zg[u].y;
+            }
+        }
     }
     __syncthreads();
     const int j = lane & 31, half = lane >> 5;
@@ -1066,7 +1132,8 @@ __global__ void __launch_bounds__(256) k_eval_mma(DevGeom g, int nlt, int nks, i
             float a[4][MT::KPL], bt[MT::KPL], bf[MT::KPL], bg[MT::KPL];
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                const float* src = evA + ((((size_t)t * nlt + lt) * nks + s) * 64 + lane) * MT::KPL;
+                const float* src = ALDS ? sA + (((size_t)t * nks + s) * 64 + lane) * MT::KPL
+                                        : evA + ((((size_t)t * nlt + lt) * nks + s) * 64 + lane) * MT::KPL;
 #pragma unroll
                 for (int e = 0; e < MT::KPL; ++e) a[t][e] = src[e];
             }
@@ -1104,14 +1171,9 @@ __global__ void __launch_bounds__(256) k_eval_mma(DevGeom g, int nlt, int nks, i
             w = k2 * Fc;
             tz = Tz;
             if (p == 0 && l < nq) {  // mean flow at k = 0 (add_dc_part)
-                const float* tb = evtab + (size_t)l * 4 * L;
-                float sF = 0.f, sG = 0.f;
-                for (int m = 0; m < L; ++m) {
-                    sF += tb[2 * L + m] * (comp ? XF[m].y : XF[m].x);
-                    sG += tb[2 * L + m] * (comp ? XG[m].y : XG[m].x);
-                }
-                u += sF;
-                v += sG;
+                const cpx<float> mF = smf[0][l - lt * 32], mG = smf[1][l - lt * 32];
+                u += comp ? mF.y : mF.x;
+                v += comp ? mG.y : mG.x;
             }
             if (pvalid && l < nq) {
                 float* o = reinterpret_cast<float*>(lev) + ((size_t)l * P + p) * 2 + comp;
@@ -1129,9 +1191,12 @@ __global__ void __launch_bounds__(256) k_eval_mma(DevGeom g, int nlt, int nks, i
 // Projection on the matrix cores: Out[m][column] = sum_l T_t[m][l] D_t[l][column]; the B
 // operands are float components of the five level combinations (column = (mode, re/im), so a
 // lane reads one float and the lanes of a wave read 128 contiguous bytes per level).
-// prA: [5][NKS][64][KPL].  Block = 64 modes, wave = 16 modes (32 columns); rows m < L <= 32.
-template <int MM, int WPB>
-__global__ void __launch_bounds__(64 * WPB) k_project_mma(DevGeom g, int nks, const float* __restrict__ Dfl,
+// prA: [5][NKS][64][KPL].  Block = WPB x KS waves: WPB groups of 16 modes (32 columns, rows
+// m < L <= 32); the KS waves of a group split the level sum into contiguous k-step ranges and
+// add their partial sums through LDS in fixed order (KS > 1 gives small grids more waves in
+// flight).
+template <int MM, int WPB, int KS>
+__global__ void __launch_bounds__(64 * WPB * KS) k_project_mma(DevGeom g, int nks, const float* __restrict__ Dfl,
                                                      const cpx<float>* __restrict__ fgw, const float* __restrict__ prA,
                                                      const float* __restrict__ prtab, const float* __restrict__ kxa,
                                                      const float* __restrict__ kya, float Pr,
@@ -1141,8 +1206,9 @@ __global__ void __launch_bounds__(64 * WPB) k_project_mma(DevGeom g, int nks, co
     using MT = Mma<MM>;
     constexpr int OS = 66;               // staging row (2L <= 64 floats plus 2 to spread banks)
     __shared__ float sO[WPB][3][16 * OS];  // per wave: [field][mode][OS] staging for the write-out
+    __shared__ f32x16 sPart[KS > 1 ? (KS - 1) * WPB : 1][KS > 1 ? 4 : 1][64];  // partials of waves wk > 0
     const int L = g.L, P = g.P, nq = g.nq;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) % WPB, wk = (threadIdx.x >> 6) / WPB;
     const int j = lane & 31, half = lane >> 5;
     const int col = wave * 32 + j;
     const int p = blockIdx.x * 16 * WPB + (col >> 1), comp = col & 1;
@@ -1153,7 +1219,8 @@ __global__ void __launch_bounds__(64 * WPB) k_project_mma(DevGeom g, int nks, co
     const float* dcol = Dfl + (size_t)2 * pc + comp;
     f32x16 cF1 = {}, cF2 = {}, cG = {}, cT = {};
     constexpr int U = MM == 0 ? 4 : 1;  // k-steps whose operand loads are issued together
-    for (int s0 = 0; s0 < nks; s0 += U) {
+    const int spw = (nks + KS - 1) / KS, sbeg = wk * spw, send = min(nks, sbeg + spw);
+    for (int s0 = sbeg; s0 < send; s0 += U) {
         float b[U][5][MT::KPL];
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -1161,12 +1228,13 @@ __global__ void __launch_bounds__(64 * WPB) k_project_mma(DevGeom g, int nks, co
             for (int e = 0; e < MT::KPL; ++e) {
                 const int l = (s0 + u) * MT::K + half * MT::KPL + e;
 #pragma unroll
-                for (int t = 0; t < 5; ++t) b[u][t][e] = (l < nq) ? dcol[t * ffs + (size_t)2 * l * P] : 0.f;
+                for (int t = 0; t < 5; ++t)
+                    b[u][t][e] = (l < nq && s0 + u < send) ? dcol[t * ffs + (size_t)2 * l * P] : 0.f;
             }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int s = s0 + u;
-            if (s >= nks) break;
+            if (s >= send) break;
             float a[5][MT::KPL];
 #pragma unroll
             for (int t = 0; t < 5; ++t) {
@@ -1180,6 +1248,25 @@ __global__ void __launch_bounds__(64 * WPB) k_project_mma(DevGeom g, int nks, co
             MT::run(cG, a[3], b[u][2]);   // wS   . D2
             MT::run(cG, a[4], b[u][3]);   // wS'  . D3
             MT::run(cT, a[3], b[u][4]);   // wS   . D4
+        }
+    }
+    if (KS > 1) {
+        if (wk > 0) {
+            f32x16* dst = sPart[(wk - 1) * WPB + wave][0];
+            dst[0 * 64 + lane] = cF1;
+            dst[1 * 64 + lane] = cF2;
+            dst[2 * 64 + lane] = cG;
+            dst[3 * 64 + lane] = cT;
+        }
+        __syncthreads();
+        if (wk > 0) return;
+#pragma unroll
+        for (int k = 0; k < KS - 1; ++k) {
+            const f32x16* src = sPart[k * WPB + wave][0];
+            cF1 += src[0 * 64 + lane];
+            cF2 += src[1 * 64 + lane];
+            cG += src[2 * 64 + lane];
+            cT += src[3 * 64 + lane];
         }
     }
     const float mt = -1.f, mp = -1.f / Pr;
@@ -1347,19 +1434,25 @@ void Launch<T>::eval(const DevGeom& g, const Buffers& b, void* const X[3], const
                      hipStream_t s) {
     if (sizeof(T) == 4 && b.mma >= 0) {
         const int K = b.mma == 0 ? 2 : 16;
-        const size_t smm = (size_t)3 * 128 * (b.ev_nks * K + 1) * sizeof(float);
+        const size_t sx = (size_t)3 * 128 * (b.ev_nks * K + 1) * sizeof(float);
+        const size_t sa = (size_t)4 * b.ev_nks * 64 * (K / 2) * sizeof(float);
+        const bool alds = sx + sa <= 65536;
+        const size_t smm = alds ? sx + sa : sx;
         const int nblk = ((g.P + 63) / 64) * b.ev_nlt, per = (nblk + 7) / 8;
         dim3 grid(8 * per);
-        if (b.mma == 0)
-            hipLaunchKernelGGL(k_eval_mma<0>, grid, dim3(256), smm, s, g, b.ev_nlt, b.ev_nks, per, (const cpx<float>*)X[0],
-                               (const cpx<float>*)X[1], (const cpx<float>*)X[2], (const cpx<float>*)XF,
-                               (const cpx<float>*)XG, (const float*)b.evA, (const float*)b.evtab,
-                               (const float*)b.kx, (const float*)b.ky, (cpx<float>*)b.lev);
-        else
-            hipLaunchKernelGGL(k_eval_mma<1>, grid, dim3(256), smm, s, g, b.ev_nlt, b.ev_nks, per, (const cpx<float>*)X[0],
-                               (const cpx<float>*)X[1], (const cpx<float>*)X[2], (const cpx<float>*)XF,
-                               (const cpx<float>*)XG, (const float*)b.evA, (const float*)b.evtab,
-                               (const float*)b.kx, (const float*)b.ky, (cpx<float>*)b.lev);
+#define RBC_EVAL_MMA(MMV, ALDSV)                                                                                 \
+    hipLaunchKernelGGL((k_eval_mma<MMV, ALDSV>), grid, dim3(256), smm, s, g, b.ev_nlt, b.ev_nks, per,             \
+                       (const cpx<float>*)X[0], (const cpx<float>*)X[1], (const cpx<float>*)X[2],                \
+                       (const cpx<float>*)XF, (const cpx<float>*)XG, (const float*)b.evA, (const float*)b.evtab, \
+                       (const float*)b.kx, (const float*)b.ky, (cpx<float>*)b.lev)
+        if (b.mma == 0) {
+            if (alds) RBC_EVAL_MMA(0, true);
+            else RBC_EVAL_MMA(0, false);
+        } else {
+            if (alds) RBC_EVAL_MMA(1, true);
+            else RBC_EVAL_MMA(1, false);
+        }
+#undef RBC_EVAL_MMA
         return;
     }
     const size_t sm = (size_t)3 * g.L * 64 * sizeof(cpx<T>);
@@ -1394,10 +1487,21 @@ void Launch<T>::rows(const DevGeom& g, const Buffers& b, hipStream_t s) {
                            (cpx<float>*)b.hout);
         return;
     }
-    const size_t sm = (size_t)6 * g.M * sizeof(cpx<T>);
     const int nth = b.tune_rows_threads > 0 ? b.tune_rows_threads : 256;
-    hipLaunchKernelGGL(k_rows<T>, grid, dim3(nth), sm, s, g, (const cpx<T>*)b.hin, (const cpx<T>*)b.twM,
-                       (cpx<T>*)b.hout);
+    // rows per block: RB M = 512 for M >= 64 (at most 8 rows, at most nth / 32 so that a row keeps
+    // 32 threads or more, and dividing N)
+    int RB = 1;
+    if (g.M >= 64)
+        while (RB * 2 * g.M <= 512 && RB * 2 <= 8 && RB * 2 <= nth / 32 && g.N % (RB * 2) == 0) RB *= 2;
+    const size_t sm = (size_t)6 * RB * g.M * sizeof(cpx<T>);
+    const bool twl = sm + (size_t)g.M * sizeof(cpx<T>) <= 65536;
+    const dim3 gr(g.N / RB, g.nq);
+    if (twl)
+        hipLaunchKernelGGL((k_rows<T, true>), gr, dim3(nth), sm + (size_t)g.M * sizeof(cpx<T>), s, g, RB,
+                           (const cpx<T>*)b.hin, (const cpx<T>*)b.twM, (cpx<T>*)b.hout);
+    else
+        hipLaunchKernelGGL((k_rows<T, false>), gr, dim3(nth), sm, s, g, RB, (const cpx<T>*)b.hin,
+                           (const cpx<T>*)b.twM, (cpx<T>*)b.hout);
 }
 
 template <typename T>
@@ -1431,11 +1535,12 @@ void Launch<T>::project(const DevGeom& g, const Buffers& b, double Pr, void* con
     constexpr int MCH = sizeof(T) == 8 ? 4 : 8;
     dim3 grid((g.P + 63) / 64);
     if (sizeof(T) == 4 && b.mma >= 0) {
-        // one-wave blocks when 4-wave blocks would leave most of the 256 CUs idle
+        // when 64-mode blocks would leave most of the 256 CUs idle: 16-mode blocks whose four
+        // waves split the level sum
         const bool small = (g.P + 63) / 64 < 512;
-        const dim3 gr = small ? dim3((g.P + 15) / 16) : grid, bl = small ? dim3(64) : dim3(256);
+        const dim3 gr = small ? dim3((g.P + 15) / 16) : grid, bl = dim3(256);
 #define RBC_PROJECT_MMA(MMV, WPBV)                                                                              \
-    hipLaunchKernelGGL((k_project_mma<MMV, WPBV>), gr, bl, 0, s, g, b.pr_nks, (const float*)b.lev,                \
+    hipLaunchKernelGGL((k_project_mma<MMV, WPBV, 4 / WPBV>), gr, bl, 0, s, g, b.pr_nks, (const float*)b.lev,      \
                        (const cpx<float>*)b.fgw, (const float*)b.prA, (const float*)b.prtab, (const float*)b.kx,   \
                        (const float*)b.ky, (float)Pr, (cpx<float>*)Nout[0], (cpx<float>*)Nout[1],                \
                        (cpx<float>*)Nout[2], (cpx<float>*)NFo, (cpx<float>*)NGo)

@@ -53,7 +53,7 @@ EXPORTS = ["rbc_create", "rbc_destroy", "rbc_last_error", "rbc_get_info", "rbc_s
            "rbc_closure_dataset", "rbc_closure_load_dataset", "rbc_closure_clear_dataset",
            "rbc_closure_fit_normalization", "rbc_closure_get_normalization", "rbc_closure_set_normalization",
            "rbc_closure_get_weights", "rbc_closure_set_weights", "rbc_closure_gradient", "rbc_closure_train",
-           "rbc_closure_predict", "rbc_closure_eval", "rbc_closure_enable", "rbc_closure_step",
+           "rbc_closure_predict", "rbc_closure_eval", "rbc_closure_enable", "rbc_closure_interval", "rbc_closure_step",
            "rbc_closure_status"]
 
 CLOSURE_NIN, CLOSURE_NOUT = 8, 3
@@ -147,6 +147,7 @@ def load_library(path=LIB_PATH):
     L.rbc_closure_predict.argtypes = [vp, i_, fp, fp]
     L.rbc_closure_eval.argtypes = [vp, i_, i_, dp, dp]
     L.rbc_closure_enable.argtypes = [vp, i_, ctypes.c_double]
+    L.rbc_closure_interval.argtypes = [vp, i_]
     L.rbc_closure_step.argtypes = [vp, i_, i_]
     L.rbc_closure_status.argtypes = [vp, ip, ip, dp]
     _lib = L
@@ -570,6 +571,11 @@ class Closure:
 
     def enable(self, on=True, max_ratio=1.0):
         self._check(self.lib.rbc_closure_enable(self.h, int(bool(on)), float(max_ratio)), "rbc_closure_enable")
+
+    def set_interval(self, every):
+        """Evaluate the network on every `every`-th corrected step; the steps between reuse the
+        last evaluated correction."""
+        self._check(self.lib.rbc_closure_interval(self.h, int(every)), "rbc_closure_interval")
 
     def step(self, nsteps, check_every=100):
         self._check(self.lib.rbc_closure_step(self.h, int(nsteps), int(check_every)), "rbc_closure_step")

@@ -167,6 +167,43 @@ def test_correction_inside_the_step():
         assert np.abs(d - e[i]).max() < 1e-5 * np.abs(e[i]).max(), i
 
 
+def test_correction_interval_reuses_the_last_correction():
+    """rbc_closure_interval(k): the network runs on every k-th corrected step and the steps
+    between add the last correction again.  With W1 = 0 the network's output does not depend on
+    the state, so re-evaluating and reusing give the same correction and k = 1, 2, 3 must agree
+    bit for bit; with a state-dependent network k = 2 must differ from k = 1 after the second
+    step (the reused correction is older) and agree after the first."""
+    print("case: closure interval", flush=True)
+    from sdc_amd import Closure
+    st = random_state(COARSE["M"], COARSE["N"], COARSE["L"], seed=14, amp=1e-1)
+    w, norm, _, _ = _net(5)
+    w_const = w.copy()
+    w_const[C.OFF_W1:C.OFF_B1] = 0.0
+
+    def run(weights, every, nsteps):
+        with _stepper(COARSE, "f32") as A:
+            A.set_state(*st)
+            cl = Closure(A, nz=8, capacity_points=0)
+            cl.set_weights(weights)
+            cl.set_normalization(*norm)
+            cl.set_interval(every)
+            cl.enable(True, max_ratio=1e30)
+            cl.step(nsteps, check_every=nsteps)
+            out = A.get_state()
+            cl.close()
+        return out
+
+    ref = run(w_const, 1, 4)
+    for k in (2, 3):
+        got = run(w_const, k, 4)
+        assert all(np.array_equal(a, b) for a, b in zip(got, ref)), k
+    one1, one2 = run(w, 1, 1), run(w, 2, 1)
+    assert all(np.array_equal(a, b) for a, b in zip(one1, one2))
+    two1, two2 = run(w, 1, 2), run(w, 2, 2)
+    assert max(float(np.abs(a - b).max()) for a, b in zip(two1, two2)) > 0.0
+    assert all(np.isfinite(a).all() for a in two2)
+
+
 def test_fallback_to_the_plain_step():
     print("case: closure fallback", flush=True)
     from sdc_amd import Closure
