@@ -682,10 +682,12 @@ int y_tile(int N) {
 void synth_points(rbc_closure* cl, int F, int os, int zcol, float* out) {
     rbc_ctx* c = cl->c;
     const DevGeom& g = cl->gz;
-    const int TX = y_tile(g.N);
-    hipLaunchKernelGGL(k_fft_y_inv<float>, dim3((g.KX + TX - 1) / TX, cl->nz, F), dim3(256),
-                       (size_t)2 * TX * g.N * sizeof(cpx<float>), c->stream, g, TX, (const cpx<float>*)cl->spec,
-                       c->b.rows, (const cpx<float>*)cl->twN, cl->hyb);
+    if (!launch_y_inv_group(g, F, cl->spec, cl->twN, cl->hyb, c->stream)) {
+        const int TX = y_tile(g.N);
+        hipLaunchKernelGGL(k_fft_y_inv<float>, dim3((g.KX + TX - 1) / TX, cl->nz, F), dim3(256),
+                           (size_t)2 * TX * g.N * sizeof(cpx<float>), c->stream, g, TX, (const cpx<float>*)cl->spec,
+                           c->b.rows, (const cpx<float>*)cl->twN, cl->hyb);
+    }
     hipLaunchKernelGGL(k_synth_rows, dim3(g.N, cl->nz), dim3(256), (size_t)2 * F * g.M * sizeof(cpx<float>),
                        c->stream, g, F, os, zcol, (const float*)cl->zl, (const cpx<float>*)cl->hyb,
                        (const cpx<float>*)cl->twM, out);
@@ -729,10 +731,12 @@ void apply_closure(rbc_ctx* c, void* user) {
     const DevGeom& g = cl->gz;
     hipLaunchKernelGGL(k_anal_rows, dim3(g.N, cl->nz), dim3(256), (size_t)2 * NOUT * g.M * sizeof(cpx<float>), c->stream, g,
                        (const float*)cl->pred, (const cpx<float>*)cl->twM, cl->hyb);
-    const int TX = y_tile(g.N);
-    hipLaunchKernelGGL(k_fft_y_fwd<float>, dim3((g.KX + TX - 1) / TX, cl->nz, 3), dim3(256),
-                       (size_t)2 * TX * g.N * sizeof(cpx<float>), c->stream, g, TX, (const cpx<float>*)cl->hyb,
-                       c->b.rows, (const cpx<float>*)cl->twN, cl->spec);
+    if (!launch_y_fwd_group(g, 3, cl->hyb, cl->twN, cl->spec, c->stream)) {
+        const int TX = y_tile(g.N);
+        hipLaunchKernelGGL(k_fft_y_fwd<float>, dim3((g.KX + TX - 1) / TX, cl->nz, 3), dim3(256),
+                           (size_t)2 * TX * g.N * sizeof(cpx<float>), c->stream, g, TX, (const cpx<float>*)cl->hyb,
+                           c->b.rows, (const cpx<float>*)cl->twN, cl->spec);
+    }
     if (c->rsz == 8) launch_project<double>(cl);
     else launch_project<float>(cl);
 }

@@ -41,13 +41,132 @@ __device__ __forceinline__ cpx<T> imul(cpx<T> a, T s) { return mk(-s * a.y, s * 
 // hardware block id to a logical id so that logical neighbours (which share cache lines) run
 // on the same XCD: XCD x takes the contiguous logical range [x*per, (x+1)*per).  Grids using
 // it are launched with 8*per blocks; blocks with logical id >= the real count return at once.
-inline __device__ __forceinline__ int xcd_logical_block(int b, int per) { return (b & 7) * per + (b >> 3); }
+__device__ __forceinline__ int xcd_logical_block(int b, int per) { return (b & 7) * per + (b >> 3); }
 
 // Orders LDS traffic between the lanes of one wavefront (no block-wide barrier).
-inline __device__ __forceinline__ void wave_sync() {
+__device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One LDS pad slot every 8 complex values spreads the strided accesses of the FFT passes over
+// the banks.
+__device__ __forceinline__ int pad8(int i) { return i + (i >> 3); }
+
+// 8-point DFT in registers (decimation in frequency, outputs reordered to natural order).
+template <typename T, bool INV>
+__device__ __forceinline__ void dft8(cpx<T> v[8]) {
+    const T h = T(0.70710678118654752440);
+    // stage 1 (span 4) with twiddles w8^i, w8 = exp(-+ i pi / 4)
+    cpx<T> a[8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        a[i] = v[i] + v[i + 4];
+        a[i + 4] = v[i] - v[i + 4];
+    }
+    // a5 *= w8, a6 *= w8^2, a7 *= w8^3
+    if (!INV) {
+        a[5] = mk(h * (a[5].x + a[5].y), h * (a[5].y - a[5].x));
+        a[6] = mk(a[6].y, -a[6].x);
+        a[7] = mk(h * (a[7].y - a[7].x), -h * (a[7].x + a[7].y));
+    } else {
+        a[5] = mk(h * (a[5].x - a[5].y), h * (a[5].y + a[5].x));
+        a[6] = mk(-a[6].y, a[6].x);
+        a[7] = mk(-h * (a[7].x + a[7].y), h * (a[7].x - a[7].y));
+    }
+    // stage 2 (span 2) with w4 = -+ i on the second element of each pair
+    cpx<T> b[8];
+#pragma unroll
+    for (int hh = 0; hh < 8; hh += 4) {
+        b[hh + 0] = a[hh + 0] + a[hh + 2];
+        b[hh + 2] = a[hh + 0] - a[hh + 2];
+        b[hh + 1] = a[hh + 1] + a[hh + 3];
+        const cpx<T> d = a[hh + 1] - a[hh + 3];
+        b[hh + 3] = INV ? mk(-d.y, d.x) : mk(d.y, -d.x);
+    }
+    // stage 3 (span 1); outputs in bit-reversed order
+    const cpx<T> c0 = b[0] + b[1], c1 = b[0] - b[1], c2 = b[2] + b[3], c3 = b[2] - b[3];
+    const cpx<T> c4 = b[4] + b[5], c5 = b[4] - b[5], c6 = b[6] + b[7], c7 = b[6] - b[7];
+    v[0] = c0;
+    v[1] = c4;
+    v[2] = c2;
+    v[3] = c6;
+    v[4] = c1;
+    v[5] = c5;
+    v[6] = c3;
+    v[7] = c7;
+}
+
+// radix-4 butterfly in place (natural order outputs): X_p = sum_q a_q (-+i)^{pq}
+template <bool INV>
+__device__ __forceinline__ void dft4(cpx<float>& a0, cpx<float>& a1, cpx<float>& a2, cpx<float>& a3) {
+    const cpx<float> t0 = a0 + a2, t1 = a0 - a2, t2 = a1 + a3, t3 = a1 - a3;
+    const cpx<float> it3 = INV ? mk(-t3.y, t3.x) : mk(t3.y, -t3.x);  // -+ i t3
+    a0 = t0 + t2;
+    a1 = t1 + it3;
+    a2 = t0 - t2;
+    a3 = t1 - it3;
+}
+
+// n-point FFT (n = 2^LOGN, 64 <= n <= 1024) of NF sequences on a group of W = n / 8 lanes in
+// natural layout: element i of a sequence sits in register i / W of lane i % W of the group,
+// on entry and on exit.  Stockham passes: radix 8 while three or more radix-2 stages remain,
+// then one radix-2^(LOGN mod 3) pass.  Lane j runs the butterflies j + W b (b < 8 / R) of a
+// radix-R pass, whose inputs x[j + W b + q n / R] are registers b + q 8 / R on the first pass
+// and whose outputs y[j + W b + p n / R] are registers b + p 8 / R on the last; passes between
+// exchange through the group's LDS scratch (element i at sb[pad8(i) S], so a column of an LDS
+// tile of row stride S can serve).  The NF sequences go through the scratch one after the
+// other.  tw[t] = exp(-2 pi i t / n); the inverse uses conjugate twiddles; unnormalised.
+template <int LOGN, bool INV, int NF, int S = 1>
+__device__ __forceinline__ void fft_group(cpx<float> (&v)[NF][8], cpx<float>* sb, const cpx<float>* tw, int j) {
+    constexpr int n = 1 << LOGN, W = n / 8;
+    constexpr int NP8 = LOGN / 3, REM = LOGN % 3, NPASS = NP8 + (REM ? 1 : 0);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        int logLs = 0;
+#pragma unroll
+        for (int ps = 0; ps < NPASS; ++ps) {
+            const int LR = ps < NP8 ? 3 : REM, R = 1 << LR, NB = 8 / R, Ls = 1 << logLs;
+            if (ps > 0) {
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+#pragma unroll
+                    for (int q = 0; q < R; ++q) v[f][b + q * NB] = sb[pad8(j + W * b + q * (n / R)) * S];
+                wave_sync();  // every lane has read the pass inputs before the scratch is rewritten
+            }
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int k = (j + W * b) & (Ls - 1);
+                if (ps > 0)
+#pragma unroll
+                    for (int q = 1; q < R; ++q) {
+                        cpx<float> w = tw[(q * k) << (LOGN - LR - logLs)];
+                        if (INV) w.y = -w.y;
+                        v[f][b + q * NB] = cmul(v[f][b + q * NB], w);
+                    }
+                if (R == 8) {
+                    dft8<float, INV>(v[f]);
+                } else if (R == 4) {
+                    dft4<INV>(v[f][b], v[f][b + 2], v[f][b + 4], v[f][b + 6]);
+                } else {
+                    const cpx<float> a0 = v[f][b], a1 = v[f][b + 4];
+                    v[f][b] = a0 + a1;
+                    v[f][b + 4] = a0 - a1;
+                }
+            }
+            if (ps + 1 < NPASS) {
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    const int jb = j + W * b, k = jb & (Ls - 1);
+#pragma unroll
+                    for (int p = 0; p < R; ++p) sb[pad8((jb - k) * R + k + p * Ls) * S] = v[f][b + p * NB];
+                }
+                wave_sync();
+            }
+            logLs += LR;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -182,6 +301,104 @@ __global__ void __launch_bounds__(256) k_fft_y_fwd(DevGeom g, int TX, const cpx<
         const int c = e / NYc, kyc = e - c * NYc;
         dst[(size_t)(kx0 + c) * NYc + kyc] = r[c * N + rows[kyc]];
     }
+}
+
+// y-FFTs at N = 64, 128, 256 (FP32): a block of 4 waves handles CB = 4 (64 / W) kx columns of
+// one (field, level), one group of W = N / 8 lanes per column (fft_group in natural layout);
+// the LDS tile [pad8(y)][CB + 1] is each column's FFT scratch and the transpose to the
+// row-major hybrid layout (rows of CB contiguous values).
+template <int LOGN>
+__global__ void __launch_bounds__(256) k_fft_y_inv_w(DevGeom g, const cpx<float>* __restrict__ lev,
+                                                     const cpx<float>* __restrict__ twN, cpx<float>* __restrict__ hin) {
+    constexpr int N = 1 << LOGN, W = N / 8, G = 64 / W, CB = 4 * G, S = CB + 1;
+    __shared__ cpx<float> tw[N];
+    __shared__ cpx<float> tile[(N + N / 8) * S];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, grp = lane / W, j = lane - grp * W;
+    const int NYc = g.NYc, P = g.P, nq = g.nq, KX = g.KX;
+    const int kx0 = blockIdx.x * CB, l = blockIdx.y, field = blockIdx.z;
+    const int c = wave * G + grp, kx = kx0 + c;
+    const int a = (NYc - 1) / 2, shift = N - NYc;  // kept rows [0, a] U [N - a, N - 1]
+    for (int e = threadIdx.x; e < N; e += blockDim.x) tw[e] = twN[e];
+    __syncthreads();
+    const cpx<float>* src = lev + ((size_t)field * nq + l) * P + (size_t)kx * NYc;
+    cpx<float> v[1][8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int n = j + W * r;
+        v[0][r] = mk(0.f, 0.f);
+        if (kx < KX) {
+            if (n <= a) v[0][r] = src[n];
+            else if (n >= N - a) v[0][r] = src[n - shift];
+        }
+    }
+    fft_group<LOGN, true, 1, S>(v, tile + c, tw, j);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) tile[pad8(j + W * r) * S + c] = v[0][r];
+    __syncthreads();
+    const int ncol = min(CB, KX - kx0);
+    cpx<float>* dst = hin + ((size_t)field * nq + l) * (size_t)N * g.KXp + kx0;
+    for (int e = threadIdx.x; e < N * CB; e += blockDim.x) {
+        const int yy = e / CB, cc = e - yy * CB;
+        if (cc < ncol) dst[(size_t)yy * g.KXp + cc] = tile[pad8(yy) * S + cc];
+    }
+}
+
+template <int LOGN>
+__global__ void __launch_bounds__(256) k_fft_y_fwd_w(DevGeom g, const cpx<float>* __restrict__ hout,
+                                                     const cpx<float>* __restrict__ twN, cpx<float>* __restrict__ prod) {
+    constexpr int N = 1 << LOGN, W = N / 8, G = 64 / W, CB = 4 * G, S = CB + 1;
+    __shared__ cpx<float> tw[N];
+    __shared__ cpx<float> tile[(N + N / 8) * S];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, grp = lane / W, j = lane - grp * W;
+    const int NYc = g.NYc, P = g.P, nq = g.nq, KX = g.KX;
+    const int kx0 = blockIdx.x * CB, l = blockIdx.y, q = blockIdx.z;
+    const int c = wave * G + grp, kx = kx0 + c;
+    const int a = (NYc - 1) / 2, shift = N - NYc;
+    const int ncol = min(CB, KX - kx0);
+    for (int e = threadIdx.x; e < N; e += blockDim.x) tw[e] = twN[e];
+    const cpx<float>* src = hout + ((size_t)q * nq + l) * (size_t)N * g.KXp + kx0;
+    for (int e = threadIdx.x; e < N * CB; e += blockDim.x) {
+        const int yy = e / CB, cc = e - yy * CB;
+        tile[pad8(yy) * S + cc] = (cc < ncol) ? src[(size_t)yy * g.KXp + cc] : mk(0.f, 0.f);
+    }
+    __syncthreads();
+    cpx<float> v[1][8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[0][r] = tile[pad8(j + W * r) * S + c];
+    wave_sync();  // the column is read before it becomes the scratch
+    fft_group<LOGN, false, 1, S>(v, tile + c, tw, j);
+    if (kx < KX) {
+        cpx<float>* dst = prod + ((size_t)q * nq + l) * P + (size_t)kx * NYc;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int n = j + W * r;
+            if (n <= a) dst[n] = v[0][r];
+            else if (n >= N - a) dst[n - shift] = v[0][r];
+        }
+    }
+}
+
+// Launchers of the group y-FFTs for nf fields of g.nq levels each ([field][l][p] spectra,
+// [field][l][y][kx] hybrid rows); false (nothing launched) unless N is 64, 128 or 256.
+inline bool launch_y_inv_group(const DevGeom& g, int nf, const cpx<float>* lev, const cpx<float>* twN,
+                               cpx<float>* hin, hipStream_t s) {
+    if (g.N < 64 || g.N > 256) return false;
+    const int CB = 4 * 512 / g.N;
+    const dim3 gr((g.KX + CB - 1) / CB, g.nq, nf);
+    if (g.N == 64) hipLaunchKernelGGL(k_fft_y_inv_w<6>, gr, dim3(256), 0, s, g, lev, twN, hin);
+    else if (g.N == 128) hipLaunchKernelGGL(k_fft_y_inv_w<7>, gr, dim3(256), 0, s, g, lev, twN, hin);
+    else hipLaunchKernelGGL(k_fft_y_inv_w<8>, gr, dim3(256), 0, s, g, lev, twN, hin);
+    return true;
+}
+inline bool launch_y_fwd_group(const DevGeom& g, int nf, const cpx<float>* hout, const cpx<float>* twN,
+                               cpx<float>* prod, hipStream_t s) {
+    if (g.N < 64 || g.N > 256) return false;
+    const int CB = 4 * 512 / g.N;
+    const dim3 gr((g.KX + CB - 1) / CB, g.nq, nf);
+    if (g.N == 64) hipLaunchKernelGGL(k_fft_y_fwd_w<6>, gr, dim3(256), 0, s, g, hout, twN, prod);
+    else if (g.N == 128) hipLaunchKernelGGL(k_fft_y_fwd_w<7>, gr, dim3(256), 0, s, g, hout, twN, prod);
+    else hipLaunchKernelGGL(k_fft_y_fwd_w<8>, gr, dim3(256), 0, s, g, hout, twN, prod);
+    return true;
 }
 
 }  // namespace rbc

@@ -749,52 +749,6 @@ __global__ void __launch_bounds__(256) k_nusselt(DevGeom g, const cpx<T>* __rest
 // Lane j holds x[j + 64 r], r = 0..7, on entry and X[j + 64 r] on exit.  Two exchanges
 // through a per-wave LDS scratch of 576 complex (one pad slot every 8 to spread banks).
 // =======================================================================================
-__device__ __forceinline__ int pad8(int i) { return i + (i >> 3); }
-
-// 8-point DFT in registers (decimation in frequency, outputs reordered to natural order).
-template <typename T, bool INV>
-__device__ __forceinline__ void dft8(cpx<T> v[8]) {
-    const T h = T(0.70710678118654752440);
-    // stage 1 (span 4) with twiddles w8^i, w8 = exp(-+ i pi / 4)
-    cpx<T> a[8];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        a[i] = v[i] + v[i + 4];
-        a[i + 4] = v[i] - v[i + 4];
-    }
-    // a5 *= w8, a6 *= w8^2, a7 *= w8^3
-    if (!INV) {
-        a[5] = mk(h * (a[5].x + a[5].y), h * (a[5].y - a[5].x));
-        a[6] = mk(a[6].y, -a[6].x);
-        a[7] = mk(h * (a[7].y - a[7].x), -h * (a[7].x + a[7].y));
-    } else {
-        a[5] = mk(h * (a[5].x - a[5].y), h * (a[5].y + a[5].x));
-        a[6] = mk(-a[6].y, a[6].x);
-        a[7] = mk(-h * (a[7].x + a[7].y), h * (a[7].x - a[7].y));
-    }
-    // stage 2 (span 2) with w4 = -+ i on the second element of each pair
-    cpx<T> b[8];
-#pragma unroll
-    for (int hh = 0; hh < 8; hh += 4) {
-        b[hh + 0] = a[hh + 0] + a[hh + 2];
-        b[hh + 2] = a[hh + 0] - a[hh + 2];
-        b[hh + 1] = a[hh + 1] + a[hh + 3];
-        const cpx<T> d = a[hh + 1] - a[hh + 3];
-        b[hh + 3] = INV ? mk(-d.y, d.x) : mk(d.y, -d.x);
-    }
-    // stage 3 (span 1); outputs in bit-reversed order
-    const cpx<T> c0 = b[0] + b[1], c1 = b[0] - b[1], c2 = b[2] + b[3], c3 = b[2] - b[3];
-    const cpx<T> c4 = b[4] + b[5], c5 = b[4] - b[5], c6 = b[6] + b[7], c7 = b[6] - b[7];
-    v[0] = c0;
-    v[1] = c4;
-    v[2] = c2;
-    v[3] = c6;
-    v[4] = c1;
-    v[5] = c5;
-    v[6] = c3;
-    v[7] = c7;
-}
-
 // 512-point FFT of one wavefront.  tw = table exp(-2 pi i k / 512), k < 512 (LDS or global).
 // The scratch element of internal index i is sb[pad8(i) * S] (S = 1 for a private buffer,
 // S = tile row stride when a column of an LDS tile doubles as the scratch).
@@ -945,6 +899,112 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, const cpx<float>* _
         const cpx<float> zk = zs[pr][pad8(k)], zm = zs[pr][pad8((512 - k) & 511)];
         o1[k] = mk(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
         o2[k] = mk(0.5f * isg * (zk.y + zm.y), -0.5f * isg * (zk.x - zm.x));
+    }
+}
+
+// x-rows at M = 64, 128, 256 (FP32): one group of W = M / 8 lanes per (level, y) row holding
+// the row's three pair transforms in natural layout (fft_group), so the products form in
+// registers between the inverse and the forward transforms; 64 / W rows per wave, 4 waves per
+// block.  Same arithmetic as k_rows512t: both members of every input and product pair are
+// balanced by a power of two (pair_scale over the row, a shuffle reduction over the group).
+template <int LOGM>
+__global__ void __launch_bounds__(256) k_rows_w(DevGeom g, const cpx<float>* __restrict__ hin,
+                                                const cpx<float>* __restrict__ twM, cpx<float>* __restrict__ hout) {
+    constexpr int M = 1 << LOGM, W = M / 8, G = 64 / W, RPB = 4 * G, SP = M + M / 8;
+    __shared__ cpx<float> tw[M];
+    __shared__ cpx<float> scr[RPB][SP];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, grp = lane / W, j = lane - grp * W;
+    const int KX = g.KX, KXp = g.KXp, nq = g.nq, N = g.N;
+    const int y = blockIdx.x * RPB + wave * G + grp, l = blockIdx.y;
+    const size_t fstride = (size_t)nq * N * KXp;
+    const size_t roff = ((size_t)l * N + y) * KXp;
+    for (int e = threadIdx.x; e < M; e += blockDim.x) tw[e] = twM[e];
+    __syncthreads();
+    cpx<float>* sb = scr[wave * G + grp];
+    auto gmax = [](float x) {
+#pragma unroll
+        for (int o = W / 2; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o));
+        return x;
+    };
+    cpx<float> v[3][8];
+    float isg[3];
+#pragma unroll
+    for (int pr = 0; pr < 3; ++pr) {
+        const cpx<float>* x1 = hin + (size_t)(2 * pr) * fstride + roff;
+        const cpx<float>* x2 = hin + (size_t)(2 * pr + 1) * fstride + roff;
+        cpx<float> a[8], b[8];
+        float ma = 0.f, mb = 0.f;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int k = j + W * r;
+            const int src = k < KX ? k : (k > M - KX ? M - k : -1);
+            a[r] = b[r] = mk(0.f, 0.f);
+            if (src >= 0) {
+                a[r] = x1[src];
+                b[r] = x2[src];
+                if (src == 0) {  // a C2R ignores the imaginary part at kx = 0
+                    a[r].y = 0.f;
+                    b[r].y = 0.f;
+                }
+            }
+            ma = fmaxf(ma, fmaxf(fabsf(a[r].x), fabsf(a[r].y)));
+            mb = fmaxf(mb, fmaxf(fabsf(b[r].x), fabsf(b[r].y)));
+        }
+        const float sg = pair_scale(gmax(ma), gmax(mb));
+        isg[pr] = 1.f / sg;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int k = j + W * r;
+            const cpx<float> bs = mk(sg * b[r].x, sg * b[r].y);
+            if (k < KX) v[pr][r] = mk(a[r].x - bs.y, a[r].y + bs.x);               // X1 + i s X2
+            else if (k > M - KX) v[pr][r] = mk(a[r].x + bs.y, -a[r].y + bs.x);     // conj(X1) + i s conj(X2)
+            else v[pr][r] = mk(0.f, 0.f);
+        }
+    }
+    fft_group<LOGM, true, 3>(v, sb, tw, j);
+    // products at x = j + W r, in place, scaled by 1 / (M N)
+    const float s = 1.f / ((float)M * (float)N);
+    float pa[3] = {0.f, 0.f, 0.f}, pb[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const float u = v[0][r].x, vv = isg[0] * v[0][r].y, w = v[1][r].x, tx = isg[1] * v[1][r].y;
+        const float ty = v[2][r].x, tz = isg[2] * v[2][r].y;
+        v[0][r] = mk(s * (u * u + vv * vv - 2.f * w * w), s * (u * u - vv * vv));
+        v[1][r] = mk(s * u * vv, s * u * w);
+        v[2][r] = mk(s * vv * w, s * (u * tx + vv * ty + w * tz));
+#pragma unroll
+        for (int pr = 0; pr < 3; ++pr) {
+            pa[pr] = fmaxf(pa[pr], fabsf(v[pr][r].x));
+            pb[pr] = fmaxf(pb[pr], fabsf(v[pr][r].y));
+        }
+    }
+#pragma unroll
+    for (int pr = 0; pr < 3; ++pr) {
+        const float sg = pair_scale(gmax(pa[pr]), gmax(pb[pr]));
+        isg[pr] = 1.f / sg;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v[pr][r].y *= sg;
+    }
+    fft_group<LOGM, false, 3>(v, sb, tw, j);
+    // unpack X1_k = (Z_k + conj Z_{M-k}) / 2, X2_k = -i (Z_k - conj Z_{M-k}) / (2 s); Z_{M-k} of
+    // another lane comes through the scratch
+#pragma unroll
+    for (int pr = 0; pr < 3; ++pr) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) sb[pad8(j + W * r)] = v[pr][r];
+        wave_sync();
+        cpx<float>* o1 = hout + (size_t)(2 * pr) * fstride + roff;
+        cpx<float>* o2 = hout + (size_t)(2 * pr + 1) * fstride + roff;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int k = j + W * r;
+            if (k < KX) {
+                const cpx<float> zk = v[pr][r], zm = sb[pad8((M - k) & (M - 1))];
+                o1[k] = mk(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+                o2[k] = mk(0.5f * isg[pr] * (zk.y + zm.y), -0.5f * isg[pr] * (zk.x - zm.x));
+            }
+        }
+        wave_sync();  // the scratch is rewritten by the next pair
     }
 }
 
@@ -1420,6 +1480,19 @@ template <typename T>
 static bool fast512_y(const DevGeom& g) {
     return sizeof(T) == 4 && g.N == 512 && !generic_fft_forced();
 }
+// group (natural-layout) transforms for FP32 grids of 64, 128 and 256 points: log2 of the size,
+// or 0 when the generic kernels run
+template <typename T>
+static int wave_rows_log(const DevGeom& g) {
+    if (sizeof(T) != 4 || generic_fft_forced() || g.M < 64 || g.M > 256) return 0;
+    if (g.N % (2048 / g.M) != 0) return 0;  // rows per block must divide N
+    return g.logM;
+}
+template <typename T>
+static int wave_y_log(const DevGeom& g) {
+    if (sizeof(T) != 4 || generic_fft_forced() || g.N < 64 || g.N > 256) return 0;
+    return g.logN;
+}
 
 static inline int fft_tile(int N, size_t csz) {
     // two LDS buffers of TX columns; stay within 64 KiB
@@ -1472,6 +1545,9 @@ void Launch<T>::fft_y_inv(const DevGeom& g, const Buffers& b, hipStream_t s) {
                            (const cpx<float>*)b.lev, (const cpx<float>*)b.twN, (cpx<float>*)b.hin);
         return;
     }
+    if (wave_y_log<T>(g) &&
+        launch_y_inv_group(g, 6, (const cpx<float>*)b.lev, (const cpx<float>*)b.twN, (cpx<float>*)b.hin, s))
+        return;
     const int TX = b.tune_ytx > 0 ? b.tune_ytx : fft_tile(g.N, sizeof(cpx<T>));
     dim3 grid((g.KX + TX - 1) / TX, g.nq, 6);
     const size_t sm = (size_t)2 * TX * g.N * sizeof(cpx<T>);
@@ -1485,6 +1561,17 @@ void Launch<T>::rows(const DevGeom& g, const Buffers& b, hipStream_t s) {
     if (fast512_rows<T>(g)) {
         hipLaunchKernelGGL(k_rows512t, grid, dim3(192), 0, s, g, (const cpx<float>*)b.hin, (const cpx<float>*)b.twM,
                            (cpx<float>*)b.hout);
+        return;
+    }
+    if (const int lg = wave_rows_log<T>(g)) {
+        const dim3 gr(g.N / (2048 / g.M), g.nq);
+#define RBC_ROWSW(LG)                                                                                          \
+    hipLaunchKernelGGL(k_rows_w<LG>, gr, dim3(256), 0, s, g, (const cpx<float>*)b.hin, (const cpx<float>*)b.twM, \
+                       (cpx<float>*)b.hout)
+        if (lg == 6) RBC_ROWSW(6);
+        else if (lg == 7) RBC_ROWSW(7);
+        else RBC_ROWSW(8);
+#undef RBC_ROWSW
         return;
     }
     const int nth = b.tune_rows_threads > 0 ? b.tune_rows_threads : 256;
@@ -1519,11 +1606,14 @@ void Launch<T>::fft_y_fwd(const DevGeom& g, const Buffers& b, hipStream_t s) {
                            (const float*)b.ky, (cpx<float>*)b.lev, (cpx<float>*)b.fgw);
         return;
     }
-    const int TX = b.tune_ytx > 0 ? b.tune_ytx : fft_tile(g.N, sizeof(cpx<T>));
-    dim3 grid((g.KX + TX - 1) / TX, g.nq, 6);
-    const size_t sm = (size_t)2 * TX * g.N * sizeof(cpx<T>);
-    hipLaunchKernelGGL(k_fft_y_fwd<T>, grid, dim3(256), sm, s, g, TX, (const cpx<T>*)b.hout, b.rows,
-                       (const cpx<T>*)b.twN, (cpx<T>*)b.prod);
+    if (!(wave_y_log<T>(g) &&
+          launch_y_fwd_group(g, 6, (const cpx<float>*)b.hout, (const cpx<float>*)b.twN, (cpx<float>*)b.prod, s))) {
+        const int TX = b.tune_ytx > 0 ? b.tune_ytx : fft_tile(g.N, sizeof(cpx<T>));
+        dim3 grid((g.KX + TX - 1) / TX, g.nq, 6);
+        const size_t sm = (size_t)2 * TX * g.N * sizeof(cpx<T>);
+        hipLaunchKernelGGL(k_fft_y_fwd<T>, grid, dim3(256), sm, s, g, TX, (const cpx<T>*)b.hout, b.rows,
+                           (const cpx<T>*)b.twN, (cpx<T>*)b.prod);
+    }
     const size_t n = (size_t)g.nq * g.P;
     hipLaunchKernelGGL(k_combine<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, (const cpx<T>*)b.prod,
                        (const T*)b.kx, (const T*)b.ky, (cpx<T>*)b.lev, (cpx<T>*)b.fgw);

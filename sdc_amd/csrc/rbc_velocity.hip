@@ -586,11 +586,13 @@ void velocity_grid(rbc_ctx* c, const GridWork& w) {
     gt.nq = w.nz;
     if (c->rsz == 8) launch_vel_levels<double>(c, gt, w.nz, w.zb, w.spec);
     else launch_vel_levels<float>(c, gt, w.nz, w.zb, w.spec);
-    int TX = (int)(65536 / (2 * (size_t)gt.N * sizeof(cpx<float>)));
-    TX = std::max(1, std::min(16, TX));
-    dim3 gy((gt.KX + TX - 1) / TX, w.nz, 3);
-    hipLaunchKernelGGL(k_fft_y_inv<float>, gy, dim3(256), (size_t)2 * TX * gt.N * sizeof(cpx<float>), c->stream, gt,
-                       TX, (const cpx<float>*)w.spec, c->b.rows, (const cpx<float>*)w.twN, w.hyb);
+    if (!launch_y_inv_group(gt, 3, (const cpx<float>*)w.spec, (const cpx<float>*)w.twN, w.hyb, c->stream)) {
+        int TX = (int)(65536 / (2 * (size_t)gt.N * sizeof(cpx<float>)));
+        TX = std::max(1, std::min(16, TX));
+        dim3 gy((gt.KX + TX - 1) / TX, w.nz, 3);
+        hipLaunchKernelGGL(k_fft_y_inv<float>, gy, dim3(256), (size_t)2 * TX * gt.N * sizeof(cpx<float>), c->stream,
+                           gt, TX, (const cpx<float>*)w.spec, c->b.rows, (const cpx<float>*)w.twN, w.hyb);
+    }
     hipLaunchKernelGGL(k_vel_rows, dim3(gt.N, w.nz), dim3(256), (size_t)4 * gt.M * sizeof(cpx<float>), c->stream, gt,
                        (const cpx<float>*)w.hyb, (const cpx<float>*)w.twM, w.grid);
     HIPCHK(hipGetLastError());

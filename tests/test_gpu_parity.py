@@ -103,6 +103,28 @@ def test_nonlinear_shapes_fp64(M, N, L):
         assert max(e) < 1e-11, e
 
 
+@pytest.mark.parametrize("M,N,L", [(64, 32, 4), (64, 128, 8), (256, 64, 8), (128, 128, 16), (256, 256, 8)])
+def test_group_fft_kernels_fp32(M, N, L):
+    """FP32 nonlinear term through the natural-layout group transforms (x-rows and y-FFTs of 64,
+    128 and 256 points; N = 32 keeps the generic y-FFT) against the generic LDS kernels and the
+    FP64 oracle; 2e-5 relative."""
+    print(f"case: group-FFT fp32 {M}x{N}x{L}", flush=True)
+    cfg = dict(M=M, N=N, L=L, Lx=3.3, Ly=4.1, Ra=2e4, Pr=0.7, dt=2e-4)
+    st = random_state(M, N, L, seed=M + 2 * N + L, amp=1e-2, decay=0.05)
+    keep = kept_mask(M, N)
+    ref = _orc(cfg).nonlinear(*st)
+    with _gpu(cfg, "f32") as G:
+        group = G.calculate_nonlinear(*st)
+        os.environ["RBC_GENERIC_FFT"] = "1"
+        try:
+            generic = G.calculate_nonlinear(*st)
+        finally:
+            del os.environ["RBC_GENERIC_FFT"]
+    e1, e2, e3 = _cmp(group, ref, keep), _cmp(generic, ref, keep), _cmp(group, generic, keep)
+    print("group vs oracle", e1, "generic vs oracle", e2, "group vs generic", e3)
+    assert max(e1) < 2e-5 and max(e2) < 2e-5 and max(e3) < 2e-5
+
+
 def test_bundled_512_state_fp32_and_fp64():
     """The reference's bundled 512^2 x 2 state (FP32) through the nonlinear operator."""
     print("case: bundled 512x512x2 state", flush=True)
