@@ -40,8 +40,8 @@ def stage_bytes(st, cs, nprod):
         "project": (5 * nq * P + 3 * L * P) * cs,
         # state, N and history in/out; factor records once per |k|^2 group
         "update": (9 * L * P + 6 * L * P) * cs + st.n_unique_k * (2 * L * L + L * L // 2) * (cs // 2),
-        "update_FG": 8 * L * cs,
-        "make_real": 2 * 3 * st.ky_kept * L * cs,
+        # make_real of the kx = 0 column and the F, G update
+        "finish": 2 * 3 * st.ky_kept * L * cs + 8 * L * cs,
     }
 
 
@@ -218,6 +218,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="f32", choices=["f32", "bf16x3", "f64"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true", help="skip timing the other precision form")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-torch", action="store_true", help="bracket the timed region with the library's "
                     "stream synchronisation only (for profiler runs)")
@@ -287,6 +288,23 @@ def main():
                 "stage_GBs": {k: round(nbytes[k] / (stages[k] * 1e-3) / 1e9, 1) for k in stages if stages[k] > 0}}
     nq, kx, ky, ng = st.nq, st.kx_kept, st.ky_kept, st.n_unique_k
     st.close()
+    # the other single-GPU precision forms on the same workload (plain step_time, no stage events)
+    variants = {}
+    if not a.no_variants:
+        for prec in ("f32", "bf16x3"):
+            if prec == a.precision:
+                continue
+            sv = sdc_amd.Stepper(W["M"], W["N"], W["L"], W["Lx"], W["Ly"], W["Ra"], W["Pr"], W["dt"],
+                                 precision=prec, device=0)
+            sv.set_state(*random_theta(W["M"], W["N"], W["L"], seed=3, amp=1e-3))
+            sv.step_time(a.warmup)
+            sv.synchronize()
+            tv = time.perf_counter()
+            sv.step_time(a.steps)
+            sv.synchronize()
+            tv = time.perf_counter() - tv
+            variants[prec] = {"timesteps_per_s": round(a.steps / tv, 2), "ms_per_step": round(1e3 * tv / a.steps, 4)}
+            sv.close()
     if a.copy_calibrate > 0:
         copy_probe(a.copy_calibrate)
 
@@ -314,6 +332,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "precision_variants": variants,
     }
     print(json.dumps(out))
 

@@ -100,7 +100,7 @@ int rbc_nusselt(rbc_ctx* c, double* nu3);
 /* Per-stage timing with HIP events on the context's stream: runs nsteps steps and writes
  * the mean milliseconds per call of each stage into ms[RBC_NSTAGES] (stage names from
  * rbc_stage_name). */
-#define RBC_NSTAGES 8
+#define RBC_NSTAGES 7
 int rbc_profile_steps(rbc_ctx* c, int nsteps, double* ms);
 const char* rbc_stage_name(int i);
 

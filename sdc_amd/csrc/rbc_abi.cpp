@@ -149,11 +149,8 @@ void one_step(rbc_ctx* c, bool timed) {
     Launch<T>::update(c->dg, c->b, c->fac.ngroups(), c->fac.L0, c->fac.L1, c->fac.stride, c->p.Pr, c->p.dt, sc,
                       UPD_STEP, c->b.X, c->b.Nn, c->b.Rp, c->stream);
     if (timed) HIPCHK(hipEventRecord(c->ev[6], c->stream));
-    Launch<T>::update_FG(c->dg, c->p.Pr, c->p.dt, sc, UPD_STEP, c->b.XF, c->b.XG, c->b.NF, c->b.NG, c->b.RpF,
-                         c->b.RpG, c->stream);
+    Launch<T>::finish(c->dg, c->b, c->p.Pr, c->p.dt, sc, c->stream);  // make_real + F, G update
     if (timed) HIPCHK(hipEventRecord(c->ev[7], c->stream));
-    Launch<T>::make_real(c->dg, c->b, c->b.X, c->stream);
-    if (timed) HIPCHK(hipEventRecord(c->ev[8], c->stream));
     HIPCHK(hipGetLastError());
     c->have_hist = true;
 }
@@ -190,7 +187,7 @@ int hip_fail(rbc_ctx* c, const HipError& e) {
 namespace {
 
 const char* kStageNames[RBC_NSTAGES] = {"eval", "fft_y_inv", "rows_products", "fft_y_fwd",
-                                        "project", "update", "update_FG", "make_real"};
+                                        "project", "update", "finish"};
 
 }  // namespace
 

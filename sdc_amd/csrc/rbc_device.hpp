@@ -303,14 +303,14 @@ __global__ void __launch_bounds__(256) k_fft_y_fwd(DevGeom g, int TX, const cpx<
     }
 }
 
-// y-FFTs at N = 64, 128, 256 (FP32): a block of 4 waves handles CB = 4 (64 / W) kx columns of
-// one (field, level), one group of W = N / 8 lanes per column (fft_group in natural layout);
+// y-FFTs at N = 64, 128, 256 (FP32): a block of WPB waves handles CB = WPB (64 / W) kx columns
+// of one (field, level), one group of W = N / 8 lanes per column (fft_group in natural layout);
 // the LDS tile [pad8(y)][CB + 1] is each column's FFT scratch and the transpose to the
 // row-major hybrid layout (rows of CB contiguous values).
-template <int LOGN>
-__global__ void __launch_bounds__(256) k_fft_y_inv_w(DevGeom g, const cpx<float>* __restrict__ lev,
+template <int LOGN, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_fft_y_inv_w(DevGeom g, const cpx<float>* __restrict__ lev,
                                                      const cpx<float>* __restrict__ twN, cpx<float>* __restrict__ hin) {
-    constexpr int N = 1 << LOGN, W = N / 8, G = 64 / W, CB = 4 * G, S = CB + 1;
+    constexpr int N = 1 << LOGN, W = N / 8, G = 64 / W, CB = WPB * G, S = CB + 1;
     __shared__ cpx<float> tw[N];
     __shared__ cpx<float> tile[(N + N / 8) * S];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, grp = lane / W, j = lane - grp * W;
@@ -343,10 +343,10 @@ __global__ void __launch_bounds__(256) k_fft_y_inv_w(DevGeom g, const cpx<float>
     }
 }
 
-template <int LOGN>
-__global__ void __launch_bounds__(256) k_fft_y_fwd_w(DevGeom g, const cpx<float>* __restrict__ hout,
+template <int LOGN, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_fft_y_fwd_w(DevGeom g, const cpx<float>* __restrict__ hout,
                                                      const cpx<float>* __restrict__ twN, cpx<float>* __restrict__ prod) {
-    constexpr int N = 1 << LOGN, W = N / 8, G = 64 / W, CB = 4 * G, S = CB + 1;
+    constexpr int N = 1 << LOGN, W = N / 8, G = 64 / W, CB = WPB * G, S = CB + 1;
     __shared__ cpx<float> tw[N];
     __shared__ cpx<float> tile[(N + N / 8) * S];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, grp = lane / W, j = lane - grp * W;
@@ -379,26 +379,39 @@ __global__ void __launch_bounds__(256) k_fft_y_fwd_w(DevGeom g, const cpx<float>
 }
 
 // Launchers of the group y-FFTs for nf fields of g.nq levels each ([field][l][p] spectra,
-// [field][l][y][kx] hybrid rows); false (nothing launched) unless N is 64, 128 or 256.
-inline bool launch_y_inv_group(const DevGeom& g, int nf, const cpx<float>* lev, const cpx<float>* twN,
-                               cpx<float>* hin, hipStream_t s) {
-    if (g.N < 64 || g.N > 256) return false;
-    const int CB = 4 * 512 / g.N;
+// [field][l][y][kx] hybrid rows); cb = kx columns per block (WPB = cb / (512 / N) waves of 2, 4
+// or 8; 0: 4 waves).  false (nothing launched) unless N is 64, 128 or 256.
+template <int LOGN, int WPB>
+inline void launch_y_group_t(bool inv, const DevGeom& g, int nf, const cpx<float>* in, const cpx<float>* twN,
+                             cpx<float>* out, hipStream_t s) {
+    constexpr int CB = WPB * 512 / (1 << LOGN);
     const dim3 gr((g.KX + CB - 1) / CB, g.nq, nf);
-    if (g.N == 64) hipLaunchKernelGGL(k_fft_y_inv_w<6>, gr, dim3(256), 0, s, g, lev, twN, hin);
-    else if (g.N == 128) hipLaunchKernelGGL(k_fft_y_inv_w<7>, gr, dim3(256), 0, s, g, lev, twN, hin);
-    else hipLaunchKernelGGL(k_fft_y_inv_w<8>, gr, dim3(256), 0, s, g, lev, twN, hin);
+    if (inv) hipLaunchKernelGGL((k_fft_y_inv_w<LOGN, WPB>), gr, dim3(64 * WPB), 0, s, g, in, twN, out);
+    else hipLaunchKernelGGL((k_fft_y_fwd_w<LOGN, WPB>), gr, dim3(64 * WPB), 0, s, g, in, twN, out);
+}
+template <int LOGN>
+inline void launch_y_group_w(bool inv, int wpb, const DevGeom& g, int nf, const cpx<float>* in,
+                             const cpx<float>* twN, cpx<float>* out, hipStream_t s) {
+    if (wpb == 2) launch_y_group_t<LOGN, 2>(inv, g, nf, in, twN, out, s);
+    else if (wpb == 8) launch_y_group_t<LOGN, 8>(inv, g, nf, in, twN, out, s);
+    else launch_y_group_t<LOGN, 4>(inv, g, nf, in, twN, out, s);
+}
+inline bool launch_y_group(bool inv, const DevGeom& g, int nf, const cpx<float>* in, const cpx<float>* twN,
+                           cpx<float>* out, hipStream_t s, int cb = 0) {
+    if (g.N < 64 || g.N > 256) return false;
+    const int wpb = cb > 0 ? cb / (512 / g.N) : 4;
+    if (g.N == 64) launch_y_group_w<6>(inv, wpb, g, nf, in, twN, out, s);
+    else if (g.N == 128) launch_y_group_w<7>(inv, wpb, g, nf, in, twN, out, s);
+    else launch_y_group_w<8>(inv, wpb, g, nf, in, twN, out, s);
     return true;
 }
+inline bool launch_y_inv_group(const DevGeom& g, int nf, const cpx<float>* lev, const cpx<float>* twN,
+                               cpx<float>* hin, hipStream_t s, int cb = 0) {
+    return launch_y_group(true, g, nf, lev, twN, hin, s, cb);
+}
 inline bool launch_y_fwd_group(const DevGeom& g, int nf, const cpx<float>* hout, const cpx<float>* twN,
-                               cpx<float>* prod, hipStream_t s) {
-    if (g.N < 64 || g.N > 256) return false;
-    const int CB = 4 * 512 / g.N;
-    const dim3 gr((g.KX + CB - 1) / CB, g.nq, nf);
-    if (g.N == 64) hipLaunchKernelGGL(k_fft_y_fwd_w<6>, gr, dim3(256), 0, s, g, hout, twN, prod);
-    else if (g.N == 128) hipLaunchKernelGGL(k_fft_y_fwd_w<7>, gr, dim3(256), 0, s, g, hout, twN, prod);
-    else hipLaunchKernelGGL(k_fft_y_fwd_w<8>, gr, dim3(256), 0, s, g, hout, twN, prod);
-    return true;
+                               cpx<float>* prod, hipStream_t s, int cb = 0) {
+    return launch_y_group(false, g, nf, hout, twN, prod, s, cb);
 }
 
 }  // namespace rbc

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <string>
 #include <tuple>
@@ -194,22 +195,39 @@ int rbc_ensemble_tune(rbc_ensemble* e, double* report) {
             Snapshot snap;
             take(c, snap);
             const size_t cs = c->csz();
-            std::vector<int> txs;
-            for (int tx = 1; tx <= 16; tx *= 2)
-                if ((size_t)2 * tx * c->dg.N * cs <= 65536) txs.push_back(tx);
-            // coordinate search: y-FFT tile first (rows at 256 threads), then the row kernel
-            int btx = txs.back(), bth = 256;
+            const char* gen = getenv("RBC_GENERIC_FFT");
+            const bool forced = gen && gen[0] == '1';
+            // FP32 grids of 64..256 points run the group transforms: their knobs are the kx
+            // columns per y-FFT block (2, 4 or 8 waves) and the threads per row block (2, 4 or
+            // 8 waves); the generic LDS kernels take the column tile and 64..256 threads
+            const bool gy = c->rsz == 4 && !forced && c->dg.N >= 64 && c->dg.N <= 256;
+            const bool gr = c->rsz == 4 && !forced && c->dg.M >= 64 && c->dg.M <= 256;
+            std::vector<int> txs, ths;
+            if (gy) {
+                for (int w : {2, 4, 8}) txs.push_back(w * 512 / c->dg.N);
+            } else {
+                for (int tx = 1; tx <= 16; tx *= 2)
+                    if ((size_t)2 * tx * c->dg.N * cs <= 65536) txs.push_back(tx);
+            }
+            if (gr) {
+                for (int w : {2, 4, 8})
+                    if (c->dg.N % (w * 512 / c->dg.M) == 0) ths.push_back(64 * w);
+            } else {
+                ths = {64, 128, 256};
+            }
+            // coordinate search: y-FFT tile first (rows at their default), then the row kernel
+            int btx = txs.back(), bth = 0;
             double bms = 1e30;
             for (int tx : txs) {
                 c->b.tune_ytx = tx;
-                c->b.tune_rows_threads = 256;
+                c->b.tune_rows_threads = 0;
                 const double ms = time_steps(c, 3);
                 if (ms < bms) {
                     bms = ms;
                     btx = tx;
                 }
             }
-            for (int th : {64, 128}) {
+            for (int th : ths) {
                 c->b.tune_ytx = btx;
                 c->b.tune_rows_threads = th;
                 const double ms = time_steps(c, 3);

@@ -157,6 +157,8 @@ struct Launch {
     static void update_FG(const DevGeom& g, double Pr, double dt, StepCoef sc, int mode,
                           void* XF, void* XG, const void* NF, const void* NG, void* RpF,
                           void* RpG, hipStream_t s);
+    // make_real of X and the F, G update of a time step (mode UPD_STEP) in one launch
+    static void finish(const DevGeom& g, const Buffers& b, double Pr, double dt, StepCoef sc, hipStream_t s);
     // make_real on the kx = 0 column (util.cu make_real)
     static void make_real(const DevGeom& g, const Buffers& b, void* const X[3], hipStream_t s);
     // gather/scatter between the reference layout (host-staged, T complex) and [p][m]

@@ -654,11 +654,11 @@ __global__ void __launch_bounds__(256) k_update32(DevGeom g, int nchunk, int L0,
     }
 }
 
-// F, G: inverse B = 2 Pr (imag 0), implicit factor 1/(1 + dt Pr (m pi)^2) (imag 0).
+// F, G: inverse B = 2 Pr (imag 0), implicit factor 1/(1 + dt Pr (m pi)^2) (imag 0); thread m.
 template <typename T>
-__global__ void k_update_FG(int L, T Pr, T dt, T a_cur, T a_prev, int use_prev, int mode, cpx<T>* XF,
-                            cpx<T>* XG, const cpx<T>* NF, const cpx<T>* NG, cpx<T>* RF, cpx<T>* RG) {
-    const int m = threadIdx.x;
+__device__ __forceinline__ void update_FG_mode(int m, int L, T Pr, T dt, T a_cur, T a_prev, int use_prev, int mode,
+                                               cpx<T>* XF, cpx<T>* XG, const cpx<T>* NF, const cpx<T>* NG,
+                                               cpx<T>* RF, cpx<T>* RG) {
     if (m >= L) return;
     const T mpi = T(m + 1) * T(M_PI);
     const T fac = T(1) / (T(1) + dt * Pr * mpi * mpi);
@@ -679,6 +679,11 @@ __global__ void k_update_FG(int L, T Pr, T dt, T a_cur, T a_prev, int use_prev, 
         }
     }
 }
+template <typename T>
+__global__ void k_update_FG(int L, T Pr, T dt, T a_cur, T a_prev, int use_prev, int mode, cpx<T>* XF,
+                            cpx<T>* XG, const cpx<T>* NF, const cpx<T>* NG, cpx<T>* RF, cpx<T>* RG) {
+    update_FG_mode<T>(threadIdx.x, L, Pr, dt, a_cur, a_prev, use_prev, mode, XF, XG, NF, NG, RF, RG);
+}
 
 // make_real (util.cu): on the kx = 0 column set row r (0 < r < N/2) to conj(row N - r), and
 // clear the imaginary part of (0, 0).  Kept rows are symmetric, so source and target are
@@ -696,6 +701,33 @@ __global__ void k_make_real(DevGeom g, const int* __restrict__ rows, cpx<T>* X0,
         X[m].y = T(0);
     } else if (row < N / 2) {
         // source row N - row is at compact index NYc - kyc (rows symmetric, 0 first)
+        const int src = NYc - kyc;
+        const cpx<T> s = X[(size_t)src * L + m];
+        X[(size_t)kyc * L + m] = conjg(s);
+    }
+}
+
+// the end of a time step in one launch: make_real on the blocks before the last, the F, G
+// update (mode UPD_STEP) on the last block
+template <typename T>
+__global__ void __launch_bounds__(256) k_finish(DevGeom g, const int* __restrict__ rows, cpx<T>* X0, cpx<T>* X1,
+                                                cpx<T>* X2, T Pr, T dt, T a_cur, T a_prev, int use_prev, cpx<T>* XF,
+                                                cpx<T>* XG, const cpx<T>* NF, const cpx<T>* NG, cpx<T>* RF,
+                                                cpx<T>* RG) {
+    if (blockIdx.x == gridDim.x - 1) {
+        update_FG_mode<T>(threadIdx.x, g.L, Pr, dt, a_cur, a_prev, use_prev, UPD_STEP, XF, XG, NF, NG, RF, RG);
+        return;
+    }
+    const int L = g.L, NYc = g.NYc, N = g.N;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 3 * NYc * L) return;
+    const int fld = e / (NYc * L), rem = e - fld * NYc * L;
+    const int kyc = rem / L, m = rem - kyc * L;
+    cpx<T>* X = fld == 0 ? X0 : (fld == 1 ? X1 : X2);
+    const int row = rows[kyc];
+    if (row == 0) {
+        X[m].y = T(0);
+    } else if (row < N / 2) {
         const int src = NYc - kyc;
         const cpx<T> s = X[(size_t)src * L + m];
         X[(size_t)kyc * L + m] = conjg(s);
@@ -904,13 +936,13 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, const cpx<float>* _
 
 // x-rows at M = 64, 128, 256 (FP32): one group of W = M / 8 lanes per (level, y) row holding
 // the row's three pair transforms in natural layout (fft_group), so the products form in
-// registers between the inverse and the forward transforms; 64 / W rows per wave, 4 waves per
+// registers between the inverse and the forward transforms; 64 / W rows per wave, WPB waves per
 // block.  Same arithmetic as k_rows512t: both members of every input and product pair are
 // balanced by a power of two (pair_scale over the row, a shuffle reduction over the group).
-template <int LOGM>
-__global__ void __launch_bounds__(256) k_rows_w(DevGeom g, const cpx<float>* __restrict__ hin,
+template <int LOGM, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_rows_w(DevGeom g, const cpx<float>* __restrict__ hin,
                                                 const cpx<float>* __restrict__ twM, cpx<float>* __restrict__ hout) {
-    constexpr int M = 1 << LOGM, W = M / 8, G = 64 / W, RPB = 4 * G, SP = M + M / 8;
+    constexpr int M = 1 << LOGM, W = M / 8, G = 64 / W, RPB = WPB * G, SP = M + M / 8;
     __shared__ cpx<float> tw[M];
     __shared__ cpx<float> scr[RPB][SP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, grp = lane / W, j = lane - grp * W;
@@ -1485,7 +1517,6 @@ static bool fast512_y(const DevGeom& g) {
 template <typename T>
 static int wave_rows_log(const DevGeom& g) {
     if (sizeof(T) != 4 || generic_fft_forced() || g.M < 64 || g.M > 256) return 0;
-    if (g.N % (2048 / g.M) != 0) return 0;  // rows per block must divide N
     return g.logM;
 }
 template <typename T>
@@ -1545,8 +1576,8 @@ void Launch<T>::fft_y_inv(const DevGeom& g, const Buffers& b, hipStream_t s) {
                            (const cpx<float>*)b.lev, (const cpx<float>*)b.twN, (cpx<float>*)b.hin);
         return;
     }
-    if (wave_y_log<T>(g) &&
-        launch_y_inv_group(g, 6, (const cpx<float>*)b.lev, (const cpx<float>*)b.twN, (cpx<float>*)b.hin, s))
+    if (wave_y_log<T>(g) && launch_y_inv_group(g, 6, (const cpx<float>*)b.lev, (const cpx<float>*)b.twN,
+                                               (cpx<float>*)b.hin, s, b.tune_ytx))
         return;
     const int TX = b.tune_ytx > 0 ? b.tune_ytx : fft_tile(g.N, sizeof(cpx<T>));
     dim3 grid((g.KX + TX - 1) / TX, g.nq, 6);
@@ -1564,15 +1595,23 @@ void Launch<T>::rows(const DevGeom& g, const Buffers& b, hipStream_t s) {
         return;
     }
     if (const int lg = wave_rows_log<T>(g)) {
-        const dim3 gr(g.N / (2048 / g.M), g.nq);
-#define RBC_ROWSW(LG)                                                                                          \
-    hipLaunchKernelGGL(k_rows_w<LG>, gr, dim3(256), 0, s, g, (const cpx<float>*)b.hin, (const cpx<float>*)b.twM, \
-                       (cpx<float>*)b.hout)
-        if (lg == 6) RBC_ROWSW(6);
-        else if (lg == 7) RBC_ROWSW(7);
-        else RBC_ROWSW(8);
+        const int wpb = b.tune_rows_threads > 0 ? b.tune_rows_threads / 64 : 4;
+        if (g.N % (wpb * 512 / g.M) == 0) {
+            const dim3 gr(g.N / (wpb * 512 / g.M), g.nq);
+#define RBC_ROWSW(LG, WPBV)                                                                                        \
+    hipLaunchKernelGGL((k_rows_w<LG, WPBV>), gr, dim3(64 * WPBV), 0, s, g, (const cpx<float>*)b.hin,                \
+                       (const cpx<float>*)b.twM, (cpx<float>*)b.hout)
+#define RBC_ROWSW_L(LG)              \
+    if (wpb == 2) RBC_ROWSW(LG, 2);  \
+    else if (wpb == 8) RBC_ROWSW(LG, 8); \
+    else RBC_ROWSW(LG, 4)
+            if (lg == 6) { RBC_ROWSW_L(6); }
+            else if (lg == 7) { RBC_ROWSW_L(7); }
+            else { RBC_ROWSW_L(8); }
+#undef RBC_ROWSW_L
 #undef RBC_ROWSW
-        return;
+            return;
+        }
     }
     const int nth = b.tune_rows_threads > 0 ? b.tune_rows_threads : 256;
     // rows per block: RB M = 512 for M >= 64 (at most 8 rows, at most nth / 32 so that a row keeps
@@ -1606,8 +1645,8 @@ void Launch<T>::fft_y_fwd(const DevGeom& g, const Buffers& b, hipStream_t s) {
                            (const float*)b.ky, (cpx<float>*)b.lev, (cpx<float>*)b.fgw);
         return;
     }
-    if (!(wave_y_log<T>(g) &&
-          launch_y_fwd_group(g, 6, (const cpx<float>*)b.hout, (const cpx<float>*)b.twN, (cpx<float>*)b.prod, s))) {
+    if (!(wave_y_log<T>(g) && launch_y_fwd_group(g, 6, (const cpx<float>*)b.hout, (const cpx<float>*)b.twN,
+                                                 (cpx<float>*)b.prod, s, b.tune_ytx))) {
         const int TX = b.tune_ytx > 0 ? b.tune_ytx : fft_tile(g.N, sizeof(cpx<T>));
         dim3 grid((g.KX + TX - 1) / TX, g.nq, 6);
         const size_t sm = (size_t)2 * TX * g.N * sizeof(cpx<T>);
@@ -1688,6 +1727,15 @@ void Launch<T>::make_real(const DevGeom& g, const Buffers& b, void* const X[3], 
     const int n = 3 * g.NYc * g.L;
     hipLaunchKernelGGL(k_make_real<T>, dim3((n + 255) / 256), dim3(256), 0, s, g, b.rows, (cpx<T>*)X[0],
                        (cpx<T>*)X[1], (cpx<T>*)X[2]);
+}
+
+template <typename T>
+void Launch<T>::finish(const DevGeom& g, const Buffers& b, double Pr, double dt, StepCoef sc, hipStream_t s) {
+    const int n = 3 * g.NYc * g.L;
+    hipLaunchKernelGGL(k_finish<T>, dim3((n + 255) / 256 + 1), dim3(256), 0, s, g, b.rows, (cpx<T>*)b.X[0],
+                       (cpx<T>*)b.X[1], (cpx<T>*)b.X[2], (T)Pr, (T)dt, (T)sc.a_cur, (T)sc.a_prev, sc.use_prev,
+                       (cpx<T>*)b.XF, (cpx<T>*)b.XG, (const cpx<T>*)b.NF, (const cpx<T>*)b.NG, (cpx<T>*)b.RpF,
+                       (cpx<T>*)b.RpG);
 }
 
 template <typename T>

@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(HERE, "libsdc_rbc.so")
 
 F64, F32, BF16X3 = 0, 1, 2
 PRECISIONS = {"f64": F64, "f32": F32, "bf16x3": BF16X3}
-NSTAGES = 8
+NSTAGES = 7
 
 
 class RbcError(RuntimeError):
