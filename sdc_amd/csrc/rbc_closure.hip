@@ -1,4 +1,4 @@
-// rbc_closure.hip -- learned closure for coarse runs: a pointwise MLP (8 -> 64 -> 64 -> 3,
+// rbc_closure.hip -- learned closure for coarse runs: a pointwise MLP (16 -> 64 -> 64 -> 3,
 // tanh, BF16 weights, FP32 arithmetic) trained on the GPU to predict, at the points of a
 // coarse grid, the part of the nonlinear term that the coarse grid misses.
 //
@@ -7,8 +7,10 @@
 // where P_c truncates to the coarse kept modes and vertical modes (the Galerkin bases are
 // nested), N_f / N_c are the fine / coarse nonlinear terms (scaled as the stepper uses them).
 // R is synthesised at the coarse points (x, y, z_l) as r_theta = sum S_m R_theta,m,
-// r_f = sum C_m R_f,m, r_g = sum S_m R_g,m; the inputs are the local coarse fields
-// (u, v, w, theta, d_x theta, d_y theta, d_z theta, z).
+// r_f = sum C_m R_f,m, r_g = sum S_m R_g,m; the inputs are the local coarse fields and
+// their horizontal derivatives (u, v, w, theta, d_x theta, d_y theta, d_z theta, d_x u, d_y u,
+// d_x v, d_y v, d_x w, d_y w, lap_h theta, lap_h w, z): the horizontal velocity gradients carry
+// the local strain that sub-grid fluxes follow.
 // Correction (rbc_closure_enable): inside every coarse step, the predicted r at the points
 // is projected back onto the coarse modes (R2C, forward y-FFT, Gauss-Legendre projection
 // on S_m / C_m) and added to N before the inverse-B / AB2 / implicit update.
@@ -28,14 +30,14 @@ using namespace rbc;
 
 namespace {
 
-constexpr int NIN = 8, NH = 64, NOUT = 3, NFEAT = 7;
+constexpr int NIN = 16, NH = 64, NOUT = 3, NFEAT = 15;
 constexpr int OFF_W1 = 0;                   // [NH][NIN]
 constexpr int OFF_B1 = OFF_W1 + NH * NIN;   // [NH]
 constexpr int OFF_W2 = OFF_B1 + NH;         // [NH][NH]  (row j = hidden unit j of layer 2)
 constexpr int OFF_B2 = OFF_W2 + NH * NH;    // [NH]
 constexpr int OFF_W3 = OFF_B2 + NH;         // [NOUT][NH]
 constexpr int OFF_B3 = OFF_W3 + NOUT * NH;  // [NOUT]
-constexpr int NW = OFF_B3 + NOUT;           // 4931 parameters
+constexpr int NW = OFF_B3 + NOUT;           // 5443 parameters
 constexpr int kTrainWaves = 512;
 
 struct Norm {
@@ -60,7 +62,8 @@ __host__ __device__ inline uint64_t mix64(uint64_t x) {
 }
 
 // ---- point synthesis ---------------------------------------------------------------------
-// features at the levels: spec[f][l][p], f = u, v, w, theta, theta_x, theta_y, theta_z;
+// features at the levels: spec[f][l][p], f = u, v, w, theta, theta_x, theta_y, theta_z, u_x, u_y,
+// v_x, v_y, w_x, w_y, lap_h theta, lap_h w;
 // zb[l][4][L] = C', C, S, S' (FP64).  Block = MB modes (MB = 64, 32 or 16, so that LDS holds
 // them): their [MB][L] slabs of theta, f, g are staged in LDS as [field][m][mode] (coalesced
 // reads, each mode read once for all levels), the mean flow at every level is staged by the
@@ -122,7 +125,10 @@ __global__ void __launch_bounds__(256) k_feat_levels(DevGeom g, int nz, int MB, 
             u = u + smf[l];
             v = v + smf[nz + l];
         }
-        const cpx<T> vals[NFEAT] = {u, v, cscale(Cc, k2), Ts, imul(Ts, kx), imul(Ts, ky), Tz};
+        const cpx<T> w = cscale(Cc, k2);
+        const cpx<T> vals[NFEAT] = {u, v, w, Ts, imul(Ts, kx), imul(Ts, ky), Tz,
+                                    imul(u, kx), imul(u, ky), imul(v, kx), imul(v, ky), imul(w, kx), imul(w, ky),
+                                    cscale(Ts, -k2), cscale(w, -k2)};
 #pragma unroll
         for (int f = 0; f < NFEAT; ++f)
             spec[f * plane + (size_t)l * P + p] = mk<float>((float)vals[f].x, (float)vals[f].y);
@@ -366,7 +372,8 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int npts, const uint16_t* __r
         bf16x8 xh, xl;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            const float v = (h == 0 && valid) ? (X[(size_t)pt * NIN + e] - nm.in_mean[e]) * nm.in_inv[e] : 0.f;
+            const int k = 8 * h + e;  // K index of this lane's B element
+            const float v = (k < NIN && valid) ? (X[(size_t)pt * NIN + k] - nm.in_mean[k]) * nm.in_inv[k] : 0.f;
             xh[e] = bf16_of(v);
             xl[e] = bf16_of(v - f32_of(xh[e]));
         }
@@ -552,8 +559,9 @@ __global__ void k_adam(const float* __restrict__ grad, float* __restrict__ wm, f
 }
 
 // per-column sums and sums of squares (FP64) of a row-major [n][nc] array, per block
-__global__ void k_colstats(int n, int nc, const float* __restrict__ A, double* __restrict__ part) {
-    __shared__ double s[2 * NIN][256];  // nc <= NIN
+__global__ void __launch_bounds__(128) k_colstats(int n, int nc, const float* __restrict__ A,
+                                                  double* __restrict__ part) {
+    __shared__ double s[2 * NIN][128];  // nc <= NIN; 128 threads
     double acc[2 * NIN];
     for (int c = 0; c < 2 * nc; ++c) acc[c] = 0.0;
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
@@ -564,7 +572,7 @@ __global__ void k_colstats(int n, int nc, const float* __restrict__ A, double* _
         }
     for (int c = 0; c < 2 * nc; ++c) s[c][threadIdx.x] = acc[c];
     __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
+    for (int h = 64; h > 0; h >>= 1) {
         if (threadIdx.x < h)
             for (int c = 0; c < 2 * nc; ++c) s[c][threadIdx.x] += s[c][threadIdx.x + h];
         __syncthreads();
@@ -600,7 +608,7 @@ struct rbc_closure {
     // scratch
     cpx<float>* spec = nullptr;       // [7][nz][P]
     cpx<float>* hyb = nullptr;        // [7][nz][N][KXp]
-    float* feat = nullptr;            // [npts][8]
+    float* feat = nullptr;            // [npts][NIN]
     float* pred = nullptr;            // [npts][3]
     double* guard = nullptr;          // [nblk][2]
     int nguard = 0;
@@ -612,7 +620,7 @@ struct rbc_closure {
     int pmap_M = 0, pmap_N = 0, pmap_L = 0;
     void* Nt[3] = {};                 // restricted fine N (coarse precision)
     // dataset
-    float* DX = nullptr;              // [cap][8]
+    float* DX = nullptr;              // [cap][NIN]
     float* DY = nullptr;              // [cap][3]
     // network
     float* wm = nullptr;              // FP32 master
@@ -861,9 +869,9 @@ int rbc_closure_create(rbc_ctx* c, int nz, int capacity_points, rbc_closure** ou
     if (nz <= 0) nz = 2 * c->dg.L;
     if (nz > 512) return fail(c, RBC_ERR_ARG, "nz must be <= 512");
     if (capacity_points < 0) return fail(c, RBC_ERR_ARG, "capacity_points must be >= 0");
-    // one LDS transform per input field and row: 2 * 7 * M complex floats within 64 KiB
+    // one LDS transform per input field and row: 2 * 15 * M complex floats within 64 KiB
     if ((size_t)2 * NFEAT * c->dg.M * sizeof(cpx<float>) > 65536)
-        return fail(c, RBC_ERR_ARG, "the closure supports coarse grids with M <= 512");
+        return fail(c, RBC_ERR_ARG, "the closure supports coarse grids with M <= 256");
     const size_t npts = (size_t)c->dg.M * c->dg.N * nz;
     if (npts > (size_t)1 << 30) return fail(c, RBC_ERR_ARG, "too many closure points");
     rbc_closure* cl = new rbc_closure;
@@ -1079,10 +1087,10 @@ int rbc_closure_fit_normalization(rbc_closure* cl, int n) {
         hipSetDevice(cl->c->p.device);
         const int nb = 256;
         double* part = nullptr;
-        HIPCHK(hipMalloc(&part, (size_t)nb * 2 * 16 * sizeof(double)));
-        std::vector<double> h((size_t)nb * 2 * 16);
+        HIPCHK(hipMalloc(&part, (size_t)nb * 2 * NIN * sizeof(double)));
+        std::vector<double> h((size_t)nb * 2 * NIN);
         auto stats = [&](const float* A, int nc, std::vector<double>& mean, std::vector<double>& sd) {
-            hipLaunchKernelGGL(k_colstats, dim3(nb), dim3(256), 0, cl->c->stream, n, nc, A, part);
+            hipLaunchKernelGGL(k_colstats, dim3(nb), dim3(128), 0, cl->c->stream, n, nc, A, part);
             HIPCHK(hipMemcpyAsync(h.data(), part, (size_t)nb * 2 * nc * sizeof(double), hipMemcpyDeviceToHost,
                                   cl->c->stream));
             HIPCHK(hipStreamSynchronize(cl->c->stream));
