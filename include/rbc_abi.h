@@ -211,9 +211,8 @@ int rbc_ensemble_tune(rbc_ensemble* e, double* report);
 int rbc_ensemble_step(rbc_ensemble* e, int nsteps, int steps_per_launch);
 
 /* ---- Learned closure for coarse runs -------------------------------------------------------
- * No counterpart in the reference.  A pointwise MLP (16 inputs: u, v, w, theta, d_x theta,
- * d_y theta, d_z theta, d_x u, d_y u, d_x v, d_y v, d_x w, d_y w, lap_h theta, lap_h w, z; two
- * tanh layers of 64; outputs r_theta, r_f, r_g; 5443 parameters,
+ * No counterpart in the reference.  A pointwise MLP (inputs u, v, w, theta, d_x theta,
+ * d_y theta, d_z theta, z; two tanh layers of 64; outputs r_theta, r_f, r_g; 4931 parameters,
  * BF16 weights, FP32 arithmetic) attached to a coarse context.  The points are the coarse
  * M x N grid at nz Gauss-Legendre levels (0: 2 L; at most 512).
  *   rbc_closure_sample        appends to the dataset (capacity_points) the inputs and the

@@ -4,7 +4,7 @@ synthesis / projection that move between coarse modes and coarse points.
 
 TEST INFRASTRUCTURE: only tests/ imports this module.
 
-Weight vector layout (5443 floats): W1[64][16], b1[64], W2[64][64], b2[64], W3[3][64], b3[3];
+Weight vector layout (4931 floats): W1[64][8], b1[64], W2[64][64], b2[64], W3[3][64], b3[3];
 hidden activations tanh; loss = mean over points and outputs of (y - t / out_scale)^2 with
 inputs normalised as (x - in_mean) * in_inv and predictions y * out_scale.
 """
@@ -13,7 +13,7 @@ import numpy as np
 from oracle import oracle as O
 from oracle import velocity as V
 
-NIN, NH, NOUT = 16, 64, 3
+NIN, NH, NOUT = 8, 64, 3
 OFF_W1 = 0
 OFF_B1 = OFF_W1 + NH * NIN
 OFF_W2 = OFF_B1 + NH
@@ -112,9 +112,8 @@ def _synth(coefs, M, N):
 
 
 def features(state, M, N, Lx, Ly, z):
-    """Inputs at the points (x, y, z_l), row-major pt = (l N + y) M + x: (npts, 16) =
-    u, v, w, theta, d_x theta, d_y theta, d_z theta, d_x u, d_y u, d_x v, d_y v, d_x w, d_y w,
-    lap_h theta, lap_h w, z (horizontal derivatives taken spectrally on the coarse grid)."""
+    """Inputs at the points (x, y, z_l), row-major pt = (l N + y) M + x: (npts, 8) =
+    u, v, w, theta, d_x theta, d_y theta, d_z theta, z."""
     th = state[0]
     L = th.shape[0]
     u, v, w = V.velocity_grid(state, M, N, Lx, Ly, z)
@@ -126,16 +125,7 @@ def features(state, M, N, Lx, Ly, z):
         Sp[i] = np.pi * (i + 1) * np.cos(np.pi * (i + 1) * (np.clip(z, -0.5, 0.5) + 0.5))
     ts = np.einsum("mz,myx->zyx", S, th)
     tz = np.einsum("mz,myx->zyx", Sp, th)
-    K2 = KX ** 2 + KY ** 2
-
-    def spectral(a):  # physical level fields -> (nz, N, mox) spectra (the inverse of _synth)
-        return np.fft.rfft2(a) / (M * N)
-
-    us, vs, ws = spectral(u), spectral(v), spectral(w)
-    fields = [u, v, w, _synth(ts, M, N), _synth(1j * KX * ts, M, N), _synth(1j * KY * ts, M, N), _synth(tz, M, N),
-              _synth(1j * KX * us, M, N), _synth(1j * KY * us, M, N), _synth(1j * KX * vs, M, N),
-              _synth(1j * KY * vs, M, N), _synth(1j * KX * ws, M, N), _synth(1j * KY * ws, M, N),
-              _synth(-K2 * ts, M, N), _synth(-K2 * ws, M, N)]
+    fields = [u, v, w, _synth(ts, M, N), _synth(1j * KX * ts, M, N), _synth(1j * KY * ts, M, N), _synth(tz, M, N)]
     nz = len(z)
     zz = np.broadcast_to(np.asarray(z, np.float64)[:, None, None], (nz, N, M))
     return np.stack([f.ravel() for f in fields] + [zz.ravel()], axis=1)

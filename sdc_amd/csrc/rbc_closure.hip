@@ -1,4 +1,4 @@
-// rbc_closure.hip -- learned closure for coarse runs: a pointwise MLP (16 -> 64 -> 64 -> 3,
+// rbc_closure.hip -- learned closure for coarse runs: a pointwise MLP (8 -> 64 -> 64 -> 3,
 // tanh, BF16 weights, FP32 arithmetic) trained on the GPU to predict, at the points of a
 // coarse grid, the part of the nonlinear term that the coarse grid misses.
 //
@@ -7,10 +7,8 @@
 // where P_c truncates to the coarse kept modes and vertical modes (the Galerkin bases are
 // nested), N_f / N_c are the fine / coarse nonlinear terms (scaled as the stepper uses them).
 // R is synthesised at the coarse points (x, y, z_l) as r_theta = sum S_m R_theta,m,
-// r_f = sum C_m R_f,m, r_g = sum S_m R_g,m; the inputs are the local coarse fields and
-// their horizontal derivatives (u, v, w, theta, d_x theta, d_y theta, d_z theta, d_x u, d_y u,
-// d_x v, d_y v, d_x w, d_y w, lap_h theta, lap_h w, z): the horizontal velocity gradients carry
-// the local strain that sub-grid fluxes follow.
+// r_f = sum C_m R_f,m, r_g = sum S_m R_g,m; the inputs are the local coarse fields
+// (u, v, w, theta, d_x theta, d_y theta, d_z theta, z).
 // Correction (rbc_closure_enable): inside every coarse step, the predicted r at the points
 // is projected back onto the coarse modes (R2C, forward y-FFT, Gauss-Legendre projection
 // on S_m / C_m) and added to N before the inverse-B / AB2 / implicit update.
@@ -30,14 +28,14 @@ using namespace rbc;
 
 namespace {
 
-constexpr int NIN = 16, NH = 64, NOUT = 3, NFEAT = 15;
+constexpr int NIN = 8, NH = 64, NOUT = 3, NFEAT = 7;
 constexpr int OFF_W1 = 0;                   // [NH][NIN]
 constexpr int OFF_B1 = OFF_W1 + NH * NIN;   // [NH]
 constexpr int OFF_W2 = OFF_B1 + NH;         // [NH][NH]  (row j = hidden unit j of layer 2)
 constexpr int OFF_B2 = OFF_W2 + NH * NH;    // [NH]
 constexpr int OFF_W3 = OFF_B2 + NH;         // [NOUT][NH]
 constexpr int OFF_B3 = OFF_W3 + NOUT * NH;  // [NOUT]
-constexpr int NW = OFF_B3 + NOUT;           // 5443 parameters
+constexpr int NW = OFF_B3 + NOUT;           // 4931 parameters
 constexpr int kTrainWaves = 512;
 
 struct Norm {
@@ -62,8 +60,7 @@ __host__ __device__ inline uint64_t mix64(uint64_t x) {
 }
 
 // ---- point synthesis ---------------------------------------------------------------------
-// features at the levels: spec[f][l][p], f = u, v, w, theta, theta_x, theta_y, theta_z, u_x, u_y,
-// v_x, v_y, w_x, w_y, lap_h theta, lap_h w;
+// features at the levels: spec[f][l][p], f = u, v, w, theta, theta_x, theta_y, theta_z;
 // zb[l][4][L] = C', C, S, S' (FP64).  Block = MB modes (MB = 64, 32 or 16, so that LDS holds
 // them): their [MB][L] slabs of theta, f, g are staged in LDS as [field][m][mode] (coalesced
 // reads, each mode read once for all levels), the mean flow at every level is staged by the
@@ -125,10 +122,7 @@ __global__ void __launch_bounds__(256) k_feat_levels(DevGeom g, int nz, int MB, 
             u = u + smf[l];
             v = v + smf[nz + l];
         }
-        const cpx<T> w = cscale(Cc, k2);
-        const cpx<T> vals[NFEAT] = {u, v, w, Ts, imul(Ts, kx), imul(Ts, ky), Tz,
-                                    imul(u, kx), imul(u, ky), imul(v, kx), imul(v, ky), imul(w, kx), imul(w, ky),
-                                    cscale(Ts, -k2), cscale(w, -k2)};
+        const cpx<T> vals[NFEAT] = {u, v, cscale(Cc, k2), Ts, imul(Ts, kx), imul(Ts, ky), Tz};
 #pragma unroll
         for (int f = 0; f < NFEAT; ++f)
             spec[f * plane + (size_t)l * P + p] = mk<float>((float)vals[f].x, (float)vals[f].y);
@@ -214,6 +208,141 @@ __global__ void __launch_bounds__(256) k_anal_rows(DevGeom g, const float* __res
     for (int e = threadIdx.x; e < NOUT * KX; e += blockDim.x) {
         const int f = e / KX, k = e - f * KX;
         hyb[(size_t)f * fstride + roff + k] = cscale(R[f * M + k], s);
+    }
+}
+
+// Group-transform versions (FP32, M = 64, 128, 256; see fft_group): one group of W = M / 8
+// lanes per (y, l) row, WPB = 4 waves per block.  The real fields go two at a time through one
+// complex transform with the second balanced to the first by a power of two (pair_scale over
+// the row), so each keeps the rounding of its own size.
+template <int LOGM>
+__global__ void __launch_bounds__(256) k_synth_rows_w(DevGeom g, int F, int os, int zcol,
+                                                      const float* __restrict__ zl, const cpx<float>* __restrict__ hyb,
+                                                      const cpx<float>* __restrict__ twM, float* __restrict__ out) {
+    constexpr int M = 1 << LOGM, W = M / 8, G = 64 / W, RPB = 4 * G, SP = M + M / 8;
+    __shared__ cpx<float> tw[M];
+    __shared__ cpx<float> scr[RPB][SP];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, grp = lane / W, j = lane - grp * W;
+    const int N = g.N, KX = g.KX, KXp = g.KXp, nz = g.nq;
+    const int y = blockIdx.x * RPB + wave * G + grp, l = blockIdx.y;
+    const size_t fstride = (size_t)nz * N * KXp, roff = ((size_t)l * N + y) * KXp;
+    for (int e = threadIdx.x; e < M; e += blockDim.x) tw[e] = twM[e];
+    __syncthreads();
+    cpx<float>* sb = scr[wave * G + grp];
+    auto gmax = [](float x) {
+#pragma unroll
+        for (int o = W / 2; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o));
+        return x;
+    };
+    float* orow = out + ((size_t)l * N + y) * M * os;
+    for (int f0 = 0; f0 < F; f0 += 2) {
+        const bool two = f0 + 1 < F;
+        const cpx<float>* x1 = hyb + (size_t)f0 * fstride + roff;
+        const cpx<float>* x2 = hyb + (size_t)(two ? f0 + 1 : f0) * fstride + roff;
+        cpx<float> a[8], b[8];
+        float ma = 0.f, mb = 0.f;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int k = j + W * r;
+            const int src = k < KX ? k : (k > M - KX ? M - k : -1);
+            a[r] = b[r] = mk(0.f, 0.f);
+            if (src >= 0) {
+                a[r] = x1[src];
+                if (two) b[r] = x2[src];
+                if (src == 0) {  // a C2R ignores the imaginary part at kx = 0
+                    a[r].y = 0.f;
+                    b[r].y = 0.f;
+                }
+            }
+            ma = fmaxf(ma, fmaxf(fabsf(a[r].x), fabsf(a[r].y)));
+            mb = fmaxf(mb, fmaxf(fabsf(b[r].x), fabsf(b[r].y)));
+        }
+        const float sg = pair_scale(gmax(ma), gmax(mb)), isg = 1.f / sg;
+        cpx<float> v[1][8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int k = j + W * r;
+            const cpx<float> bs = mk(sg * b[r].x, sg * b[r].y);
+            if (k < KX) v[0][r] = mk(a[r].x - bs.y, a[r].y + bs.x);            // X1 + i s X2
+            else if (k > M - KX) v[0][r] = mk(a[r].x + bs.y, -a[r].y + bs.x);  // conj(X1) + i s conj(X2)
+            else v[0][r] = mk(0.f, 0.f);
+        }
+        fft_group<LOGM, true, 1>(v, sb, tw, j);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            float* o = orow + (size_t)(j + W * r) * os + f0;
+            o[0] = v[0][r].x;
+            if (two) o[1] = isg * v[0][r].y;
+        }
+        wave_sync();  // the scratch serves the next pair
+    }
+    if (zcol >= 0) {
+        const float zv = zl[l];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) orow[(size_t)(j + W * r) * os + zcol] = zv;
+    }
+}
+
+// R2C of the three predicted fields per (y, l) row: (r_theta + i s r_f) and r_g as two
+// complex transforms, unpacked and scaled by 1 / (M N)
+template <int LOGM>
+__global__ void __launch_bounds__(256) k_anal_rows_w(DevGeom g, const float* __restrict__ in,
+                                                     const cpx<float>* __restrict__ twM, cpx<float>* __restrict__ hyb) {
+    constexpr int M = 1 << LOGM, W = M / 8, G = 64 / W, RPB = 4 * G, SP = M + M / 8;
+    __shared__ cpx<float> tw[M];
+    __shared__ cpx<float> scr[RPB][SP];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, grp = lane / W, j = lane - grp * W;
+    const int N = g.N, KX = g.KX, KXp = g.KXp, nz = g.nq;
+    const int y = blockIdx.x * RPB + wave * G + grp, l = blockIdx.y;
+    const size_t fstride = (size_t)nz * N * KXp, roff = ((size_t)l * N + y) * KXp;
+    for (int e = threadIdx.x; e < M; e += blockDim.x) tw[e] = twM[e];
+    __syncthreads();
+    cpx<float>* sb = scr[wave * G + grp];
+    auto gmax = [](float x) {
+#pragma unroll
+        for (int o = W / 2; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o));
+        return x;
+    };
+    const float* irow = in + ((size_t)l * N + y) * M * NOUT;
+    float r0[8], r1[8], r2[8], m0 = 0.f, m1 = 0.f;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const float* p = irow + (size_t)(j + W * r) * NOUT;
+        r0[r] = p[0];
+        r1[r] = p[1];
+        r2[r] = p[2];
+        m0 = fmaxf(m0, fabsf(r0[r]));
+        m1 = fmaxf(m1, fabsf(r1[r]));
+    }
+    const float sg = pair_scale(gmax(m0), gmax(m1)), isg = 1.f / sg;
+    const float s = 1.0f / ((float)M * (float)N);
+    cpx<float> v[2][8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        v[0][r] = mk(r0[r], sg * r1[r]);
+        v[1][r] = mk(r2[r], 0.f);
+    }
+    fft_group<LOGM, false, 2>(v, sb, tw, j);
+    // field 2 directly; fields 0, 1 from Z_k and Z_{M-k} (another lane: through the scratch)
+    cpx<float>* o2 = hyb + (size_t)2 * fstride + roff;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int k = j + W * r;
+        if (k < KX) o2[k] = cscale(v[1][r], s);
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) sb[pad8(j + W * r)] = v[0][r];
+    wave_sync();
+    cpx<float>* o0 = hyb + roff;
+    cpx<float>* o1 = hyb + fstride + roff;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int k = j + W * r;
+        if (k < KX) {
+            const cpx<float> zk = v[0][r], zm = sb[pad8((M - k) & (M - 1))];
+            o0[k] = mk(0.5f * s * (zk.x + zm.x), 0.5f * s * (zk.y - zm.y));
+            o1[k] = mk(0.5f * s * isg * (zk.y + zm.y), -0.5f * s * isg * (zk.x - zm.x));
+        }
     }
 }
 
@@ -372,8 +501,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int npts, const uint16_t* __r
         bf16x8 xh, xl;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            const int k = 8 * h + e;  // K index of this lane's B element
-            const float v = (k < NIN && valid) ? (X[(size_t)pt * NIN + k] - nm.in_mean[k]) * nm.in_inv[k] : 0.f;
+            const float v = (h == 0 && valid) ? (X[(size_t)pt * NIN + e] - nm.in_mean[e]) * nm.in_inv[e] : 0.f;
             xh[e] = bf16_of(v);
             xl[e] = bf16_of(v - f32_of(xh[e]));
         }
@@ -559,9 +687,8 @@ __global__ void k_adam(const float* __restrict__ grad, float* __restrict__ wm, f
 }
 
 // per-column sums and sums of squares (FP64) of a row-major [n][nc] array, per block
-__global__ void __launch_bounds__(128) k_colstats(int n, int nc, const float* __restrict__ A,
-                                                  double* __restrict__ part) {
-    __shared__ double s[2 * NIN][128];  // nc <= NIN; 128 threads
+__global__ void k_colstats(int n, int nc, const float* __restrict__ A, double* __restrict__ part) {
+    __shared__ double s[2 * NIN][256];  // nc <= NIN
     double acc[2 * NIN];
     for (int c = 0; c < 2 * nc; ++c) acc[c] = 0.0;
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
@@ -572,7 +699,7 @@ __global__ void __launch_bounds__(128) k_colstats(int n, int nc, const float* __
         }
     for (int c = 0; c < 2 * nc; ++c) s[c][threadIdx.x] = acc[c];
     __syncthreads();
-    for (int h = 64; h > 0; h >>= 1) {
+    for (int h = 128; h > 0; h >>= 1) {
         if (threadIdx.x < h)
             for (int c = 0; c < 2 * nc; ++c) s[c][threadIdx.x] += s[c][threadIdx.x + h];
         __syncthreads();
@@ -608,7 +735,7 @@ struct rbc_closure {
     // scratch
     cpx<float>* spec = nullptr;       // [7][nz][P]
     cpx<float>* hyb = nullptr;        // [7][nz][N][KXp]
-    float* feat = nullptr;            // [npts][NIN]
+    float* feat = nullptr;            // [npts][8]
     float* pred = nullptr;            // [npts][3]
     double* guard = nullptr;          // [nblk][2]
     int nguard = 0;
@@ -620,7 +747,7 @@ struct rbc_closure {
     int pmap_M = 0, pmap_N = 0, pmap_L = 0;
     void* Nt[3] = {};                 // restricted fine N (coarse precision)
     // dataset
-    float* DX = nullptr;              // [cap][NIN]
+    float* DX = nullptr;              // [cap][8]
     float* DY = nullptr;              // [cap][3]
     // network
     float* wm = nullptr;              // FP32 master
@@ -696,6 +823,17 @@ void synth_points(rbc_closure* cl, int F, int os, int zcol, float* out) {
                            (size_t)2 * TX * g.N * sizeof(cpx<float>), c->stream, g, TX, (const cpx<float>*)cl->spec,
                            c->b.rows, (const cpx<float>*)cl->twN, cl->hyb);
     }
+    if (g.M >= 64 && g.M <= 256 && g.N % (2048 / g.M) == 0) {
+        const dim3 gr(g.N / (2048 / g.M), cl->nz);
+#define RBC_SYNTHW(LG)                                                                                    \
+    hipLaunchKernelGGL(k_synth_rows_w<LG>, gr, dim3(256), 0, c->stream, g, F, os, zcol, (const float*)cl->zl, \
+                       (const cpx<float>*)cl->hyb, (const cpx<float>*)cl->twM, out)
+        if (g.M == 64) RBC_SYNTHW(6);
+        else if (g.M == 128) RBC_SYNTHW(7);
+        else RBC_SYNTHW(8);
+#undef RBC_SYNTHW
+        return;
+    }
     hipLaunchKernelGGL(k_synth_rows, dim3(g.N, cl->nz), dim3(256), (size_t)2 * F * g.M * sizeof(cpx<float>),
                        c->stream, g, F, os, zcol, (const float*)cl->zl, (const cpx<float>*)cl->hyb,
                        (const cpx<float>*)cl->twM, out);
@@ -737,8 +875,21 @@ void apply_closure(rbc_ctx* c, void* user) {
     hipLaunchKernelGGL(k_mlp_infer, dim3(infer_blocks(cl->npts)), dim3(256), 0, c->stream, cl->npts,
                        (const uint16_t*)cl->wbf, cl->nm, (const float*)cl->feat, cl->pred);
     const DevGeom& g = cl->gz;
-    hipLaunchKernelGGL(k_anal_rows, dim3(g.N, cl->nz), dim3(256), (size_t)2 * NOUT * g.M * sizeof(cpx<float>), c->stream, g,
-                       (const float*)cl->pred, (const cpx<float>*)cl->twM, cl->hyb);
+    if (g.M >= 64 && g.M <= 256 && g.N % (2048 / g.M) == 0) {
+        const dim3 gr(g.N / (2048 / g.M), cl->nz);
+        if (g.M == 64)
+            hipLaunchKernelGGL(k_anal_rows_w<6>, gr, dim3(256), 0, c->stream, g, (const float*)cl->pred,
+                               (const cpx<float>*)cl->twM, cl->hyb);
+        else if (g.M == 128)
+            hipLaunchKernelGGL(k_anal_rows_w<7>, gr, dim3(256), 0, c->stream, g, (const float*)cl->pred,
+                               (const cpx<float>*)cl->twM, cl->hyb);
+        else
+            hipLaunchKernelGGL(k_anal_rows_w<8>, gr, dim3(256), 0, c->stream, g, (const float*)cl->pred,
+                               (const cpx<float>*)cl->twM, cl->hyb);
+    } else {
+        hipLaunchKernelGGL(k_anal_rows, dim3(g.N, cl->nz), dim3(256), (size_t)2 * NOUT * g.M * sizeof(cpx<float>),
+                           c->stream, g, (const float*)cl->pred, (const cpx<float>*)cl->twM, cl->hyb);
+    }
     if (!launch_y_fwd_group(g, 3, cl->hyb, cl->twN, cl->spec, c->stream)) {
         const int TX = y_tile(g.N);
         hipLaunchKernelGGL(k_fft_y_fwd<float>, dim3((g.KX + TX - 1) / TX, cl->nz, 3), dim3(256),
@@ -869,9 +1020,9 @@ int rbc_closure_create(rbc_ctx* c, int nz, int capacity_points, rbc_closure** ou
     if (nz <= 0) nz = 2 * c->dg.L;
     if (nz > 512) return fail(c, RBC_ERR_ARG, "nz must be <= 512");
     if (capacity_points < 0) return fail(c, RBC_ERR_ARG, "capacity_points must be >= 0");
-    // one LDS transform per input field and row: 2 * 15 * M complex floats within 64 KiB
+    // one LDS transform per input field and row: 2 * 7 * M complex floats within 64 KiB
     if ((size_t)2 * NFEAT * c->dg.M * sizeof(cpx<float>) > 65536)
-        return fail(c, RBC_ERR_ARG, "the closure supports coarse grids with M <= 256");
+        return fail(c, RBC_ERR_ARG, "the closure supports coarse grids with M <= 512");
     const size_t npts = (size_t)c->dg.M * c->dg.N * nz;
     if (npts > (size_t)1 << 30) return fail(c, RBC_ERR_ARG, "too many closure points");
     rbc_closure* cl = new rbc_closure;
@@ -1087,10 +1238,10 @@ int rbc_closure_fit_normalization(rbc_closure* cl, int n) {
         hipSetDevice(cl->c->p.device);
         const int nb = 256;
         double* part = nullptr;
-        HIPCHK(hipMalloc(&part, (size_t)nb * 2 * NIN * sizeof(double)));
-        std::vector<double> h((size_t)nb * 2 * NIN);
+        HIPCHK(hipMalloc(&part, (size_t)nb * 2 * 16 * sizeof(double)));
+        std::vector<double> h((size_t)nb * 2 * 16);
         auto stats = [&](const float* A, int nc, std::vector<double>& mean, std::vector<double>& sd) {
-            hipLaunchKernelGGL(k_colstats, dim3(nb), dim3(128), 0, cl->c->stream, n, nc, A, part);
+            hipLaunchKernelGGL(k_colstats, dim3(nb), dim3(256), 0, cl->c->stream, n, nc, A, part);
             HIPCHK(hipMemcpyAsync(h.data(), part, (size_t)nb * 2 * nc * sizeof(double), hipMemcpyDeviceToHost,
                                   cl->c->stream));
             HIPCHK(hipStreamSynchronize(cl->c->stream));

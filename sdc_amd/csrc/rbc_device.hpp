@@ -98,6 +98,22 @@ __device__ __forceinline__ void dft8(cpx<T> v[8]) {
     v[7] = c7;
 }
 
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+// power of two that brings the row maximum mb of the second field of a pair to that of the
+// first (ma), so that one complex transform carries both with the rounding of each set by its
+// own size; scaling by a power of two is exact
+__device__ __forceinline__ float pair_scale(float ma, float mb) {
+    if (!(mb > 0.f) || !(ma > 0.f)) return 1.f;
+    int ea, eb;
+    frexpf(ma, &ea);
+    frexpf(mb, &eb);
+    return ldexpf(1.f, max(-100, min(100, ea - eb)));
+}
+
 // radix-4 butterfly in place (natural order outputs): X_p = sum_q a_q (-+i)^{pq}
 template <bool INV>
 __device__ __forceinline__ void dft4(cpx<float>& a0, cpx<float>& a1, cpx<float>& a2, cpx<float>& a3) {

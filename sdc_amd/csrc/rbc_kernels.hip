@@ -815,22 +815,6 @@ __device__ __forceinline__ void fft512_wave(cpx<T> v[8], cpx<T>* sb, const cpx<T
     wave_sync();  // the scratch may be reused right after
 }
 
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
-    return v;
-}
-// power of two that brings the row maximum mb of the second field of a pair to that of the
-// first (ma), so that one complex transform carries both with the rounding of each set by its
-// own size; scaling by a power of two is exact
-__device__ __forceinline__ float pair_scale(float ma, float mb) {
-    if (!(mb > 0.f) || !(ma > 0.f)) return 1.f;
-    int ea, eb;
-    frexpf(ma, &ea);
-    frexpf(mb, &eb);
-    return ldexpf(1.f, max(-100, min(100, ea - eb)));
-}
-
 // x-rows at M = N = 512 for the fused forward y-FFT: three waves per (level, y) row.  Phase 1:
 // wave pr inverse-transforms the pair (2 pr, 2 pr + 1) of level fields.  Phase 2: the six
 // real product fields h = uu + vv - 2 ww, d = uu - vv, uv, uw, vw, u.grad(theta) go back in

@@ -56,7 +56,7 @@ EXPORTS = ["rbc_create", "rbc_destroy", "rbc_last_error", "rbc_get_info", "rbc_s
            "rbc_closure_predict", "rbc_closure_eval", "rbc_closure_enable", "rbc_closure_interval", "rbc_closure_step",
            "rbc_closure_status"]
 
-CLOSURE_NIN, CLOSURE_NOUT = 16, 3
+CLOSURE_NIN, CLOSURE_NOUT = 8, 3
 
 TRACER_MODES = {"rk4_trilinear": 0, "spectral_rk2": 1}
 PART_INIT = {"random": 0, "center": 1, "center_sigma": 2, "diagonal": 3, "pairs": 4, "coord_sigma": 5}

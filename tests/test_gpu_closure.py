@@ -104,30 +104,32 @@ def test_training_reduces_loss_on_a_learnable_map():
     assert mse < 0.2 * zero
 
 
-def test_point_data_vs_oracle():
-    """Inputs and residual targets of a sample (fine 64^2 x 8 -> coarse 32^2 x 4, FP64
-    contexts) against the oracle's nonlinear terms, truncation and synthesis."""
-    print("case: closure sample", flush=True)
+@pytest.mark.parametrize("fm,cm", [(64, 32), (128, 64)])
+def test_point_data_vs_oracle(fm, cm):
+    """Inputs and residual targets of a sample (fine fm^2 x 8 -> coarse cm^2 x 4, FP64
+    contexts; cm = 64 runs the group-transform synthesis) against the oracle's nonlinear terms,
+    truncation and synthesis."""
+    print(f"case: closure sample {fm} -> {cm}", flush=True)
     from sdc_amd import Closure
-    st = random_state(FINE["M"], FINE["N"], FINE["L"], seed=9, amp=1e-1)
-    with _stepper(FINE) as F, _stepper(COARSE) as Cc:
+    fine, coarse = dict(FINE, M=fm, N=fm), dict(COARSE, M=cm, N=cm)
+    st = random_state(fm, fm, fine["L"], seed=9, amp=1e-1)
+    with _stepper(fine) as F, _stepper(coarse) as Cc:
         F.set_state(*st)
-        cl = Closure(Cc, nz=8, capacity_points=32 * 32 * 8)
+        cl = Closure(Cc, nz=8, capacity_points=cm * cm * 8)
         X, Y = cl.points(F)
         nz = cl.nz
         cl.close()
-    of = O.Stepper(FINE["M"], FINE["N"], FINE["L"], FINE["Lx"], FINE["Ly"], FINE["Ra"], FINE["Pr"], FINE["dt"],
+    of = O.Stepper(fm, fm, fine["L"], fine["Lx"], fine["Ly"], fine["Ra"], fine["Pr"], fine["dt"], nthreads=8)
+    oc = O.Stepper(cm, cm, coarse["L"], coarse["Lx"], coarse["Ly"], coarse["Ra"], coarse["Pr"], coarse["dt"],
                    nthreads=8)
-    oc = O.Stepper(COARSE["M"], COARSE["N"], COARSE["L"], COARSE["Lx"], COARSE["Ly"], COARSE["Ra"], COARSE["Pr"],
-                   COARSE["dt"], nthreads=8)
     Nf = of.nonlinear(*st)
-    stc = [C.truncate(a, 32, 32, 4) for a in st]
+    stc = [C.truncate(a, cm, cm, 4) for a in st]
     Nc = oc.nonlinear(*stc)
-    keep = C.kept_mask(32, 32)  # the oracle also returns the removed band, which no step sees
-    R = [C.truncate(Nf[i], 32, 32, 4) - Nc[i] * keep for i in range(3)]
+    keep = C.kept_mask(cm, cm)  # the oracle also returns the removed band, which no step sees
+    R = [C.truncate(Nf[i], cm, cm, 4) - Nc[i] * keep for i in range(3)]
     z, _ = O.gauss_legendre(nz)
-    Xr = C.features(stc, 32, 32, COARSE["Lx"], COARSE["Ly"], z)
-    Yr = C.residual_points(R, 32, 32, z)
+    Xr = C.features(stc, cm, cm, coarse["Lx"], coarse["Ly"], z)
+    Yr = C.residual_points(R, cm, cm, z)
     of.close()
     oc.close()
     ex = [float(np.abs(X[:, j] - Xr[:, j]).max() / max(np.abs(Xr[:, j]).max(), 1e-30)) for j in range(C.NIN)]
@@ -136,17 +138,19 @@ def test_point_data_vs_oracle():
     assert max(ex) <= 2e-5 and max(ey) <= 2e-5
 
 
-def test_correction_inside_the_step():
+@pytest.mark.parametrize("cm", [32, 64])
+def test_correction_inside_the_step(cm):
     """One corrected Euler step differs from the plain one by Implicit(dt B^-1 dN), dN the
-    projection of the network's prediction at the points."""
-    print("case: closure correction", flush=True)
+    projection of the network's prediction at the points (cm = 64: group-transform analysis)."""
+    print(f"case: closure correction {cm}", flush=True)
     from sdc_amd import Closure
-    st = random_state(COARSE["M"], COARSE["N"], COARSE["L"], seed=12, amp=1e-1)
+    coarse = dict(COARSE, M=cm, N=cm)
+    st = random_state(cm, cm, coarse["L"], seed=12, amp=1e-1)
     w, norm, _, _ = _net(3)
-    with _stepper(COARSE) as A, _stepper(COARSE) as B:
+    with _stepper(coarse) as A, _stepper(coarse) as B:
         A.set_state(*st)
         B.set_state(*st)
-        cl = Closure(A, nz=8, capacity_points=32 * 32 * 8)
+        cl = Closure(A, nz=8, capacity_points=cm * cm * 8)
         cl.set_weights(w)
         cl.set_normalization(*norm)
         X, _ = cl.points(A)  # inputs of A's own state
@@ -157,10 +161,10 @@ def test_correction_inside_the_step():
         B.step_time(1)
         xa, xb = A.get_state(), B.get_state()
         z, wz = O.gauss_legendre(cl.nz)
-        dN = C.project_points(P.astype(np.float64), 32, 32, 4, z, wz)
+        dN = C.project_points(P.astype(np.float64), cm, cm, 4, z, wz)
         zeros = np.zeros(4, complex)
         r = B.calculate_inverse(dN[0], dN[1], dN[2], zeros, zeros)
-        e = B.calculate_implicit(*[COARSE["dt"] * a for a in r[:3]], zeros, zeros)
+        e = B.calculate_implicit(*[coarse["dt"] * a for a in r[:3]], zeros, zeros)
         cl.close()
     for i in range(3):
         d = xa[i] - xb[i]
