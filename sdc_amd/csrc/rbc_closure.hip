@@ -257,7 +257,7 @@ __global__ void __launch_bounds__(256) k_synth_rows_w(DevGeom g, int F, int os, 
             ma = fmaxf(ma, fmaxf(fabsf(a[r].x), fabsf(a[r].y)));
             mb = fmaxf(mb, fmaxf(fabsf(b[r].x), fabsf(b[r].y)));
         }
-        const float sg = pair_scale(gmax(ma), gmax(mb)), isg = 1.f / sg;
+        const float sg = pair_scale(gmax(ma), gmax(mb)), isg = inv_pow2(sg);
         cpx<float> v[1][8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
@@ -314,7 +314,7 @@ __global__ void __launch_bounds__(256) k_anal_rows_w(DevGeom g, const float* __r
         m0 = fmaxf(m0, fabsf(r0[r]));
         m1 = fmaxf(m1, fabsf(r1[r]));
     }
-    const float sg = pair_scale(gmax(m0), gmax(m1)), isg = 1.f / sg;
+    const float sg = pair_scale(gmax(m0), gmax(m1)), isg = inv_pow2(sg);
     const float s = 1.0f / ((float)M * (float)N);
     cpx<float> v[2][8];
 #pragma unroll

@@ -113,6 +113,12 @@ __device__ __forceinline__ float pair_scale(float ma, float mb) {
     frexpf(mb, &eb);
     return ldexpf(1.f, max(-100, min(100, ea - eb)));
 }
+// 1 / s of a power of two s (exact; two bit-field instructions instead of a division)
+__device__ __forceinline__ float inv_pow2(float s) {
+    int e;
+    frexpf(s, &e);  // s = 0.5 * 2^e
+    return ldexpf(1.f, 1 - e);
+}
 
 // radix-4 butterfly in place (natural order outputs): X_p = sum_q a_q (-+i)^{pq}
 template <bool INV>

@@ -815,6 +815,11 @@ __device__ __forceinline__ void fft512_wave(cpx<T> v[8], cpx<T>* sb, const cpx<T
     wave_sync();  // the scratch may be reused right after
 }
 
+// Kept columns at M = 512 and kept rows at N = 512 (the dealias rule of make_geometry: kx <= 171,
+// |ky| <= 170); the 512-point kernels take them as constants so that the zero band and the
+// mirrored half resolve at compile time (the launchers check them).
+constexpr int KX512 = 172, NYC512 = 341;
+
 // x-rows at M = N = 512 for the fused forward y-FFT: three waves per (level, y) row.  Phase 1:
 // wave pr inverse-transforms the pair (2 pr, 2 pr + 1) of level fields.  Phase 2: the six
 // real product fields h = uu + vv - 2 ww, d = uu - vv, uv, uw, vw, u.grad(theta) go back in
@@ -827,7 +832,8 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, const cpx<float>* _
     __shared__ cpx<float> tw[512];
     __shared__ cpx<float> zs[3][576];
     const int lane = threadIdx.x & 63, pr = threadIdx.x >> 6;
-    const int KX = g.KX, KXp = g.KXp, nq = g.nq, N = g.N;
+    constexpr int KX = KX512;
+    const int KXp = g.KXp, nq = g.nq, N = g.N;
     const int y = blockIdx.x, l = blockIdx.y;
     const size_t fstride = (size_t)nq * N * KXp;
     const size_t roff = ((size_t)l * N + y) * KXp;
@@ -853,7 +859,7 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, const cpx<float>* _
             ma = fmaxf(ma, fmaxf(fabsf(av[r].x), fabsf(av[r].y)));
             mb = fmaxf(mb, fmaxf(fabsf(bv[r].x), fabsf(bv[r].y)));
         }
-        const float sg = pair_scale(wave_max(ma), wave_max(mb)), isg = 1.f / sg;
+        const float sg = pair_scale(wave_max(ma), wave_max(mb)), isg = inv_pow2(sg);
         // spectrum n < KX from (av, bv) at r = n / 64; n > 512 - KX mirrors 512 - n < KX,
         // which sits in lane (512 - n) % 64 of slot (512 - n) / 64 of another lane: read it
         // from LDS after a staging pass
@@ -901,7 +907,7 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, const cpx<float>* _
         ma = fmaxf(ma, fabsf(p[r].x));
         mb = fmaxf(mb, fabsf(p[r].y));
     }
-    const float sg = pair_scale(wave_max(ma), wave_max(mb)), isg = 1.f / sg;
+    const float sg = pair_scale(wave_max(ma), wave_max(mb)), isg = inv_pow2(sg);
 #pragma unroll
     for (int r = 0; r < 8; ++r) p[r].y *= sg;
     __syncthreads();  // every wave has read the slots before they become scratch
@@ -967,7 +973,7 @@ __global__ void __launch_bounds__(64 * WPB) k_rows_w(DevGeom g, const cpx<float>
             mb = fmaxf(mb, fmaxf(fabsf(b[r].x), fabsf(b[r].y)));
         }
         const float sg = pair_scale(gmax(ma), gmax(mb));
-        isg[pr] = 1.f / sg;
+        isg[pr] = inv_pow2(sg);
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
             const int k = j + W * r;
@@ -997,7 +1003,7 @@ __global__ void __launch_bounds__(64 * WPB) k_rows_w(DevGeom g, const cpx<float>
 #pragma unroll
     for (int pr = 0; pr < 3; ++pr) {
         const float sg = pair_scale(gmax(pa[pr]), gmax(pb[pr]));
-        isg[pr] = 1.f / sg;
+        isg[pr] = inv_pow2(sg);
 #pragma unroll
         for (int r = 0; r < 8; ++r) v[pr][r].y *= sg;
     }
@@ -1038,11 +1044,12 @@ __global__ void __launch_bounds__(512) k_fft_y_inv512(DevGeom g, int ntile, int 
     const int lb = xcd_logical_block(blockIdx.x, per);
     if (lb >= nblk) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int NYc = g.NYc, P = g.P, nq = g.nq, KX = g.KX;
+    constexpr int NYc = NYC512;
+    const int P = g.P, nq = g.nq, KX = g.KX;
     const int tix = lb % ntile, rest = lb / ntile;
     const int l = rest % nq, field = rest / nq;
     const int kx0 = tix * YT;
-    const int a = (NYc - 1) / 2, shift = 512 - NYc;  // kept rows [0, a] U [512 - a, 511]
+    constexpr int a = (NYc - 1) / 2, shift = 512 - NYc;  // kept rows [0, a] U [512 - a, 511]
     const cpx<float>* src = lev + ((size_t)field * nq + l) * P;
     {
         const int c = wave, kx = kx0 + c;  // one column per wave
@@ -1419,10 +1426,11 @@ __global__ void __launch_bounds__(512) k_fft_y_fwd512c(DevGeom g, int ntile, int
     const int lb = xcd_logical_block(blockIdx.x, per);
     if (lb >= nblk) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int NYc = g.NYc, P = g.P, nq = g.nq, KX = g.KX;
+    constexpr int NYc = NYC512;
+    const int P = g.P, nq = g.nq, KX = g.KX;
     const int tix = lb % ntile, l = lb / ntile;
     const int kx0 = tix * YT;
-    const int a = (NYc - 1) / 2, shift = 512 - NYc;
+    constexpr int a = (NYc - 1) / 2, shift = 512 - NYc;
     const int ncol = min(YT, KX - kx0);
     const int c = wave, kxc = kx0 + c;
     const bool cvalid = kxc < KX;
@@ -1490,11 +1498,11 @@ static bool generic_fft_forced() {
 }
 template <typename T>
 static bool fast512_rows(const DevGeom& g) {
-    return sizeof(T) == 4 && g.M == 512 && g.KX <= 192 && !generic_fft_forced();
+    return sizeof(T) == 4 && g.M == 512 && g.KX == KX512 && !generic_fft_forced();
 }
 template <typename T>
 static bool fast512_y(const DevGeom& g) {
-    return sizeof(T) == 4 && g.N == 512 && !generic_fft_forced();
+    return sizeof(T) == 4 && g.N == 512 && g.NYc == NYC512 && !generic_fft_forced();
 }
 // group (natural-layout) transforms for FP32 grids of 64, 128 and 256 points: log2 of the size,
 // or 0 when the generic kernels run
