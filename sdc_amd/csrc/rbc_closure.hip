@@ -219,21 +219,17 @@ template <int LOGM>
 __global__ void __launch_bounds__(256) k_synth_rows_w(DevGeom g, int F, int os, int zcol,
                                                       const float* __restrict__ zl, const cpx<float>* __restrict__ hyb,
                                                       const cpx<float>* __restrict__ twM, float* __restrict__ out) {
-    constexpr int M = 1 << LOGM, W = M / 8, G = 64 / W, RPB = 4 * G, SP = M + M / 8;
+    constexpr int M = 1 << LOGM, W = M / 8, G = 64 / W, RPB = 4 * G, SP = M + M / 8, KX = kept_cols(M);
     __shared__ cpx<float> tw[M];
     __shared__ cpx<float> scr[RPB][SP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, grp = lane / W, j = lane - grp * W;
-    const int N = g.N, KX = g.KX, KXp = g.KXp, nz = g.nq;
+    const int N = g.N, KXp = g.KXp, nz = g.nq;
     const int y = blockIdx.x * RPB + wave * G + grp, l = blockIdx.y;
     const size_t fstride = (size_t)nz * N * KXp, roff = ((size_t)l * N + y) * KXp;
     for (int e = threadIdx.x; e < M; e += blockDim.x) tw[e] = twM[e];
     __syncthreads();
     cpx<float>* sb = scr[wave * G + grp];
-    auto gmax = [](float x) {
-#pragma unroll
-        for (int o = W / 2; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o));
-        return x;
-    };
+    auto gmax = [](float x) { return group_max<W>(x); };
     float* orow = out + ((size_t)l * N + y) * M * os;
     for (int f0 = 0; f0 < F; f0 += 2) {
         const bool two = f0 + 1 < F;
@@ -288,21 +284,17 @@ __global__ void __launch_bounds__(256) k_synth_rows_w(DevGeom g, int F, int os, 
 template <int LOGM>
 __global__ void __launch_bounds__(256) k_anal_rows_w(DevGeom g, const float* __restrict__ in,
                                                      const cpx<float>* __restrict__ twM, cpx<float>* __restrict__ hyb) {
-    constexpr int M = 1 << LOGM, W = M / 8, G = 64 / W, RPB = 4 * G, SP = M + M / 8;
+    constexpr int M = 1 << LOGM, W = M / 8, G = 64 / W, RPB = 4 * G, SP = M + M / 8, KX = kept_cols(M);
     __shared__ cpx<float> tw[M];
     __shared__ cpx<float> scr[RPB][SP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, grp = lane / W, j = lane - grp * W;
-    const int N = g.N, KX = g.KX, KXp = g.KXp, nz = g.nq;
+    const int N = g.N, KXp = g.KXp, nz = g.nq;
     const int y = blockIdx.x * RPB + wave * G + grp, l = blockIdx.y;
     const size_t fstride = (size_t)nz * N * KXp, roff = ((size_t)l * N + y) * KXp;
     for (int e = threadIdx.x; e < M; e += blockDim.x) tw[e] = twM[e];
     __syncthreads();
     cpx<float>* sb = scr[wave * G + grp];
-    auto gmax = [](float x) {
-#pragma unroll
-        for (int o = W / 2; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o));
-        return x;
-    };
+    auto gmax = [](float x) { return group_max<W>(x); };
     const float* irow = in + ((size_t)l * N + y) * M * NOUT;
     float r0[8], r1[8], r2[8], m0 = 0.f, m1 = 0.f;
 #pragma unroll
@@ -823,7 +815,7 @@ void synth_points(rbc_closure* cl, int F, int os, int zcol, float* out) {
                            (size_t)2 * TX * g.N * sizeof(cpx<float>), c->stream, g, TX, (const cpx<float>*)cl->spec,
                            c->b.rows, (const cpx<float>*)cl->twN, cl->hyb);
     }
-    if (g.M >= 64 && g.M <= 256 && g.N % (2048 / g.M) == 0) {
+    if (g.M >= 64 && g.M <= 256 && g.N % (2048 / g.M) == 0 && g.KX == kept_cols(g.M)) {
         const dim3 gr(g.N / (2048 / g.M), cl->nz);
 #define RBC_SYNTHW(LG)                                                                                    \
     hipLaunchKernelGGL(k_synth_rows_w<LG>, gr, dim3(256), 0, c->stream, g, F, os, zcol, (const float*)cl->zl, \
@@ -875,7 +867,7 @@ void apply_closure(rbc_ctx* c, void* user) {
     hipLaunchKernelGGL(k_mlp_infer, dim3(infer_blocks(cl->npts)), dim3(256), 0, c->stream, cl->npts,
                        (const uint16_t*)cl->wbf, cl->nm, (const float*)cl->feat, cl->pred);
     const DevGeom& g = cl->gz;
-    if (g.M >= 64 && g.M <= 256 && g.N % (2048 / g.M) == 0) {
+    if (g.M >= 64 && g.M <= 256 && g.N % (2048 / g.M) == 0 && g.KX == kept_cols(g.M)) {
         const dim3 gr(g.N / (2048 / g.M), cl->nz);
         if (g.M == 64)
             hipLaunchKernelGGL(k_anal_rows_w<6>, gr, dim3(256), 0, c->stream, g, (const float*)cl->pred,

@@ -37,6 +37,25 @@ __device__ __forceinline__ cpx<T> conjg(cpx<T> a) { return mk(a.x, -a.y); }
 template <typename T>
 __device__ __forceinline__ cpx<T> imul(cpx<T> a, T s) { return mk(-s * a.y, s * a.x); }
 
+// Kept columns (kx = 0 .. kept_cols - 1) and kept rows of the dealias rule of make_geometry,
+// as compile-time functions of the grid size (same double expressions), so that the transform
+// kernels of a fixed size resolve the zero band and the mirrored half at compile time.
+constexpr int kept_cols(int M) {
+    const int mox = M / 2 + 1;
+    int kx = 0;
+    for (int col = 0; col < mox; ++col)
+        if (!(mox - col <= (1.0 / 3.0) * mox)) kx = col + 1;
+    return kx;
+}
+constexpr int kept_rows(int N) {
+    int n = 0;
+    for (int r = 0; r < N; ++r) {
+        const double d = 0.5 * N - r;
+        if (!((d < 0 ? -d : d) <= (1.0 / 3.0) * 0.5 * N)) ++n;
+    }
+    return n;
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs (each with its own L2).  This maps the
 // hardware block id to a logical id so that logical neighbours (which share cache lines) run
 // on the same XCD: XCD x takes the contiguous logical range [x*per, (x+1)*per).  Grids using
@@ -98,11 +117,35 @@ __device__ __forceinline__ void dft8(cpx<T> v[8]) {
     v[7] = c7;
 }
 
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+// max over aligned groups of W lanes (W = 8, 16, 32 or 64), the result in every lane of the
+// group: DPP lane permutations inside 16-lane rows (quad_perm [1,0,3,2], quad_perm [2,3,0,1],
+// row_half_mirror, row_mirror; after the earlier steps each is a max with the partner half),
+// then a shuffle across rows (W = 32) or lane reads of the four rows (W = 64).  All lanes active.
+__device__ __forceinline__ float dpp_max_step(float v, int x, int ctrl_id) {
+    int y;
+    switch (ctrl_id) {
+        case 0: y = __builtin_amdgcn_update_dpp(x, x, 0xB1, 0xF, 0xF, false); break;
+        case 1: y = __builtin_amdgcn_update_dpp(x, x, 0x4E, 0xF, 0xF, false); break;
+        case 2: y = __builtin_amdgcn_update_dpp(x, x, 0x141, 0xF, 0xF, false); break;
+        default: y = __builtin_amdgcn_update_dpp(x, x, 0x140, 0xF, 0xF, false); break;
+    }
+    return fmaxf(v, __int_as_float(y));
+}
+template <int W>
+__device__ __forceinline__ float group_max(float v) {
+    v = dpp_max_step(v, __float_as_int(v), 0);
+    v = dpp_max_step(v, __float_as_int(v), 1);
+    v = dpp_max_step(v, __float_as_int(v), 2);
+    if (W >= 16) v = dpp_max_step(v, __float_as_int(v), 3);
+    if (W == 32) v = fmaxf(v, __shfl_xor(v, 16));
+    if (W == 64) {
+        const int x = __float_as_int(v);
+        v = fmaxf(fmaxf(__int_as_float(__builtin_amdgcn_readlane(x, 0)), __int_as_float(__builtin_amdgcn_readlane(x, 16))),
+                  fmaxf(__int_as_float(__builtin_amdgcn_readlane(x, 32)), __int_as_float(__builtin_amdgcn_readlane(x, 48))));
+    }
     return v;
 }
+__device__ __forceinline__ float wave_max(float v) { return group_max<64>(v); }
 // power of two that brings the row maximum mb of the second field of a pair to that of the
 // first (ma), so that one complex transform carries both with the rounding of each set by its
 // own size; scaling by a power of two is exact
@@ -336,10 +379,10 @@ __global__ void __launch_bounds__(64 * WPB) k_fft_y_inv_w(DevGeom g, const cpx<f
     __shared__ cpx<float> tw[N];
     __shared__ cpx<float> tile[(N + N / 8) * S];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, grp = lane / W, j = lane - grp * W;
-    const int NYc = g.NYc, P = g.P, nq = g.nq, KX = g.KX;
+    constexpr int NYc = kept_rows(N), a = (NYc - 1) / 2, shift = N - NYc;  // kept rows [0, a] U [N - a, N)
+    const int P = g.P, nq = g.nq, KX = g.KX;
     const int kx0 = blockIdx.x * CB, l = blockIdx.y, field = blockIdx.z;
     const int c = wave * G + grp, kx = kx0 + c;
-    const int a = (NYc - 1) / 2, shift = N - NYc;  // kept rows [0, a] U [N - a, N - 1]
     for (int e = threadIdx.x; e < N; e += blockDim.x) tw[e] = twN[e];
     __syncthreads();
     const cpx<float>* src = lev + ((size_t)field * nq + l) * P + (size_t)kx * NYc;
@@ -372,10 +415,10 @@ __global__ void __launch_bounds__(64 * WPB) k_fft_y_fwd_w(DevGeom g, const cpx<f
     __shared__ cpx<float> tw[N];
     __shared__ cpx<float> tile[(N + N / 8) * S];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, grp = lane / W, j = lane - grp * W;
-    const int NYc = g.NYc, P = g.P, nq = g.nq, KX = g.KX;
+    constexpr int NYc = kept_rows(N), a = (NYc - 1) / 2, shift = N - NYc;
+    const int P = g.P, nq = g.nq, KX = g.KX;
     const int kx0 = blockIdx.x * CB, l = blockIdx.y, q = blockIdx.z;
     const int c = wave * G + grp, kx = kx0 + c;
-    const int a = (NYc - 1) / 2, shift = N - NYc;
     const int ncol = min(CB, KX - kx0);
     for (int e = threadIdx.x; e < N; e += blockDim.x) tw[e] = twN[e];
     const cpx<float>* src = hout + ((size_t)q * nq + l) * (size_t)N * g.KXp + kx0;
@@ -420,7 +463,7 @@ inline void launch_y_group_w(bool inv, int wpb, const DevGeom& g, int nf, const 
 }
 inline bool launch_y_group(bool inv, const DevGeom& g, int nf, const cpx<float>* in, const cpx<float>* twN,
                            cpx<float>* out, hipStream_t s, int cb = 0) {
-    if (g.N < 64 || g.N > 256) return false;
+    if (g.N < 64 || g.N > 256 || g.NYc != kept_rows(g.N)) return false;
     const int wpb = cb > 0 ? cb / (512 / g.N) : 4;
     if (g.N == 64) launch_y_group_w<6>(inv, wpb, g, nf, in, twN, out, s);
     else if (g.N == 128) launch_y_group_w<7>(inv, wpb, g, nf, in, twN, out, s);

@@ -818,7 +818,8 @@ __device__ __forceinline__ void fft512_wave(cpx<T> v[8], cpx<T>* sb, const cpx<T
 // Kept columns at M = 512 and kept rows at N = 512 (the dealias rule of make_geometry: kx <= 171,
 // |ky| <= 170); the 512-point kernels take them as constants so that the zero band and the
 // mirrored half resolve at compile time (the launchers check them).
-constexpr int KX512 = 172, NYC512 = 341;
+constexpr int KX512 = kept_cols(512), NYC512 = kept_rows(512);
+static_assert(KX512 == 172 && NYC512 == 341, "dealias rule at 512 points");
 
 // x-rows at M = N = 512 for the fused forward y-FFT: three waves per (level, y) row.  Phase 1:
 // wave pr inverse-transforms the pair (2 pr, 2 pr + 1) of level fields.  Phase 2: the six
@@ -932,22 +933,18 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, const cpx<float>* _
 template <int LOGM, int WPB>
 __global__ void __launch_bounds__(64 * WPB) k_rows_w(DevGeom g, const cpx<float>* __restrict__ hin,
                                                 const cpx<float>* __restrict__ twM, cpx<float>* __restrict__ hout) {
-    constexpr int M = 1 << LOGM, W = M / 8, G = 64 / W, RPB = WPB * G, SP = M + M / 8;
+    constexpr int M = 1 << LOGM, W = M / 8, G = 64 / W, RPB = WPB * G, SP = M + M / 8, KX = kept_cols(M);
     __shared__ cpx<float> tw[M];
     __shared__ cpx<float> scr[RPB][SP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, grp = lane / W, j = lane - grp * W;
-    const int KX = g.KX, KXp = g.KXp, nq = g.nq, N = g.N;
+    const int KXp = g.KXp, nq = g.nq, N = g.N;
     const int y = blockIdx.x * RPB + wave * G + grp, l = blockIdx.y;
     const size_t fstride = (size_t)nq * N * KXp;
     const size_t roff = ((size_t)l * N + y) * KXp;
     for (int e = threadIdx.x; e < M; e += blockDim.x) tw[e] = twM[e];
     __syncthreads();
     cpx<float>* sb = scr[wave * G + grp];
-    auto gmax = [](float x) {
-#pragma unroll
-        for (int o = W / 2; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o));
-        return x;
-    };
+    auto gmax = [](float x) { return group_max<W>(x); };
     cpx<float> v[3][8];
     float isg[3];
 #pragma unroll
@@ -1508,7 +1505,7 @@ static bool fast512_y(const DevGeom& g) {
 // or 0 when the generic kernels run
 template <typename T>
 static int wave_rows_log(const DevGeom& g) {
-    if (sizeof(T) != 4 || generic_fft_forced() || g.M < 64 || g.M > 256) return 0;
+    if (sizeof(T) != 4 || generic_fft_forced() || g.M < 64 || g.M > 256 || g.KX != kept_cols(g.M)) return 0;
     return g.logM;
 }
 template <typename T>
