@@ -380,8 +380,10 @@ int rbc_create(const rbc_params* p, rbc_ctx** out) {
         HIPCHK(hipMemcpy(b.gstart, c->fac.group_start.data(), c->fac.group_start.size() * sizeof(int),
                          hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(b.gmem, c->fac.members.data(), c->fac.members.size() * sizeof(int), hipMemcpyHostToDevice));
-        {  // chunks of at most 8 members per wave of the group update (bounded tail latency)
-            constexpr int kChunk = 8;
+        {  // chunks of at most kChunk members per wave of the group update: a wave runs its members
+           // one after another, so small grids (few modes, thousands of waves at most) take short
+           // chunks to keep the longest wave short; large grids take 8 to share the matrix loads
+            const int kChunk = std::max(1, std::min(8, (int)(g.P() / 4096)));
             std::vector<int> cg, cb;
             const auto& gs = c->fac.group_start;
             for (int gi = 0; gi + 1 < (int)gs.size(); ++gi)

@@ -1484,6 +1484,84 @@ __global__ void __launch_bounds__(512) k_fft_y_fwd512c(DevGeom g, int ntile, int
     }
 }
 
+// Forward y-FFT at N = 64, 128, 256 fused with the level combinations (FP32): blockIdx.z = the
+// combination group of YGroup (0: h, d, uv -> D0, D2; 1: uw, vw -> D1, D3 and the k = 0 values
+// of P_uw, P_vw; 2: u.grad(theta) -> D4).  A block of WPB waves handles CB kx columns of one
+// level, one group of W = N / 8 lanes per column (fft_group, natural layout); the group's
+// products are transformed in turn through the LDS tile and their combinations accumulate in
+// registers.  Writes D[t][l][p] into the level buffer (free once the inverse y-FFT has read it).
+template <int LOGN, int WPB, int GRP>
+__device__ __forceinline__ void y_fwd_group_comb(const DevGeom& g, cpx<float>* tile, const cpx<float>* tw,
+                                                 const cpx<float>* __restrict__ hout, const float* __restrict__ kxa,
+                                                 const float* __restrict__ kya, cpx<float>* __restrict__ D,
+                                                 cpx<float>* __restrict__ fgw) {
+    using Gr = YGroup<GRP>;
+    constexpr int N = 1 << LOGN, W = N / 8, G = 64 / W, CB = WPB * G, S = CB + 1;
+    constexpr int NYc = kept_rows(N), a = (NYc - 1) / 2, shift = N - NYc;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, grp = lane / W, j = lane - grp * W;
+    const int P = g.P, nq = g.nq, KX = g.KX;
+    const int kx0 = blockIdx.x * CB, l = blockIdx.y;
+    const int c = wave * G + grp, kx = kx0 + c;
+    const bool cvalid = kx < KX;
+    const int ncol = min(CB, KX - kx0);
+    auto kyc_of = [](int n) { return n <= a ? n : (n >= N - a ? n - shift : -1); };
+    const float kxv = cvalid ? kxa[(size_t)kx * NYc] : 0.f;
+    float kyv[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int kc = kyc_of(j + W * r);
+        kyv[r] = (cvalid && kc >= 0) ? kya[(size_t)kx * NYc + kc] : 0.f;
+    }
+    cpx<float> d0[8], d1[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) d0[r] = d1[r] = mk(0.f, 0.f);
+#pragma unroll
+    for (int iq = 0; iq < Gr::n; ++iq) {
+        const cpx<float>* src = hout + ((size_t)Gr::q(iq) * nq + l) * (size_t)N * g.KXp + kx0;
+        for (int e = threadIdx.x; e < N * CB; e += blockDim.x) {
+            const int yy = e / CB, cc = e - yy * CB;
+            tile[pad8(yy) * S + cc] = (cc < ncol) ? src[(size_t)yy * g.KXp + cc] : mk(0.f, 0.f);
+        }
+        __syncthreads();
+        cpx<float> v[1][8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v[0][r] = tile[pad8(j + W * r) * S + c];
+        wave_sync();  // the column is read before it becomes the scratch
+        fft_group<LOGN, false, 1, S>(v, tile + c, tw, j);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) Gr::add(iq, v[0][r], kxv, kyv[r], d0[r], d1[r]);
+        if (GRP == 1 && kx == 0 && j == 0) fgw[2 * l + iq] = v[0][0];  // P_uw, P_vw at (0, 0)
+        __syncthreads();  // the tile is refilled by the next product
+    }
+    if (cvalid) {
+        const size_t fs = (size_t)nq * P;
+        cpx<float>* o = D + (size_t)l * P + (size_t)kx * NYc;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int kc = kyc_of(j + W * r);
+            if (kc >= 0) {
+                o[Gr::t0 * fs + kc] = d0[r];
+                if (Gr::t1 >= 0) o[Gr::t1 * fs + kc] = d1[r];
+            }
+        }
+    }
+}
+
+template <int LOGN, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_fft_y_fwd_wc(DevGeom g, const cpx<float>* __restrict__ hout,
+                                                          const cpx<float>* __restrict__ twN,
+                                                          const float* __restrict__ kxa, const float* __restrict__ kya,
+                                                          cpx<float>* __restrict__ D, cpx<float>* __restrict__ fgw) {
+    constexpr int N = 1 << LOGN, W = N / 8, G = 64 / W, CB = WPB * G, S = CB + 1;
+    __shared__ cpx<float> tw[N];
+    __shared__ cpx<float> tile[(N + N / 8) * S];
+    for (int e = threadIdx.x; e < N; e += blockDim.x) tw[e] = twN[e];
+    __syncthreads();
+    if (blockIdx.z == 0) y_fwd_group_comb<LOGN, WPB, 0>(g, tile, tw, hout, kxa, kya, D, fgw);
+    else if (blockIdx.z == 1) y_fwd_group_comb<LOGN, WPB, 1>(g, tile, tw, hout, kxa, kya, D, fgw);
+    else y_fwd_group_comb<LOGN, WPB, 2>(g, tile, tw, hout, kxa, kya, D, fgw);
+}
+
 // ---------------------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------------------
@@ -1634,8 +1712,26 @@ void Launch<T>::fft_y_fwd(const DevGeom& g, const Buffers& b, hipStream_t s) {
                            (const float*)b.ky, (cpx<float>*)b.lev, (cpx<float>*)b.fgw);
         return;
     }
-    if (!(wave_y_log<T>(g) && launch_y_fwd_group(g, 6, (const cpx<float>*)b.hout, (const cpx<float>*)b.twN,
-                                                 (cpx<float>*)b.prod, s, b.tune_ytx))) {
+    if (wave_y_log<T>(g) && g.NYc == kept_rows(g.N)) {
+        // default: 2 waves per block at N = 64 (more blocks on the small grids), else 4
+        const int G = 512 / g.N, wpb = b.tune_ytx > 0 && b.tune_ytx / G >= 2 ? b.tune_ytx / G : (g.N <= 64 ? 2 : 4);
+        const dim3 gr((g.KX + wpb * G - 1) / (wpb * G), g.nq, 3);
+#define RBC_YFWDC(LG, WPBV)                                                                                        \
+    hipLaunchKernelGGL((k_fft_y_fwd_wc<LG, WPBV>), gr, dim3(64 * WPBV), 0, s, g, (const cpx<float>*)b.hout,        \
+                       (const cpx<float>*)b.twN, (const float*)b.kx, (const float*)b.ky, (cpx<float>*)b.lev,   \
+                       (cpx<float>*)b.fgw)
+#define RBC_YFWDC_L(LG)                  \
+    if (wpb == 2) RBC_YFWDC(LG, 2);      \
+    else if (wpb == 8) RBC_YFWDC(LG, 8); \
+    else RBC_YFWDC(LG, 4)
+        if (g.N == 64) { RBC_YFWDC_L(6); }
+        else if (g.N == 128) { RBC_YFWDC_L(7); }
+        else { RBC_YFWDC_L(8); }
+#undef RBC_YFWDC_L
+#undef RBC_YFWDC
+        return;
+    }
+    {
         const int TX = b.tune_ytx > 0 ? b.tune_ytx : fft_tile(g.N, sizeof(cpx<T>));
         dim3 grid((g.KX + TX - 1) / TX, g.nq, 6);
         const size_t sm = (size_t)2 * TX * g.N * sizeof(cpx<T>);
