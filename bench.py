@@ -47,9 +47,10 @@ def stage_bytes(st, cs, nprod):
 
 # kernels of each stage on the 512 x 512 FP32 path (BF16X3 runs the <1> variants of the two
 # matrix-core kernels, with the same memory traffic)
-STAGE_KERNELS = {"eval": ["k_eval_mma<0>"], "fft_y_inv": ["k_fft_y_inv512"], "rows_products": ["k_rows512t"],
+STAGE_KERNELS = {"eval": ["k_eval_mma<0, false>"], "fft_y_inv": ["k_fft_y_inv512"], "rows_products": ["k_rows512t"],
                  "fft_y_fwd": ["k_fft_y_fwd512c<0>", "k_fft_y_fwd512c<1>", "k_fft_y_fwd512c<2>"],
-                 "project": ["k_project_mma<0>"], "update": ["k_update32<float>"]}
+                 "project": ["k_project_mma<0, 4, 1>"], "update": ["k_update32<float>"],
+                 "finish": ["k_finish<float>"]}
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_512x512x32_f32.md")
 
 
@@ -218,6 +219,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="f32", choices=["f32", "bf16x3", "f64"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--grid", type=int, nargs=3, metavar=("M", "N", "L"), default=None,
+                    help="another grid for profiling (the reported workload is 512 512 32)")
     ap.add_argument("--no-variants", action="store_true", help="skip timing the other precision form")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-torch", action="store_true", help="bracket the timed region with the library's "
@@ -252,6 +255,8 @@ def main():
 
     import sdc_amd
     from sdc_amd.initial import random_theta
+    if a.grid:
+        WORKLOAD.update(M=a.grid[0], N=a.grid[1], L=a.grid[2])
     W = WORKLOAD
     st = sdc_amd.Stepper(W["M"], W["N"], W["L"], W["Lx"], W["Ly"], W["Ra"], W["Pr"], W["dt"],
                          precision=a.precision, device=0)
@@ -280,7 +285,7 @@ def main():
     nbytes = stage_bytes(st, cs, nprod)
     dom = max(stages, key=lambda k: stages[k])
     achieved = nbytes[dom] / (stages[dom] * 1e-3) / 1e9
-    traffic = measured_traffic(dom) if a.precision != "f64" else None
+    traffic = measured_traffic(dom) if a.precision != "f64" and not a.grid else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                 "algorithmic_bytes": nbytes[dom],
@@ -326,8 +331,8 @@ def main():
         "vs_baseline": None,
         "dtype": {"f32": "f32", "bf16x3": "bf16x3+f32", "f64": "f64"}[a.precision],
         "data": "synthetic (seeded theta ~ 1e-3 N(0,1) in every kept mode)",
-        "config": {"workload": "RBC IMEX step, 512x512 grid x 32 vertical modes, Ra=1e6, Pr=0.7, 4x4x1 box, "
-                               "dt=5e-5", "M": W["M"], "N": W["N"], "L": W["L"], "nq_levels": nq,
+        "config": {"workload": f"RBC IMEX step, {W['M']}x{W['N']} grid x {W['L']} vertical modes, Ra=1e6, Pr=0.7, "
+                               "4x4x1 box, dt=5e-5", "M": W["M"], "N": W["N"], "L": W["L"], "nq_levels": nq,
                    "kept_modes": [kx, ky], "unique_k2_groups": ng, "precision": a.precision,
                    "parallelism": "single GPU"},
         "roofline": roofline,
