@@ -292,6 +292,15 @@ def main():
                 "stage_ms": {k: round(v, 4) for k, v in stages.items()},
                 "stage_GBs": {k: round(nbytes[k] / (stages[k] * 1e-3) / 1e9, 1) for k in stages if stages[k] > 0}}
     nq, kx, ky, ng = st.nq, st.kx_kept, st.ky_kept, st.n_unique_k
+    # the boundary's host buffers: one state in (reference layout, FP64 host arrays), one step, one
+    # state out, including PCIe and the host-side packing; reported beside `value`, which excludes it
+    state = st.get_state()
+    st.synchronize()
+    tb = time.perf_counter()
+    st.set_state(*state)
+    st.step_time(1)
+    state = st.get_state()
+    boundary_ms = 1e3 * (time.perf_counter() - tb)
     st.close()
     # the other single-GPU precision forms on the same workload (plain step_time, no stage events)
     variants = {}
@@ -338,6 +347,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "precision_variants": variants,
+        "boundary_roundtrip_ms": round(boundary_ms, 2),
     }
     print(json.dumps(out))
 
