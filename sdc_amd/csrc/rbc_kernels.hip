@@ -1412,23 +1412,24 @@ struct YGroup<2> {  // u.grad(theta) -> D4
     __device__ static void add(int, cpx<float> V, float, float, cpx<float>& d0, cpx<float>&) { d0 = d0 + V; }
 };
 
-template <int GRP>
-__global__ void __launch_bounds__(512) k_fft_y_fwd512c(DevGeom g, int ntile, int nblk, int per,
+template <int GRP, int CT>
+__global__ void __launch_bounds__(64 * CT) k_fft_y_fwd512c(DevGeom g, int ntile, int nblk, int per,
                                                        const cpx<float>* __restrict__ hout,
                                                        const cpx<float>* __restrict__ tw,
                                                        const float* __restrict__ kxa, const float* __restrict__ kya,
                                                        cpx<float>* __restrict__ D, cpx<float>* __restrict__ fgw) {
     using G = YGroup<GRP>;
-    __shared__ cpx<float> tile[576 * YS];
+    constexpr int CS = CT + 1;  // tile row stride
+    __shared__ cpx<float> tile[576 * CS];
     const int lb = xcd_logical_block(blockIdx.x, per);
     if (lb >= nblk) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int NYc = NYC512;
     const int P = g.P, nq = g.nq, KX = g.KX;
     const int tix = lb % ntile, l = lb / ntile;
-    const int kx0 = tix * YT;
+    const int kx0 = tix * CT;
     constexpr int a = (NYc - 1) / 2, shift = 512 - NYc;
-    const int ncol = min(YT, KX - kx0);
+    const int ncol = min(CT, KX - kx0);
     const int c = wave, kxc = kx0 + c;
     const bool cvalid = kxc < KX;
     int kyc[8];
@@ -1443,13 +1444,13 @@ __global__ void __launch_bounds__(512) k_fft_y_fwd512c(DevGeom g, int ntile, int
     cpx<float> d0[8], d1[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) d0[r] = d1[r] = mk(0.f, 0.f);
-    // tile element e = threadIdx.x + 512 i (i < 8): y = e / YT, column e % YT
+    // tile element e = threadIdx.x + 64 CT i (i < 8): y = e / CT, column e % CT
     cpx<float> pre[8];
     auto fetch = [&](int q) {
         const cpx<float>* src = hout + ((size_t)q * nq + l) * (size_t)512 * g.KXp + kx0;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int e = threadIdx.x + 512 * i, y = e / YT, cc = e - y * YT;
+            const int e = threadIdx.x + 64 * CT * i, y = e / CT, cc = e - y * CT;
             pre[i] = (cc < ncol) ? src[(size_t)y * g.KXp + cc] : mk(0.f, 0.f);
         }
     };
@@ -1458,15 +1459,15 @@ __global__ void __launch_bounds__(512) k_fft_y_fwd512c(DevGeom g, int ntile, int
     for (int iq = 0; iq < G::n; ++iq) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int e = threadIdx.x + 512 * i, y = e / YT, cc = e - y * YT;
-            tile[pad8(y) * YS + cc] = pre[i];
+            const int e = threadIdx.x + 64 * CT * i, y = e / CT, cc = e - y * CT;
+            tile[pad8(y) * CS + cc] = pre[i];
         }
         __syncthreads();
         if (iq + 1 < G::n) fetch(G::q(iq + 1));
         cpx<float> v[8];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) v[r] = tile[pad8(lane + 64 * r) * YS + c];
-        fft512_wave<float, false, YS>(v, tile + c, tw, lane);
+        for (int r = 0; r < 8; ++r) v[r] = tile[pad8(lane + 64 * r) * CS + c];
+        fft512_wave<float, false, CS>(v, tile + c, tw, lane);
 #pragma unroll
         for (int r = 0; r < 8; ++r) G::add(iq, v[r], kxv, kyv[r], d0[r], d1[r]);
         if (GRP == 1 && kxc == 0 && lane == 0) fgw[2 * l + iq] = v[0];  // P_uw, P_vw at row 0
@@ -1700,16 +1701,18 @@ void Launch<T>::rows(const DevGeom& g, const Buffers& b, hipStream_t s) {
 template <typename T>
 void Launch<T>::fft_y_fwd(const DevGeom& g, const Buffers& b, hipStream_t s) {
     if (fast512_y<T>(g)) {
-        const int ntile = (g.KX + YT - 1) / YT, nblk = ntile * g.nq, per = (nblk + 7) / 8;
-        hipLaunchKernelGGL(k_fft_y_fwd512c<0>, dim3(8 * per), dim3(512), 0, s, g, ntile, nblk, per,
-                           (const cpx<float>*)b.hout, (const cpx<float>*)b.twN, (const float*)b.kx,
-                           (const float*)b.ky, (cpx<float>*)b.lev, (cpx<float>*)b.fgw);
-        hipLaunchKernelGGL(k_fft_y_fwd512c<1>, dim3(8 * per), dim3(512), 0, s, g, ntile, nblk, per,
-                           (const cpx<float>*)b.hout, (const cpx<float>*)b.twN, (const float*)b.kx,
-                           (const float*)b.ky, (cpx<float>*)b.lev, (cpx<float>*)b.fgw);
-        hipLaunchKernelGGL(k_fft_y_fwd512c<2>, dim3(8 * per), dim3(512), 0, s, g, ntile, nblk, per,
-                           (const cpx<float>*)b.hout, (const cpx<float>*)b.twN, (const float*)b.kx,
-                           (const float*)b.ky, (cpx<float>*)b.lev, (cpx<float>*)b.fgw);
+        // 4 kx columns (4 waves) per block: the combination accumulators hold 3 waves per SIMD,
+        // so smaller blocks keep 12 waves per CU busy where 8-wave blocks fit only one block
+        constexpr int CT = 4;
+        const int ntile = (g.KX + CT - 1) / CT, nblk = ntile * g.nq, per = (nblk + 7) / 8;
+#define RBC_YF512(GRPV)                                                                                          \
+    hipLaunchKernelGGL((k_fft_y_fwd512c<GRPV, CT>), dim3(8 * per), dim3(64 * CT), 0, s, g, ntile, nblk, per,     \
+                       (const cpx<float>*)b.hout, (const cpx<float>*)b.twN, (const float*)b.kx,                  \
+                       (const float*)b.ky, (cpx<float>*)b.lev, (cpx<float>*)b.fgw)
+        RBC_YF512(0);
+        RBC_YF512(1);
+        RBC_YF512(2);
+#undef RBC_YF512
         return;
     }
     if (wave_y_log<T>(g) && g.NYc == kept_rows(g.N)) {
