@@ -16,6 +16,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -212,6 +214,142 @@ def run_closure(a):
     print(json.dumps(out))
 
 
+def _max_rel(a, b, keep):
+    """Largest relative difference over theta, f, g (kept modes) and F, G (each against its own
+    maximum)."""
+    errs = []
+    for i in range(5):
+        x, y = (a[i][:, keep], b[i][:, keep]) if i < 3 else (a[i], b[i])
+        scale = np.abs(y).max()
+        errs.append(float(np.abs(x - y).max() / scale) if scale > 0 else float(np.abs(x).max()))
+    return max(errs)
+
+
+def run_validate(a):
+    """The accuracy forms, each against an FP64 solver on the same seeded input:
+    C1 (32^2 x 8, Ra 5000) FP64 on the GPU against the FP64 CPU oracle, 1000 steps;
+    C2 (128^2 x 16, Ra 1e5) FP32 against FP64 on the GPU (the FP64 path is pinned to the oracle),
+       2000 steps, 1e5 spectral-RK2 tracers riding the FP32 run (at dt = 2.5e-5: the stated 2e-4 is
+       beyond the explicit advection's limit, which the run records);
+    C3 (512^2 x 32, Ra 1e6) BF16X3 and FP32 against FP64 on the GPU from a developed FP64 state."""
+    import sdc_amd
+    from oracle import oracle as O
+    from sdc_amd.initial import kept_mask, low_mode_theta, random_theta
+    say = lambda *m: print("#", *m, file=sys.stderr, flush=True)  # noqa: E731
+    threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+    O.build()
+    out = {"metric": "validation: relative error of fields, Nu and spectra against FP64 runs"}
+
+    # C1
+    c1 = (32, 32, 8, 4.032, 4.032, 5000.0, 1.0, 1e-3)
+    st = low_mode_theta(32, 32, 8, seed=1, amp=1e-3)
+    keep = kept_mask(32, 32)
+    G = sdc_amd.Stepper(*c1, precision="f64")
+    G.set_state(*st)
+    o = O.Stepper(*c1, nthreads=threads)
+    o.set_state(*st)
+    nu_g, nu_o, err100 = [], [], None
+    for k in range(10):
+        G.step_time(100)
+        o.step(100)
+        if k == 0:
+            err100 = _max_rel(G.get_state(), o.get_state(), keep)
+        nu_g.append(G.nusselt().tolist())
+        nu_o.append(o.nusselt().tolist())
+    err1000 = _max_rel(G.get_state(), o.get_state(), keep)
+    G.close()
+    o.close()
+    nu_g, nu_o = np.array(nu_g), np.array(nu_o)
+    out["C1"] = {"config": "32x32x8 Ra=5000 Pr=1 box 4.032^2 dt=1e-3, low-mode theta seed 1, FP64 GPU vs FP64 oracle",
+                 "field_rel_err_100_steps": err100, "field_rel_err_1000_steps": err1000,
+                 "nu_rel_diff_max": float(np.abs(nu_g - nu_o).max() / np.abs(nu_o).max()),
+                 "nu_volume_every_100_steps": nu_g[:, 2].tolist()}
+    say("C1", out["C1"])
+
+    # C2: first the stated dt = 2e-4 (FP64), recording when it stops being finite; then the
+    # comparison at a stable step
+    st = random_theta(128, 128, 16, seed=2, amp=1e-3)
+    keep = kept_mask(128, 128)
+    T = sdc_amd.Stepper(128, 128, 16, 4.0, 4.0, 1e5, 0.7, 2e-4, precision="f64")
+    T.set_state(*st)
+    blowup = None
+    for k in range(20):
+        T.step_time(100)
+        if not np.all(np.isfinite(T.nusselt())):
+            blowup = 100 * (k + 1)
+            break
+    T.close()
+    c2 = (128, 128, 16, 4.0, 4.0, 1e5, 0.7, 2.5e-5)
+    A = sdc_amd.Stepper(*c2, precision="f32")
+    B = sdc_amd.Stepper(*c2, precision="f64")
+    A.set_state(*st)
+    B.set_state(*st)
+    tr = sdc_amd.Tracer(A, 100000, buffer_size=101, mode="spectral_rk2", seed=2)
+    tr.init_positions("random")
+    nu_a, nu_b = [], []
+    t0 = time.perf_counter()
+    for k in range(20):
+        for _ in range(100):
+            A.step_time(1)
+            tr.step_particles(c2[7])
+        tr.get_particle_positions(clear=True)  # the ring holds 100 steps (the reference's output cadence)
+        B.step_time(100)
+        nu_a.append(A.nusselt().tolist())
+        nu_b.append(B.nusselt().tolist())
+    wall = time.perf_counter() - t0
+    Ea, _ = A.energy_spectrum()
+    Eb, _ = B.energy_spectrum()
+    n = min(len(Ea), len(Eb))
+    nu_a, nu_b = np.array(nu_a), np.array(nu_b)
+    mom = tr.moments()
+    out["C2"] = {"config": "128x128x16 Ra=1e5 Pr=0.7 box 4^2, random theta 1e-3 seed 2, 2000 steps at dt=2.5e-5, FP32 "
+                           "vs FP64 on the GPU, 1e5 spectral-RK2 tracers",
+                 "stated_dt_2e-4_fp64_nonfinite_by_step": blowup,
+                 "field_rel_err_2000_steps": _max_rel(A.get_state(), B.get_state(), keep),
+                 "nu_rel_diff_max": float(np.abs(nu_a - nu_b).max() / np.abs(nu_b).max()),
+                 "spectrum_rel_err": float(np.linalg.norm(Ea[:n] - Eb[:n]) / np.linalg.norm(Eb[:n])),
+                 "nu_volume_every_100_steps_f32": nu_a[:, 2].tolist(),
+                 "tracer_moments": [float(x) for x in mom], "wall_s_f32_with_tracers_plus_f64": round(wall, 2)}
+    tr.close()
+    A.close()
+    B.close()
+    say("C2", {k: v for k, v in out["C2"].items() if "every" not in k})
+
+    # C3
+    c3 = (512, 512, 32, 4.0, 4.0, 1e6, 0.7, 2.5e-6)
+    keep = kept_mask(512, 512)
+    F = sdc_amd.Stepper(*c3, precision="f64")
+    F.set_state(*random_theta(512, 512, 32, seed=3, amp=1e-3))
+    t0 = time.perf_counter()
+    F.step_time(2000)
+    F.synchronize()
+    dev = F.get_state()
+    say("C3 developed state", round(time.perf_counter() - t0, 1), "s")
+    res = {}
+    ref_nu, ref_E = None, None
+    F.reset_history()
+    F.step_time(200)
+    ref = F.get_state()
+    ref_nu = F.nusselt()
+    ref_E, _ = F.energy_spectrum()
+    F.close()
+    for prec in ("bf16x3", "f32"):
+        S = sdc_amd.Stepper(*c3, precision=prec)
+        S.set_state(*dev)
+        S.step_time(200)
+        E, _ = S.energy_spectrum()
+        n = min(len(E), len(ref_E))
+        res[prec] = {"field_rel_err": _max_rel(S.get_state(), ref, keep),
+                     "nu_rel_err": float(np.abs(S.nusselt() - ref_nu).max() / np.abs(ref_nu).max()),
+                     "spectrum_rel_err": float(np.linalg.norm(E[:n] - ref_E[:n]) / np.linalg.norm(ref_E[:n]))}
+        S.close()
+    out["C3"] = {"config": "512x512x32 Ra=1e6 Pr=0.7 box 4^2; FP64 2000 steps from random theta 1e-3 seed 3, then 200 "
+                           "steps in each form; dt = 2.5e-6 (5e-5 is unstable through the onset transient)",
+                 "nu_fp64": ref_nu.tolist(), **res}
+    say("C3", out["C3"])
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,6 +368,7 @@ def main():
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--ensemble", action="store_true", help="run the batched-ensemble workload instead")
     ap.add_argument("--closure", action="store_true", help="run the learned-closure experiment instead")
+    ap.add_argument("--validate", action="store_true", help="run the accuracy forms (C1, C2, C3) instead")
     ap.add_argument("--closure-T", type=float, default=1.0)
     ap.add_argument("--closure-every", default="1", help="network intervals of the corrected runs, e.g. 1,2,4 "
                     "(the first is reported)")
@@ -251,6 +390,9 @@ def main():
         return
     if a.closure:
         run_closure(a)
+        return
+    if a.validate:
+        run_validate(a)
         return
 
     import sdc_amd
