@@ -1,6 +1,7 @@
 // rbc_device.hpp -- device-side helpers shared by the kernel translation units of
 // libsdc_rbc.so: complex arithmetic, XCD-aware block ids, wavefront LDS ordering, the
-// radix-2 LDS FFT and the generic inverse y-FFT of level fields.
+// LDS Stockham FFT with the generic y-FFTs, the natural-layout group FFT with the group y-FFTs, and
+// the lane reductions.
 #pragma once
 
 #include <hip/hip_runtime.h>

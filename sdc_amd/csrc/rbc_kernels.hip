@@ -1566,8 +1566,8 @@ __global__ void __launch_bounds__(64 * WPB) k_fft_y_fwd_wc(DevGeom g, const cpx<
 // ---------------------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------------------
-// RBC_GENERIC_FFT=1 in the environment selects the generic radix-2 kernels everywhere (used
-// by the tests to pin the 512-point kernels against them).
+// RBC_GENERIC_FFT=1 in the environment selects the generic LDS Stockham kernels everywhere (used
+// by the tests to pin the 512-point and group-transform kernels against them).
 static bool generic_fft_forced() {
     const char* e = getenv("RBC_GENERIC_FFT");
     return e && e[0] == '1';

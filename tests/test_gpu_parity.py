@@ -153,7 +153,7 @@ def test_bundled_512_state_fp32_and_fp64():
     e = _cmp(fast, ref, keep)
     print("512^2x2 fp32 (512-point kernels) nonlinear rel errors vs oracle", e)
     eg = _cmp(generic, ref, keep)
-    print("512^2x2 fp32 (generic radix-2 kernels) nonlinear rel errors vs oracle", eg)
+    print("512^2x2 fp32 (generic LDS kernels) nonlinear rel errors vs oracle", eg)
     # FP32 on a developed state: the f/g terms carry k^2..k^3 derivative weights
     assert max(e) < 1e-4 and max(eg) < 1e-4, (e, eg)
     ef = _cmp(fast, generic, keep)
