@@ -70,6 +70,14 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Block-wide barrier that orders LDS traffic only: unlike __syncthreads it does not wait for
+// the wave's outstanding global loads and stores, so prefetches stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // One LDS pad slot every 8 complex values spreads the strided accesses of the FFT passes over
 // the banks.
 __device__ __forceinline__ int pad8(int i) { return i + (i >> 3); }

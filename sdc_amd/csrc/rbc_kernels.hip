@@ -1027,49 +1027,67 @@ __global__ void __launch_bounds__(64 * WPB) k_rows_w(DevGeom g, const cpx<float>
     }
 }
 
-// y inverse at N = 512: a block of 8 waves handles YT = 8 kx columns of one (field, level);
-// each wave transforms one column.  The LDS tile [pad8(y)][YT + 1] holds the results and
+// y inverse at N = 512: a block of 8 waves handles YT = 8 kx columns of one (field, level) at a
+// time (a tile), one column per wave.  The LDS tile [pad8(y)][YT + 1] holds the results and
 // doubles as the FFT scratch of each column; it is written out row-wise (64-byte segments).
-// Launched on a 1-D grid of 8 * per blocks (XCD-aware order, see xcd_logical_block).
+// The blocks stay resident: the tiles are split into 8 contiguous ranges, one per XCD (block b
+// runs on XCD b % 8), and the bpx blocks of an XCD walk their range together, so that the tiles
+// in flight on an XCD are neighbours (their output rows share cache lines).  Each wave loads
+// its column of the next tile before it transforms the current one.
 constexpr int YT = 8;
 constexpr int YS = YT + 1;
-__global__ void __launch_bounds__(512) k_fft_y_inv512(DevGeom g, int ntile, int nblk, int per,
+__global__ void __launch_bounds__(512) k_fft_y_inv512(DevGeom g, int ntile, int nblk, int bpx,
                                                       const cpx<float>* __restrict__ lev,
                                                       const cpx<float>* __restrict__ tw,
                                                       cpx<float>* __restrict__ hin) {
     __shared__ cpx<float> tile[576 * YS];
-    const int lb = xcd_logical_block(blockIdx.x, per);
-    if (lb >= nblk) return;
+    __shared__ cpx<float> twl[512];  // in LDS, so that no table entry stays in registers
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int NYc = NYC512;
-    const int P = g.P, nq = g.nq, KX = g.KX;
-    const int tix = lb % ntile, rest = lb / ntile;
-    const int l = rest % nq, field = rest / nq;
-    const int kx0 = tix * YT;
     constexpr int a = (NYc - 1) / 2, shift = 512 - NYc;  // kept rows [0, a] U [512 - a, 511]
-    const cpx<float>* src = lev + ((size_t)field * nq + l) * P;
-    {
-        const int c = wave, kx = kx0 + c;  // one column per wave
-        cpx<float> v[8];
+    const int P = g.P, KX = g.KX;
+    twl[threadIdx.x] = tw[threadIdx.x];
+    lds_barrier();
+    const int range = (nblk + 7) / 8, xcd = blockIdx.x & 7;
+    const int end = min(nblk, (xcd + 1) * range);
+    auto load = [&](int lb, cpx<float> (&v)[8]) {
+        const int tix = lb % ntile, rest = lb / ntile;
+        const int kx = tix * YT + wave;
+        const cpx<float>* src = lev + (size_t)rest * P + (size_t)kx * NYc;  // rest = field * nq + l
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
             const int n = lane + 64 * r;
             v[r] = mk(0.f, 0.f);
             if (kx < KX) {
-                if (n <= a) v[r] = src[(size_t)kx * NYc + n];
-                else if (n >= 512 - a) v[r] = src[(size_t)kx * NYc + n - shift];
+                if (n <= a) v[r] = src[n];
+                else if (n >= 512 - a) v[r] = src[n - shift];
             }
         }
-        fft512_wave<float, true, YS>(v, tile + c, tw, lane);
+    };
+    int lb = xcd * range + (blockIdx.x >> 3);
+    cpx<float> nxt[8];
+    if (lb < end) load(lb, nxt);
+    for (; lb < end; lb += bpx) {
+        cpx<float> v[8];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) tile[pad8(lane + 64 * r) * YS + c] = v[r];
-    }
-    __syncthreads();
-    const int ncol = min(YT, KX - kx0);
-    cpx<float>* dst = hin + ((size_t)field * nq + l) * (size_t)512 * g.KXp + kx0;
-    for (int e = threadIdx.x; e < 512 * YT; e += 512) {
-        const int y = e / YT, c = e - y * YT;
-        if (c < ncol) dst[(size_t)y * g.KXp + c] = tile[pad8(y) * YS + c];
+        for (int r = 0; r < 8; ++r) v[r] = nxt[r];
+        if (lb + bpx < end) load(lb + bpx, nxt);
+        fft512_wave<float, true, YS>(v, tile + wave, twl, lane);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) tile[pad8(lane + 64 * r) * YS + wave] = v[r];
+        lds_barrier();
+        const int tix = lb % ntile, rest = lb / ntile, kx0 = tix * YT;
+        // every lane stores on every pass (columns beyond the last kept one repeat it), so that
+        // the number of stores after the prefetch is fixed and the prefetch wait can leave them
+        // in flight
+        const int cc = min((int)threadIdx.x % YT, KX - kx0 - 1);
+        cpx<float>* dst = hin + (size_t)rest * 512 * g.KXp + kx0 + cc;
+#pragma unroll
+        for (int i = 0; i < YT; ++i) {
+            const int y = threadIdx.x / YT + (512 / YT) * i;
+            dst[(size_t)y * g.KXp] = tile[pad8(y) * YS + cc];
+        }
+        lds_barrier();  // the tile columns are the next tile's FFT scratch
     }
 }
 
@@ -1298,38 +1316,48 @@ __global__ void __launch_bounds__(64 * WPB * KS) k_project_mma(DevGeom g, int nk
     const size_t ffs = (size_t)2 * nq * P;  // floats per combination
     const float* dcol = Dfl + (size_t)2 * pc + comp;
     f32x16 cF1 = {}, cF2 = {}, cG = {}, cT = {};
-    constexpr int U = MM == 0 ? 4 : 1;  // k-steps whose operand loads are issued together
     const int spw = (nks + KS - 1) / KS, sbeg = wk * spw, send = min(nks, sbeg + spw);
-    for (int s0 = sbeg; s0 < send; s0 += U) {
+    // B operands of k-step s; levels past nq are clamped to nq - 1 (their table entries are
+    // zero), so that no load is predicated
+    auto fetch = [&](int s, float (&b)[5][MT::KPL]) {
+#pragma unroll
+        for (int e = 0; e < MT::KPL; ++e) {
+            const int l = min(s * MT::K + half * MT::KPL + e, nq - 1);
+#pragma unroll
+            for (int t = 0; t < 5; ++t) b[t][e] = dcol[t * ffs + (size_t)2 * l * P];
+        }
+    };
+    auto step = [&](int s, const float (&b)[5][MT::KPL]) {
+        float a[5][MT::KPL];
+#pragma unroll
+        for (int t = 0; t < 5; ++t) {
+            const float* src = prA + (((size_t)t * nks + s) * 64 + lane) * MT::KPL;
+#pragma unroll
+            for (int e = 0; e < MT::KPL; ++e) a[t][e] = src[e];
+        }
+        MT::run(cF1, a[0], b[0]);  // wC'  . D0
+        MT::run(cF1, a[1], b[1]);  // wC'' . D1
+        MT::run(cF2, a[2], b[1]);  // wC   . D1   (times k^2 below)
+        MT::run(cG, a[3], b[2]);   // wS   . D2
+        MT::run(cG, a[4], b[3]);   // wS'  . D3
+        MT::run(cT, a[3], b[4]);   // wS   . D4
+    };
+    // U k-steps whose operand loads are issued together while whole batches last, then singly
+    constexpr int U = MM == 0 ? 4 : 1;
+    int s = sbeg;
+    for (; s + U <= send; s += U) {
         float b[U][5][MT::KPL];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
+        for (int u = 0; u < U; ++u) fetch(s + u, b[u]);
 #pragma unroll
-            for (int e = 0; e < MT::KPL; ++e) {
-                const int l = (s0 + u) * MT::K + half * MT::KPL + e;
-#pragma unroll
-                for (int t = 0; t < 5; ++t)
-                    b[u][t][e] = (l < nq && s0 + u < send) ? dcol[t * ffs + (size_t)2 * l * P] : 0.f;
-            }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int s = s0 + u;
-            if (s >= send) break;
-            float a[5][MT::KPL];
-#pragma unroll
-            for (int t = 0; t < 5; ++t) {
-                const float* src = prA + (((size_t)t * nks + s) * 64 + lane) * MT::KPL;
-#pragma unroll
-                for (int e = 0; e < MT::KPL; ++e) a[t][e] = src[e];
-            }
-            MT::run(cF1, a[0], b[u][0]);  // wC'  . D0
-            MT::run(cF1, a[1], b[u][1]);  // wC'' . D1
-            MT::run(cF2, a[2], b[u][1]);  // wC   . D1   (times k^2 below)
-            MT::run(cG, a[3], b[u][2]);   // wS   . D2
-            MT::run(cG, a[4], b[u][3]);   // wS'  . D3
-            MT::run(cT, a[3], b[u][4]);   // wS   . D4
-        }
+        for (int u = 0; u < U; ++u) step(s + u, b[u]);
     }
+    if constexpr (U > 1)
+        for (; s < send; ++s) {
+            float b[5][MT::KPL];
+            fetch(s, b);
+            step(s, b);
+        }
     if (KS > 1) {
         if (wk > 0) {
             f32x16* dst = sPart[(wk - 1) * WPB + wave][0];
@@ -1420,9 +1448,15 @@ __global__ void __launch_bounds__(64 * CT) k_fft_y_fwd512c(DevGeom g, int ntile,
                                                        cpx<float>* __restrict__ D, cpx<float>* __restrict__ fgw) {
     using G = YGroup<GRP>;
     constexpr int CS = CT + 1;  // tile row stride
+    // groups of several products read the twiddles from LDS: the barrier between products keeps
+    // the compiler from holding the table entries in registers across the transforms
+    constexpr bool TWL = G::n > 1;
     __shared__ cpx<float> tile[576 * CS];
+    __shared__ cpx<float> twl[TWL ? 512 : 1];
     const int lb = xcd_logical_block(blockIdx.x, per);
     if (lb >= nblk) return;
+    if (TWL)
+        for (int e = threadIdx.x; e < 512; e += 64 * CT) twl[e] = tw[e];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int NYc = NYC512;
     const int P = g.P, nq = g.nq, KX = g.KX;
@@ -1467,7 +1501,7 @@ __global__ void __launch_bounds__(64 * CT) k_fft_y_fwd512c(DevGeom g, int ntile,
         cpx<float> v[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) v[r] = tile[pad8(lane + 64 * r) * CS + c];
-        fft512_wave<float, false, CS>(v, tile + c, tw, lane);
+        fft512_wave<float, false, CS>(v, tile + c, TWL ? twl : tw, lane);
 #pragma unroll
         for (int r = 0; r < 8; ++r) G::add(iq, v[r], kxv, kyv[r], d0[r], d1[r]);
         if (GRP == 1 && kxc == 0 && lane == 0) fgw[2 * l + iq] = v[0];  // P_uw, P_vw at row 0
@@ -1568,6 +1602,17 @@ __global__ void __launch_bounds__(64 * WPB) k_fft_y_fwd_wc(DevGeom g, const cpx<
 // ---------------------------------------------------------------------------------------
 // RBC_GENERIC_FFT=1 in the environment selects the generic LDS Stockham kernels everywhere (used
 // by the tests to pin the 512-point and group-transform kernels against them).
+// Blocks of `kernel` (blocks of `threads`, static LDS only) that one XCD of the current device
+// holds at once: resident blocks per CU x CUs / 8.
+static int resident_blocks_per_xcd(const void* kernel, int threads) {
+    int dev = 0, ncu = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess)
+        return 32;
+    return std::max(1, per_cu * ncu / 8);
+}
+
 static bool generic_fft_forced() {
     const char* e = getenv("RBC_GENERIC_FFT");
     return e && e[0] == '1';
@@ -1639,8 +1684,10 @@ void Launch<T>::eval(const DevGeom& g, const Buffers& b, void* const X[3], const
 template <typename T>
 void Launch<T>::fft_y_inv(const DevGeom& g, const Buffers& b, hipStream_t s) {
     if (fast512_y<T>(g)) {
-        const int ntile = (g.KX + YT - 1) / YT, nblk = ntile * g.nq * 6, per = (nblk + 7) / 8;
-        hipLaunchKernelGGL(k_fft_y_inv512, dim3(8 * per), dim3(512), 0, s, g, ntile, nblk, per,
+        const int ntile = (g.KX + YT - 1) / YT, nblk = ntile * g.nq * 6;
+        static const int resident = resident_blocks_per_xcd((const void*)k_fft_y_inv512, 512);
+        const int bpx = std::max(1, std::min(resident, (nblk + 7) / 8));
+        hipLaunchKernelGGL(k_fft_y_inv512, dim3(8 * bpx), dim3(512), 0, s, g, ntile, nblk, bpx,
                            (const cpx<float>*)b.lev, (const cpx<float>*)b.twN, (cpx<float>*)b.hin);
         return;
     }
@@ -1701,9 +1748,10 @@ void Launch<T>::rows(const DevGeom& g, const Buffers& b, hipStream_t s) {
 template <typename T>
 void Launch<T>::fft_y_fwd(const DevGeom& g, const Buffers& b, hipStream_t s) {
     if (fast512_y<T>(g)) {
-        // 4 kx columns (4 waves) per block: the combination accumulators hold 3 waves per SIMD,
-        // so smaller blocks keep 12 waves per CU busy where 8-wave blocks fit only one block
-        constexpr int CT = 4;
+        // 8 kx columns (8 waves) per block: 64-byte row segments; group 0 holds 4 waves per
+        // SIMD (2 blocks per CU), groups 1 and 2 more (measured: 4 columns 194 us, 8 178 us and
+        // 16 195 us for the three launches together)
+        constexpr int CT = 8;
         const int ntile = (g.KX + CT - 1) / CT, nblk = ntile * g.nq, per = (nblk + 7) / 8;
 #define RBC_YF512(GRPV)                                                                                          \
     hipLaunchKernelGGL((k_fft_y_fwd512c<GRPV, CT>), dim3(8 * per), dim3(64 * CT), 0, s, g, ntile, nblk, per,     \
