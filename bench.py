@@ -131,7 +131,9 @@ def ensemble_configs(n=64, seed=4):
 
 
 def run_ensemble(a):
-    """Member-steps per second of the batched workload: graph replay vs eager launches."""
+    """Member-steps per second of the ensemble workload: graph replay with the size classes
+    batched (one launch per stage and class), graph replay with one branch per member, and
+    eager launches."""
     import numpy as np
     from sdc_amd import Ensemble
     from sdc_amd.initial import random_theta
@@ -145,18 +147,38 @@ def run_ensemble(a):
     for i, (m, c) in enumerate(zip(E.members, cfgs)):
         m.set_state(*random_theta(c["M"], c["N"], c["L"], seed=1000 + i, amp=1e-3))
     E.step(1 + a.warmup, steps_per_launch=a.ens_k)  # Euler start, capture, warm-up
+    batched = E.batched_members
     res = {}
-    for mode, k in (("graph", a.ens_k), ("eager", 0)):
+    for mode, k in (("graph", a.ens_k), ("graph_unbatched", a.ens_k), ("eager", 0)):
+        if mode == "graph_unbatched":
+            E.set_batching(False)
+            E.step(a.ens_k, steps_per_launch=a.ens_k)  # capture outside the timed region
         t0 = time.perf_counter()
         E.step(a.steps, steps_per_launch=k)
         res[mode] = time.perf_counter() - t0
     nu = np.array([m.nusselt()[2] for m in E.members])
     finite = int(np.isfinite(nu).sum())
     E.close()
+    # the same draw with every grid at 64^2: members too small to fill the device, where the
+    # batched classes pay off
+    small = [dict(c, M=64, N=64, dt=0.25 * (4.0 / 64) / (c["Ra"] * 0.7) ** 0.5) for c in cfgs]
+    small_rate = {}
+    for mode, on in (("batched", True), ("unbatched", False)):
+        S = Ensemble(small)
+        S.set_batching(on)
+        for i, (m, c) in enumerate(zip(S.members, small)):
+            m.set_state(*random_theta(c["M"], c["N"], c["L"], seed=1000 + i, amp=1e-3))
+        S.step(1 + 2 * a.ens_k, steps_per_launch=a.ens_k)
+        t0 = time.perf_counter()
+        S.step(a.steps, steps_per_launch=a.ens_k)
+        small_rate[mode] = round(len(small) * a.steps / (time.perf_counter() - t0), 1)
+        if on:
+            small_rate["batched_members"] = S.batched_members
+        S.close()
     n = len(cfgs)
     classes = sorted({(c["M"], c["L"]) for c in cfgs})
     out = {
-        "metric": "ensemble member-steps/s (64 uneven RBC runs, one captured graph per launch)",
+        "metric": "ensemble member-steps/s (64 uneven RBC runs, size classes batched in one captured graph per launch)",
         "value": round(n * a.steps / res["graph"], 1),
         "unit": "member-steps/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -166,11 +188,16 @@ def run_ensemble(a):
         "config": {"workload": f"ensemble of {n} runs, grid in {{64^2,128^2,256^2}}, L in {{8,16,24}}, "
                                "Ra log-uniform [1e4,1e6], Pr=0.7, 4x4x1 box",
                    "steps_per_launch": a.ens_k, "size_classes": [list(c) for c in classes]},
+        "batched_members": batched,
+        "unbatched_graph_member_steps_per_s": round(n * a.steps / res["graph_unbatched"], 1),
         "eager_member_steps_per_s": round(n * a.steps / res["eager"], 1),
+        "batched_speedup_vs_unbatched_graph": round(res["graph_unbatched"] / res["graph"], 3),
         "graph_speedup_vs_eager": round(res["eager"] / res["graph"], 3),
         "create_s": round(t_create, 2), "tune_s": round(t_tune, 2),
         "tuned_shapes": {f"{c['M']}^2xL{c['L']}": [int(r[0]), int(r[1]), round(float(r[2]), 4)]
                          for c, r in zip(cfgs, rep)},
+        "small_grid_ensemble": {"workload": "the same 64 runs, every grid at 64^2 (L, Ra as drawn)",
+                                "member_steps_per_s": small_rate},
         "members_finite": finite,
         "nu_vol_range": [round(float(np.nanmin(nu)), 6), round(float(np.nanmax(nu)), 6)],
     }

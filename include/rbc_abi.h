@@ -200,7 +200,12 @@ int rbc_energy_spectrum_ref(rbc_ctx* c, double z, int cap, double* E3, int* nbin
  * member alone.  rbc_ensemble_tune times launch shapes of the generic FFT kernels per size
  * class (M, N, L, precision) on the first member of the class (its state is restored) and
  * applies the fastest to the class; report (optional, 3 doubles per member) receives
- * {y-FFT columns per block, row-kernel threads, ms per step}. */
+ * {y-FFT columns per block, row-kernel threads, ms per step}.
+ * In the graph, the members of one size class (same M, N, L, nq, precision and tuned shapes)
+ * on the FP32 / BF16X3 path of 64..256-point grids with L <= 32 advance together, one launch
+ * per stage for the class (same per-member arithmetic as a run of its own); others are graph
+ * branches of their own.  rbc_ensemble_batching(e, 0) turns this off (1: on, the default);
+ * rbc_ensemble_batched_members gives how many members the current graph steps in batches. */
 typedef struct rbc_ensemble rbc_ensemble;
 int rbc_ensemble_create(int n, const rbc_params* params, rbc_ensemble** out);
 void rbc_ensemble_destroy(rbc_ensemble* e);
@@ -209,6 +214,8 @@ int rbc_ensemble_size(const rbc_ensemble* e);
 rbc_ctx* rbc_ensemble_member(rbc_ensemble* e, int i);
 int rbc_ensemble_tune(rbc_ensemble* e, double* report);
 int rbc_ensemble_step(rbc_ensemble* e, int nsteps, int steps_per_launch);
+int rbc_ensemble_batching(rbc_ensemble* e, int on);
+int rbc_ensemble_batched_members(const rbc_ensemble* e);
 
 /* ---- Learned closure for coarse runs -------------------------------------------------------
  * No counterpart in the reference.  A pointwise MLP (inputs u, v, w, theta, d_x theta,

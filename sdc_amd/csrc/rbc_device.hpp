@@ -382,7 +382,7 @@ __global__ void __launch_bounds__(256) k_fft_y_fwd(DevGeom g, int TX, const cpx<
 // the LDS tile [pad8(y)][CB + 1] is each column's FFT scratch and the transpose to the
 // row-major hybrid layout (rows of CB contiguous values).
 template <int LOGN, int WPB>
-__global__ void __launch_bounds__(64 * WPB) k_fft_y_inv_w(DevGeom g, const cpx<float>* __restrict__ lev,
+__device__ __forceinline__ void k_fft_y_inv_w_body(int bz, DevGeom g, const cpx<float>* __restrict__ lev,
                                                      const cpx<float>* __restrict__ twN, cpx<float>* __restrict__ hin) {
     constexpr int N = 1 << LOGN, W = N / 8, G = 64 / W, CB = WPB * G, S = CB + 1;
     __shared__ cpx<float> tw[N];
@@ -390,7 +390,7 @@ __global__ void __launch_bounds__(64 * WPB) k_fft_y_inv_w(DevGeom g, const cpx<f
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, grp = lane / W, j = lane - grp * W;
     constexpr int NYc = kept_rows(N), a = (NYc - 1) / 2, shift = N - NYc;  // kept rows [0, a] U [N - a, N)
     const int P = g.P, nq = g.nq, KX = g.KX;
-    const int kx0 = blockIdx.x * CB, l = blockIdx.y, field = blockIdx.z;
+    const int kx0 = blockIdx.x * CB, l = blockIdx.y, field = bz;
     const int c = wave * G + grp, kx = kx0 + c;
     for (int e = threadIdx.x; e < N; e += blockDim.x) tw[e] = twN[e];
     __syncthreads();
@@ -416,6 +416,25 @@ __global__ void __launch_bounds__(64 * WPB) k_fft_y_inv_w(DevGeom g, const cpx<f
         if (cc < ncol) dst[(size_t)yy * g.KXp + cc] = tile[pad8(yy) * S + cc];
     }
 }
+
+template <int LOGN, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_fft_y_inv_w(DevGeom g, const cpx<float>* __restrict__ lev,
+                                                     const cpx<float>* __restrict__ twN, cpx<float>* __restrict__ hin) {
+    k_fft_y_inv_w_body<LOGN, WPB>(blockIdx.z, g, lev, twN, hin);
+}
+struct k_fft_y_inv_w_args {
+    DevGeom g;
+    const cpx<float>* lev;
+    const cpx<float>* twN;
+    cpx<float>* hin;
+};
+template <int LOGN, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_fft_y_inv_w_batch(const k_fft_y_inv_w_args* __restrict__ tab, int zdim) {
+    const int mb = blockIdx.z / zdim;
+    const k_fft_y_inv_w_args& a = tab[mb];
+    k_fft_y_inv_w_body<LOGN, WPB>(blockIdx.z - mb * zdim, a.g, a.lev, a.twN, a.hin);
+}
+
 
 template <int LOGN, int WPB>
 __global__ void __launch_bounds__(64 * WPB) k_fft_y_fwd_w(DevGeom g, const cpx<float>* __restrict__ hout,

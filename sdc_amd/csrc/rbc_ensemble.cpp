@@ -3,10 +3,12 @@
 //
 // The reference runs one simulation per process (run/script_onset_sdc.sh:8 loops over
 // processes).  Here every member is a full context (its own factors, buffers and stream);
-// K steps of every member are captured once into a graph whose branches (one per member)
-// are independent, and each rbc_ensemble_step replays it.  The first step of a member with
-// no AB2 history is an Euler step with other coefficients, so it runs eagerly before the
-// graph is used.
+// K steps of every member are captured once into a graph and each rbc_ensemble_step replays
+// it.  Members of one size class on the batched path (FP32 storage, 64..256-point grids,
+// L <= 32) advance together: one launch per stage for the whole class (batch_step), so a
+// small member no longer pays a full launch per stage; the other members are independent
+// branches of the graph.  The first step of a member with no AB2 history is an Euler step
+// with other coefficients, so it runs eagerly before the graph is used.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,8 +33,13 @@ struct rbc_ensemble {
     int graph_steps = 0;
     std::string err;
     std::vector<double> tune_report;  // per member: ytx, rows threads, ms per step
+    bool batching = true;
+    std::vector<StepBatch*> batches;       // one per size class run as a batch
+    std::vector<std::vector<int>> bmembers;  // member indices of each batch
+    std::vector<int> solo;                 // members stepped on their own
     ~rbc_ensemble() {
         if (s0) hipStreamSynchronize(s0);
+        for (StepBatch* b : batches) batch_destroy(b);
         if (exec) hipGraphExecDestroy(exec);
         if (graph) hipGraphDestroy(graph);
         for (hipEvent_t e : join) hipEventDestroy(e);
@@ -55,6 +62,62 @@ void drop_graph(rbc_ensemble* e) {
     e->exec = nullptr;
     e->graph = nullptr;
     e->graph_steps = 0;
+    for (StepBatch* b : e->batches) batch_destroy(b);
+    e->batches.clear();
+    e->bmembers.clear();
+    e->solo.clear();
+}
+
+// Level-space bytes of one member that a step writes in one stage and reads in the next (the
+// hybrid rows, 6 fields x nq levels x N x KXp complex).
+size_t stage_bytes(const rbc_ctx* c) { return (size_t)6 * c->dg.nq * c->dg.N * c->dg.KXp * c->csz(); }
+
+// Batching pays where a member's kernels are too small to fill the device: their launches and
+// ramp-up dominate, and a batch runs the class's blocks together (measured on the 64-run
+// workload: the 64^2 members 2.5x faster).  Members of 128^2 and more already fill it and
+// overlap with each other as graph branches; batched they ran 5-7 % slower (their stage data,
+// in flight for the whole batch at once, no longer stays in the last-level cache between
+// stages).  So only classes with at most kBatchMemberBytes of stage data per member batch, in
+// batches of at most kBatchBytes.
+constexpr size_t kBatchMemberBytes = (size_t)8 << 20;
+constexpr size_t kBatchBytes = (size_t)64 << 20;
+
+// size classes of two or more members with the same launch shapes become batches; the rest,
+// and members with a hook after the nonlinear term, step on their own
+void build_batches(rbc_ensemble* e) {
+    std::map<std::tuple<int, int, int, int, int, int, int>, std::vector<int>> cls;
+    for (size_t i = 0; i < e->m.size(); ++i) {
+        rbc_ctx* c = e->m[i];
+        if (!e->batching || c->post_nonlinear || c->rsz != 4 || stage_bytes(c) > kBatchMemberBytes) {
+            e->solo.push_back((int)i);
+            continue;
+        }
+        cls[std::make_tuple(c->dg.M, c->dg.N, c->dg.L, c->dg.nq, c->prec, c->b.tune_ytx, c->b.tune_rows_threads)]
+            .push_back((int)i);
+    }
+    for (auto& kv : cls) {
+        const std::vector<int>& all = kv.second;
+        const size_t per = std::max<size_t>(1, kBatchBytes / stage_bytes(e->m[all[0]]));
+        for (size_t j0 = 0; j0 < all.size(); j0 += per) {
+            const std::vector<int> idx(all.begin() + j0, all.begin() + std::min(all.size(), j0 + per));
+            StepBatch* sb = nullptr;
+            if (idx.size() >= 2) {
+                std::vector<BatchMember> bm;
+                for (int i : idx) {
+                    rbc_ctx* c = e->m[i];
+                    bm.push_back({&c->dg, &c->b, c->p.Pr, c->p.dt, c->fac.L0, c->fac.L1, c->fac.stride});
+                }
+                std::string why;
+                sb = batch_create(bm, why);
+            }
+            if (sb) {
+                e->batches.push_back(sb);
+                e->bmembers.push_back(idx);
+            } else {
+                e->solo.insert(e->solo.end(), idx.begin(), idx.end());
+            }
+        }
+    }
 }
 
 // members without AB2 history take their Euler step eagerly (on their own streams)
@@ -65,15 +128,26 @@ void euler_starts(rbc_ensemble* e) {
 
 void capture(rbc_ensemble* e, int K) {
     drop_graph(e);
+    build_batches(e);  // device tables are allocated before the capture begins
     HIPCHK(hipStreamBeginCapture(e->s0, hipStreamCaptureModeThreadLocal));
     HIPCHK(hipEventRecord(e->fork, e->s0));
-    for (size_t i = 0; i < e->m.size(); ++i) {
+    for (int i : e->solo) {
         rbc_ctx* c = e->m[i];
         HIPCHK(hipStreamWaitEvent(c->stream, e->fork, 0));
         run_steps(c, K);
         HIPCHK(hipEventRecord(e->join[i], c->stream));
-        HIPCHK(hipStreamWaitEvent(e->s0, e->join[i], 0));
     }
+    // each batch is a branch of its own (on its first member's stream), so that size classes
+    // overlap as the members' branches do
+    for (size_t k = 0; k < e->batches.size(); ++k) {
+        const int i0 = e->bmembers[k][0];
+        hipStream_t bs = e->m[i0]->stream;
+        HIPCHK(hipStreamWaitEvent(bs, e->fork, 0));
+        for (int st = 0; st < K; ++st) batch_step(e->batches[k], bs);
+        HIPCHK(hipEventRecord(e->join[i0], bs));
+    }
+    for (int i : e->solo) HIPCHK(hipStreamWaitEvent(e->s0, e->join[i], 0));
+    for (const auto& v : e->bmembers) HIPCHK(hipStreamWaitEvent(e->s0, e->join[v[0]], 0));
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamEndCapture(e->s0, &g));
     e->graph = g;
@@ -255,6 +329,26 @@ int rbc_ensemble_tune(rbc_ensemble* e, double* report) {
         return efail(e, RBC_ERR_HIP, std::string(err.what) + ": " + hipGetErrorString(err.e));
     }
     return RBC_OK;
+}
+
+int rbc_ensemble_batching(rbc_ensemble* e, int on) {
+    if (!e || (on != 0 && on != 1)) return RBC_ERR_ARG;
+    try {
+        hipSetDevice(e->device);
+        HIPCHK(hipStreamSynchronize(e->s0));
+        drop_graph(e);
+    } catch (const HipError& err) {
+        return efail(e, RBC_ERR_HIP, std::string(err.what) + ": " + hipGetErrorString(err.e));
+    }
+    e->batching = on == 1;
+    return RBC_OK;
+}
+
+int rbc_ensemble_batched_members(const rbc_ensemble* e) {
+    if (!e) return 0;
+    int n = 0;
+    for (const auto& v : e->bmembers) n += (int)v.size();
+    return n;
 }
 
 int rbc_ensemble_step(rbc_ensemble* e, int nsteps, int steps_per_launch) {

@@ -165,4 +165,20 @@ struct Launch {
     static void nusselt(const DevGeom& g, const Buffers& b, const void* ia1, hipStream_t s);
 };
 
+// Batched steps of the members of one ensemble size class (rbc_kernels.hip).  A member is
+// described by its geometry, buffers and step constants; batch_create returns nullptr (reason
+// in why) when the members are not on the path the batched kernels cover (FP32 storage on the
+// matrix cores, group transforms of 64..256 points, L <= 32) or differ in launch shape.
+struct BatchMember {
+    const DevGeom* g;
+    const Buffers* b;
+    double Pr, dt;
+    int L0, L1, stride;
+};
+struct StepBatch;
+StepBatch* batch_create(const std::vector<BatchMember>& m, std::string& why);
+// one AB2 step of every member (all of them must have their AB2 history)
+void batch_step(const StepBatch* sb, hipStream_t s);
+void batch_destroy(StepBatch* sb);
+
 }  // namespace rbc

@@ -11,6 +11,7 @@
 
 #include <cstdlib>
 
+#include "rbc_ctx.hpp"
 #include "rbc_device.hpp"
 
 namespace rbc {
@@ -547,7 +548,7 @@ __global__ void __launch_bounds__(64) k_update(DevGeom g, int L0, int L1, int st
 // B^-1_f (L_c <= 16 values) for every member of the group.  Same arithmetic and rounding
 // order as k_update; mode UPD_STEP only.
 template <typename T>
-__global__ void __launch_bounds__(256) k_update32(DevGeom g, int nchunk, int L0, int L1, int stride, T Pr, T dt,
+__device__ __forceinline__ void k_update32_body(DevGeom g, int nchunk, int L0, int L1, int stride, T Pr, T dt,
                                                   T a_cur, T a_prev, int use_prev, const int* __restrict__ cgrp,
                                                   const int* __restrict__ cbeg,
                                                   const int* __restrict__ gmem, const T* __restrict__ gmat,
@@ -654,6 +655,53 @@ __global__ void __launch_bounds__(256) k_update32(DevGeom g, int nchunk, int L0,
     }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256) k_update32(DevGeom g, int nchunk, int L0, int L1, int stride, T Pr, T dt,
+                                                  T a_cur, T a_prev, int use_prev, const int* __restrict__ cgrp,
+                                                  const int* __restrict__ cbeg,
+                                                  const int* __restrict__ gmem, const T* __restrict__ gmat,
+                                                  const T* __restrict__ k2a, cpx<T>* __restrict__ Xt,
+                                                  cpx<T>* __restrict__ Xf, cpx<T>* __restrict__ Xg,
+                                                  const cpx<T>* __restrict__ Nt, const cpx<T>* __restrict__ Nf,
+                                                  const cpx<T>* __restrict__ Ng, cpx<T>* __restrict__ Rt,
+                                                  cpx<T>* __restrict__ Rf, cpx<T>* __restrict__ Rg) {
+    k_update32_body<T>(g, nchunk, L0, L1, stride, Pr, dt, a_cur, a_prev, use_prev, cgrp, cbeg, gmem, gmat, k2a, Xt, Xf, Xg, Nt, Nf, Ng, Rt, Rf, Rg);
+}
+template <typename T>
+struct k_update32_args {
+    DevGeom g;
+    int nchunk;
+    int L0;
+    int L1;
+    int stride;
+    T Pr;
+    T dt;
+    T a_cur;
+    T a_prev;
+    int use_prev;
+    const int* cgrp;
+    const int* cbeg;
+    const int* gmem;
+    const T* gmat;
+    const T* k2a;
+    cpx<T>* Xt;
+    cpx<T>* Xf;
+    cpx<T>* Xg;
+    const cpx<T>* Nt;
+    const cpx<T>* Nf;
+    const cpx<T>* Ng;
+    cpx<T>* Rt;
+    cpx<T>* Rf;
+    cpx<T>* Rg;
+};
+template <typename T>
+__global__ void __launch_bounds__(256) k_update32_batch(const k_update32_args<T>* __restrict__ tab, int zdim) {
+    const int mb = blockIdx.z / zdim;
+    const k_update32_args<T>& a = tab[mb];
+    k_update32_body<T>(a.g, a.nchunk, a.L0, a.L1, a.stride, a.Pr, a.dt, a.a_cur, a.a_prev, a.use_prev, a.cgrp, a.cbeg, a.gmem, a.gmat, a.k2a, a.Xt, a.Xf, a.Xg, a.Nt, a.Nf, a.Ng, a.Rt, a.Rf, a.Rg);
+}
+
+
 // F, G: inverse B = 2 Pr (imag 0), implicit factor 1/(1 + dt Pr (m pi)^2) (imag 0); thread m.
 template <typename T>
 __device__ __forceinline__ void update_FG_mode(int m, int L, T Pr, T dt, T a_cur, T a_prev, int use_prev, int mode,
@@ -710,7 +758,7 @@ __global__ void k_make_real(DevGeom g, const int* __restrict__ rows, cpx<T>* X0,
 // the end of a time step in one launch: make_real on the blocks before the last, the F, G
 // update (mode UPD_STEP) on the last block
 template <typename T>
-__global__ void __launch_bounds__(256) k_finish(DevGeom g, const int* __restrict__ rows, cpx<T>* X0, cpx<T>* X1,
+__device__ __forceinline__ void k_finish_body(DevGeom g, const int* __restrict__ rows, cpx<T>* X0, cpx<T>* X1,
                                                 cpx<T>* X2, T Pr, T dt, T a_cur, T a_prev, int use_prev, cpx<T>* XF,
                                                 cpx<T>* XG, const cpx<T>* NF, const cpx<T>* NG, cpx<T>* RF,
                                                 cpx<T>* RG) {
@@ -733,6 +781,40 @@ __global__ void __launch_bounds__(256) k_finish(DevGeom g, const int* __restrict
         X[(size_t)kyc * L + m] = conjg(s);
     }
 }
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_finish(DevGeom g, const int* __restrict__ rows, cpx<T>* X0, cpx<T>* X1,
+                                                cpx<T>* X2, T Pr, T dt, T a_cur, T a_prev, int use_prev, cpx<T>* XF,
+                                                cpx<T>* XG, const cpx<T>* NF, const cpx<T>* NG, cpx<T>* RF,
+                                                cpx<T>* RG) {
+    k_finish_body<T>(g, rows, X0, X1, X2, Pr, dt, a_cur, a_prev, use_prev, XF, XG, NF, NG, RF, RG);
+}
+template <typename T>
+struct k_finish_args {
+    DevGeom g;
+    const int* rows;
+    cpx<T>* X0;
+    cpx<T>* X1;
+    cpx<T>* X2;
+    T Pr;
+    T dt;
+    T a_cur;
+    T a_prev;
+    int use_prev;
+    cpx<T>* XF;
+    cpx<T>* XG;
+    const cpx<T>* NF;
+    const cpx<T>* NG;
+    cpx<T>* RF;
+    cpx<T>* RG;
+};
+template <typename T>
+__global__ void __launch_bounds__(256) k_finish_batch(const k_finish_args<T>* __restrict__ tab, int zdim) {
+    const int mb = blockIdx.z / zdim;
+    const k_finish_args<T>& a = tab[mb];
+    k_finish_body<T>(a.g, a.rows, a.X0, a.X1, a.X2, a.Pr, a.dt, a.a_cur, a.a_prev, a.use_prev, a.XF, a.XG, a.NF, a.NG, a.RF, a.RG);
+}
+
 
 // Nusselt reductions (FP64): blocks write partial sums of sum_p w_p sum_{m,n} Ia1(m,n)
 // Re(k^2 f_m conj theta_n) to red[blockIdx]; red[nblk] = plate sums from mode p = 0.
@@ -931,7 +1013,7 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, const cpx<float>* _
 // block.  Same arithmetic as k_rows512t: both members of every input and product pair are
 // balanced by a power of two (pair_scale over the row, a shuffle reduction over the group).
 template <int LOGM, int WPB>
-__global__ void __launch_bounds__(64 * WPB) k_rows_w(DevGeom g, const cpx<float>* __restrict__ hin,
+__device__ __forceinline__ void k_rows_w_body(DevGeom g, const cpx<float>* __restrict__ hin,
                                                 const cpx<float>* __restrict__ twM, cpx<float>* __restrict__ hout) {
     constexpr int M = 1 << LOGM, W = M / 8, G = 64 / W, RPB = WPB * G, SP = M + M / 8, KX = kept_cols(M);
     __shared__ cpx<float> tw[M];
@@ -1026,6 +1108,25 @@ __global__ void __launch_bounds__(64 * WPB) k_rows_w(DevGeom g, const cpx<float>
         wave_sync();  // the scratch is rewritten by the next pair
     }
 }
+
+template <int LOGM, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_rows_w(DevGeom g, const cpx<float>* __restrict__ hin,
+                                                const cpx<float>* __restrict__ twM, cpx<float>* __restrict__ hout) {
+    k_rows_w_body<LOGM, WPB>(g, hin, twM, hout);
+}
+struct k_rows_w_args {
+    DevGeom g;
+    const cpx<float>* hin;
+    const cpx<float>* twM;
+    cpx<float>* hout;
+};
+template <int LOGM, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_rows_w_batch(const k_rows_w_args* __restrict__ tab, int zdim) {
+    const int mb = blockIdx.z / zdim;
+    const k_rows_w_args& a = tab[mb];
+    k_rows_w_body<LOGM, WPB>(a.g, a.hin, a.twM, a.hout);
+}
+
 
 // y inverse at N = 512: a block of 8 waves handles YT = 8 kx columns of one (field, level) at a
 // time (a tile), one column per wave.  The LDS tile [pad8(y)][YT + 1] holds the results and
@@ -1142,7 +1243,7 @@ struct Mma<1> {
 // LDS as [field][m][128 columns].  evA: [4][NLT][NKS][64][KPL] floats; ALDS: the block's A
 // operands are copied to LDS with the modes (when both fit in 64 KiB).
 template <int MM, bool ALDS>
-__global__ void __launch_bounds__(256) k_eval_mma(DevGeom g, int nlt, int nks, int per,
+__device__ __forceinline__ void k_eval_mma_body(DevGeom g, int nlt, int nks, int per,
                                                   const cpx<float>* __restrict__ Xt,
                                                   const cpx<float>* __restrict__ Xf, const cpx<float>* __restrict__ Xg,
                                                   const cpx<float>* __restrict__ XF, const cpx<float>* __restrict__ XG,
@@ -1286,6 +1387,40 @@ __global__ void __launch_bounds__(256) k_eval_mma(DevGeom g, int nlt, int nks, i
     }
 }
 
+template <int MM, bool ALDS>
+__global__ void __launch_bounds__(256) k_eval_mma(DevGeom g, int nlt, int nks, int per,
+                                                  const cpx<float>* __restrict__ Xt,
+                                                  const cpx<float>* __restrict__ Xf, const cpx<float>* __restrict__ Xg,
+                                                  const cpx<float>* __restrict__ XF, const cpx<float>* __restrict__ XG,
+                                                  const float* __restrict__ evA, const float* __restrict__ evtab,
+                                                  const float* __restrict__ kxa, const float* __restrict__ kya,
+                                                  cpx<float>* __restrict__ lev) {
+    k_eval_mma_body<MM, ALDS>(g, nlt, nks, per, Xt, Xf, Xg, XF, XG, evA, evtab, kxa, kya, lev);
+}
+struct k_eval_mma_args {
+    DevGeom g;
+    int nlt;
+    int nks;
+    int per;
+    const cpx<float>* Xt;
+    const cpx<float>* Xf;
+    const cpx<float>* Xg;
+    const cpx<float>* XF;
+    const cpx<float>* XG;
+    const float* evA;
+    const float* evtab;
+    const float* kxa;
+    const float* kya;
+    cpx<float>* lev;
+};
+template <int MM, bool ALDS>
+__global__ void __launch_bounds__(256) k_eval_mma_batch(const k_eval_mma_args* __restrict__ tab, int zdim) {
+    const int mb = blockIdx.z / zdim;
+    const k_eval_mma_args& a = tab[mb];
+    k_eval_mma_body<MM, ALDS>(a.g, a.nlt, a.nks, a.per, a.Xt, a.Xf, a.Xg, a.XF, a.XG, a.evA, a.evtab, a.kxa, a.kya, a.lev);
+}
+
+
 // Projection on the matrix cores: Out[m][column] = sum_l T_t[m][l] D_t[l][column]; the B
 // operands are float components of the five level combinations (column = (mode, re/im), so a
 // lane reads one float and the lanes of a wave read 128 contiguous bytes per level).
@@ -1294,7 +1429,7 @@ __global__ void __launch_bounds__(256) k_eval_mma(DevGeom g, int nlt, int nks, i
 // add their partial sums through LDS in fixed order (KS > 1 gives small grids more waves in
 // flight).
 template <int MM, int WPB, int KS>
-__global__ void __launch_bounds__(64 * WPB * KS) k_project_mma(DevGeom g, int nks, const float* __restrict__ Dfl,
+__device__ __forceinline__ void k_project_mma_body(DevGeom g, int nks, const float* __restrict__ Dfl,
                                                      const cpx<float>* __restrict__ fgw, const float* __restrict__ prA,
                                                      const float* __restrict__ prtab, const float* __restrict__ kxa,
                                                      const float* __restrict__ kya, float Pr,
@@ -1402,6 +1537,40 @@ __global__ void __launch_bounds__(64 * WPB * KS) k_project_mma(DevGeom g, int nk
     }
     mean_flow_projection(g, fgw, prtab, Pr, NF, NG);
 }
+
+template <int MM, int WPB, int KS>
+__global__ void __launch_bounds__(64 * WPB * KS) k_project_mma(DevGeom g, int nks, const float* __restrict__ Dfl,
+                                                     const cpx<float>* __restrict__ fgw, const float* __restrict__ prA,
+                                                     const float* __restrict__ prtab, const float* __restrict__ kxa,
+                                                     const float* __restrict__ kya, float Pr,
+                                                     cpx<float>* __restrict__ Nt, cpx<float>* __restrict__ Nf,
+                                                     cpx<float>* __restrict__ Ng, cpx<float>* __restrict__ NF,
+                                                     cpx<float>* __restrict__ NG) {
+    k_project_mma_body<MM, WPB, KS>(g, nks, Dfl, fgw, prA, prtab, kxa, kya, Pr, Nt, Nf, Ng, NF, NG);
+}
+struct k_project_mma_args {
+    DevGeom g;
+    int nks;
+    const float* Dfl;
+    const cpx<float>* fgw;
+    const float* prA;
+    const float* prtab;
+    const float* kxa;
+    const float* kya;
+    float Pr;
+    cpx<float>* Nt;
+    cpx<float>* Nf;
+    cpx<float>* Ng;
+    cpx<float>* NF;
+    cpx<float>* NG;
+};
+template <int MM, int WPB, int KS>
+__global__ void __launch_bounds__(64 * WPB * KS) k_project_mma_batch(const k_project_mma_args* __restrict__ tab, int zdim) {
+    const int mb = blockIdx.z / zdim;
+    const k_project_mma_args& a = tab[mb];
+    k_project_mma_body<MM, WPB, KS>(a.g, a.nks, a.Dfl, a.fgw, a.prA, a.prtab, a.kxa, a.kya, a.Pr, a.Nt, a.Nf, a.Ng, a.NF, a.NG);
+}
+
 
 // Forward y-FFT at N = 512 fused with the level combinations.  A block of 8 waves handles 8
 // kx columns of one level and one product group:
@@ -1583,7 +1752,7 @@ __device__ __forceinline__ void y_fwd_group_comb(const DevGeom& g, cpx<float>* t
 }
 
 template <int LOGN, int WPB>
-__global__ void __launch_bounds__(64 * WPB) k_fft_y_fwd_wc(DevGeom g, const cpx<float>* __restrict__ hout,
+__device__ __forceinline__ void k_fft_y_fwd_wc_body(int bz, DevGeom g, const cpx<float>* __restrict__ hout,
                                                           const cpx<float>* __restrict__ twN,
                                                           const float* __restrict__ kxa, const float* __restrict__ kya,
                                                           cpx<float>* __restrict__ D, cpx<float>* __restrict__ fgw) {
@@ -1592,10 +1761,34 @@ __global__ void __launch_bounds__(64 * WPB) k_fft_y_fwd_wc(DevGeom g, const cpx<
     __shared__ cpx<float> tile[(N + N / 8) * S];
     for (int e = threadIdx.x; e < N; e += blockDim.x) tw[e] = twN[e];
     __syncthreads();
-    if (blockIdx.z == 0) y_fwd_group_comb<LOGN, WPB, 0>(g, tile, tw, hout, kxa, kya, D, fgw);
-    else if (blockIdx.z == 1) y_fwd_group_comb<LOGN, WPB, 1>(g, tile, tw, hout, kxa, kya, D, fgw);
+    if (bz == 0) y_fwd_group_comb<LOGN, WPB, 0>(g, tile, tw, hout, kxa, kya, D, fgw);
+    else if (bz == 1) y_fwd_group_comb<LOGN, WPB, 1>(g, tile, tw, hout, kxa, kya, D, fgw);
     else y_fwd_group_comb<LOGN, WPB, 2>(g, tile, tw, hout, kxa, kya, D, fgw);
 }
+
+template <int LOGN, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_fft_y_fwd_wc(DevGeom g, const cpx<float>* __restrict__ hout,
+                                                          const cpx<float>* __restrict__ twN,
+                                                          const float* __restrict__ kxa, const float* __restrict__ kya,
+                                                          cpx<float>* __restrict__ D, cpx<float>* __restrict__ fgw) {
+    k_fft_y_fwd_wc_body<LOGN, WPB>(blockIdx.z, g, hout, twN, kxa, kya, D, fgw);
+}
+struct k_fft_y_fwd_wc_args {
+    DevGeom g;
+    const cpx<float>* hout;
+    const cpx<float>* twN;
+    const float* kxa;
+    const float* kya;
+    cpx<float>* D;
+    cpx<float>* fgw;
+};
+template <int LOGN, int WPB>
+__global__ void __launch_bounds__(64 * WPB) k_fft_y_fwd_wc_batch(const k_fft_y_fwd_wc_args* __restrict__ tab, int zdim) {
+    const int mb = blockIdx.z / zdim;
+    const k_fft_y_fwd_wc_args& a = tab[mb];
+    k_fft_y_fwd_wc_body<LOGN, WPB>(blockIdx.z - mb * zdim, a.g, a.hout, a.twN, a.kxa, a.kya, a.D, a.fgw);
+}
+
 
 // ---------------------------------------------------------------------------------------
 // Launchers
@@ -1883,5 +2076,228 @@ void Launch<T>::nusselt(const DevGeom& g, const Buffers& b, const void* ia1, hip
 
 template struct Launch<float>;
 template struct Launch<double>;
+
+// ---------------------------------------------------------------------------------------
+// Batched steps: the members of one ensemble size class advance together, one launch per stage
+// for all of them (blockIdx.z = member, or member x fields for the kernels whose own grid has a
+// z extent), each block reading its member's arguments from a table in device memory.  The
+// kernels are the single-run kernels' bodies with the single-run launch shapes, so a member's
+// arithmetic is the same as in a run of its own.
+// ---------------------------------------------------------------------------------------
+struct StepBatch {
+    int B = 0;
+    DevGeom g;            // the class geometry (launch shapes; Lx, Ly come from each member's table)
+    int mma = 0;          // 0: FP32 MFMA, 1: BF16 x3
+    bool alds = false;    // evaluation operand tables in LDS
+    size_t ev_smem = 0;   // evaluation dynamic LDS
+    int ev_blocks = 0;    // evaluation blocks per member (XCD-ordered 1-D grid)
+    int yi_wpb = 0;       // waves per block of the inverse group y-FFT
+    int yf_wpb = 0;       // waves per block of the forward group y-FFT
+    int rows_wpb = 0;     // waves per row block
+    bool small_proj = false;
+    int nchunk_max = 0;
+    k_eval_mma_args* ev = nullptr;
+    k_fft_y_inv_w_args* yi = nullptr;
+    k_rows_w_args* rw = nullptr;
+    k_fft_y_fwd_wc_args* yf = nullptr;
+    k_project_mma_args* pj = nullptr;
+    k_update32_args<float>* up = nullptr;
+    k_finish_args<float>* fi = nullptr;
+};
+
+template <typename A>
+static A* upload_table(const std::vector<A>& v) {
+    void* p = nullptr;
+    HIPCHK(hipMalloc(&p, v.size() * sizeof(A)));
+    HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(A), hipMemcpyHostToDevice));
+    return static_cast<A*>(p);
+}
+
+void batch_destroy(StepBatch* sb) {
+    if (!sb) return;
+    for (void* p : {(void*)sb->ev, (void*)sb->yi, (void*)sb->rw, (void*)sb->yf, (void*)sb->pj, (void*)sb->up,
+                    (void*)sb->fi})
+        if (p) hipFree(p);
+    delete sb;
+}
+
+StepBatch* batch_create(const std::vector<BatchMember>& m, std::string& why) {
+    if (m.empty()) {
+        why = "no members";
+        return nullptr;
+    }
+    const DevGeom& g = *m[0].g;
+    const Buffers& b0 = *m[0].b;
+    // the single-run path these kernels stand for: FP32 storage on the matrix cores, group
+    // transforms in both directions, the register group update
+    if (b0.mma < 0 || g.L > 32 || !wave_rows_log<float>(g) || !wave_y_log<float>(g) || g.NYc != kept_rows(g.N) ||
+        fast512_rows<float>(g) || fast512_y<float>(g)) {
+        why = "not on the batched path (FP32/BF16X3 storage, 64..256-point group transforms, L <= 32)";
+        return nullptr;
+    }
+    // launch shapes as the single-run launchers choose them from the tuned values
+    const int G = 512 / g.N;
+    int yi_wpb = b0.tune_ytx > 0 ? b0.tune_ytx / G : 4;
+    int yf_wpb = b0.tune_ytx > 0 && b0.tune_ytx / G >= 2 ? b0.tune_ytx / G : (g.N <= 64 ? 2 : 4);
+    const int rows_wpb = b0.tune_rows_threads > 0 ? b0.tune_rows_threads / 64 : 4;
+    if (yi_wpb != 2 && yi_wpb != 8) yi_wpb = 4;
+    if (yf_wpb != 2 && yf_wpb != 8) yf_wpb = 4;
+    if ((rows_wpb != 2 && rows_wpb != 4 && rows_wpb != 8) || g.N % (rows_wpb * 512 / g.M) != 0) {
+        why = "row launch shape outside the group kernels";
+        return nullptr;
+    }
+    for (const BatchMember& x : m) {
+        const DevGeom& h = *x.g;
+        const Buffers& b = *x.b;
+        if (h.M != g.M || h.N != g.N || h.L != g.L || h.nq != g.nq || h.KX != g.KX || h.KXp != g.KXp ||
+            h.NYc != g.NYc || h.P != g.P || b.mma != b0.mma || b.ev_nlt != b0.ev_nlt || b.ev_nks != b0.ev_nks ||
+            b.pr_nks != b0.pr_nks || b.tune_ytx != b0.tune_ytx || b.tune_rows_threads != b0.tune_rows_threads) {
+            why = "members differ in launch shape";
+            return nullptr;
+        }
+    }
+    StepBatch* sb = new StepBatch;
+    sb->B = (int)m.size();
+    sb->g = g;
+    sb->mma = b0.mma;
+    sb->yi_wpb = yi_wpb;
+    sb->yf_wpb = yf_wpb;
+    sb->rows_wpb = rows_wpb;
+    {
+        const int K = b0.mma == 0 ? 2 : 16;
+        const size_t sx = (size_t)3 * 128 * (b0.ev_nks * K + 1) * sizeof(float);
+        const size_t sa = (size_t)4 * b0.ev_nks * 64 * (K / 2) * sizeof(float);
+        sb->alds = sx + sa <= 65536;
+        sb->ev_smem = sb->alds ? sx + sa : sx;
+    }
+    const int nblk = ((g.P + 63) / 64) * b0.ev_nlt, per = (nblk + 7) / 8;
+    sb->ev_blocks = 8 * per;
+    sb->small_proj = (g.P + 63) / 64 < 512;
+    std::vector<k_eval_mma_args> ev;
+    std::vector<k_fft_y_inv_w_args> yi;
+    std::vector<k_rows_w_args> rw;
+    std::vector<k_fft_y_fwd_wc_args> yf;
+    std::vector<k_project_mma_args> pj;
+    std::vector<k_update32_args<float>> up;
+    std::vector<k_finish_args<float>> fi;
+    for (const BatchMember& x : m) {
+        const DevGeom& h = *x.g;
+        const Buffers& b = *x.b;
+        using C = cpx<float>;
+        ev.push_back({h, b.ev_nlt, b.ev_nks, per, (const C*)b.X[0], (const C*)b.X[1], (const C*)b.X[2],
+                      (const C*)b.XF, (const C*)b.XG, (const float*)b.evA, (const float*)b.evtab,
+                      (const float*)b.kx, (const float*)b.ky, (C*)b.lev});
+        yi.push_back({h, (const C*)b.lev, (const C*)b.twN, (C*)b.hin});
+        rw.push_back({h, (const C*)b.hin, (const C*)b.twM, (C*)b.hout});
+        yf.push_back({h, (const C*)b.hout, (const C*)b.twN, (const float*)b.kx, (const float*)b.ky, (C*)b.lev,
+                      (C*)b.fgw});
+        pj.push_back({h, b.pr_nks, (const float*)b.lev, (const C*)b.fgw, (const float*)b.prA, (const float*)b.prtab,
+                      (const float*)b.kx, (const float*)b.ky, (float)x.Pr, (C*)b.Nn[0], (C*)b.Nn[1], (C*)b.Nn[2],
+                      (C*)b.NF, (C*)b.NG});
+        // AB2 coefficients: a batch only runs members that have their history
+        const float a_cur = (float)(1.5 * x.dt), a_prev = (float)(-0.5 * x.dt);
+        up.push_back({h, b.nchunk, x.L0, x.L1, x.stride, (float)x.Pr, (float)x.dt, a_cur, a_prev, 1, b.cgrp, b.cbeg,
+                      b.gmem, (const float*)b.gmat, (const float*)b.k2, (C*)b.X[0], (C*)b.X[1], (C*)b.X[2],
+                      (const C*)b.Nn[0], (const C*)b.Nn[1], (const C*)b.Nn[2], (C*)b.Rp[0], (C*)b.Rp[1],
+                      (C*)b.Rp[2]});
+        fi.push_back({h, b.rows, (C*)b.X[0], (C*)b.X[1], (C*)b.X[2], (float)x.Pr, (float)x.dt, a_cur, a_prev, 1,
+                      (C*)b.XF, (C*)b.XG, (const C*)b.NF, (const C*)b.NG, (C*)b.RpF, (C*)b.RpG});
+        sb->nchunk_max = std::max(sb->nchunk_max, b.nchunk);
+    }
+    try {
+        sb->ev = upload_table(ev);
+        sb->yi = upload_table(yi);
+        sb->rw = upload_table(rw);
+        sb->yf = upload_table(yf);
+        sb->pj = upload_table(pj);
+        sb->up = upload_table(up);
+        sb->fi = upload_table(fi);
+    } catch (const HipError&) {
+        batch_destroy(sb);
+        throw;
+    }
+    return sb;
+}
+
+// One AB2 step of every member, in the stage order of a single step (evaluation, inverse
+// y-FFT, rows, forward y-FFT with the combinations, projection, group update, finish).
+void batch_step(const StepBatch* sb, hipStream_t s) {
+    const DevGeom& g = sb->g;
+    const int B = sb->B;
+    {
+        const dim3 gr(sb->ev_blocks, 1, B);
+#define RBC_EVB(MMV, ALDSV) \
+    hipLaunchKernelGGL((k_eval_mma_batch<MMV, ALDSV>), gr, dim3(256), sb->ev_smem, s, sb->ev, 1)
+        if (sb->mma == 0) {
+            if (sb->alds) RBC_EVB(0, true);
+            else RBC_EVB(0, false);
+        } else {
+            if (sb->alds) RBC_EVB(1, true);
+            else RBC_EVB(1, false);
+        }
+#undef RBC_EVB
+    }
+    const int G = 512 / g.N;
+    {
+        const int ywpb = sb->yi_wpb, cb = ywpb * G;
+        const dim3 gr((g.KX + cb - 1) / cb, g.nq, 6 * B);
+#define RBC_YIB(LG, WPBV) hipLaunchKernelGGL((k_fft_y_inv_w_batch<LG, WPBV>), gr, dim3(64 * WPBV), 0, s, sb->yi, 6)
+#define RBC_YIB_L(LG)                 \
+    if (ywpb == 2) RBC_YIB(LG, 2);      \
+    else if (ywpb == 8) RBC_YIB(LG, 8); \
+    else RBC_YIB(LG, 4)
+        if (g.N == 64) { RBC_YIB_L(6); }
+        else if (g.N == 128) { RBC_YIB_L(7); }
+        else { RBC_YIB_L(8); }
+#undef RBC_YIB_L
+#undef RBC_YIB
+    }
+    {
+        const int wpb = sb->rows_wpb;
+        const dim3 gr(g.N / (wpb * 512 / g.M), g.nq, B);
+#define RBC_RWB(LG, WPBV) hipLaunchKernelGGL((k_rows_w_batch<LG, WPBV>), gr, dim3(64 * WPBV), 0, s, sb->rw, 1)
+#define RBC_RWB_L(LG)                \
+    if (wpb == 2) RBC_RWB(LG, 2);      \
+    else if (wpb == 8) RBC_RWB(LG, 8); \
+    else RBC_RWB(LG, 4)
+        if (g.M == 64) { RBC_RWB_L(6); }
+        else if (g.M == 128) { RBC_RWB_L(7); }
+        else { RBC_RWB_L(8); }
+#undef RBC_RWB_L
+#undef RBC_RWB
+    }
+    {
+        const int wpb = sb->yf_wpb;
+        const dim3 gr((g.KX + wpb * G - 1) / (wpb * G), g.nq, 3 * B);
+#define RBC_YFB(LG, WPBV) hipLaunchKernelGGL((k_fft_y_fwd_wc_batch<LG, WPBV>), gr, dim3(64 * WPBV), 0, s, sb->yf, 3)
+#define RBC_YFB_L(LG)                \
+    if (wpb == 2) RBC_YFB(LG, 2);      \
+    else if (wpb == 8) RBC_YFB(LG, 8); \
+    else RBC_YFB(LG, 4)
+        if (g.N == 64) { RBC_YFB_L(6); }
+        else if (g.N == 128) { RBC_YFB_L(7); }
+        else { RBC_YFB_L(8); }
+#undef RBC_YFB_L
+#undef RBC_YFB
+    }
+    {
+        const dim3 gr(sb->small_proj ? (g.P + 15) / 16 : (g.P + 63) / 64, 1, B);
+#define RBC_PJB(MMV, WPBV) \
+    hipLaunchKernelGGL((k_project_mma_batch<MMV, WPBV, 4 / WPBV>), gr, dim3(256), 0, s, sb->pj, 1)
+        if (sb->mma == 0) {
+            if (sb->small_proj) RBC_PJB(0, 1);
+            else RBC_PJB(0, 4);
+        } else {
+            if (sb->small_proj) RBC_PJB(1, 1);
+            else RBC_PJB(1, 4);
+        }
+#undef RBC_PJB
+    }
+    hipLaunchKernelGGL(k_update32_batch<float>, dim3((sb->nchunk_max + 3) / 4, 1, B), dim3(256), 0, s, sb->up, 1);
+    {
+        const int n = 3 * g.NYc * g.L;
+        hipLaunchKernelGGL(k_finish_batch<float>, dim3((n + 255) / 256 + 1, 1, B), dim3(256), 0, s, sb->fi, 1);
+    }
+}
 
 }  // namespace rbc

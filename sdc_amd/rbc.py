@@ -48,7 +48,8 @@ EXPORTS = ["rbc_create", "rbc_destroy", "rbc_last_error", "rbc_get_info", "rbc_s
            "rbc_tracer_get_positions", "rbc_tracer_current", "rbc_tracer_info", "rbc_tracer_moments",
            "rbc_energy_spectrum", "rbc_energy_spectrum_ref", "rbc_ensemble_create", "rbc_ensemble_destroy",
            "rbc_ensemble_last_error", "rbc_ensemble_size", "rbc_ensemble_member", "rbc_ensemble_tune",
-           "rbc_ensemble_step", "rbc_closure_create", "rbc_closure_destroy", "rbc_closure_last_error",
+           "rbc_ensemble_step", "rbc_ensemble_batching", "rbc_ensemble_batched_members",
+           "rbc_closure_create", "rbc_closure_destroy", "rbc_closure_last_error",
            "rbc_closure_info", "rbc_closure_restrict_state", "rbc_closure_sample", "rbc_closure_points",
            "rbc_closure_dataset", "rbc_closure_load_dataset", "rbc_closure_clear_dataset",
            "rbc_closure_fit_normalization", "rbc_closure_get_normalization", "rbc_closure_set_normalization",
@@ -124,6 +125,8 @@ def load_library(path=LIB_PATH):
     L.rbc_ensemble_member.restype = vp
     L.rbc_ensemble_tune.argtypes = [vp, dp]
     L.rbc_ensemble_step.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.rbc_ensemble_batching.argtypes = [vp, ctypes.c_int]
+    L.rbc_ensemble_batched_members.argtypes = [vp]
     i_ = ctypes.c_int
     L.rbc_closure_create.argtypes = [vp, i_, i_, ctypes.POINTER(vp)]
     L.rbc_closure_destroy.argtypes = [vp]
@@ -428,6 +431,16 @@ class Ensemble:
 
     def step(self, nsteps, steps_per_launch=10):
         self._check(self.lib.rbc_ensemble_step(self.h, int(nsteps), int(steps_per_launch)), "rbc_ensemble_step")
+
+    def set_batching(self, on):
+        """Size classes stepped as batches inside the graph (True, the default) or every member as
+        a graph branch of its own (False)."""
+        self._check(self.lib.rbc_ensemble_batching(self.h, 1 if on else 0), "rbc_ensemble_batching")
+
+    @property
+    def batched_members(self):
+        """Members the current graph steps in batches (0 before the first graph launch)."""
+        return int(self.lib.rbc_ensemble_batched_members(self.h))
 
     def close(self):
         if getattr(self, "h", None):
