@@ -383,7 +383,7 @@ __global__ void __launch_bounds__(256) k_fft_y_fwd(DevGeom g, int TX, const cpx<
 // row-major hybrid layout (rows of CB contiguous values).
 template <int LOGN, int WPB>
 __device__ __forceinline__ void k_fft_y_inv_w_body(int bz, DevGeom g, const cpx<float>* __restrict__ lev,
-                                                     const cpx<float>* __restrict__ twN, cpx<float>* __restrict__ hin) {
+                                                   const cpx<float>* __restrict__ twN, cpx<float>* __restrict__ hin) {
     constexpr int N = 1 << LOGN, W = N / 8, G = 64 / W, CB = WPB * G, S = CB + 1;
     __shared__ cpx<float> tw[N];
     __shared__ cpx<float> tile[(N + N / 8) * S];
@@ -422,6 +422,8 @@ __global__ void __launch_bounds__(64 * WPB) k_fft_y_inv_w(DevGeom g, const cpx<f
                                                      const cpx<float>* __restrict__ twN, cpx<float>* __restrict__ hin) {
     k_fft_y_inv_w_body<LOGN, WPB>(blockIdx.z, g, lev, twN, hin);
 }
+// one member's arguments of k_fft_y_inv_w_batch, which runs the k_fft_y_inv_w body for member
+// blockIdx.z / zdim of a batch (the rest of blockIdx.z is the body's own z index)
 struct k_fft_y_inv_w_args {
     DevGeom g;
     const cpx<float>* lev;

@@ -549,14 +549,14 @@ __global__ void __launch_bounds__(64) k_update(DevGeom g, int L0, int L1, int st
 // order as k_update; mode UPD_STEP only.
 template <typename T>
 __device__ __forceinline__ void k_update32_body(DevGeom g, int nchunk, int L0, int L1, int stride, T Pr, T dt,
-                                                  T a_cur, T a_prev, int use_prev, const int* __restrict__ cgrp,
-                                                  const int* __restrict__ cbeg,
-                                                  const int* __restrict__ gmem, const T* __restrict__ gmat,
-                                                  const T* __restrict__ k2a, cpx<T>* __restrict__ Xt,
-                                                  cpx<T>* __restrict__ Xf, cpx<T>* __restrict__ Xg,
-                                                  const cpx<T>* __restrict__ Nt, const cpx<T>* __restrict__ Nf,
-                                                  const cpx<T>* __restrict__ Ng, cpx<T>* __restrict__ Rt,
-                                                  cpx<T>* __restrict__ Rf, cpx<T>* __restrict__ Rg) {
+                                                T a_cur, T a_prev, int use_prev, const int* __restrict__ cgrp,
+                                                const int* __restrict__ cbeg,
+                                                const int* __restrict__ gmem, const T* __restrict__ gmat,
+                                                const T* __restrict__ k2a, cpx<T>* __restrict__ Xt,
+                                                cpx<T>* __restrict__ Xf, cpx<T>* __restrict__ Xg,
+                                                const cpx<T>* __restrict__ Nt, const cpx<T>* __restrict__ Nf,
+                                                const cpx<T>* __restrict__ Ng, cpx<T>* __restrict__ Rt,
+                                                cpx<T>* __restrict__ Rf, cpx<T>* __restrict__ Rg) {
     __shared__ cpx<T> shm[4][2][2][32];  // [wave][class][0: N_f, 1: rhs][j]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int ch = blockIdx.x * 4 + wave;  // one chunk (<= kChunk members of one group) per wave
@@ -665,8 +665,11 @@ __global__ void __launch_bounds__(256) k_update32(DevGeom g, int nchunk, int L0,
                                                   const cpx<T>* __restrict__ Nt, const cpx<T>* __restrict__ Nf,
                                                   const cpx<T>* __restrict__ Ng, cpx<T>* __restrict__ Rt,
                                                   cpx<T>* __restrict__ Rf, cpx<T>* __restrict__ Rg) {
-    k_update32_body<T>(g, nchunk, L0, L1, stride, Pr, dt, a_cur, a_prev, use_prev, cgrp, cbeg, gmem, gmat, k2a, Xt, Xf, Xg, Nt, Nf, Ng, Rt, Rf, Rg);
+    k_update32_body<T>(g, nchunk, L0, L1, stride, Pr, dt, a_cur, a_prev, use_prev, cgrp, cbeg, gmem, gmat, k2a, Xt,
+                       Xf, Xg, Nt, Nf, Ng, Rt, Rf, Rg);
 }
+// one member's arguments of k_update32_batch, which runs the k_update32 body for member
+// blockIdx.z / zdim of a batch (the rest of blockIdx.z is the body's own z index)
 template <typename T>
 struct k_update32_args {
     DevGeom g;
@@ -698,7 +701,8 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_update32_batch(const k_update32_args<T>* __restrict__ tab, int zdim) {
     const int mb = blockIdx.z / zdim;
     const k_update32_args<T>& a = tab[mb];
-    k_update32_body<T>(a.g, a.nchunk, a.L0, a.L1, a.stride, a.Pr, a.dt, a.a_cur, a.a_prev, a.use_prev, a.cgrp, a.cbeg, a.gmem, a.gmat, a.k2a, a.Xt, a.Xf, a.Xg, a.Nt, a.Nf, a.Ng, a.Rt, a.Rf, a.Rg);
+    k_update32_body<T>(a.g, a.nchunk, a.L0, a.L1, a.stride, a.Pr, a.dt, a.a_cur, a.a_prev, a.use_prev, a.cgrp, a.cbeg,
+                       a.gmem, a.gmat, a.k2a, a.Xt, a.Xf, a.Xg, a.Nt, a.Nf, a.Ng, a.Rt, a.Rf, a.Rg);
 }
 
 
@@ -759,9 +763,9 @@ __global__ void k_make_real(DevGeom g, const int* __restrict__ rows, cpx<T>* X0,
 // update (mode UPD_STEP) on the last block
 template <typename T>
 __device__ __forceinline__ void k_finish_body(DevGeom g, const int* __restrict__ rows, cpx<T>* X0, cpx<T>* X1,
-                                                cpx<T>* X2, T Pr, T dt, T a_cur, T a_prev, int use_prev, cpx<T>* XF,
-                                                cpx<T>* XG, const cpx<T>* NF, const cpx<T>* NG, cpx<T>* RF,
-                                                cpx<T>* RG) {
+                                              cpx<T>* X2, T Pr, T dt, T a_cur, T a_prev, int use_prev, cpx<T>* XF,
+                                              cpx<T>* XG, const cpx<T>* NF, const cpx<T>* NG, cpx<T>* RF,
+                                              cpx<T>* RG) {
     if (blockIdx.x == gridDim.x - 1) {
         update_FG_mode<T>(threadIdx.x, g.L, Pr, dt, a_cur, a_prev, use_prev, UPD_STEP, XF, XG, NF, NG, RF, RG);
         return;
@@ -789,6 +793,8 @@ __global__ void __launch_bounds__(256) k_finish(DevGeom g, const int* __restrict
                                                 cpx<T>* RG) {
     k_finish_body<T>(g, rows, X0, X1, X2, Pr, dt, a_cur, a_prev, use_prev, XF, XG, NF, NG, RF, RG);
 }
+// one member's arguments of k_finish_batch, which runs the k_finish body for member
+// blockIdx.z / zdim of a batch (the rest of blockIdx.z is the body's own z index)
 template <typename T>
 struct k_finish_args {
     DevGeom g;
@@ -812,7 +818,8 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_finish_batch(const k_finish_args<T>* __restrict__ tab, int zdim) {
     const int mb = blockIdx.z / zdim;
     const k_finish_args<T>& a = tab[mb];
-    k_finish_body<T>(a.g, a.rows, a.X0, a.X1, a.X2, a.Pr, a.dt, a.a_cur, a.a_prev, a.use_prev, a.XF, a.XG, a.NF, a.NG, a.RF, a.RG);
+    k_finish_body<T>(a.g, a.rows, a.X0, a.X1, a.X2, a.Pr, a.dt, a.a_cur, a.a_prev, a.use_prev, a.XF, a.XG, a.NF, a.NG,
+                     a.RF, a.RG);
 }
 
 
@@ -1014,7 +1021,7 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, const cpx<float>* _
 // balanced by a power of two (pair_scale over the row, a shuffle reduction over the group).
 template <int LOGM, int WPB>
 __device__ __forceinline__ void k_rows_w_body(DevGeom g, const cpx<float>* __restrict__ hin,
-                                                const cpx<float>* __restrict__ twM, cpx<float>* __restrict__ hout) {
+                                              const cpx<float>* __restrict__ twM, cpx<float>* __restrict__ hout) {
     constexpr int M = 1 << LOGM, W = M / 8, G = 64 / W, RPB = WPB * G, SP = M + M / 8, KX = kept_cols(M);
     __shared__ cpx<float> tw[M];
     __shared__ cpx<float> scr[RPB][SP];
@@ -1114,6 +1121,8 @@ __global__ void __launch_bounds__(64 * WPB) k_rows_w(DevGeom g, const cpx<float>
                                                 const cpx<float>* __restrict__ twM, cpx<float>* __restrict__ hout) {
     k_rows_w_body<LOGM, WPB>(g, hin, twM, hout);
 }
+// one member's arguments of k_rows_w_batch, which runs the k_rows_w body for member
+// blockIdx.z / zdim of a batch (the rest of blockIdx.z is the body's own z index)
 struct k_rows_w_args {
     DevGeom g;
     const cpx<float>* hin;
@@ -1244,12 +1253,12 @@ struct Mma<1> {
 // operands are copied to LDS with the modes (when both fit in 64 KiB).
 template <int MM, bool ALDS>
 __device__ __forceinline__ void k_eval_mma_body(DevGeom g, int nlt, int nks, int per,
-                                                  const cpx<float>* __restrict__ Xt,
-                                                  const cpx<float>* __restrict__ Xf, const cpx<float>* __restrict__ Xg,
-                                                  const cpx<float>* __restrict__ XF, const cpx<float>* __restrict__ XG,
-                                                  const float* __restrict__ evA, const float* __restrict__ evtab,
-                                                  const float* __restrict__ kxa, const float* __restrict__ kya,
-                                                  cpx<float>* __restrict__ lev) {
+                                                const cpx<float>* __restrict__ Xt,
+                                                const cpx<float>* __restrict__ Xf, const cpx<float>* __restrict__ Xg,
+                                                const cpx<float>* __restrict__ XF, const cpx<float>* __restrict__ XG,
+                                                const float* __restrict__ evA, const float* __restrict__ evtab,
+                                                const float* __restrict__ kxa, const float* __restrict__ kya,
+                                                cpx<float>* __restrict__ lev) {
     using MT = Mma<MM>;
     extern __shared__ float sX[];  // 3 * 128 * (Lp + 1) modes, then sA: the block's A operands
     const int L = g.L, P = g.P, nq = g.nq;
@@ -1397,6 +1406,8 @@ __global__ void __launch_bounds__(256) k_eval_mma(DevGeom g, int nlt, int nks, i
                                                   cpx<float>* __restrict__ lev) {
     k_eval_mma_body<MM, ALDS>(g, nlt, nks, per, Xt, Xf, Xg, XF, XG, evA, evtab, kxa, kya, lev);
 }
+// one member's arguments of k_eval_mma_batch, which runs the k_eval_mma body for member
+// blockIdx.z / zdim of a batch (the rest of blockIdx.z is the body's own z index)
 struct k_eval_mma_args {
     DevGeom g;
     int nlt;
@@ -1417,7 +1428,8 @@ template <int MM, bool ALDS>
 __global__ void __launch_bounds__(256) k_eval_mma_batch(const k_eval_mma_args* __restrict__ tab, int zdim) {
     const int mb = blockIdx.z / zdim;
     const k_eval_mma_args& a = tab[mb];
-    k_eval_mma_body<MM, ALDS>(a.g, a.nlt, a.nks, a.per, a.Xt, a.Xf, a.Xg, a.XF, a.XG, a.evA, a.evtab, a.kxa, a.kya, a.lev);
+    k_eval_mma_body<MM, ALDS>(a.g, a.nlt, a.nks, a.per, a.Xt, a.Xf, a.Xg, a.XF, a.XG, a.evA, a.evtab, a.kxa, a.kya,
+                              a.lev);
 }
 
 
@@ -1430,12 +1442,12 @@ __global__ void __launch_bounds__(256) k_eval_mma_batch(const k_eval_mma_args* _
 // flight).
 template <int MM, int WPB, int KS>
 __device__ __forceinline__ void k_project_mma_body(DevGeom g, int nks, const float* __restrict__ Dfl,
-                                                     const cpx<float>* __restrict__ fgw, const float* __restrict__ prA,
-                                                     const float* __restrict__ prtab, const float* __restrict__ kxa,
-                                                     const float* __restrict__ kya, float Pr,
-                                                     cpx<float>* __restrict__ Nt, cpx<float>* __restrict__ Nf,
-                                                     cpx<float>* __restrict__ Ng, cpx<float>* __restrict__ NF,
-                                                     cpx<float>* __restrict__ NG) {
+                                                   const cpx<float>* __restrict__ fgw, const float* __restrict__ prA,
+                                                   const float* __restrict__ prtab, const float* __restrict__ kxa,
+                                                   const float* __restrict__ kya, float Pr,
+                                                   cpx<float>* __restrict__ Nt, cpx<float>* __restrict__ Nf,
+                                                   cpx<float>* __restrict__ Ng, cpx<float>* __restrict__ NF,
+                                                   cpx<float>* __restrict__ NG) {
     using MT = Mma<MM>;
     constexpr int OS = 66;               // staging row (2L <= 64 floats plus 2 to spread banks)
     __shared__ float sO[WPB][3][16 * OS];  // per wave: [field][mode][OS] staging for the write-out
@@ -1548,6 +1560,8 @@ __global__ void __launch_bounds__(64 * WPB * KS) k_project_mma(DevGeom g, int nk
                                                      cpx<float>* __restrict__ NG) {
     k_project_mma_body<MM, WPB, KS>(g, nks, Dfl, fgw, prA, prtab, kxa, kya, Pr, Nt, Nf, Ng, NF, NG);
 }
+// one member's arguments of k_project_mma_batch, which runs the k_project_mma body for member
+// blockIdx.z / zdim of a batch (the rest of blockIdx.z is the body's own z index)
 struct k_project_mma_args {
     DevGeom g;
     int nks;
@@ -1568,7 +1582,8 @@ template <int MM, int WPB, int KS>
 __global__ void __launch_bounds__(64 * WPB * KS) k_project_mma_batch(const k_project_mma_args* __restrict__ tab, int zdim) {
     const int mb = blockIdx.z / zdim;
     const k_project_mma_args& a = tab[mb];
-    k_project_mma_body<MM, WPB, KS>(a.g, a.nks, a.Dfl, a.fgw, a.prA, a.prtab, a.kxa, a.kya, a.Pr, a.Nt, a.Nf, a.Ng, a.NF, a.NG);
+    k_project_mma_body<MM, WPB, KS>(a.g, a.nks, a.Dfl, a.fgw, a.prA, a.prtab, a.kxa, a.kya, a.Pr, a.Nt, a.Nf, a.Ng,
+                                    a.NF, a.NG);
 }
 
 
@@ -1753,9 +1768,9 @@ __device__ __forceinline__ void y_fwd_group_comb(const DevGeom& g, cpx<float>* t
 
 template <int LOGN, int WPB>
 __device__ __forceinline__ void k_fft_y_fwd_wc_body(int bz, DevGeom g, const cpx<float>* __restrict__ hout,
-                                                          const cpx<float>* __restrict__ twN,
-                                                          const float* __restrict__ kxa, const float* __restrict__ kya,
-                                                          cpx<float>* __restrict__ D, cpx<float>* __restrict__ fgw) {
+                                                    const cpx<float>* __restrict__ twN,
+                                                    const float* __restrict__ kxa, const float* __restrict__ kya,
+                                                    cpx<float>* __restrict__ D, cpx<float>* __restrict__ fgw) {
     constexpr int N = 1 << LOGN, W = N / 8, G = 64 / W, CB = WPB * G, S = CB + 1;
     __shared__ cpx<float> tw[N];
     __shared__ cpx<float> tile[(N + N / 8) * S];
@@ -1773,6 +1788,8 @@ __global__ void __launch_bounds__(64 * WPB) k_fft_y_fwd_wc(DevGeom g, const cpx<
                                                           cpx<float>* __restrict__ D, cpx<float>* __restrict__ fgw) {
     k_fft_y_fwd_wc_body<LOGN, WPB>(blockIdx.z, g, hout, twN, kxa, kya, D, fgw);
 }
+// one member's arguments of k_fft_y_fwd_wc_batch, which runs the k_fft_y_fwd_wc body for member
+// blockIdx.z / zdim of a batch (the rest of blockIdx.z is the body's own z index)
 struct k_fft_y_fwd_wc_args {
     DevGeom g;
     const cpx<float>* hout;
