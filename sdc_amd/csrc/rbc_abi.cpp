@@ -238,7 +238,9 @@ int rbc_create(const rbc_params* p, rbc_ctx** out) {
         d.L = p->L;
         d.nq = nq;
         d.KX = g.KX;
-        d.KXp = (g.KX + 3) & ~3;
+        // hybrid row stride; the 512 x 512 kernels store the hybrid rows tile-major in tiles of 8
+        // kx columns ([tile][y][8]), so the stride there is a whole number of tiles
+        d.KXp = (p->M == 512 && p->N == 512) ? (g.KX + 7) & ~7 : (g.KX + 3) & ~3;
         d.NYc = NYc;
         d.P = g.P();
         d.logM = 0;

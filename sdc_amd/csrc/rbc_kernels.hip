@@ -910,6 +910,13 @@ __device__ __forceinline__ void fft512_wave(cpx<T> v[8], cpx<T>* sb, const cpx<T
 constexpr int KX512 = kept_cols(512), NYC512 = kept_rows(512);
 static_assert(KX512 == 172 && NYC512 == 341, "dealias rule at 512 points");
 
+// Hybrid rows of the 512 x 512 kernels, tile-major: plane (field, level) = [KXT512 / 8 tiles][512
+// y][8 kx], so that the y-FFT blocks (8 kx columns each) read and write one contiguous 32 KiB
+// tile, and the x-rows read 64-byte pieces of a row (the other half of each 128-byte line is
+// the neighbouring row, run on the same XCD).  Element (y, kx) of a plane:
+constexpr int KXT512 = (KX512 + 7) & ~7;  // 176 columns, 22 tiles
+__device__ __forceinline__ int hyb512(int y, int kx) { return (kx >> 3) * (512 * 8) + y * 8 + (kx & 7); }
+
 // x-rows at M = N = 512 for the fused forward y-FFT: three waves per (level, y) row.  Phase 1:
 // wave pr inverse-transforms the pair (2 pr, 2 pr + 1) of level fields.  Phase 2: the six
 // real product fields h = uu + vv - 2 ww, d = uu - vv, uv, uw, vw, u.grad(theta) go back in
@@ -917,16 +924,20 @@ static_assert(KX512 == 172 && NYC512 == 341, "dealias rule at 512 points");
 // kx^2 Puu + ky^2 Pvv - k^2 Pww = (k^2 / 2) h + ((kx^2 - ky^2) / 2) d and Puu - Pvv = d, with
 // d formed before any rounding of the larger ww (which would swamp uu - vv).  The second
 // field of every pair is scaled to the size of the first for the transform (pair_scale).
-__global__ void __launch_bounds__(192) k_rows512t(DevGeom g, const cpx<float>* __restrict__ hin,
+__global__ void __launch_bounds__(192) k_rows512t(DevGeom g, int nblk, int per, const cpx<float>* __restrict__ hin,
                                                   const cpx<float>* __restrict__ twM, cpx<float>* __restrict__ hout) {
     __shared__ cpx<float> tw[512];
     __shared__ cpx<float> zs[3][576];
+    // one (level, y) row per block; consecutive rows (which share the 128-byte lines of the
+    // tile-major layout) are consecutive logical blocks and so run on one XCD
+    const int lb = xcd_logical_block(blockIdx.x, per);
+    if (lb >= nblk) return;
     const int lane = threadIdx.x & 63, pr = threadIdx.x >> 6;
     constexpr int KX = KX512;
-    const int KXp = g.KXp, nq = g.nq, N = g.N;
-    const int y = blockIdx.x, l = blockIdx.y;
-    const size_t fstride = (size_t)nq * N * KXp;
-    const size_t roff = ((size_t)l * N + y) * KXp;
+    const int nq = g.nq;
+    const int y = lb % 512, l = lb / 512;
+    const size_t fstride = (size_t)nq * 512 * KXT512;
+    const size_t roff = (size_t)l * 512 * KXT512 + (size_t)y * 8;
     for (int e = threadIdx.x; e < 512; e += 192) tw[e] = twM[e];
     __syncthreads();
     {
@@ -939,8 +950,8 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, const cpx<float>* _
             const int n = lane + 64 * r;
             av[r] = bv[r] = mk(0.f, 0.f);
             if (n < KX) {
-                av[r] = x1[n];
-                bv[r] = x2[n];
+                av[r] = x1[hyb512(0, n)];
+                bv[r] = x2[hyb512(0, n)];
                 if (n == 0) {
                     av[r].y = 0.f;
                     bv[r].y = 0.f;
@@ -983,7 +994,7 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, const cpx<float>* _
         for (int r = 0; r < 8; ++r) zs[pr][pad8(lane + 64 * r)] = mk(v[r].x, isg * v[r].y);
     }
     __syncthreads();
-    const float s = 1.f / (512.f * (float)N);
+    const float s = 1.f / (512.f * 512.f);
     cpx<float> p[8];
     float ma = 0.f, mb = 0.f;
 #pragma unroll
@@ -1009,8 +1020,8 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, const cpx<float>* _
     cpx<float>* o2 = hout + (size_t)(2 * pr + 1) * fstride + roff;
     for (int k = lane; k < KX; k += 64) {
         const cpx<float> zk = zs[pr][pad8(k)], zm = zs[pr][pad8((512 - k) & 511)];
-        o1[k] = mk(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-        o2[k] = mk(0.5f * isg * (zk.y + zm.y), -0.5f * isg * (zk.x - zm.x));
+        o1[hyb512(0, k)] = mk(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+        o2[hyb512(0, k)] = mk(0.5f * isg * (zk.y + zm.y), -0.5f * isg * (zk.x - zm.x));
     }
 }
 
@@ -1186,16 +1197,15 @@ __global__ void __launch_bounds__(512) k_fft_y_inv512(DevGeom g, int ntile, int 
 #pragma unroll
         for (int r = 0; r < 8; ++r) tile[pad8(lane + 64 * r) * YS + wave] = v[r];
         lds_barrier();
-        const int tix = lb % ntile, rest = lb / ntile, kx0 = tix * YT;
-        // every lane stores on every pass (columns beyond the last kept one repeat it), so that
-        // the number of stores after the prefetch is fixed and the prefetch wait can leave them
-        // in flight
-        const int cc = min((int)threadIdx.x % YT, KX - kx0 - 1);
-        cpx<float>* dst = hin + (size_t)rest * 512 * g.KXp + kx0 + cc;
+        const int tix = lb % ntile, rest = lb / ntile;
+        // the tile is one contiguous 32 KiB piece of the tile-major plane; every lane stores on
+        // every pass (the columns past KX are the transforms of zero columns), so that the number
+        // of stores after the prefetch is fixed and the prefetch wait leaves them in flight
+        cpx<float>* dst = hin + (size_t)rest * 512 * KXT512 + (size_t)tix * 512 * YT;
 #pragma unroll
         for (int i = 0; i < YT; ++i) {
-            const int y = threadIdx.x / YT + (512 / YT) * i;
-            dst[(size_t)y * g.KXp] = tile[pad8(y) * YS + cc];
+            const int e = threadIdx.x + 512 * i, y = e / YT;
+            dst[e] = tile[pad8(y) * YS + e % YT];
         }
         lds_barrier();  // the tile columns are the next tile's FFT scratch
     }
@@ -1664,12 +1674,14 @@ __global__ void __launch_bounds__(64 * CT) k_fft_y_fwd512c(DevGeom g, int ntile,
     for (int r = 0; r < 8; ++r) d0[r] = d1[r] = mk(0.f, 0.f);
     // tile element e = threadIdx.x + 64 CT i (i < 8): y = e / CT, column e % CT
     cpx<float> pre[8];
+    // the block's 8 columns are one contiguous tile of the tile-major plane
+    static_assert(CT == 8, "one 8-column tile of the hybrid layout per block");
     auto fetch = [&](int q) {
-        const cpx<float>* src = hout + ((size_t)q * nq + l) * (size_t)512 * g.KXp + kx0;
+        const cpx<float>* src = hout + ((size_t)q * nq + l) * (size_t)512 * KXT512 + (size_t)tix * 512 * CT;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int e = threadIdx.x + 64 * CT * i, y = e / CT, cc = e - y * CT;
-            pre[i] = (cc < ncol) ? src[(size_t)y * g.KXp + cc] : mk(0.f, 0.f);
+            const int e = threadIdx.x + 64 * CT * i, cc = e % CT;
+            pre[i] = (cc < ncol) ? src[e] : mk(0.f, 0.f);
         }
     };
     fetch(G::q(0));
@@ -1829,11 +1841,13 @@ static bool generic_fft_forced() {
 }
 template <typename T>
 static bool fast512_rows(const DevGeom& g) {
-    return sizeof(T) == 4 && g.M == 512 && g.KX == KX512 && !generic_fft_forced();
+    return sizeof(T) == 4 && g.M == 512 && g.N == 512 && g.KX == KX512 && g.NYc == NYC512 && g.KXp == KXT512 &&
+           !generic_fft_forced();
 }
+// the 512-point y and x kernels share the tile-major hybrid layout: both run, or neither
 template <typename T>
 static bool fast512_y(const DevGeom& g) {
-    return sizeof(T) == 4 && g.N == 512 && g.NYc == NYC512 && !generic_fft_forced();
+    return fast512_rows<T>(g);
 }
 // group (natural-layout) transforms for FP32 grids of 64, 128 and 256 points: log2 of the size,
 // or 0 when the generic kernels run
@@ -1913,10 +1927,10 @@ void Launch<T>::fft_y_inv(const DevGeom& g, const Buffers& b, hipStream_t s) {
 
 template <typename T>
 void Launch<T>::rows(const DevGeom& g, const Buffers& b, hipStream_t s) {
-    dim3 grid(g.N, g.nq);
     if (fast512_rows<T>(g)) {
-        hipLaunchKernelGGL(k_rows512t, grid, dim3(192), 0, s, g, (const cpx<float>*)b.hin, (const cpx<float>*)b.twM,
-                           (cpx<float>*)b.hout);
+        const int nblk = g.N * g.nq, per = (nblk + 7) / 8;
+        hipLaunchKernelGGL(k_rows512t, dim3(8 * per), dim3(192), 0, s, g, nblk, per, (const cpx<float>*)b.hin,
+                           (const cpx<float>*)b.twM, (cpx<float>*)b.hout);
         return;
     }
     if (const int lg = wave_rows_log<T>(g)) {
