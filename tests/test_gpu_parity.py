@@ -206,6 +206,22 @@ def test_512_fast_path_vs_oracle_L4():
         assert max(e) < 2e-5, e
 
 
+@pytest.mark.parametrize("M,N", [(512, 256), (256, 512)])
+def test_512_one_side_fp32_vs_oracle(M, N):
+    """Grids with 512 points on one side only: the 512-point kernels need both sides (their
+    hybrid rows are tile-major), so these run the group / generic kernels on the row-major
+    layout with the 512-point row stride."""
+    print(f"case: {M}x{N}x4 fp32 nonlinear vs oracle", flush=True)
+    cfg = dict(M=M, N=N, L=4, Lx=4.0, Ly=4.0 * N / M, Ra=1e6, Pr=0.7, dt=5e-5)
+    st = random_state(M, N, 4, seed=23, amp=1e-2, decay=0.02)
+    keep = kept_mask(M, N)
+    ref = _orc(cfg, nthreads=16).nonlinear(*st)
+    with _gpu(cfg, "f32") as G:
+        e = _cmp(G.calculate_nonlinear(*st), ref, keep)
+        print(f"{M}x{N}x4 fp32 rel errors", e)
+        assert max(e) < 2e-5, e
+
+
 def test_c3_fp32_vs_fp64_gpu_steps():
     """Full 512^2 x 32 workload: FP32 fast path vs the FP64 GPU path (itself pinned to the
     oracle above) after 20 steps."""
