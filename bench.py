@@ -67,6 +67,20 @@ def measured_traffic(stage):
         return None
 
 
+ISSUE_PROFILE = os.path.join(ROOT, "profiles", "sq_issue_512x512x32_f32.md")
+
+
+def measured_valu_issue(stage):
+    """VALU instructions issued per CU per clock (at most 1: packed FP32 is off) of a stage's
+    kernels, from the committed rocprofv3 SQ-counter summary; None when they are not in it."""
+    try:
+        txt = open(ISSUE_PROFILE).read()
+        table = json.loads(txt.split("<!-- valu-issue ", 1)[1].split(" -->", 1)[0])
+        return {k: table[k] for k in STAGE_KERNELS[stage]}
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
 def copy_probe(mib):
     """One device-to-device copy of `mib` MiB by the HIP runtime (its blit kernel reads and writes
     exactly that many bytes), used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
@@ -457,6 +471,7 @@ def main():
     traffic = measured_traffic(dom) if a.precision != "f64" and not a.grid else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
+                "valu_issue_per_cu_cycle": measured_valu_issue(dom) if a.precision != "f64" and not a.grid else None,
                 "algorithmic_bytes": nbytes[dom],
                 "stage_ms": {k: round(v, 4) for k, v in stages.items()},
                 "stage_GBs": {k: round(nbytes[k] / (stages[k] * 1e-3) / 1e9, 1) for k in stages if stages[k] > 0}}
