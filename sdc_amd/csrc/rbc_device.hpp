@@ -130,29 +130,33 @@ __device__ __forceinline__ void dft8(cpx<T> v[8]) {
 // group: DPP lane permutations inside 16-lane rows (quad_perm [1,0,3,2], quad_perm [2,3,0,1],
 // row_half_mirror, row_mirror; after the earlier steps each is a max with the partner half),
 // then a shuffle across rows (W = 32) or lane reads of the four rows (W = 64).  All lanes active.
-__device__ __forceinline__ float dpp_max_step(float v, int x, int ctrl_id) {
+// The values are non-negative (row maxima of |x|), whose IEEE bit patterns order as the values:
+// the lanes compare them as unsigned integers, which needs no NaN canonicalisation and lets the
+// compiler fold each DPP move into the max.
+__device__ __forceinline__ unsigned dpp_max_step(unsigned x, int ctrl_id) {
     int y;
     switch (ctrl_id) {
-        case 0: y = __builtin_amdgcn_update_dpp(x, x, 0xB1, 0xF, 0xF, false); break;
-        case 1: y = __builtin_amdgcn_update_dpp(x, x, 0x4E, 0xF, 0xF, false); break;
-        case 2: y = __builtin_amdgcn_update_dpp(x, x, 0x141, 0xF, 0xF, false); break;
-        default: y = __builtin_amdgcn_update_dpp(x, x, 0x140, 0xF, 0xF, false); break;
+        case 0: y = __builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, true); break;
+        case 1: y = __builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, true); break;
+        case 2: y = __builtin_amdgcn_mov_dpp((int)x, 0x141, 0xF, 0xF, true); break;
+        default: y = __builtin_amdgcn_mov_dpp((int)x, 0x140, 0xF, 0xF, true); break;
     }
-    return fmaxf(v, __int_as_float(y));
+    return max(x, (unsigned)y);
 }
 template <int W>
 __device__ __forceinline__ float group_max(float v) {
-    v = dpp_max_step(v, __float_as_int(v), 0);
-    v = dpp_max_step(v, __float_as_int(v), 1);
-    v = dpp_max_step(v, __float_as_int(v), 2);
-    if (W >= 16) v = dpp_max_step(v, __float_as_int(v), 3);
-    if (W == 32) v = fmaxf(v, __shfl_xor(v, 16));
+    unsigned x = __float_as_uint(v);
+    x = dpp_max_step(x, 0);
+    x = dpp_max_step(x, 1);
+    x = dpp_max_step(x, 2);
+    if (W >= 16) x = dpp_max_step(x, 3);
+    if (W == 32) x = max(x, (unsigned)__shfl_xor((int)x, 16));
     if (W == 64) {
-        const int x = __float_as_int(v);
-        v = fmaxf(fmaxf(__int_as_float(__builtin_amdgcn_readlane(x, 0)), __int_as_float(__builtin_amdgcn_readlane(x, 16))),
-                  fmaxf(__int_as_float(__builtin_amdgcn_readlane(x, 32)), __int_as_float(__builtin_amdgcn_readlane(x, 48))));
+        const int xi = (int)x;
+        x = max(max((unsigned)__builtin_amdgcn_readlane(xi, 0), (unsigned)__builtin_amdgcn_readlane(xi, 16)),
+                max((unsigned)__builtin_amdgcn_readlane(xi, 32), (unsigned)__builtin_amdgcn_readlane(xi, 48)));
     }
-    return v;
+    return __uint_as_float(x);
 }
 __device__ __forceinline__ float wave_max(float v) { return group_max<64>(v); }
 // power of two that brings the row maximum mb of the second field of a pair to that of the
