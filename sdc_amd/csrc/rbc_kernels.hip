@@ -1460,7 +1460,11 @@ __device__ __forceinline__ void k_project_mma_body(DevGeom g, int nks, const flo
                                                    cpx<float>* __restrict__ NG) {
     using MT = Mma<MM>;
     constexpr int OS = 66;               // staging row (2L <= 64 floats plus 2 to spread banks)
-    __shared__ float sO[WPB][3][16 * OS];  // per wave: [field][mode][OS] staging for the write-out
+    // the FP32 form of the large grids stages one field at a time: 17 KB of LDS per block and
+    // (with the hint on the kernel) 125 registers, so that 4 blocks fit a CU and a 512^2 x 32
+    // projection (917 blocks) runs in one round
+    constexpr int NST = (MM == 0 && WPB == 4) ? 1 : 3;
+    __shared__ float sO[WPB][NST][16 * OS];  // per wave: [field][mode][OS] staging for the write-out
     __shared__ f32x16 sPart[KS > 1 ? (KS - 1) * WPB : 1][KS > 1 ? 4 : 1][64];  // partials of waves wk > 0
     const int L = g.L, P = g.P, nq = g.nq;
     const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) % WPB, wk = (threadIdx.x >> 6) / WPB;
@@ -1536,32 +1540,57 @@ __device__ __forceinline__ void k_project_mma_body(DevGeom g, int nks, const flo
     }
     const float mt = -1.f, mp = -1.f / Pr;
     const int lm = j >> 1;  // local mode 0..15 within the wave
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int m = 8 * (r >> 2) + 4 * half + (r & 3);
-        if (m < L) {
-            sO[wave][0][lm * OS + 2 * m + comp] = mt * cT[r];
-            sO[wave][1][lm * OS + 2 * m + comp] = mp * (cF1[r] + k2 * cF2[r]);
-            sO[wave][2][lm * OS + 2 * m + comp] = mp * cG[r];
-        }
-    }
-    wave_sync();
-    // write 16 modes x L complex per field, contiguous in memory ([p][m] layout)
     const int pw0 = blockIdx.x * 16 * WPB + wave * 16;
-    for (int e = lane; e < 16 * 2 * L; e += 64) {
-        const int lmode = e / (2 * L), q = e - lmode * 2 * L;
-        const int pp = pw0 + lmode;
-        if (pp < P) {
-            reinterpret_cast<float*>(Nt)[(size_t)pp * 2 * L + q] = sO[wave][0][lmode * OS + q];
-            reinterpret_cast<float*>(Nf)[(size_t)pp * 2 * L + q] = sO[wave][1][lmode * OS + q];
-            reinterpret_cast<float*>(Ng)[(size_t)pp * 2 * L + q] = sO[wave][2][lmode * OS + q];
+    // 16 modes x L complex per field written contiguously ([p][m] layout) through the wave's
+    // staging rows: all three fields at once, or (NST = 1) one after another, not unrolled, so
+    // that one field's values are live at a time
+    if constexpr (NST == 3) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int m = 8 * (r >> 2) + 4 * half + (r & 3);
+            if (m < L) {
+                sO[wave][0][lm * OS + 2 * m + comp] = mt * cT[r];
+                sO[wave][1][lm * OS + 2 * m + comp] = mp * (cF1[r] + k2 * cF2[r]);
+                sO[wave][2][lm * OS + 2 * m + comp] = mp * cG[r];
+            }
+        }
+        wave_sync();
+        for (int e = lane; e < 16 * 2 * L; e += 64) {
+            const int lmode = e / (2 * L), q = e - lmode * 2 * L;
+            const int pp = pw0 + lmode;
+            if (pp < P) {
+                reinterpret_cast<float*>(Nt)[(size_t)pp * 2 * L + q] = sO[wave][0][lmode * OS + q];
+                reinterpret_cast<float*>(Nf)[(size_t)pp * 2 * L + q] = sO[wave][1][lmode * OS + q];
+                reinterpret_cast<float*>(Ng)[(size_t)pp * 2 * L + q] = sO[wave][2][lmode * OS + q];
+            }
+        }
+    } else {
+        float* so = sO[wave][0];
+#pragma unroll 1
+        for (int fld = 0; fld < 3; ++fld) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int m = 8 * (r >> 2) + 4 * half + (r & 3);
+                if (m < L)
+                    so[lm * OS + 2 * m + comp] =
+                        fld == 0 ? mt * cT[r] : (fld == 1 ? mp * (cF1[r] + k2 * cF2[r]) : mp * cG[r]);
+            }
+            wave_sync();
+            float* out = reinterpret_cast<float*>(fld == 0 ? Nt : (fld == 1 ? Nf : Ng));
+            for (int e = lane; e < 16 * 2 * L; e += 64) {
+                const int lmode = e / (2 * L), q = e - lmode * 2 * L;
+                const int pp = pw0 + lmode;
+                if (pp < P) out[(size_t)pp * 2 * L + q] = so[lmode * OS + q];
+            }
+            wave_sync();  // the rows are read before the next field is staged
         }
     }
     mean_flow_projection(g, fgw, prtab, Pr, NF, NG);
 }
 
 template <int MM, int WPB, int KS>
-__global__ void __launch_bounds__(64 * WPB * KS) k_project_mma(DevGeom g, int nks, const float* __restrict__ Dfl,
+__global__ void __launch_bounds__(64 * WPB * KS) __attribute__((amdgpu_waves_per_eu(MM == 0 && WPB == 4 ? 4 : 1)))
+k_project_mma(DevGeom g, int nks, const float* __restrict__ Dfl,
                                                      const cpx<float>* __restrict__ fgw, const float* __restrict__ prA,
                                                      const float* __restrict__ prtab, const float* __restrict__ kxa,
                                                      const float* __restrict__ kya, float Pr,
@@ -1589,7 +1618,8 @@ struct k_project_mma_args {
     cpx<float>* NG;
 };
 template <int MM, int WPB, int KS>
-__global__ void __launch_bounds__(64 * WPB * KS) k_project_mma_batch(const k_project_mma_args* __restrict__ tab, int zdim) {
+__global__ void __launch_bounds__(64 * WPB * KS) __attribute__((amdgpu_waves_per_eu(MM == 0 && WPB == 4 ? 4 : 1)))
+k_project_mma_batch(const k_project_mma_args* __restrict__ tab, int zdim) {
     const int mb = blockIdx.z / zdim;
     const k_project_mma_args& a = tab[mb];
     k_project_mma_body<MM, WPB, KS>(a.g, a.nks, a.Dfl, a.fgw, a.prA, a.prtab, a.kxa, a.kya, a.Pr, a.Nt, a.Nf, a.Ng,
