@@ -1368,26 +1368,20 @@ __device__ __forceinline__ void k_eval_mma_body(DevGeom g, int nlt, int nks, int
             MT::run(cTs, a[2], bt);  // S  . theta
             MT::run(cTz, a[3], bt);  // S' . theta
         }
-        // epilogue: partner component of the same mode sits in lane ^ 1
+        // epilogue: the partner component of the same mode sits in lane ^ 1 (a DPP swap of
+        // adjacent lanes, no LDS); i k multiplies the real parts by -1 times the partner's
+        // imaginary part (sg = -1) and the imaginary parts by the partner's real part (sg = 1)
+        const float sg = comp ? 1.f : -1.f;
+        auto partner = [](float x) {
+            return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, true));
+        };
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int l = lt * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
-            const float Fp = cFp[r], Fc = cFc[r], Gs = cGs[r], Ts = cTs[r], Tz = cTz[r];
-            const float oFp = __shfl_xor(Fp, 1), oGs = __shfl_xor(Gs, 1), oTs = __shfl_xor(Ts, 1);
-            float u, v, w, tx, ty, tz;
-            if (comp == 0) {  // real parts; o* hold the imaginary parts
-                u = -kx * oFp - ky * oGs;
-                v = -ky * oFp + kx * oGs;
-                tx = -kx * oTs;
-                ty = -ky * oTs;
-            } else {  // imaginary parts; o* hold the real parts
-                u = kx * oFp + ky * oGs;
-                v = ky * oFp - kx * oGs;
-                tx = kx * oTs;
-                ty = ky * oTs;
-            }
-            w = k2 * Fc;
-            tz = Tz;
+            const float oFp = partner(cFp[r]), oGs = partner(cGs[r]), oTs = partner(cTs[r]);
+            float u = sg * (kx * oFp + ky * oGs);
+            float v = sg * (ky * oFp - kx * oGs);
+            const float w = k2 * cFc[r], tx = sg * (kx * oTs), ty = sg * (ky * oTs), tz = cTz[r];
             if (p == 0 && l < nq) {  // mean flow at k = 0 (add_dc_part)
                 const cpx<float> mF = smf[0][l - lt * 32], mG = smf[1][l - lt * 32];
                 u += comp ? mF.y : mF.x;
