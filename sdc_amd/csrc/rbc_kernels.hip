@@ -558,7 +558,9 @@ __device__ __forceinline__ void k_update32_body(DevGeom g, int nchunk, int L0, i
                                                 const cpx<T>* __restrict__ Ng, cpx<T>* __restrict__ Rt,
                                                 cpx<T>* __restrict__ Rf, cpx<T>* __restrict__ Rg) {
     __shared__ cpx<T> shm[4][2][2][32];  // [wave][class][0: N_f, 1: rhs][j]
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wave index as a wave-uniform value: chunk, group and member indices then live in
+    // scalar registers and come from scalar loads, which the vector loads do not wait for
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int ch = blockIdx.x * 4 + wave;  // one chunk (<= kChunk members of one group) per wave
     if (ch >= nchunk) return;               // whole wave leaves; no block-wide barrier below
     const int grp = cgrp[ch];
@@ -568,16 +570,35 @@ __device__ __forceinline__ void k_update32_body(DevGeom g, int nchunk, int L0, i
     const T* rec = gmat + (size_t)grp * stride;
     const T* Cinv = rec + (cls ? (size_t)n0 * n0 : 0);
     const T* Binv = rec + (size_t)n0 * n0 + (size_t)(2 * L1) * (2 * L1) + (cls ? (size_t)L0 * L0 : 0);
+    // Every load below is issued by every lane (lanes outside the class read element 0 of a
+    // valid array and discard it), and the matrix-vector sums run over the full 16 / 32 terms
+    // with zero matrix entries and zero LDS slots beyond the class size: no branch depends on
+    // the class sizes, so the compiler can keep the next member's loads in flight across the
+    // current member's arithmetic.  The extra terms add exact zeros (bit-identical sums).
     T crow[32], brow[16];
+    {
+        const T* csrc = nc > 0 ? Cinv : rec;
+        const T* bsrc = Lc > 0 ? Binv : rec;
 #pragma unroll
-    for (int j = 0; j < 32; ++j) crow[j] = (j < nc && r < nc) ? Cinv[(size_t)j * nc + r] : T(0);
+        for (int j = 0; j < 32; ++j) {
+            const bool in = j < nc && r < nc;
+            const T v = csrc[in ? (size_t)j * nc + r : 0];
+            crow[j] = in ? v : T(0);
+        }
 #pragma unroll
-    for (int j = 0; j < 16; ++j) brow[j] = (j < Lc && r < Lc) ? Binv[(size_t)j * Lc + r] : T(0);
+        for (int j = 0; j < 16; ++j) {
+            const bool in = j < Lc && r < Lc;
+            const T v = bsrc[in ? (size_t)j * Lc + r : 0];
+            brow[j] = in ? v : T(0);
+        }
+    }
     cpx<T>* snf = shm[wave][cls][0];
     cpx<T>* srhs = shm[wave][cls][1];
+    for (int e = r; e < 64; e += 32) shm[wave][cls][e >> 5][e & 31] = mk<T>(0, 0);  // slots beyond the class stay 0
     const int b0 = cbeg[ch], b1 = cbeg[ch + 1];
-    const int m = cls + 2 * r;           // theta/f mode of this lane (valid for r < Lc)
     const bool tf = r < Lc, gl = lane < L;
+    const int m = tf ? cls + 2 * r : 0;  // theta/f mode of this lane (element 0 outside the class)
+    const int mg = gl ? lane : 0;        // g mode of this lane
     const cpx<T> z = mk<T>(0, 0);
     // all loads of one member are issued together; the next member's are in flight while
     // the current one is computed
@@ -589,21 +610,15 @@ __device__ __forceinline__ void k_update32_body(DevGeom g, int nchunk, int L0, i
         const int p = gmem[mi];
         base_ = (size_t)p * L;
         k2_ = k2a[p];
-        if (tf) {
-            nf_ = Nf[base_ + m];
-            xt_ = Xt[base_ + m];
-            xf_ = Xf[base_ + m];
-            nt_ = Nt[base_ + m];
-            if (use_prev) {
-                rt_ = Rt[base_ + m];
-                rf_ = Rf[base_ + m];
-            }
-        }
-        if (gl) {
-            ng_ = Ng[base_ + lane];
-            xg_ = Xg[base_ + lane];
-            if (use_prev) rg_ = Rg[base_ + lane];
-        }
+        nf_ = Nf[base_ + m];
+        xt_ = Xt[base_ + m];
+        xf_ = Xf[base_ + m];
+        nt_ = Nt[base_ + m];
+        rt_ = Rt[base_ + m];  // read also without history (then unused)
+        rf_ = Rf[base_ + m];
+        ng_ = Ng[base_ + mg];
+        xg_ = Xg[base_ + mg];
+        rg_ = Rg[base_ + mg];
     };
     if (b0 < b1) fetch(b0, nf, xt, xf, nt, rt0, rf0, ng, xg, rg0, k2, base);
     for (int mi = b0; mi < b1; ++mi) {
@@ -616,8 +631,7 @@ __device__ __forceinline__ void k_update32_body(DevGeom g, int nchunk, int L0, i
         if (tf) {
             cpx<T> rf = mk<T>(0, 0);
 #pragma unroll
-            for (int j = 0; j < 16; ++j)
-                if (j < Lc) rf = rf + cscale(snf[j], brow[j]);
+            for (int j = 0; j < 16; ++j) rf = rf + cscale(snf[j], brow[j]);
             const cpx<T> rt = cscale(nt, T(2));
             cpx<T> x1 = xt + cscale(rt, a_cur), x2 = xf + cscale(rf, a_cur);
             if (use_prev) {
@@ -633,8 +647,7 @@ __device__ __forceinline__ void k_update32_body(DevGeom g, int nchunk, int L0, i
         if (r < nc) {
             cpx<T> acc = mk<T>(0, 0);
 #pragma unroll
-            for (int j = 0; j < 32; ++j)
-                if (j < nc) acc = acc + cscale(srhs[j], crow[j]);
+            for (int j = 0; j < 32; ++j) acc = acc + cscale(srhs[j], crow[j]);
             const int mm = cls + 2 * (r < Lc ? r : r - Lc);
             if (r < Lc) Xt[base + mm] = acc;
             else Xf[base + mm] = acc;
