@@ -945,7 +945,7 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, int nblk, int per, 
     // tile-major layout) are consecutive logical blocks and so run on one XCD
     const int lb = xcd_logical_block(blockIdx.x, per);
     if (lb >= nblk) return;
-    const int lane = threadIdx.x & 63, pr = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, pr = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     constexpr int KX = KX512;
     const int nq = g.nq;
     const int y = lb % 512, l = lb / 512;
@@ -1291,7 +1291,7 @@ __device__ __forceinline__ void k_eval_mma_body(DevGeom g, int nlt, int nks, int
     const int lb = xcd_logical_block(blockIdx.x, per);
     if (lb >= ((P + 63) / 64) * nlt) return;
     const int p0 = (lb / nlt) * 64, lt_block = lb % nlt;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // LDS layout [field][column][LP] with column = 2 mode + re/im and an odd row length LP, so
     // that both the staging writes (m fastest) and the operand reads (column fastest) spread
     // over the banks
@@ -1688,7 +1688,7 @@ __global__ void __launch_bounds__(64 * CT) k_fft_y_fwd512c(DevGeom g, int ntile,
     if (lb >= nblk) return;
     if (TWL)
         for (int e = threadIdx.x; e < 512; e += 64 * CT) twl[e] = tw[e];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     constexpr int NYc = NYC512;
     const int P = g.P, nq = g.nq, KX = g.KX;
     const int tix = lb % ntile, l = lb / ntile;
