@@ -544,10 +544,10 @@ __global__ void __launch_bounds__(64) k_update(DevGeom g, int L0, int L1, int st
 }
 
 // Group update for L <= 32 with the class matrices held in registers: one wave per group
-// (4 groups per block), lane (class c, row r) keeps row r of C^-1 (n_c <= 32 values) and of
-// B^-1_f (L_c <= 16 values) for every member of the group.  Same arithmetic and rounding
-// order as k_update; mode UPD_STEP only.
-template <typename T>
+// (4 groups per block), lane (class c, row r) keeps row r of C^-1 (n_c <= NC values) and of
+// B^-1_f (L_c <= NC / 2 values) for every member of the group; NC = 32, or 16 for L <= 16.
+// Same arithmetic and rounding order as k_update; mode UPD_STEP only.
+template <typename T, int NC>
 __device__ __forceinline__ void k_update32_body(DevGeom g, int nchunk, int L0, int L1, int stride, T Pr, T dt,
                                                 T a_cur, T a_prev, int use_prev, const int* __restrict__ cgrp,
                                                 const int* __restrict__ cbeg,
@@ -575,18 +575,18 @@ __device__ __forceinline__ void k_update32_body(DevGeom g, int nchunk, int L0, i
     // with zero matrix entries and zero LDS slots beyond the class size: no branch depends on
     // the class sizes, so the compiler can keep the next member's loads in flight across the
     // current member's arithmetic.  The extra terms add exact zeros (bit-identical sums).
-    T crow[32], brow[16];
+    T crow[NC], brow[NC / 2];
     {
         const T* csrc = nc > 0 ? Cinv : rec;
         const T* bsrc = Lc > 0 ? Binv : rec;
 #pragma unroll
-        for (int j = 0; j < 32; ++j) {
+        for (int j = 0; j < NC; ++j) {
             const bool in = j < nc && r < nc;
             const T v = csrc[in ? (size_t)j * nc + r : 0];
             crow[j] = in ? v : T(0);
         }
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
+        for (int j = 0; j < NC / 2; ++j) {
             const bool in = j < Lc && r < Lc;
             const T v = bsrc[in ? (size_t)j * Lc + r : 0];
             brow[j] = in ? v : T(0);
@@ -631,7 +631,7 @@ __device__ __forceinline__ void k_update32_body(DevGeom g, int nchunk, int L0, i
         if (tf) {
             cpx<T> rf = mk<T>(0, 0);
 #pragma unroll
-            for (int j = 0; j < 16; ++j) rf = rf + cscale(snf[j], brow[j]);
+            for (int j = 0; j < NC / 2; ++j) rf = rf + cscale(snf[j], brow[j]);
             const cpx<T> rt = cscale(nt, T(2));
             cpx<T> x1 = xt + cscale(rt, a_cur), x2 = xf + cscale(rf, a_cur);
             if (use_prev) {
@@ -647,7 +647,7 @@ __device__ __forceinline__ void k_update32_body(DevGeom g, int nchunk, int L0, i
         if (r < nc) {
             cpx<T> acc = mk<T>(0, 0);
 #pragma unroll
-            for (int j = 0; j < 32; ++j) acc = acc + cscale(srhs[j], crow[j]);
+            for (int j = 0; j < NC; ++j) acc = acc + cscale(srhs[j], crow[j]);
             const int mm = cls + 2 * (r < Lc ? r : r - Lc);
             if (r < Lc) Xt[base + mm] = acc;
             else Xf[base + mm] = acc;
@@ -668,7 +668,7 @@ __device__ __forceinline__ void k_update32_body(DevGeom g, int nchunk, int L0, i
     }
 }
 
-template <typename T>
+template <typename T, int NC>
 __global__ void __launch_bounds__(256) k_update32(DevGeom g, int nchunk, int L0, int L1, int stride, T Pr, T dt,
                                                   T a_cur, T a_prev, int use_prev, const int* __restrict__ cgrp,
                                                   const int* __restrict__ cbeg,
@@ -678,7 +678,7 @@ __global__ void __launch_bounds__(256) k_update32(DevGeom g, int nchunk, int L0,
                                                   const cpx<T>* __restrict__ Nt, const cpx<T>* __restrict__ Nf,
                                                   const cpx<T>* __restrict__ Ng, cpx<T>* __restrict__ Rt,
                                                   cpx<T>* __restrict__ Rf, cpx<T>* __restrict__ Rg) {
-    k_update32_body<T>(g, nchunk, L0, L1, stride, Pr, dt, a_cur, a_prev, use_prev, cgrp, cbeg, gmem, gmat, k2a, Xt,
+    k_update32_body<T, NC>(g, nchunk, L0, L1, stride, Pr, dt, a_cur, a_prev, use_prev, cgrp, cbeg, gmem, gmat, k2a, Xt,
                        Xf, Xg, Nt, Nf, Ng, Rt, Rf, Rg);
 }
 // one member's arguments of k_update32_batch, which runs the k_update32 body for member
@@ -710,11 +710,11 @@ struct k_update32_args {
     cpx<T>* Rf;
     cpx<T>* Rg;
 };
-template <typename T>
+template <typename T, int NC>
 __global__ void __launch_bounds__(256) k_update32_batch(const k_update32_args<T>* __restrict__ tab, int zdim) {
     const int mb = blockIdx.z / zdim;
     const k_update32_args<T>& a = tab[mb];
-    k_update32_body<T>(a.g, a.nchunk, a.L0, a.L1, a.stride, a.Pr, a.dt, a.a_cur, a.a_prev, a.use_prev, a.cgrp, a.cbeg,
+    k_update32_body<T, NC>(a.g, a.nchunk, a.L0, a.L1, a.stride, a.Pr, a.dt, a.a_cur, a.a_prev, a.use_prev, a.cgrp, a.cbeg,
                        a.gmem, a.gmat, a.k2a, a.Xt, a.Xf, a.Xg, a.Nt, a.Nf, a.Ng, a.Rt, a.Rf, a.Rg);
 }
 
@@ -2096,11 +2096,15 @@ void Launch<T>::update(const DevGeom& g, const Buffers& b, int ngroups, int L0, 
                        double dt, StepCoef sc, int mode, void* const Xio[3], void* const Nin[3], void* const Rp[3],
                        hipStream_t s) {
     if (mode == UPD_STEP && g.L <= 32) {
-        hipLaunchKernelGGL(k_update32<T>, dim3((b.nchunk + 3) / 4), dim3(256), 0, s, g, b.nchunk, L0, L1, stride,
-                           (T)Pr, (T)dt, (T)sc.a_cur, (T)sc.a_prev, sc.use_prev, b.cgrp, b.cbeg, b.gmem, (const T*)b.gmat,
-                           (const T*)b.k2, (cpx<T>*)Xio[0], (cpx<T>*)Xio[1], (cpx<T>*)Xio[2], (const cpx<T>*)Nin[0],
-                           (const cpx<T>*)Nin[1], (const cpx<T>*)Nin[2], (cpx<T>*)Rp[0], (cpx<T>*)Rp[1],
-                           (cpx<T>*)Rp[2]);
+#define RBC_UPD32(NCV)                                                                                            \
+    hipLaunchKernelGGL((k_update32<T, NCV>), dim3((b.nchunk + 3) / 4), dim3(256), 0, s, g, b.nchunk, L0, L1,       \
+                       stride, (T)Pr, (T)dt, (T)sc.a_cur, (T)sc.a_prev, sc.use_prev, b.cgrp, b.cbeg, b.gmem,           \
+                       (const T*)b.gmat,                                                                               \
+                       (const T*)b.k2, (cpx<T>*)Xio[0], (cpx<T>*)Xio[1], (cpx<T>*)Xio[2], (const cpx<T>*)Nin[0],       \
+                       (const cpx<T>*)Nin[1], (const cpx<T>*)Nin[2], (cpx<T>*)Rp[0], (cpx<T>*)Rp[1], (cpx<T>*)Rp[2])
+        if (g.L <= 16) RBC_UPD32(16);
+        else RBC_UPD32(32);
+#undef RBC_UPD32
         return;
     }
     const size_t sm = (size_t)3 * g.L * sizeof(cpx<T>);
@@ -2361,7 +2365,11 @@ void batch_step(const StepBatch* sb, hipStream_t s) {
         }
 #undef RBC_PJB
     }
-    hipLaunchKernelGGL(k_update32_batch<float>, dim3((sb->nchunk_max + 3) / 4, 1, B), dim3(256), 0, s, sb->up, 1);
+    {
+        const dim3 gr((sb->nchunk_max + 3) / 4, 1, B);
+        if (g.L <= 16) hipLaunchKernelGGL((k_update32_batch<float, 16>), gr, dim3(256), 0, s, sb->up, 1);
+        else hipLaunchKernelGGL((k_update32_batch<float, 32>), gr, dim3(256), 0, s, sb->up, 1);
+    }
     {
         const int n = 3 * g.NYc * g.L;
         hipLaunchKernelGGL(k_finish_batch<float>, dim3((n + 255) / 256 + 1, 1, B), dim3(256), 0, s, sb->fi, 1);
