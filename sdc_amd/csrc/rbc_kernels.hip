@@ -1469,7 +1469,7 @@ __device__ __forceinline__ void k_project_mma_body(DevGeom g, int nks, const flo
     constexpr int OS = 66;               // staging row (2L <= 64 floats plus 2 to spread banks)
     // the FP32 form of the large grids stages one field at a time: 17 KB of LDS per block and
     // (with the hint on the kernel) 125 registers, so that 4 blocks fit a CU and a 512^2 x 32
-    // projection (917 blocks) runs in one round
+    // projection (917 blocks) has all its blocks resident at once
     constexpr int NST = (MM == 0 && WPB == 4) ? 1 : 3;
     __shared__ float sO[WPB][NST][16 * OS];  // per wave: [field][mode][OS] staging for the write-out
     __shared__ f32x16 sPart[KS > 1 ? (KS - 1) * WPB : 1][KS > 1 ? 4 : 1][64];  // partials of waves wk > 0
