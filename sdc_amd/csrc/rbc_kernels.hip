@@ -1647,9 +1647,9 @@ template <>
 // product fields of k_rows512t: 0 h = uu + vv - 2 ww, 1 d = uu - vv, 2 uv, 3 uw, 4 vw, 5 u.grad(theta)
 // h, d, uv -> D0 = (k^2 / 2) h + ((kx^2 - ky^2) / 2) d + 2 kx ky uv, D2 = -kx ky d + (kx^2 - ky^2) uv
 struct YGroup<0> {
-    static constexpr int n = 3, t0 = 0, t1 = 2;
+    static constexpr int n = 3, t0 = 0, t1 = 2, t2 = -1;
     __device__ static int q(int i) { return i; }
-    __device__ static void add(int i, cpx<float> V, float kx, float ky, cpx<float>& d0, cpx<float>& d1) {
+    __device__ static void add(int i, cpx<float> V, float kx, float ky, cpx<float>& d0, cpx<float>& d1, cpx<float>&) {
         if (i == 0) { d0 = d0 + cscale(V, 0.5f * (kx * kx + ky * ky)); }
         else if (i == 1) { d0 = d0 + cscale(V, 0.5f * (kx * kx - ky * ky)); d1 = d1 + cscale(V, -kx * ky); }
         else { d0 = d0 + cscale(V, 2.f * kx * ky); d1 = d1 + cscale(V, kx * kx - ky * ky); }
@@ -1657,18 +1657,31 @@ struct YGroup<0> {
 };
 template <>
 struct YGroup<1> {  // uw, vw -> D1, D3
-    static constexpr int n = 2, t0 = 1, t1 = 3;
+    static constexpr int n = 2, t0 = 1, t1 = 3, t2 = -1;
     __device__ static int q(int i) { return 3 + i; }
-    __device__ static void add(int i, cpx<float> V, float kx, float ky, cpx<float>& d0, cpx<float>& d1) {
+    __device__ static void add(int i, cpx<float> V, float kx, float ky, cpx<float>& d0, cpx<float>& d1, cpx<float>&) {
         if (i == 0) { d0 = d0 + imul(V, kx); d1 = d1 - imul(V, ky); }
         else { d0 = d0 + imul(V, ky); d1 = d1 + imul(V, kx); }
     }
 };
 template <>
 struct YGroup<2> {  // u.grad(theta) -> D4
-    static constexpr int n = 1, t0 = 4, t1 = -1;
+    static constexpr int n = 1, t0 = 4, t1 = -1, t2 = -1;
     __device__ static int q(int) { return 5; }
-    __device__ static void add(int, cpx<float> V, float, float, cpx<float>& d0, cpx<float>&) { d0 = d0 + V; }
+    __device__ static void add(int, cpx<float> V, float, float, cpx<float>& d0, cpx<float>&, cpx<float>&) {
+        d0 = d0 + V;
+    }
+};
+template <>
+struct YGroup<3> {  // groups 1 and 2 in one pass: uw, vw, u.grad(theta) -> D1, D3, D4
+    static constexpr int n = 3, t0 = 1, t1 = 3, t2 = 4;
+    __device__ static int q(int i) { return 3 + i; }
+    __device__ static void add(int i, cpx<float> V, float kx, float ky, cpx<float>& d0, cpx<float>& d1,
+                               cpx<float>& d2) {
+        if (i == 0) { d0 = d0 + imul(V, kx); d1 = d1 - imul(V, ky); }
+        else if (i == 1) { d0 = d0 + imul(V, ky); d1 = d1 + imul(V, kx); }
+        else { d2 = d2 + V; }
+    }
 };
 
 template <int GRP, int CT>
@@ -1706,9 +1719,9 @@ __global__ void __launch_bounds__(64 * CT) k_fft_y_fwd512c(DevGeom g, int ntile,
         kyc[r] = (n <= a) ? n : ((n >= 512 - a) ? n - shift : -1);
         kyv[r] = (cvalid && kyc[r] >= 0) ? kya[(size_t)kxc * NYc + kyc[r]] : 0.f;
     }
-    cpx<float> d0[8], d1[8];
+    cpx<float> d0[8], d1[8], d2[8];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) d0[r] = d1[r] = mk(0.f, 0.f);
+    for (int r = 0; r < 8; ++r) d0[r] = d1[r] = d2[r] = mk(0.f, 0.f);
     // tile element e = threadIdx.x + 64 CT i (i < 8): y = e / CT, column e % CT
     cpx<float> pre[8];
     // the block's 8 columns are one contiguous tile of the tile-major plane
@@ -1736,8 +1749,8 @@ __global__ void __launch_bounds__(64 * CT) k_fft_y_fwd512c(DevGeom g, int ntile,
         for (int r = 0; r < 8; ++r) v[r] = tile[pad8(lane + 64 * r) * CS + c];
         fft512_wave<float, false, CS>(v, tile + c, TWL ? twl : tw, lane);
 #pragma unroll
-        for (int r = 0; r < 8; ++r) G::add(iq, v[r], kxv, kyv[r], d0[r], d1[r]);
-        if (GRP == 1 && kxc == 0 && lane == 0) fgw[2 * l + iq] = v[0];  // P_uw, P_vw at row 0
+        for (int r = 0; r < 8; ++r) G::add(iq, v[r], kxv, kyv[r], d0[r], d1[r], d2[r]);
+        if ((GRP == 1 || GRP == 3) && iq < 2 && kxc == 0 && lane == 0) fgw[2 * l + iq] = v[0];  // P_uw, P_vw, row 0
         __syncthreads();  // the tile is rewritten for the next product
     }
     if (cvalid) {
@@ -1748,6 +1761,7 @@ __global__ void __launch_bounds__(64 * CT) k_fft_y_fwd512c(DevGeom g, int ntile,
                 cpx<float>* o = D + (size_t)l * P + (size_t)kxc * NYc + kyc[r];
                 o[G::t0 * fs] = d0[r];
                 if (G::t1 >= 0) o[G::t1 * fs] = d1[r];
+                if (G::t2 >= 0) o[G::t2 * fs] = d2[r];
             }
     }
 }
@@ -1797,7 +1811,10 @@ __device__ __forceinline__ void y_fwd_group_comb(const DevGeom& g, cpx<float>* t
         wave_sync();  // the column is read before it becomes the scratch
         fft_group<LOGN, false, 1, S>(v, tile + c, tw, j);
 #pragma unroll
-        for (int r = 0; r < 8; ++r) Gr::add(iq, v[0][r], kxv, kyv[r], d0[r], d1[r]);
+        for (int r = 0; r < 8; ++r) {
+            cpx<float> unused;
+            Gr::add(iq, v[0][r], kxv, kyv[r], d0[r], d1[r], unused);
+        }
         if (GRP == 1 && kx == 0 && j == 0) fgw[2 * l + iq] = v[0][0];  // P_uw, P_vw at (0, 0)
         __syncthreads();  // the tile is refilled by the next product
     }
@@ -2009,9 +2026,10 @@ void Launch<T>::rows(const DevGeom& g, const Buffers& b, hipStream_t s) {
 template <typename T>
 void Launch<T>::fft_y_fwd(const DevGeom& g, const Buffers& b, hipStream_t s) {
     if (fast512_y<T>(g)) {
-        // 8 kx columns (8 waves) per block: 64-byte row segments; group 0 holds 4 waves per
-        // SIMD (2 blocks per CU), groups 1 and 2 more (measured: 4 columns 194 us, 8 178 us and
-        // 16 195 us for the three launches together)
+        // 8 kx columns (8 waves) per block, one tile of the tile-major layout (measured before
+        // that layout: 4 columns 194 us, 8 178 us and 16 195 us for the launches together); two
+        // launches of three products each: group 0 (h, d, uv -> D0, D2) and group 3 (uw, vw,
+        // u.grad(theta) -> D1, D3, D4)
         constexpr int CT = 8;
         const int ntile = (g.KX + CT - 1) / CT, nblk = ntile * g.nq, per = (nblk + 7) / 8;
 #define RBC_YF512(GRPV)                                                                                          \
@@ -2019,8 +2037,7 @@ void Launch<T>::fft_y_fwd(const DevGeom& g, const Buffers& b, hipStream_t s) {
                        (const cpx<float>*)b.hout, (const cpx<float>*)b.twN, (const float*)b.kx,                  \
                        (const float*)b.ky, (cpx<float>*)b.lev, (cpx<float>*)b.fgw)
         RBC_YF512(0);
-        RBC_YF512(1);
-        RBC_YF512(2);
+        RBC_YF512(3);
 #undef RBC_YF512
         return;
     }
