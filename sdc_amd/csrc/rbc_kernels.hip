@@ -1634,13 +1634,14 @@ k_project_mma_batch(const k_project_mma_args* __restrict__ tab, int zdim) {
 }
 
 
-// Forward y-FFT at N = 512 fused with the level combinations.  A block of 8 waves handles 8
-// kx columns of one level and one product group:
-//   group 0: uu, uv, vv, ww -> D0, D2     group 1: uw, vw -> D1, D3     group 2: u.grad -> D4
-// The products of the group are transformed in turn (tile in LDS, one column per wave) and the
-// group's combinations of the kept rows accumulate in registers; the next product's tile is
-// read into registers while the current one is transformed.  Group 1 also writes the k = 0
-// values of P_uw and P_vw (fgw) for the mean flow.
+// Forward y-FFT fused with the level combinations, by product group:
+//   group 0: h, d, uv -> D0, D2     group 1: uw, vw -> D1, D3     group 2: u.grad -> D4
+//   group 3: groups 1 and 2 in one pass (uw, vw, u.grad -> D1, D3, D4)
+// At N = 512 a block of 8 waves handles the 8 kx columns of one tile of one level, for group 0
+// or group 3; the products of the group are transformed in turn (tile in LDS, one column per
+// wave) and the group's combinations of the kept rows accumulate in registers; the next
+// product's tile is read into registers while the current one is transformed.  The groups with
+// uw and vw also write the k = 0 values of P_uw and P_vw (fgw) for the mean flow.
 template <int GRP>
 struct YGroup;
 template <>
