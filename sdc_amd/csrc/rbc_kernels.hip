@@ -881,35 +881,66 @@ __global__ void __launch_bounds__(256) k_nusselt(DevGeom g, const cpx<T>* __rest
 // =======================================================================================
 // 512-point fast paths (FP32): one wavefront per transform, radix-8 Stockham in registers.
 // Lane j holds x[j + 64 r], r = 0..7, on entry and X[j + 64 r] on exit.  Two exchanges
-// through a per-wave LDS scratch of 576 complex (one pad slot every 8 to spread banks).
+// through a per-wave LDS scratch whose slots are laid out to spread the banks (below).
 // =======================================================================================
 // 512-point FFT of one wavefront.  tw = table exp(-2 pi i k / 512), k < 512 (LDS or global).
-// The scratch element of internal index i is sb[pad8(i) * S] (S = 1 for a private buffer,
+// The scratch element of internal index i is sb[Map::at(i) * S] (S = 1 for a private buffer,
 // S = tile row stride when a column of an LDS tile doubles as the scratch).
-template <typename T, bool INV, int S = 1>
+//   Pad8Map: one pad slot every 8 (576 slots).
+//   XorMap: bits 0-2 ^= bits 3-5 and bits 3-5 ^= bits 6-8, a permutation of the 512 slots.  With
+//   8-byte slots it puts the 16 (32) lanes served per LDS cycle on distinct banks for all three
+//   exchange patterns (8 j + r, j + 64 r, 64 (j / 8) + j % 8 + 8 r); the pad-8 layout leaves
+//   the contiguous j + 64 r reads 2-way conflicted.
+struct Pad8Map {
+    static constexpr int slots = 576;
+    __device__ static __forceinline__ int at(int i) { return pad8(i); }
+};
+struct XorMap {
+    static constexpr int slots = 512;
+    __device__ static __forceinline__ int at(int i) { return i ^ ((((i >> 6) & 7) << 3) | ((i >> 3) & 7)); }
+};
+// Twiddle tables.  Tw512: tw[k] = exp(-2 pi i k / 512), k < 512.  TwByPass: the two passes' reads
+// laid out contiguously, tw[64 r + j] = w^(j r) (pass 3, lane j) and tw[512 + 8 r + j] = w^(8 j r)
+// (pass 2, lane group j = lane % 8), 576 entries: each read touches consecutive slots, where
+// the strided reads of Tw512 share 2 to 4 banks.
+struct Tw512 {
+    static constexpr int size = 512;
+    __device__ static __forceinline__ int pass2(int lane, int r) { return (lane & 7) * r * 8; }
+    __device__ static __forceinline__ int pass3(int lane, int r) { return lane * r; }
+};
+struct TwByPass {
+    static constexpr int size = 576;
+    __device__ static __forceinline__ int pass2(int lane, int r) { return 512 + 8 * r + (lane & 7); }
+    __device__ static __forceinline__ int pass3(int lane, int r) { return 64 * r + lane; }
+    // the Tw512 exponent k of entry e
+    __device__ static __forceinline__ int source(int e) {
+        return e < 512 ? (e & 63) * (e >> 6) : 8 * (e & 7) * ((e - 512) >> 3);
+    }
+};
+template <typename T, bool INV, int S = 1, typename Map = Pad8Map, typename Tw = Tw512>
 __device__ __forceinline__ void fft512_wave(cpx<T> v[8], cpx<T>* sb, const cpx<T>* tw, int lane) {
     dft8<T, INV>(v);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) sb[pad8(8 * lane + r) * S] = v[r];
+    for (int r = 0; r < 8; ++r) sb[Map::at(8 * lane + r) * S] = v[r];
     wave_sync();
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = sb[pad8(lane + 64 * r) * S];
+    for (int r = 0; r < 8; ++r) v[r] = sb[Map::at(lane + 64 * r) * S];
 #pragma unroll
     for (int r = 1; r < 8; ++r) {
-        cpx<T> w = tw[(lane & 7) * r * 8];
+        cpx<T> w = tw[Tw::pass2(lane, r)];
         if (INV) w.y = -w.y;
         v[r] = cmul(v[r], w);
     }
     dft8<T, INV>(v);
     wave_sync();
 #pragma unroll
-    for (int r = 0; r < 8; ++r) sb[pad8((lane >> 3) * 64 + (lane & 7) + 8 * r) * S] = v[r];
+    for (int r = 0; r < 8; ++r) sb[Map::at((lane >> 3) * 64 + (lane & 7) + 8 * r) * S] = v[r];
     wave_sync();
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = sb[pad8(lane + 64 * r) * S];
+    for (int r = 0; r < 8; ++r) v[r] = sb[Map::at(lane + 64 * r) * S];
 #pragma unroll
     for (int r = 1; r < 8; ++r) {
-        cpx<T> w = tw[lane * r];
+        cpx<T> w = tw[Tw::pass3(lane, r)];
         if (INV) w.y = -w.y;
         v[r] = cmul(v[r], w);
     }
@@ -939,8 +970,8 @@ __device__ __forceinline__ int hyb512(int y, int kx) { return (kx >> 3) * (512 *
 // field of every pair is scaled to the size of the first for the transform (pair_scale).
 __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, int nblk, int per, const cpx<float>* __restrict__ hin,
                                                   const cpx<float>* __restrict__ twM, cpx<float>* __restrict__ hout) {
-    __shared__ cpx<float> tw[512];
-    __shared__ cpx<float> zs[3][576];
+    __shared__ cpx<float> tw[TwByPass::size];
+    __shared__ cpx<float> zs[3][XorMap::slots];
     // one (level, y) row per block; consecutive rows (which share the 128-byte lines of the
     // tile-major layout) are consecutive logical blocks and so run on one XCD
     const int lb = xcd_logical_block(blockIdx.x, per);
@@ -951,7 +982,7 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, int nblk, int per, 
     const int y = lb % 512, l = lb / 512;
     const size_t fstride = (size_t)nq * 512 * KXT512;
     const size_t roff = (size_t)l * 512 * KXT512 + (size_t)y * 8;
-    for (int e = threadIdx.x; e < 512; e += 192) tw[e] = twM[e];
+    for (int e = threadIdx.x; e < TwByPass::size; e += 192) tw[e] = twM[TwByPass::source(e)];
     __syncthreads();
     {
         const cpx<float>* x1 = hin + (size_t)(2 * pr) * fstride + roff;
@@ -1002,22 +1033,42 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, int nblk, int per, 
             v[r] = z;
         }
         wave_sync();  // the staging area becomes the transform's scratch
-        fft512_wave<float, true>(v, zs[pr], tw, lane);
+        fft512_wave<float, true, 1, XorMap, TwByPass>(v, zs[pr], tw, lane);
 #pragma unroll
-        for (int r = 0; r < 8; ++r) zs[pr][pad8(lane + 64 * r)] = mk(v[r].x, isg * v[r].y);
+        for (int r = 0; r < 8; ++r) zs[pr][XorMap::at(lane + 64 * r)] = mk(v[r].x, isg * v[r].y);
     }
     __syncthreads();
     const float s = 1.f / (512.f * 512.f);
     cpx<float> p[8];
     float ma = 0.f, mb = 0.f;
+    // pr is uniform over the wave: one branch around the whole slot loop, not one per slot
+    if (pr == 0) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int i = XorMap::at(lane + 64 * r);
+            const cpx<float> z0 = zs[0][i], z1 = zs[1][i];
+            const float u = z0.x, v = z0.y, w = z1.x;
+            p[r] = mk(s * (u * u + v * v - 2.f * w * w), s * (u * u - v * v));
+        }
+    } else if (pr == 1) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int i = XorMap::at(lane + 64 * r);
+            const cpx<float> z0 = zs[0][i], z1 = zs[1][i];
+            const float u = z0.x, v = z0.y, w = z1.x;
+            p[r] = mk(s * u * v, s * u * w);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int i = XorMap::at(lane + 64 * r);
+            const cpx<float> z0 = zs[0][i], z1 = zs[1][i], z2 = zs[2][i];
+            const float u = z0.x, v = z0.y, w = z1.x, tx = z1.y, ty = z2.x, tz = z2.y;
+            p[r] = mk(s * v * w, s * (u * tx + v * ty + w * tz));
+        }
+    }
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-        const int i = pad8(lane + 64 * r);
-        const cpx<float> z0 = zs[0][i], z1 = zs[1][i], z2 = zs[2][i];
-        const float u = z0.x, v = z0.y, w = z1.x, tx = z1.y, ty = z2.x, tz = z2.y;
-        if (pr == 0) p[r] = mk(s * (u * u + v * v - 2.f * w * w), s * (u * u - v * v));
-        else if (pr == 1) p[r] = mk(s * u * v, s * u * w);
-        else p[r] = mk(s * v * w, s * (u * tx + v * ty + w * tz));
         ma = fmaxf(ma, fabsf(p[r].x));
         mb = fmaxf(mb, fabsf(p[r].y));
     }
@@ -1025,16 +1076,21 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, int nblk, int per, 
 #pragma unroll
     for (int r = 0; r < 8; ++r) p[r].y *= sg;
     __syncthreads();  // every wave has read the slots before they become scratch
-    fft512_wave<float, false>(p, zs[pr], tw, lane);
+    fft512_wave<float, false, 1, XorMap, TwByPass>(p, zs[pr], tw, lane);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) zs[pr][pad8(lane + 64 * r)] = p[r];
+    for (int r = 0; r < 8; ++r) zs[pr][XorMap::at(lane + 64 * r)] = p[r];
     wave_sync();
     cpx<float>* o1 = hout + (size_t)(2 * pr) * fstride + roff;
     cpx<float>* o2 = hout + (size_t)(2 * pr + 1) * fstride + roff;
-    for (int k = lane; k < KX; k += 64) {
-        const cpx<float> zk = zs[pr][pad8(k)], zm = zs[pr][pad8((512 - k) & 511)];
-        o1[hyb512(0, k)] = mk(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-        o2[hyb512(0, k)] = mk(0.5f * isg * (zk.y + zm.y), -0.5f * isg * (zk.x - zm.x));
+    // bins k = lane + 64 r < KX (r < 3) and their mirrors (512 - k) % 512
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const int k = lane + 64 * r;
+        if (r < 2 || k < KX) {
+            const cpx<float> zk = zs[pr][XorMap::at(k)], zm = zs[pr][XorMap::at((512 - k) & 511)];
+            o1[hyb512(0, k)] = mk(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+            o2[hyb512(0, k)] = mk(0.5f * isg * (zk.y + zm.y), -0.5f * isg * (zk.x - zm.x));
+        }
     }
 }
 
