@@ -1231,12 +1231,12 @@ __global__ void __launch_bounds__(512) k_fft_y_inv512(DevGeom g, int ntile, int 
                                                       const cpx<float>* __restrict__ tw,
                                                       cpx<float>* __restrict__ hin) {
     __shared__ cpx<float> tile[576 * YS];
-    __shared__ cpx<float> twl[512];  // in LDS, so that no table entry stays in registers
+    __shared__ cpx<float> twl[TwByPass::size];  // in LDS, so that no table entry stays in registers
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int NYc = NYC512;
     constexpr int a = (NYc - 1) / 2, shift = 512 - NYc;  // kept rows [0, a] U [512 - a, 511]
     const int P = g.P, KX = g.KX;
-    twl[threadIdx.x] = tw[threadIdx.x];
+    for (int e = threadIdx.x; e < TwByPass::size; e += 512) twl[e] = tw[TwByPass::source(e)];
     lds_barrier();
     const int range = (nblk + 7) / 8, xcd = blockIdx.x & 7;
     const int end = min(nblk, (xcd + 1) * range);
@@ -1262,7 +1262,7 @@ __global__ void __launch_bounds__(512) k_fft_y_inv512(DevGeom g, int ntile, int 
 #pragma unroll
         for (int r = 0; r < 8; ++r) v[r] = nxt[r];
         if (lb + bpx < end) load(lb + bpx, nxt);
-        fft512_wave<float, true, YS>(v, tile + wave, twl, lane);
+        fft512_wave<float, true, YS, Pad8Map, TwByPass>(v, tile + wave, twl, lane);
 #pragma unroll
         for (int r = 0; r < 8; ++r) tile[pad8(lane + 64 * r) * YS + wave] = v[r];
         lds_barrier();
@@ -1753,11 +1753,11 @@ __global__ void __launch_bounds__(64 * CT) k_fft_y_fwd512c(DevGeom g, int ntile,
     // the compiler from holding the table entries in registers across the transforms
     constexpr bool TWL = G::n > 1;
     __shared__ cpx<float> tile[576 * CS];
-    __shared__ cpx<float> twl[TWL ? 512 : 1];
+    __shared__ cpx<float> twl[TWL ? TwByPass::size : 1];
     const int lb = xcd_logical_block(blockIdx.x, per);
     if (lb >= nblk) return;
     if (TWL)
-        for (int e = threadIdx.x; e < 512; e += 64 * CT) twl[e] = tw[e];
+        for (int e = threadIdx.x; e < TwByPass::size; e += 64 * CT) twl[e] = tw[TwByPass::source(e)];
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     constexpr int NYc = NYC512;
     const int P = g.P, nq = g.nq, KX = g.KX;
@@ -1804,7 +1804,8 @@ __global__ void __launch_bounds__(64 * CT) k_fft_y_fwd512c(DevGeom g, int ntile,
         cpx<float> v[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) v[r] = tile[pad8(lane + 64 * r) * CS + c];
-        fft512_wave<float, false, CS>(v, tile + c, TWL ? twl : tw, lane);
+        if constexpr (TWL) fft512_wave<float, false, CS, Pad8Map, TwByPass>(v, tile + c, twl, lane);
+        else fft512_wave<float, false, CS>(v, tile + c, tw, lane);
 #pragma unroll
         for (int r = 0; r < 8; ++r) G::add(iq, v[r], kxv, kyv[r], d0[r], d1[r], d2[r]);
         if ((GRP == 1 || GRP == 3) && iq < 2 && kxc == 0 && lane == 0) fgw[2 * l + iq] = v[0];  // P_uw, P_vw, row 0
