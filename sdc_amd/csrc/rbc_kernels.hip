@@ -891,13 +891,30 @@ __global__ void __launch_bounds__(256) k_nusselt(DevGeom g, const cpx<T>* __rest
 //   8-byte slots it puts the 16 (32) lanes served per LDS cycle on distinct banks for all three
 //   exchange patterns (8 j + r, j + 64 r, 64 (j / 8) + j % 8 + 8 r); the pad-8 layout leaves
 //   the contiguous j + 64 r reads 2-way conflicted.
+// Each map also gives the slots of the three exchange patterns of lane j (j < 64) for r < 8:
+// col(j, r) = at(j + 64 r), row(j, r) = at(8 j + r), mid(j, r) = at(64 (j / 8) + j % 8 + 8 r).
 struct Pad8Map {
     static constexpr int slots = 576;
     __device__ static __forceinline__ int at(int i) { return pad8(i); }
+    __device__ static __forceinline__ int col(int j, int r) { return pad8(j + 64 * r); }
+    __device__ static __forceinline__ int row(int j, int r) { return pad8(8 * j + r); }
+    __device__ static __forceinline__ int mid(int j, int r) { return pad8((j >> 3) * 64 + (j & 7) + 8 * r); }
 };
+// XorMap's pattern slots as one lane constant XOR an immediate (the fields that r occupies are
+// disjoint from the ones it is XORed into), so each costs one VALU instruction:
+//   col = (8 (j / 8) + (j % 8 ^ j / 8)) ^ 8 r, plus 64 r;
+//   row = (64 (j / 8) + 8 (j % 8 ^ j / 8) + j % 8) ^ r;
+//   mid = (72 (j / 8) + j % 8) ^ 9 r.
 struct XorMap {
     static constexpr int slots = 512;
     __device__ static __forceinline__ int at(int i) { return i ^ ((((i >> 6) & 7) << 3) | ((i >> 3) & 7)); }
+    __device__ static __forceinline__ int col(int j, int r) {
+        return ((8 * (j >> 3) + ((j & 7) ^ (j >> 3))) ^ (8 * r)) + 64 * r;
+    }
+    __device__ static __forceinline__ int row(int j, int r) {
+        return ((64 * (j >> 3) + 8 * ((j & 7) ^ (j >> 3)) + (j & 7)) ^ r);
+    }
+    __device__ static __forceinline__ int mid(int j, int r) { return (72 * (j >> 3) + (j & 7)) ^ (9 * r); }
 };
 // Twiddle tables.  Tw512: tw[k] = exp(-2 pi i k / 512), k < 512.  TwByPass: the two passes' reads
 // laid out contiguously, tw[64 r + j] = w^(j r) (pass 3, lane j) and tw[512 + 8 r + j] = w^(8 j r)
@@ -921,10 +938,10 @@ template <typename T, bool INV, int S = 1, typename Map = Pad8Map, typename Tw =
 __device__ __forceinline__ void fft512_wave(cpx<T> v[8], cpx<T>* sb, const cpx<T>* tw, int lane) {
     dft8<T, INV>(v);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) sb[Map::at(8 * lane + r) * S] = v[r];
+    for (int r = 0; r < 8; ++r) sb[Map::row(lane, r) * S] = v[r];
     wave_sync();
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = sb[Map::at(lane + 64 * r) * S];
+    for (int r = 0; r < 8; ++r) v[r] = sb[Map::col(lane, r) * S];
 #pragma unroll
     for (int r = 1; r < 8; ++r) {
         cpx<T> w = tw[Tw::pass2(lane, r)];
@@ -934,10 +951,10 @@ __device__ __forceinline__ void fft512_wave(cpx<T> v[8], cpx<T>* sb, const cpx<T
     dft8<T, INV>(v);
     wave_sync();
 #pragma unroll
-    for (int r = 0; r < 8; ++r) sb[Map::at((lane >> 3) * 64 + (lane & 7) + 8 * r) * S] = v[r];
+    for (int r = 0; r < 8; ++r) sb[Map::mid(lane, r) * S] = v[r];
     wave_sync();
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = sb[Map::at(lane + 64 * r) * S];
+    for (int r = 0; r < 8; ++r) v[r] = sb[Map::col(lane, r) * S];
 #pragma unroll
     for (int r = 1; r < 8; ++r) {
         cpx<T> w = tw[Tw::pass3(lane, r)];
@@ -1035,7 +1052,7 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, int nblk, int per, 
         wave_sync();  // the staging area becomes the transform's scratch
         fft512_wave<float, true, 1, XorMap, TwByPass>(v, zs[pr], tw, lane);
 #pragma unroll
-        for (int r = 0; r < 8; ++r) zs[pr][XorMap::at(lane + 64 * r)] = mk(v[r].x, isg * v[r].y);
+        for (int r = 0; r < 8; ++r) zs[pr][XorMap::col(lane, r)] = mk(v[r].x, isg * v[r].y);
     }
     __syncthreads();
     const float s = 1.f / (512.f * 512.f);
@@ -1045,7 +1062,7 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, int nblk, int per, 
     if (pr == 0) {
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
-            const int i = XorMap::at(lane + 64 * r);
+            const int i = XorMap::col(lane, r);
             const cpx<float> z0 = zs[0][i], z1 = zs[1][i];
             const float u = z0.x, v = z0.y, w = z1.x;
             p[r] = mk(s * (u * u + v * v - 2.f * w * w), s * (u * u - v * v));
@@ -1053,7 +1070,7 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, int nblk, int per, 
     } else if (pr == 1) {
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
-            const int i = XorMap::at(lane + 64 * r);
+            const int i = XorMap::col(lane, r);
             const cpx<float> z0 = zs[0][i], z1 = zs[1][i];
             const float u = z0.x, v = z0.y, w = z1.x;
             p[r] = mk(s * u * v, s * u * w);
@@ -1061,7 +1078,7 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, int nblk, int per, 
     } else {
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
-            const int i = XorMap::at(lane + 64 * r);
+            const int i = XorMap::col(lane, r);
             const cpx<float> z0 = zs[0][i], z1 = zs[1][i], z2 = zs[2][i];
             const float u = z0.x, v = z0.y, w = z1.x, tx = z1.y, ty = z2.x, tz = z2.y;
             p[r] = mk(s * v * w, s * (u * tx + v * ty + w * tz));
@@ -1078,7 +1095,7 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, int nblk, int per, 
     __syncthreads();  // every wave has read the slots before they become scratch
     fft512_wave<float, false, 1, XorMap, TwByPass>(p, zs[pr], tw, lane);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) zs[pr][XorMap::at(lane + 64 * r)] = p[r];
+    for (int r = 0; r < 8; ++r) zs[pr][XorMap::col(lane, r)] = p[r];
     wave_sync();
     cpx<float>* o1 = hout + (size_t)(2 * pr) * fstride + roff;
     cpx<float>* o2 = hout + (size_t)(2 * pr + 1) * fstride + roff;
@@ -1087,7 +1104,7 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, int nblk, int per, 
     for (int r = 0; r < 3; ++r) {
         const int k = lane + 64 * r;
         if (r < 2 || k < KX) {
-            const cpx<float> zk = zs[pr][XorMap::at(k)], zm = zs[pr][XorMap::at((512 - k) & 511)];
+            const cpx<float> zk = zs[pr][XorMap::col(lane, r)], zm = zs[pr][XorMap::at((512 - k) & 511)];
             o1[hyb512(0, k)] = mk(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
             o2[hyb512(0, k)] = mk(0.5f * isg * (zk.y + zm.y), -0.5f * isg * (zk.x - zm.x));
         }
