@@ -1235,7 +1235,7 @@ __global__ void __launch_bounds__(64 * WPB) k_rows_w_batch(const k_rows_w_args* 
 
 
 // y inverse at N = 512: a block of 8 waves handles YT = 8 kx columns of one (field, level) at a
-// time (a tile), one column per wave.  The LDS tile [pad8(y)][YT + 1] holds the results and
+// time (a tile), one column per wave.  The LDS tile [XorMap::at(y)][YT + 1] holds the results and
 // doubles as the FFT scratch of each column; it is written out row-wise (64-byte segments).
 // The blocks stay resident: the tiles are split into 8 contiguous ranges, one per XCD (block b
 // runs on XCD b % 8), and the bpx blocks of an XCD walk their range together, so that the tiles
@@ -1243,11 +1243,11 @@ __global__ void __launch_bounds__(64 * WPB) k_rows_w_batch(const k_rows_w_args* 
 // its column of the next tile before it transforms the current one.
 constexpr int YT = 8;
 constexpr int YS = YT + 1;
-__global__ void __launch_bounds__(512) k_fft_y_inv512(DevGeom g, int ntile, int nblk, int bpx,
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) k_fft_y_inv512(DevGeom g, int ntile, int nblk, int bpx,
                                                       const cpx<float>* __restrict__ lev,
                                                       const cpx<float>* __restrict__ tw,
                                                       cpx<float>* __restrict__ hin) {
-    __shared__ cpx<float> tile[576 * YS];
+    __shared__ cpx<float> tile[XorMap::slots * YS];
     __shared__ cpx<float> twl[TwByPass::size];  // in LDS, so that no table entry stays in registers
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int NYc = NYC512;
@@ -1279,9 +1279,9 @@ __global__ void __launch_bounds__(512) k_fft_y_inv512(DevGeom g, int ntile, int 
 #pragma unroll
         for (int r = 0; r < 8; ++r) v[r] = nxt[r];
         if (lb + bpx < end) load(lb + bpx, nxt);
-        fft512_wave<float, true, YS, Pad8Map, TwByPass>(v, tile + wave, twl, lane);
+        fft512_wave<float, true, YS, XorMap, TwByPass>(v, tile + wave, twl, lane);
 #pragma unroll
-        for (int r = 0; r < 8; ++r) tile[pad8(lane + 64 * r) * YS + wave] = v[r];
+        for (int r = 0; r < 8; ++r) tile[XorMap::col(lane, r) * YS + wave] = v[r];
         lds_barrier();
         const int tix = lb % ntile, rest = lb / ntile;
         // the tile is one contiguous 32 KiB piece of the tile-major plane; every lane stores on
@@ -1291,7 +1291,7 @@ __global__ void __launch_bounds__(512) k_fft_y_inv512(DevGeom g, int ntile, int 
 #pragma unroll
         for (int i = 0; i < YT; ++i) {
             const int e = threadIdx.x + 512 * i, y = e / YT;
-            dst[e] = tile[pad8(y) * YS + e % YT];
+            dst[e] = tile[XorMap::at(y) * YS + e % YT];
         }
         lds_barrier();  // the tile columns are the next tile's FFT scratch
     }
@@ -1769,7 +1769,7 @@ __global__ void __launch_bounds__(64 * CT) k_fft_y_fwd512c(DevGeom g, int ntile,
     // groups of several products read the twiddles from LDS: the barrier between products keeps
     // the compiler from holding the table entries in registers across the transforms
     constexpr bool TWL = G::n > 1;
-    __shared__ cpx<float> tile[576 * CS];
+    __shared__ cpx<float> tile[XorMap::slots * CS];
     __shared__ cpx<float> twl[TWL ? TwByPass::size : 1];
     const int lb = xcd_logical_block(blockIdx.x, per);
     if (lb >= nblk) return;
@@ -1814,15 +1814,15 @@ __global__ void __launch_bounds__(64 * CT) k_fft_y_fwd512c(DevGeom g, int ntile,
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int e = threadIdx.x + 64 * CT * i, y = e / CT, cc = e - y * CT;
-            tile[pad8(y) * CS + cc] = pre[i];
+            tile[XorMap::at(y) * CS + cc] = pre[i];
         }
         __syncthreads();
         if (iq + 1 < G::n) fetch(G::q(iq + 1));
         cpx<float> v[8];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) v[r] = tile[pad8(lane + 64 * r) * CS + c];
-        if constexpr (TWL) fft512_wave<float, false, CS, Pad8Map, TwByPass>(v, tile + c, twl, lane);
-        else fft512_wave<float, false, CS>(v, tile + c, tw, lane);
+        for (int r = 0; r < 8; ++r) v[r] = tile[XorMap::col(lane, r) * CS + c];
+        if constexpr (TWL) fft512_wave<float, false, CS, XorMap, TwByPass>(v, tile + c, twl, lane);
+        else fft512_wave<float, false, CS, XorMap>(v, tile + c, tw, lane);
 #pragma unroll
         for (int r = 0; r < 8; ++r) G::add(iq, v[r], kxv, kyv[r], d0[r], d1[r], d2[r]);
         if ((GRP == 1 || GRP == 3) && iq < 2 && kxc == 0 && lane == 0) fgw[2 * l + iq] = v[0];  // P_uw, P_vw, row 0
