@@ -47,6 +47,37 @@ def stage_bytes(st, cs, nprod):
     }
 
 
+FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak (half of it without packed FP32)
+
+
+def stage_flops(st):
+    """Algorithmic FP32 operations per step of the x-row stage: per (level, y) row 6 inverse and
+    6 forward real 512-point transforms (2.5 M log2 M each) and the products and combinations
+    (uu, vv, ww, uv, uw, vw, u.grad(theta), h = uu + vv - 2 ww, d = uu - vv: 15 per point)."""
+    M, N = WORKLOAD["M"], WORKLOAD["N"]
+    lg = M.bit_length() - 1
+    rows = st.nq * N
+    return {"rows_products": rows * (12 * 2.5 * M * lg + 15 * M)}
+
+
+def fft_product_stage(st, stages, cs):
+    """The FFT + product stage (inverse y-FFT, x rows with the products, forward y-FFT with the
+    level combinations) against HBM: algorithmic bytes (F_in + F_out) Nq K cs with K = (M/2+1) N
+    (the fused formula), for this formulation (F_in = 6 level fields, F_out = 5 level
+    combinations, Nq = the level count) and for the canonical one (F_in = 4, F_out = 9, Nq = 48)."""
+    K = (WORKLOAD["M"] // 2 + 1) * WORKLOAD["N"]
+    names = ("fft_y_inv", "rows_products", "fft_y_fwd")
+    ms = sum(stages[k] for k in names)
+    out = {"stages": list(names), "stage_ms": round(ms, 4), "K": K, "bytes_per_complex": cs}
+    for tag, fi, fo, nq in (("this", 6, 5, st.nq), ("canonical", 4, 9, 48)):
+        b = (fi + fo) * nq * K * cs
+        gbs = b / (ms * 1e-3) / 1e9
+        out[tag] = {"F_in": fi, "F_out": fo, "Nq": nq, "bytes": b, "achieved_GBs": round(gbs, 1),
+                    "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    out["frac"] = out["this"]["frac"]
+    return out
+
+
 # kernels of each stage on the 512 x 512 FP32 path (BF16X3 runs the <1> variants of the two
 # matrix-core kernels, with the same memory traffic)
 STAGE_KERNELS = {"eval": ["k_eval_mma<0, false>"], "fft_y_inv": ["k_fft_y_inv512"], "rows_products": ["k_rows512t"],
@@ -97,14 +128,30 @@ def copy_probe(mib):
     hip.hipFree(dst)
 
 
-def cpu_baseline(threads, budget_s=20.0):
-    """Oracle (FP64 C restatement of the reference algorithm) on this host: one step of the
-    same workload with an evenly strided sample of the nonlinear term's (i,j) pair loop,
-    extrapolated to all L^2 pairs."""
+def host_cpu_info():
+    """nproc, the usable CPUs of this process, the CPU model and the socket count (from
+    /proc/cpuinfo, as lscpu reports them)."""
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        model, sockets = None, set()
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name") and model is None:
+                model = line.split(":", 1)[1].strip()
+            elif line.startswith("physical id"):
+                sockets.add(line.split(":", 1)[1].strip())
+        info["model"] = model
+        info["sockets"] = len(sockets) or None
+    except OSError:
+        pass
+    return info
+
+
+def _c3_sampled(threads, budget_s):
+    """One 512^2 x 32 oracle step with an evenly strided sample of the nonlinear term's (i, j)
+    pair loop, extrapolated to all L^2 pairs (the full step takes minutes)."""
     from oracle import oracle as O
     from sdc_amd.initial import random_theta
     W = WORKLOAD
-    O.build()
     S = O.Stepper(W["M"], W["N"], W["L"], W["Lx"], W["Ly"], W["Ra"], W["Pr"], W["dt"], nthreads=threads)
     S.set_state(*random_theta(W["M"], W["N"], W["L"], seed=3, amp=1e-3))
     # calibrate with a coarse sample, then pick the stride that fits the budget
@@ -116,14 +163,54 @@ def cpu_baseline(threads, budget_s=20.0):
     stride = max(1, L2 // n_pairs)
     r = O.time_step_sampled(S, stride)
     S.close()
+    return r, stride
+
+
+def _oracle_steps(cfg, state, steps, threads):
+    """Wall seconds per step of the oracle on a configuration (setup excluded)."""
+    from oracle import oracle as O
+    S = O.Stepper(*cfg, nthreads=threads)
+    S.set_state(*state)
+    t0 = time.perf_counter()
+    S.step(steps)
+    dt = time.perf_counter() - t0
+    S.close()
+    return dt / steps
+
+
+def cpu_baseline(threads, budget_s=15.0, c1_steps=100, c2_steps=2):
+    """The oracle (FP64 C restatement of the reference algorithm, OpenMP) on this host, on all
+    `threads` cores and on one thread: the headline workload as a sampled step, and the two
+    smaller configurations (32^2 x 8 Ra 5000, 128^2 x 16 Ra 1e5) as whole steps."""
+    from oracle import oracle as O
+    from sdc_amd.initial import low_mode_theta, random_theta
+    O.build()
+    W = WORKLOAD
+    r, stride = _c3_sampled(threads, budget_s)
+    r1, stride1 = _c3_sampled(1, budget_s)
+    c1 = (32, 32, 8, 4.032, 4.032, 5000.0, 1.0, 1e-3)
+    c2 = (128, 128, 16, 4.0, 4.0, 1e5, 0.7, 2e-4)
+    s1 = low_mode_theta(32, 32, 8, seed=1, amp=1e-3)
+    s2 = random_theta(128, 128, 16, seed=2, amp=1e-3)
+    small = {}
+    for name, cfg, st, n in (("32x32x8", c1, s1, c1_steps), ("128x128x16", c2, s2, c2_steps)):
+        small[name] = {"steps_timed": n,
+                       "timesteps_per_s": {f"{t}_threads": round(1.0 / _oracle_steps(cfg, st, n, t), 3)
+                                           for t in sorted({threads, 1})}}
     return {
-        "value": 1.0 / r["s_per_step"],
+        "value": round(1.0 / r["s_per_step"], 6),
         "unit": "timesteps/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"one 512x512x32 step (FP64 oracle, OpenMP {threads} threads) with every {stride}-th of the "
-                   f"{L2} (i,j) pairs of the nonlinear term, extrapolated; phases s: physical fields "
-                   f"{r['phys']:.2f}, sampled pairs {r['pairs_sampled']:.2f}, rest {r['rest']:.2f}"),
+        "sample": (f"one {W['M']}x{W['N']}x{W['L']} step of the FP64 oracle (OpenMP, {threads} threads) with every "
+                   f"{stride}-th of the {W['L'] ** 2} (i,j) pairs of the nonlinear term, extrapolated to all pairs "
+                   f"(sampled, not a whole step); phases s: physical fields {r['phys']:.2f}, sampled pairs "
+                   f"{r['pairs_sampled']:.2f}, rest {r['rest']:.2f}"),
+        "single_thread": {"value": round(1.0 / r1["s_per_step"], 6), "unit": "timesteps/s", "cores": 1,
+                          "sample": f"as above on one thread, every {stride1}-th pair"},
+        "whole_steps_other_configs": small,
+        "host": host_cpu_info(),
+        "threads_note": "threads = the GPU box's CPU share for one GPU (16), capped by the usable CPUs",
     }
 
 
@@ -452,32 +539,71 @@ def main():
                 tsync = torch.cuda.synchronize
         except Exception:
             pass
+    seed_state = random_theta(W["M"], W["N"], W["L"], seed=3, amp=1e-3)
+
+    def reseed(s):
+        """Fresh seeded state and an Euler start: each timed phase starts finite (the explicit
+        advection at dt = 5e-5 leaves the stable range after about 200 steps from this seed;
+        the cost of a step does not depend on the data)."""
+        s.set_state(*seed_state)
+        s.reset_history()
+
+    def finite_after(s):
+        from sdc_amd import RbcNumericError
+        try:
+            s.synchronize()
+            return True
+        except RbcNumericError:
+            return False
+
+    # timed region: plain rbc_step, no events inside, bracketed by stream + device waits
     st.step_time(a.warmup)
     st.synchronize()
+    reseed(st)
     tsync()
     t0 = time.perf_counter()
-    stages = st.profile_steps(a.steps)  # K steps, HIP events around every stage on the library's stream
-    st.synchronize()
+    st.step_time(a.steps)
+    finite = finite_after(st)
     tsync()
     t1 = time.perf_counter()
     sec = t1 - t0
     value = a.steps / sec
+    # a separate pass with HIP events around every stage on the library's stream
+    reseed(st)
+    nprof = min(a.steps, 100)
+    stages = st.profile_steps(nprof)
+    finite_prof = finite_after(st)
 
     cs = 16 if a.precision == "f64" else 8
     nprod = 6
     nbytes = stage_bytes(st, cs, nprod)
     dom = max(stages, key=lambda k: stages[k])
     achieved = nbytes[dom] / (stages[dom] * 1e-3) / 1e9
-    traffic = measured_traffic(dom) if a.precision != "f64" and not a.grid else None
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
-                "valu_issue_per_cu_cycle": measured_valu_issue(dom) if a.precision != "f64" and not a.grid else None,
-                "algorithmic_bytes": nbytes[dom],
-                "stage_ms": {k: round(v, 4) for k, v in stages.items()},
-                "stage_GBs": {k: round(nbytes[k] / (stages[k] * 1e-3) / 1e9, 1) for k in stages if stages[k] > 0}}
+    full = a.precision != "f64" and not a.grid
+    traffic = measured_traffic(dom) if full else None
+    issue = measured_valu_issue(dom) if full else None
+    hbm = {"achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+           "algorithmic_bytes": nbytes[dom]}
+    # the dominant kernel's bound from its measured VALU issue rate (rocprofv3 SQ counters): at
+    # >= 0.8 wave64 VALU instructions per CU-clock it is the vector ALU that limits it
+    valu_bound = issue is not None and max(issue.values()) >= 0.8
+    if valu_bound:
+        fl = stage_flops(st)[dom]
+        tf = fl / (stages[dom] * 1e-3) / 1e12
+        roofline = {"bound": "valu", "achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(tf / FP32_PEAK_TFLOPS, 4), "algorithmic_flops": fl, "hbm": hbm}
+    else:
+        roofline = dict(hbm, bound="hbm")
+    roofline.update({"traffic": traffic, "kernel": dom, "valu_issue_per_cu_cycle": issue,
+                     "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+                     "stage_GBs": {k: round(nbytes[k] / (stages[k] * 1e-3) / 1e9, 1)
+                                   for k in stages if stages[k] > 0},
+                     "fft_product_stage": fft_product_stage(st, stages, cs),
+                     "stage_events_steps": nprof})
     nq, kx, ky, ng = st.nq, st.kx_kept, st.ky_kept, st.n_unique_k
     # the boundary's host buffers: one state in (reference layout, FP64 host arrays), one step, one
     # state out, including PCIe and the host-side packing; reported beside `value`, which excludes it
+    reseed(st)
     state = st.get_state()
     st.synchronize()
     tb = time.perf_counter()
@@ -485,6 +611,27 @@ def main():
     st.step_time(1)
     state = st.get_state()
     boundary_ms = 1e3 * (time.perf_counter() - tb)
+    # the same exchange on device buffers in the reference layout (rbc_set_state_device /
+    # rbc_get_state_device): state in and out without a step, mean of 20
+    dev_boundary_us = None
+    try:
+        import torch
+        if torch.cuda.is_available():
+            ct = torch.complex128 if a.precision == "f64" else torch.complex64
+            bufs = [torch.from_numpy(x).to("cuda:0", dtype=ct) for x in state]
+            torch.cuda.synchronize()
+            ptrs = [t.data_ptr() for t in bufs]
+            st.set_state_device(*ptrs)
+            st.get_state_device(*ptrs)
+            st.synchronize()
+            tb = time.perf_counter()
+            for _ in range(20):
+                st.set_state_device(*ptrs)
+                st.get_state_device(*ptrs)
+            st.synchronize()
+            dev_boundary_us = 1e6 * (time.perf_counter() - tb) / 20
+    except ImportError:
+        pass
     st.close()
     # the other single-GPU precision forms on the same workload (plain step_time, no stage events)
     variants = {}
@@ -508,7 +655,7 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = a.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         cpu = cpu_baseline(threads)
 
     out = {
@@ -532,6 +679,9 @@ def main():
         "cpu_baseline": cpu,
         "precision_variants": variants,
         "boundary_roundtrip_ms": round(boundary_ms, 2),
+        "device_boundary_roundtrip_us": round(dev_boundary_us, 1) if dev_boundary_us is not None else None,
+        "state_finite_after_timed_steps": finite,
+        "state_finite_after_profiled_steps": finite_prof,
     }
     print(json.dumps(out))
 

@@ -72,7 +72,14 @@ int rbc_get_state(rbc_ctx* c, double* theta, double* f, double* g, double* F, do
 
 /* ≙ implicit_explicit_step::step_time + make_real_signal, nsteps times (main.cu:612-616).
  * The context owns the AB2 history; the first step after create/reset is Euler
- * (implicit_explicit_stepping.cu:43-52). */
+ * (implicit_explicit_stepping.cu:43-52).
+ * Non-finite state (≙ find_NaN, util/util.cu:83-105, which the reference defines but never
+ * calls): the end-of-step kernel records the first step that ends with a non-finite value;
+ * rbc_synchronize, rbc_get_state and rbc_nusselt then return RBC_ERR_NUMERIC (the text names
+ * the step, counted from rbc_create; get_state still copies the state out), and rbc_step
+ * refuses with RBC_ERR_NUMERIC until rbc_set_state or rbc_read_state_files replaces the state
+ * (reset the AB2 history too: it holds the non-finite step's terms).  rbc_step enqueues and
+ * does not wait, so the report comes from the next call that waits. */
 int rbc_step(rbc_ctx* c, int nsteps);
 int rbc_reset_history(rbc_ctx* c);
 
@@ -93,6 +100,53 @@ int rbc_implicit(rbc_ctx* c, const double* theta, const double* f, const double*
                  const double* F, const double* G, double* o_theta, double* o_f, double* o_g,
                  double* o_F, double* o_G);
 
+/* ---- Device-resident exchange --------------------------------------------------------------
+ * The same calls on device memory of the context's GPU, in the reference layout above with
+ * the element type of the context's precision: complex128 (interleaved doubles) for RBC_F64,
+ * complex64 (interleaved floats) for RBC_F32 and RBC_BF16X3 -- the matrix_folder data the
+ * reference's operators take and return (nonlinear_operator_rayleigh_noslip.h:56,
+ * inverse_B_matrix.h:18, linear_implicit_operator.h:65; implicit_explicit_stepping.cu:32-97).
+ * Work is enqueued on the context's stream (rbc_stream) and the call returns without waiting:
+ * inputs must be ready before the call (synchronise the producing stream first) and outputs
+ * are ready once rbc_synchronize returns.  No call allocates.  Outputs hold zeros in the
+ * removed band; set_state NULL fields are left unchanged, get_state NULL fields are skipped.
+ * rbc_set_state_device ≙ rbc_set_state (clears a recorded non-finite step). */
+int rbc_set_state_device(rbc_ctx* c, const void* theta, const void* f, const void* g, const void* F,
+                         const void* G);
+int rbc_get_state_device(rbc_ctx* c, void* theta, void* f, void* g, void* F, void* G);
+/* ≙ calculate_operator (nonlinear_operator_rayleigh_noslip.h:56) */
+int rbc_nonlinear_device(rbc_ctx* c, const void* theta, const void* f, const void* g, const void* F,
+                         const void* G, void* o_theta, void* o_f, void* o_g, void* o_F, void* o_G);
+/* ≙ calculate_inverse (inverse_B_matrix.h:18) */
+int rbc_inverse_B_device(rbc_ctx* c, const void* theta, const void* f, const void* g, const void* F,
+                         const void* G, void* o_theta, void* o_f, void* o_g, void* o_F, void* o_G);
+/* ≙ linear_implicit_operator::calculate_operator (linear_implicit_operator.h:65) */
+int rbc_implicit_device(rbc_ctx* c, const void* theta, const void* f, const void* g, const void* F,
+                        const void* G, void* o_theta, void* o_f, void* o_g, void* o_F, void* o_G);
+/* The context's HIP stream (a hipStream_t), for callers that order their own work after it. */
+void* rbc_stream(rbc_ctx* c);
+
+/* ---- Initial states and mean-flow forcing ---------------------------------------------------
+ * rbc_init_state ≙ the initial data main.cu:290-352 creates when no -files are given, followed
+ * by make_real_signal (main.cu:396); clears the AB2 history (the first step is Euler).
+ *   RBC_INIT_ZERO   every field zero
+ *   RBC_INIT_RANDOM matrix_device_init::init_random (init.cu:9-20, 348-365): every theta
+ *                   coefficient uniform in [-amplitude, amplitude] (real and imaginary part;
+ *                   the reference's default amplitude is 1e-6), f, g, F, G zero.  The numbers
+ *                   come from SplitMix64 counters on (seed, reference-layout index): cuRAND's
+ *                   XORWOW stream cannot be reproduced.
+ *   RBC_INIT_TEST   matrix_device_init::init_test (init.cu:263-310): theta and f hold
+ *                   1e-6 sin(kx) at ky = 0, kx != 0; F_m = 1e-6 cos(m); g = G = 0 (amplitude and
+ *                   seed unused).  The one initial state that needs no random numbers.
+ * Only the kept modes are stored (the stepper dealiases on entry). */
+#define RBC_INIT_ZERO 0
+#define RBC_INIT_RANDOM 1
+#define RBC_INIT_TEST 2
+int rbc_init_state(rbc_ctx* c, int kind, unsigned long long seed, double amplitude);
+/* ≙ force_mean (util/util.cu:74-81), which main.cu:567-568 applies before every step under
+ * -mean_F: F_0 = mean_F + 0i.  Enqueued on the context's stream. */
+int rbc_force_mean(rbc_ctx* c, double mean_F);
+
 /* Diagnostics (FP64 reductions on the device; the reference computes none):
  *   nu[0] plate Nusselt number at z=-1/2, nu[1] at z=+1/2, nu[2] = 1 + <w theta>. */
 int rbc_nusselt(rbc_ctx* c, double* nu3);
@@ -104,7 +158,8 @@ int rbc_nusselt(rbc_ctx* c, double* nu3);
 int rbc_profile_steps(rbc_ctx* c, int nsteps, double* ms);
 const char* rbc_stage_name(int i);
 
-/* Blocks until all work queued on the context's stream has finished. */
+/* Blocks until all work queued on the context's stream has finished; RBC_ERR_NUMERIC once a
+ * step has ended non-finite (see rbc_step). */
 int rbc_synchronize(rbc_ctx* c);
 
 /* ---- State files in the reference's binary format ---------------------------------------
@@ -129,12 +184,24 @@ int rbc_set_history(rbc_ctx* c, const double* theta, const double* f, const doub
 /* ---- Velocity operator ------------------------------------------------------------------
  * ≙ calculate_velocity_operator::calculate_operator_at (calculate_velocity_operator.cu:
  * 106-158): u_x, u_y, u_z of the current state on the M x N grid at nz heights z (clamped to
- * [-1/2, 1/2]), FP32, layout [l][y][x] per component.  Allocates scratch for the call. */
+ * [-1/2, 1/2]), FP32, layout [l][y][x] per component.  The scratch (3 nz M N floats and the
+ * level spectra) belongs to the context and grows only when nz exceeds every earlier call's. */
 int rbc_velocity_grid(rbc_ctx* c, int nz, const double* z, float* ux, float* uy, float* uz);
+/* Characteristic velocity U = <|u|> of the current state (main_extract_vel.cu:182-218): the mean
+ * of |u| over the M x N grid at nz heights linspace(-1/2, 1/2, nz) (the reference tool uses
+ * nz = 10), components in FP32 as the reference's velocity operator gives them, squared, summed
+ * and averaged in FP64, points with a NaN skipped.  Uses the rbc_velocity_grid scratch. */
+int rbc_characteristic_velocity(rbc_ctx* c, int nz, double* U);
 /* The velocity of the current state at n arbitrary points, summed exactly over the kept
- * modes (no counterpart in the reference; the spectral tracer's evaluation).  L <= 32. */
+ * modes (no counterpart in the reference; the spectral tracer's evaluation).  L <= 32.
+ * Host arrays; the context's point scratch grows only when n exceeds every earlier call's. */
 int rbc_velocity_at(rbc_ctx* c, int n, const float* x, const float* y, const float* z,
                     float* u, float* v, float* w);
+/* rbc_velocity_at on device arrays of the context's GPU, enqueued on its stream without a
+ * wait (results ready after rbc_synchronize); the partial sums live in the context's point
+ * scratch, grown (with a wait) when n exceeds every earlier call's. */
+int rbc_velocity_at_device(rbc_ctx* c, int n, const float* x, const float* y, const float* z,
+                           float* u, float* v, float* w);
 
 /* ---- Passive tracers (≙ particle_tracer, particle/particle_tracer.h:21-89) ----------------
  * Positions are FP32, stored as a ring of buffer_size time slots (the reference's layout
@@ -194,8 +261,10 @@ int rbc_energy_spectrum_ref(rbc_ctx* c, double z, int cap, double* E3, int* nbin
  * (run/script_onset_sdc.sh:8).  Members may differ in M, N, L, Ra, Pr, dt and precision but
  * share one device; each member is a full context (rbc_ensemble_member gives it for
  * rbc_set_state, rbc_get_state, rbc_nusselt, ...; it stays owned by the ensemble).
- * rbc_ensemble_step advances every member by nsteps: a member without AB2 history takes its
- * Euler step eagerly, then steps_per_launch steps of all members are captured once into one
+ * rbc_ensemble_step advances every member by nsteps: when a member has no AB2 history, the
+ * first of the nsteps runs eagerly for every member (an Euler step for members without
+ * history, an AB2 step for the others), then steps_per_launch steps of all members are
+ * captured once into one
  * HIP graph and replayed (steps_per_launch = 0: eager launches).  Results equal stepping each
  * member alone.  rbc_ensemble_tune times launch shapes of the generic FFT kernels per size
  * class (M, N, L, precision) on the first member of the class (its state is restored) and
@@ -268,6 +337,20 @@ int rbc_closure_enable(rbc_closure* cl, int on, double max_ratio);
 int rbc_closure_interval(rbc_closure* cl, int every);
 int rbc_closure_step(rbc_closure* cl, int nsteps, int check_every);
 int rbc_closure_status(const rbc_closure* cl, int* active, int* fallback, double* last_ratio);
+
+/* ---- Simulation driver ---------------------------------------------------------------------
+ * ≙ main() of gpu_flow/code/main.cu: the reference's argument set (main.cu:61-229; -delta_t and
+ * -final_t, the names README.txt:38-39 lists, are accepted for -dt and -t), iCeil(t/dt) steps
+ * (main.cu:279), -mean_F forcing before each step (main.cu:567-568), tracers per -le value,
+ * state files theta_%08d.bin .. G_%08d.bin every steps/nfiles steps (main.cu:697-728), tracks
+ * (.trkss) and second-moment files, simulation.log; plus -precision f32|f64|bf16x3,
+ * -tracer_mode rk4|spectral, -write_history / -history_files, -nq.  Returns the process exit
+ * status (0, or 1 with the message on stderr).  argv[0] is the program name. */
+int rbc_run_main(int argc, const char* const* argv);
+/* ≙ main() of gpu_flow/code/main_extract_vel.cu for its characteristic-velocity output:
+ * -clength, -files (five paths), -write_charvel_nz n > 0 with -outfile_charvel writes <|u|> over
+ * 10 heights (the reference tool's fixed count) as a data-point file; -description. */
+int rbc_extract_vel_main(int argc, const char* const* argv);
 
 #ifdef __cplusplus
 }

@@ -105,6 +105,25 @@ def dealias(field):
     return field
 
 
+def init_test_state(M, N, L, fp32=False):
+    """init_test_device (util/init.cu:263-301) on (L, N, M/2+1) arrays: theta and f hold
+    1e-6 sin(x) (real part) in row y = 0 for columns x != 0, g = 0, F_z = 1e-6 cos(z), G = 0,
+    evaluated in the build's real type (float32 for an FP32 build)."""
+    mox = M // 2 + 1
+    rt = np.float32 if fp32 else np.float64
+    amp = rt(1e-6)
+    row = np.zeros(mox, np.complex128)
+    x = np.arange(1, mox)
+    row[1:] = (amp * np.sin(x.astype(rt))).astype(np.float64)
+    th = np.zeros((L, N, mox), np.complex128)
+    th[:, 0, :] = row[None, :]
+    f = th.copy()
+    g = np.zeros_like(th)
+    F = (amp * np.cos(np.arange(L).astype(rt))).astype(np.float64).astype(np.complex128)
+    G = np.zeros(L, np.complex128)
+    return [th, f, g, F, G]
+
+
 def make_real(field):
     L, N, mox = field.shape
     lib().orc_make_real(mox, N, L, _p(field.view(np.float64)))

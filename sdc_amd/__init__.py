@@ -4,6 +4,7 @@ The compute path is libsdc_rbc.so (hand-written HIP for gfx950 behind the C-ABI 
 include/rbc_abi.h); this package holds its sources (csrc/), the build script and the
 Python host binding.
 """
-from .rbc import Closure, Ensemble, Stepper, Tracer, RbcError, load_library, LIB_PATH  # noqa: F401
+from .rbc import (Closure, Ensemble, Stepper, Tracer, RbcError, RbcNumericError, load_library,  # noqa: F401
+                  LIB_PATH)
 
-__all__ = ["Closure", "Ensemble", "Stepper", "Tracer", "RbcError", "load_library", "LIB_PATH"]
+__all__ = ["Closure", "Ensemble", "Stepper", "Tracer", "RbcError", "RbcNumericError", "load_library", "LIB_PATH"]

@@ -13,8 +13,10 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsdc_rbc.so")
 
-SOURCES = ["rbc_setup.cpp", "rbc_abi.cpp", "rbc_io.cpp", "rbc_ensemble.cpp", "rbc_kernels.hip",
+SOURCES = ["rbc_setup.cpp", "rbc_abi.cpp", "rbc_io.cpp", "rbc_ensemble.cpp", "rbc_driver.cpp", "rbc_kernels.hip",
            "rbc_velocity.hip", "rbc_closure.hip"]
+RUN_EXE = os.path.join(HERE, "rbc_run")
+EXTRACT_EXE = os.path.join(HERE, "rbc_extract_vel")
 ARCH = os.environ.get("SDC_AMD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -44,8 +46,16 @@ def build(jobs=4, verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(LIB + ".tmp", LIB)
+    # the executables: main()s around rbc_run_main and rbc_extract_vel_main, next to the library
+    for src, exe in (("rbc_run.cpp", RUN_EXE), ("rbc_extract_vel.cpp", EXTRACT_EXE)):
+        cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, src),
+               "-L", HERE, "-lsdc_rbc", "-Wl,-rpath,$ORIGIN", "-o", exe + ".tmp"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(exe + ".tmp", exe)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, RUN_EXE, EXTRACT_EXE)
     return LIB
 
 

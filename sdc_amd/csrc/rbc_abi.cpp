@@ -172,6 +172,23 @@ void run_steps(rbc_ctx* c, int n) {
     }
 }
 
+int health_status(rbc_ctx* c) {
+    int h[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(h, c->b.health, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (h[0] > 0 && c->bad_step == 0) c->bad_step = h[0];
+    if (c->bad_step > 0)
+        return fail(c, RBC_ERR_NUMERIC,
+                    "non-finite state: step " + std::to_string(c->bad_step) +
+                        " (counted from context creation) ended with a non-finite value (find_NaN)");
+    return RBC_OK;
+}
+
+void clear_health(rbc_ctx* c) {
+    c->bad_step = 0;
+    HIPCHK(hipMemsetAsync(c->b.health, 0, sizeof(int), c->stream));
+}
+
 int fail(rbc_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
     else g_create_error = msg;
@@ -406,10 +423,13 @@ int rbc_create(const rbc_params* p, rbc_ctx** out) {
         upload_reals(c.get(), b.gmat, c->fac.mats);
         b.nred = (int)((P + 255) / 256) + 2;
         b.red = (double*)c->dalloc((size_t)b.nred * sizeof(double));
+        b.health = (int*)c->dalloc(2 * sizeof(int));
+        HIPCHK(hipMemset(b.health, 0, 2 * sizeof(int)));
         std::vector<double> iac, ia1;
         closed_form_iac_ia1(p->L, iac, ia1);
         c->ia1 = c->dalloc(ia1.size() * sizeof(double));
         HIPCHK(hipMemcpy(c->ia1, ia1.data(), ia1.size() * sizeof(double), hipMemcpyHostToDevice));
+        velocity_setup(c.get());
         HIPCHK(hipDeviceSynchronize());
     } catch (const HipError& e) {
         return fail(nullptr, RBC_ERR_HIP, std::string(e.what) + ": " + hipGetErrorString(e.e));
@@ -446,6 +466,7 @@ int rbc_set_state(rbc_ctx* c, const double* th, const double* f, const double* g
         if (g) put_field(c, g, c->b.X[2]);
         if (F) put_vec(c, F, c->b.XF);
         if (G) put_vec(c, G, c->b.XG);
+        clear_health(c);
     } catch (const HipError& e) {
         return hip_fail(c, e);
     }
@@ -461,14 +482,18 @@ int rbc_get_state(rbc_ctx* c, double* th, double* f, double* g, double* F, doubl
         if (g) get_field(c, c->b.X[2], g);
         if (F) get_vec(c, c->b.XF, F);
         if (G) get_vec(c, c->b.XG, G);
+        return health_status(c);  // the state is copied out either way
     } catch (const HipError& e) {
         return hip_fail(c, e);
     }
-    return RBC_OK;
 }
 
 int rbc_step(rbc_ctx* c, int nsteps) {
     if (!c || nsteps < 0) return RBC_ERR_ARG;
+    if (c->bad_step > 0)
+        return fail(c, RBC_ERR_NUMERIC,
+                    "refusing to step a non-finite state (step " + std::to_string(c->bad_step) +
+                        " ended non-finite); replace the state (rbc_set_state) and reset the AB2 history first");
     try {
         hipSetDevice(c->p.device);
         run_steps(c, nsteps);
@@ -487,14 +512,52 @@ int rbc_reset_history(rbc_ctx* c) {
 int rbc_synchronize(rbc_ctx* c) {
     if (!c) return RBC_ERR_ARG;
     try {
-        HIPCHK(hipStreamSynchronize(c->stream));
+        hipSetDevice(c->p.device);
+        return health_status(c);
     } catch (const HipError& e) {
         return hip_fail(c, e);
     }
-    return RBC_OK;
 }
 
-static int op_call(rbc_ctx* c, int which, const double* th, const double* f, const double* g, const double* F,
+}  // extern "C"
+
+namespace {
+
+// The operator-level calls on the staging buffers: input in Xo / XoF / XoG, output in the
+// buffers returned through outs / oF / oG (Oo for the nonlinear term and inverse B, Xo for the
+// implicit solve, which works in place).
+template <typename T>
+void op_core_t(rbc_ctx* c, int which, void** outs, void** oF, void** oG) {
+    Buffers& b = c->b;
+    StepCoef sc{0.0, 0.0, 0};
+    for (int i = 0; i < 3; ++i) outs[i] = b.Oo[i];
+    *oF = b.OoF;
+    *oG = b.OoG;
+    if (which == 0) {
+        nonlinear_into<T>(c, b.Xo, b.XoF, b.XoG, b.Oo, b.OoF, b.OoG, -1);
+    } else if (which == 1) {  // inverse B: N in Xo -> R in Oo
+        Launch<T>::update(c->dg, b, c->fac.ngroups(), c->fac.L0, c->fac.L1, c->fac.stride, c->p.Pr, c->p.dt, sc,
+                          UPD_INVB, b.Xo, b.Xo, b.Oo, c->stream);
+        Launch<T>::update_FG(c->dg, c->p.Pr, c->p.dt, sc, UPD_INVB, b.XoF, b.XoG, b.XoF, b.XoG, b.OoF, b.OoG,
+                             c->stream);
+    } else {  // implicit: in place on Xo
+        Launch<T>::update(c->dg, b, c->fac.ngroups(), c->fac.L0, c->fac.L1, c->fac.stride, c->p.Pr, c->p.dt, sc,
+                          UPD_IMPLICIT, b.Xo, b.Xo, b.Oo, c->stream);
+        Launch<T>::update_FG(c->dg, c->p.Pr, c->p.dt, sc, UPD_IMPLICIT, b.XoF, b.XoG, b.XoF, b.XoG, b.OoF, b.OoG,
+                             c->stream);
+        for (int i = 0; i < 3; ++i) outs[i] = b.Xo[i];
+        *oF = b.XoF;
+        *oG = b.XoG;
+    }
+    HIPCHK(hipGetLastError());
+}
+
+void op_core(rbc_ctx* c, int which, void** outs, void** oF, void** oG) {
+    if (c->rsz == 8) op_core_t<double>(c, which, outs, oF, oG);
+    else op_core_t<float>(c, which, outs, oF, oG);
+}
+
+int op_call(rbc_ctx* c, int which, const double* th, const double* f, const double* g, const double* F,
                    const double* G, double* oth, double* of, double* og, double* oF, double* oG) {
     if (!c || !th || !f || !g || !F || !G || !oth || !of || !og || !oF || !oG) return RBC_ERR_ARG;
     try {
@@ -505,41 +568,9 @@ static int op_call(rbc_ctx* c, int which, const double* th, const double* f, con
         put_field(c, g, b.Xo[2]);
         put_vec(c, F, b.XoF);
         put_vec(c, G, b.XoG);
-        StepCoef sc{0.0, 0.0, 0};
-        void** outs = b.Oo;
-        void *oFp = b.OoF, *oGp = b.OoG;
-        if (which == 0) {
-            if (c->rsz == 8) nonlinear_into<double>(c, b.Xo, b.XoF, b.XoG, b.Oo, b.OoF, b.OoG, -1);
-            else nonlinear_into<float>(c, b.Xo, b.XoF, b.XoG, b.Oo, b.OoF, b.OoG, -1);
-        } else if (which == 1) {  // inverse B: N in Xo -> R in Oo
-            if (c->rsz == 8) {
-                Launch<double>::update(c->dg, b, c->fac.ngroups(), c->fac.L0, c->fac.L1, c->fac.stride, c->p.Pr,
-                                       c->p.dt, sc, UPD_INVB, b.Xo, b.Xo, b.Oo, c->stream);
-                Launch<double>::update_FG(c->dg, c->p.Pr, c->p.dt, sc, UPD_INVB, b.XoF, b.XoG, b.XoF, b.XoG, b.OoF,
-                                          b.OoG, c->stream);
-            } else {
-                Launch<float>::update(c->dg, b, c->fac.ngroups(), c->fac.L0, c->fac.L1, c->fac.stride, c->p.Pr,
-                                      c->p.dt, sc, UPD_INVB, b.Xo, b.Xo, b.Oo, c->stream);
-                Launch<float>::update_FG(c->dg, c->p.Pr, c->p.dt, sc, UPD_INVB, b.XoF, b.XoG, b.XoF, b.XoG, b.OoF,
-                                         b.OoG, c->stream);
-            }
-        } else {  // implicit: in place on Xo
-            if (c->rsz == 8) {
-                Launch<double>::update(c->dg, b, c->fac.ngroups(), c->fac.L0, c->fac.L1, c->fac.stride, c->p.Pr,
-                                       c->p.dt, sc, UPD_IMPLICIT, b.Xo, b.Xo, b.Oo, c->stream);
-                Launch<double>::update_FG(c->dg, c->p.Pr, c->p.dt, sc, UPD_IMPLICIT, b.XoF, b.XoG, b.XoF, b.XoG,
-                                          b.OoF, b.OoG, c->stream);
-            } else {
-                Launch<float>::update(c->dg, b, c->fac.ngroups(), c->fac.L0, c->fac.L1, c->fac.stride, c->p.Pr,
-                                      c->p.dt, sc, UPD_IMPLICIT, b.Xo, b.Xo, b.Oo, c->stream);
-                Launch<float>::update_FG(c->dg, c->p.Pr, c->p.dt, sc, UPD_IMPLICIT, b.XoF, b.XoG, b.XoF, b.XoG,
-                                         b.OoF, b.OoG, c->stream);
-            }
-            outs = b.Xo;
-            oFp = b.XoF;
-            oGp = b.XoG;
-        }
-        HIPCHK(hipGetLastError());
+        void* outs[3];
+        void *oFp, *oGp;
+        op_core(c, which, outs, &oFp, &oGp);
         get_field(c, outs[0], oth);
         get_field(c, outs[1], of);
         get_field(c, outs[2], og);
@@ -550,6 +581,50 @@ static int op_call(rbc_ctx* c, int which, const double* th, const double* f, con
     }
     return RBC_OK;
 }
+
+// device buffers in the reference layout <-> the context's compact fields, on its stream
+void dev_in(rbc_ctx* c, const void* const f3[3], const void* F, const void* G, void* const X[3], void* XF,
+                   void* XG) {
+    if (c->rsz == 8) Launch<double>::from_ref(c->dg, f3, X, c->stream);
+    else Launch<float>::from_ref(c->dg, f3, X, c->stream);
+    HIPCHK(hipGetLastError());
+    const size_t vb = (size_t)c->dg.L * c->csz();
+    if (F) HIPCHK(hipMemcpyAsync(XF, F, vb, hipMemcpyDeviceToDevice, c->stream));
+    if (G) HIPCHK(hipMemcpyAsync(XG, G, vb, hipMemcpyDeviceToDevice, c->stream));
+}
+
+void dev_out(rbc_ctx* c, void* const X[3], const void* XF, const void* XG, void* const f3[3], void* F,
+                    void* G) {
+    if (c->rsz == 8) Launch<double>::to_ref(c->dg, X, f3, c->stream);
+    else Launch<float>::to_ref(c->dg, X, f3, c->stream);
+    HIPCHK(hipGetLastError());
+    const size_t vb = (size_t)c->dg.L * c->csz();
+    if (F) HIPCHK(hipMemcpyAsync(F, XF, vb, hipMemcpyDeviceToDevice, c->stream));
+    if (G) HIPCHK(hipMemcpyAsync(G, XG, vb, hipMemcpyDeviceToDevice, c->stream));
+}
+
+int op_call_device(rbc_ctx* c, int which, const void* th, const void* f, const void* g, const void* F,
+                          const void* G, void* oth, void* of, void* og, void* oF, void* oG) {
+    if (!c || !th || !f || !g || !F || !G || !oth || !of || !og || !oF || !oG) return RBC_ERR_ARG;
+    try {
+        hipSetDevice(c->p.device);
+        Buffers& b = c->b;
+        const void* in3[3] = {th, f, g};
+        dev_in(c, in3, F, G, b.Xo, b.XoF, b.XoG);
+        void* outs[3];
+        void *oFp, *oGp;
+        op_core(c, which, outs, &oFp, &oGp);
+        void* out3[3] = {oth, of, og};
+        dev_out(c, outs, oFp, oGp, out3, oF, oG);
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+}  // namespace
+
+extern "C" {
 
 int rbc_nonlinear(rbc_ctx* c, const double* th, const double* f, const double* g, const double* F, const double* G,
                   double* oth, double* of, double* og, double* oF, double* oG) {
@@ -562,6 +637,79 @@ int rbc_inverse_B(rbc_ctx* c, const double* th, const double* f, const double* g
 int rbc_implicit(rbc_ctx* c, const double* th, const double* f, const double* g, const double* F, const double* G,
                  double* oth, double* of, double* og, double* oF, double* oG) {
     return op_call(c, 2, th, f, g, F, G, oth, of, og, oF, oG);
+}
+
+int rbc_set_state_device(rbc_ctx* c, const void* th, const void* f, const void* g, const void* F, const void* G) {
+    if (!c) return RBC_ERR_ARG;
+    try {
+        hipSetDevice(c->p.device);
+        const void* in3[3] = {th, f, g};
+        dev_in(c, in3, F, G, c->b.X, c->b.XF, c->b.XG);
+        clear_health(c);
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+int rbc_get_state_device(rbc_ctx* c, void* th, void* f, void* g, void* F, void* G) {
+    if (!c) return RBC_ERR_ARG;
+    try {
+        hipSetDevice(c->p.device);
+        void* out3[3] = {th, f, g};
+        dev_out(c, c->b.X, c->b.XF, c->b.XG, out3, F, G);
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+int rbc_nonlinear_device(rbc_ctx* c, const void* th, const void* f, const void* g, const void* F, const void* G,
+                         void* oth, void* of, void* og, void* oF, void* oG) {
+    return op_call_device(c, 0, th, f, g, F, G, oth, of, og, oF, oG);
+}
+int rbc_inverse_B_device(rbc_ctx* c, const void* th, const void* f, const void* g, const void* F, const void* G,
+                         void* oth, void* of, void* og, void* oF, void* oG) {
+    return op_call_device(c, 1, th, f, g, F, G, oth, of, og, oF, oG);
+}
+int rbc_implicit_device(rbc_ctx* c, const void* th, const void* f, const void* g, const void* F, const void* G,
+                        void* oth, void* of, void* og, void* oF, void* oG) {
+    return op_call_device(c, 2, th, f, g, F, G, oth, of, og, oF, oG);
+}
+
+void* rbc_stream(rbc_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int rbc_init_state(rbc_ctx* c, int kind, unsigned long long seed, double amplitude) {
+    if (!c || kind < RBC_INIT_ZERO || kind > RBC_INIT_TEST || !std::isfinite(amplitude)) return RBC_ERR_ARG;
+    try {
+        hipSetDevice(c->p.device);
+        if (c->rsz == 8) {
+            Launch<double>::init_state(c->dg, c->b, kind, seed, amplitude, c->stream);
+            Launch<double>::make_real(c->dg, c->b, c->b.X, c->stream);  // main.cu:396
+        } else {
+            Launch<float>::init_state(c->dg, c->b, kind, seed, amplitude, c->stream);
+            Launch<float>::make_real(c->dg, c->b, c->b.X, c->stream);
+        }
+        HIPCHK(hipGetLastError());
+        c->have_hist = false;
+        clear_health(c);
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+int rbc_force_mean(rbc_ctx* c, double mean_F) {
+    if (!c || !std::isfinite(mean_F)) return RBC_ERR_ARG;
+    try {
+        hipSetDevice(c->p.device);
+        if (c->rsz == 8) Launch<double>::force_mean(c->b, mean_F, c->stream);
+        else Launch<float>::force_mean(c->b, mean_F, c->stream);
+        HIPCHK(hipGetLastError());
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
 }
 
 int rbc_nusselt(rbc_ctx* c, double* nu3) {
@@ -580,10 +728,10 @@ int rbc_nusselt(rbc_ctx* c, double* nu3) {
         nu3[0] = 1.0 - red[nblk];
         nu3[1] = 1.0 - red[nblk + 1];
         nu3[2] = 1.0 + s;
+        return health_status(c);
     } catch (const HipError& e) {
         return hip_fail(c, e);
     }
-    return RBC_OK;
 }
 
 int rbc_profile_steps(rbc_ctx* c, int nsteps, double* ms) {

@@ -970,22 +970,27 @@ void launch_res_levels(rbc_closure* cl) {
                        (const cpx<T>*)c->b.Oo[0], (const cpx<T>*)c->b.Oo[1], (const cpx<T>*)c->b.Oo[2], cl->spec);
 }
 
+// bytes of snapshot entry i: three fields, F, G, three history fields, their F, G, and the
+// step-health word pair (a chunk redone from the snapshot must not keep a non-finite flag)
+size_t snap_bytes(const rbc_ctx* c, int i) {
+    if (i == 10) return 2 * sizeof(int);
+    return (i % 5) < 3 ? (size_t)c->dg.P * c->dg.L * c->csz() : (size_t)c->dg.L * c->csz();
+}
+
 void take_snapshot(rbc_closure* cl) {
     rbc_ctx* c = cl->c;
-    const size_t field = (size_t)c->dg.P * c->dg.L * c->csz(), vec = (size_t)c->dg.L * c->csz();
-    void* src[10] = {c->b.X[0], c->b.X[1], c->b.X[2], c->b.XF, c->b.XG, c->b.Rp[0], c->b.Rp[1], c->b.Rp[2],
-                     c->b.RpF, c->b.RpG};
-    for (int i = 0; i < 10; ++i)
-        HIPCHK(hipMemcpyAsync(cl->snap[i], src[i], (i % 5) < 3 ? field : vec, hipMemcpyDeviceToDevice, c->stream));
+    void* src[11] = {c->b.X[0], c->b.X[1], c->b.X[2], c->b.XF, c->b.XG, c->b.Rp[0], c->b.Rp[1], c->b.Rp[2],
+                     c->b.RpF, c->b.RpG, c->b.health};
+    for (int i = 0; i < 11; ++i)
+        HIPCHK(hipMemcpyAsync(cl->snap[i], src[i], snap_bytes(c, i), hipMemcpyDeviceToDevice, c->stream));
 }
 
 void restore_snapshot(rbc_closure* cl) {
     rbc_ctx* c = cl->c;
-    const size_t field = (size_t)c->dg.P * c->dg.L * c->csz(), vec = (size_t)c->dg.L * c->csz();
-    void* dst[10] = {c->b.X[0], c->b.X[1], c->b.X[2], c->b.XF, c->b.XG, c->b.Rp[0], c->b.Rp[1], c->b.Rp[2],
-                     c->b.RpF, c->b.RpG};
-    for (int i = 0; i < 10; ++i)
-        HIPCHK(hipMemcpyAsync(dst[i], cl->snap[i], (i % 5) < 3 ? field : vec, hipMemcpyDeviceToDevice, c->stream));
+    void* dst[11] = {c->b.X[0], c->b.X[1], c->b.X[2], c->b.XF, c->b.XG, c->b.Rp[0], c->b.Rp[1], c->b.Rp[2],
+                     c->b.RpF, c->b.RpG, c->b.health};
+    for (int i = 0; i < 11; ++i)
+        HIPCHK(hipMemcpyAsync(dst[i], cl->snap[i], snap_bytes(c, i), hipMemcpyDeviceToDevice, c->stream));
 }
 
 // |dN| / |N| over the last correction (all three fields)
@@ -1096,8 +1101,7 @@ int rbc_closure_create(rbc_ctx* c, int nz, int capacity_points, rbc_closure** ou
         cl->m2 = (float*)cl->al(NW * sizeof(float));
         cl->gpart = (float*)cl->al((size_t)kTrainWaves * NW * sizeof(float));
         cl->lpart = (double*)cl->al(kTrainWaves * sizeof(double));
-        for (int i = 0; i < 10; ++i)
-            cl->snap.push_back(cl->al((i % 5) < 3 ? (size_t)g.P * L * c->csz() : (size_t)L * c->csz()));
+        for (int i = 0; i < 11; ++i) cl->snap.push_back(cl->al(snap_bytes(c, i)));
         // Glorot-uniform initial weights from SplitMix64, zero biases
         std::vector<float> w0(NW, 0.f);
         uint64_t ctr = 0;

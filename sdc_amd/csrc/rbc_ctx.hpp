@@ -39,6 +39,7 @@ struct rbc_ctx {
     void* ia1 = nullptr;  // L*L doubles (Nusselt)
     std::vector<void*> allocs;
     bool have_hist = false;
+    int bad_step = 0;  // host copy of the first non-finite step (health[0]) once read; 0 = none
     std::string err;
     std::vector<unsigned char> stage;  // host staging for one compact field
     static constexpr int kRing = 64;                 // steps of stage events in flight
@@ -49,6 +50,11 @@ struct rbc_ctx {
     // formed; the learned closure adds its correction there.  Null = plain method.
     void (*post_nonlinear)(rbc_ctx*, void*) = nullptr;
     void* post_user = nullptr;
+    // velocity and spectrum workspace of rbc_velocity.hip: constants and the spectrum scratch
+    // are made at create; the grid and point scratch of the host-buffer diagnostics grow on
+    // demand (a call allocates only when it needs more than any earlier call)
+    void* vel_ws = nullptr;
+    void (*vel_ws_free)(void*) = nullptr;
 
     size_t csz() const { return 2 * rsz; }
     void* dalloc(size_t bytes) {
@@ -59,6 +65,7 @@ struct rbc_ctx {
     }
     ~rbc_ctx() {
         if (stream) hipStreamSynchronize(stream);
+        if (vel_ws && vel_ws_free) vel_ws_free(vel_ws);
         for (void* a : allocs) hipFree(a);
         if (events)
             for (auto& row : ev_ring)
@@ -82,6 +89,15 @@ void get_vec(rbc_ctx* c, const void* dev, double* host);
 
 // n IMEX steps on the context's stream (no host synchronisation)
 void run_steps(rbc_ctx* c, int n);
+
+// waits for the context's stream, then reads the end-of-step health words: RBC_OK, or
+// RBC_ERR_NUMERIC (with the step in the error text) once a step has ended non-finite
+int health_status(rbc_ctx* c);
+// forgets a recorded non-finite step (the state was replaced); stream-ordered, no wait
+void clear_health(rbc_ctx* c);
+
+// velocity and spectrum workspace of a new context (rbc_velocity.hip), called by rbc_create
+void velocity_setup(rbc_ctx* c);
 
 // the nonlinear term of a state held in the context's device layout, on its stream
 void nonlinear_op(rbc_ctx* c, void* const X[3], const void* XF, const void* XG, void* const Nout[3], void* NF,

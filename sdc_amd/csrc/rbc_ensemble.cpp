@@ -8,7 +8,8 @@
 // L <= 32) advance together: one launch per stage for the whole class (batch_step), so a
 // small member no longer pays a full launch per stage; the other members are independent
 // branches of the graph.  The first step of a member with no AB2 history is an Euler step
-// with other coefficients, so it runs eagerly before the graph is used.
+// with other coefficients, so a call whose members include one without history runs its
+// first step eagerly for every member before the graph is used.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -120,10 +121,11 @@ void build_batches(rbc_ensemble* e) {
     }
 }
 
-// members without AB2 history take their Euler step eagerly (on their own streams)
-void euler_starts(rbc_ensemble* e) {
-    for (rbc_ctx* c : e->m)
-        if (!c->have_hist) run_steps(c, 1);
+// The first of a call's steps when some member has no AB2 history: every member advances by
+// exactly one step here (an Euler step for members without history, an AB2 step for the others),
+// so that the graph replays that follow start from members that all have history.
+void first_step_all(rbc_ensemble* e) {
+    for (rbc_ctx* c : e->m) run_steps(c, 1);
 }
 
 void capture(rbc_ensemble* e, int K) {
@@ -165,7 +167,8 @@ std::vector<std::pair<void*, size_t>> state_spans(rbc_ctx* c) {
     const size_t field = (size_t)c->dg.P * c->dg.L * c->csz(), vec = (size_t)c->dg.L * c->csz();
     Buffers& b = c->b;
     return {{b.X[0], field}, {b.X[1], field}, {b.X[2], field}, {b.XF, vec}, {b.XG, vec},
-            {b.Rp[0], field}, {b.Rp[1], field}, {b.Rp[2], field}, {b.RpF, vec}, {b.RpG, vec}};
+            {b.Rp[0], field}, {b.Rp[1], field}, {b.Rp[2], field}, {b.RpF, vec}, {b.RpG, vec},
+            {b.health, 2 * sizeof(int)}};
 }
 
 void take(rbc_ctx* c, Snapshot& s) {
@@ -360,7 +363,7 @@ int rbc_ensemble_step(rbc_ensemble* e, int nsteps, int steps_per_launch) {
             bool any_euler = false;
             for (rbc_ctx* c : e->m) any_euler = any_euler || !c->have_hist;
             if (any_euler) {
-                euler_starts(e);
+                first_step_all(e);
                 left -= 1;
             }
         }

@@ -117,6 +117,10 @@ struct Buffers {
     void* gmat;   // ngroups * stride real
     double* red;  // reduction scratch (doubles)
     int nred;
+    // step health: health[0] = 1 + the step count at which the end-of-step check first saw a
+    // non-finite value (0 = none), health[1] = steps finished since create (both written by the
+    // last block of the end-of-step kernel only)
+    int* health;
     // matrix-core vertical transforms (FP32 storage): -1 = off, 0 = FP32 MFMA, 1 = BF16 x3
     int mma = -1;
     void* evA = nullptr;  // [4][ev_nlt][ev_nks][64][KPL] floats (C', C, S, S')
@@ -161,8 +165,16 @@ struct Launch {
     static void finish(const DevGeom& g, const Buffers& b, double Pr, double dt, StepCoef sc, hipStream_t s);
     // make_real on the kx = 0 column (util.cu make_real)
     static void make_real(const DevGeom& g, const Buffers& b, void* const X[3], hipStream_t s);
-    // gather/scatter between the reference layout (host-staged, T complex) and [p][m]
     static void nusselt(const DevGeom& g, const Buffers& b, const void* ia1, hipStream_t s);
+    // device fields in the reference layout ((M/2+1) x N x L, T complex) -> compact [p][m]
+    // (kept modes); null source fields are skipped
+    static void from_ref(const DevGeom& g, const void* const src[3], void* const dst[3], hipStream_t s);
+    // compact [p][m] -> reference layout with zeros in the removed band; null targets skipped
+    static void to_ref(const DevGeom& g, const void* const src[3], void* const dst[3], hipStream_t s);
+    // initial state of the context's X, XF, XG (kind: 0 zeros, 1 random theta, 2 init_test)
+    static void init_state(const DevGeom& g, const Buffers& b, int kind, uint64_t seed, double amp, hipStream_t s);
+    // F_0 = v (force_mean)
+    static void force_mean(const Buffers& b, double v, hipStream_t s);
 };
 
 // Batched steps of the members of one ensemble size class (rbc_kernels.hip).  A member is

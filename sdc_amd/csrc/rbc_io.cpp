@@ -144,6 +144,7 @@ int rbc_read_state_files(rbc_ctx* c, const char* const paths[5], int which) {
         if (which == 0) {  // main.cu:396: the initial state is made real after reading
             if (c->rsz == 8) make_real_now<double>(c);
             else make_real_now<float>(c);
+            clear_health(c);
         } else {
             c->have_hist = true;
         }

@@ -773,14 +773,30 @@ __global__ void k_make_real(DevGeom g, const int* __restrict__ rows, cpx<T>* X0,
 }
 
 // the end of a time step in one launch: make_real on the blocks before the last, the F, G
-// update (mode UPD_STEP) on the last block
+// update (mode UPD_STEP) and the health check on the last block.  A non-finite coefficient
+// anywhere in the state reaches every mode within one nonlinear evaluation (each Fourier mode
+// of a product sums every grid point), so the mean modes theta(0,0), F and G, which the last
+// block reads anyway, show it by the end of the step that consumed it.  health[1] counts the
+// steps; health[0] keeps 1 + the count of the first step that ended non-finite.
 template <typename T>
 __device__ __forceinline__ void k_finish_body(DevGeom g, const int* __restrict__ rows, cpx<T>* X0, cpx<T>* X1,
                                               cpx<T>* X2, T Pr, T dt, T a_cur, T a_prev, int use_prev, cpx<T>* XF,
                                               cpx<T>* XG, const cpx<T>* NF, const cpx<T>* NG, cpx<T>* RF,
-                                              cpx<T>* RG) {
+                                              cpx<T>* RG, int* __restrict__ health) {
     if (blockIdx.x == gridDim.x - 1) {
         update_FG_mode<T>(threadIdx.x, g.L, Pr, dt, a_cur, a_prev, use_prev, UPD_STEP, XF, XG, NF, NG, RF, RG);
+        int bad = 0;
+        if ((int)threadIdx.x < g.L) {
+            const int m = threadIdx.x;
+            // theta(0,0) real parts only: the make_real blocks clear its imaginary parts meanwhile
+            bad = !(isfinite(XF[m].x) && isfinite(XG[m].x) && isfinite(X0[m].x));
+        }
+        bad = __syncthreads_or(bad);
+        if (threadIdx.x == 0) {
+            const int n = health[1];
+            if (bad && health[0] == 0) health[0] = n + 1;
+            health[1] = n + 1;
+        }
         return;
     }
     const int L = g.L, NYc = g.NYc, N = g.N;
@@ -803,8 +819,8 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_finish(DevGeom g, const int* __restrict__ rows, cpx<T>* X0, cpx<T>* X1,
                                                 cpx<T>* X2, T Pr, T dt, T a_cur, T a_prev, int use_prev, cpx<T>* XF,
                                                 cpx<T>* XG, const cpx<T>* NF, const cpx<T>* NG, cpx<T>* RF,
-                                                cpx<T>* RG) {
-    k_finish_body<T>(g, rows, X0, X1, X2, Pr, dt, a_cur, a_prev, use_prev, XF, XG, NF, NG, RF, RG);
+                                                cpx<T>* RG, int* __restrict__ health) {
+    k_finish_body<T>(g, rows, X0, X1, X2, Pr, dt, a_cur, a_prev, use_prev, XF, XG, NF, NG, RF, RG, health);
 }
 // one member's arguments of k_finish_batch, which runs the k_finish body for member
 // blockIdx.z / zdim of a batch (the rest of blockIdx.z is the body's own z index)
@@ -826,15 +842,125 @@ struct k_finish_args {
     const cpx<T>* NG;
     cpx<T>* RF;
     cpx<T>* RG;
+    int* health;
 };
 template <typename T>
 __global__ void __launch_bounds__(256) k_finish_batch(const k_finish_args<T>* __restrict__ tab, int zdim) {
     const int mb = blockIdx.z / zdim;
     const k_finish_args<T>& a = tab[mb];
     k_finish_body<T>(a.g, a.rows, a.X0, a.X1, a.X2, a.Pr, a.dt, a.a_cur, a.a_prev, a.use_prev, a.XF, a.XG, a.NF, a.NG,
-                     a.RF, a.RG);
+                     a.RF, a.RG, a.health);
 }
 
+
+// ---------------------------------------------------------------------------------------
+// Device-resident exchange with the reference layout ((M/2+1) x N x L complex, kx fastest,
+// matrix/matrix_device.h:13-63) in the context's element type.  The kept rows are
+// [0, a] U [N - a, N - 1] (a = (NYc - 1) / 2, checked at create), so a row's compact index
+// follows from the row itself.
+// ---------------------------------------------------------------------------------------
+
+// reference layout -> compact [p][m] (kept modes only); one thread per (p, m), three fields
+template <typename T>
+__global__ void __launch_bounds__(256) k_ref_to_compact(DevGeom g, const cpx<T>* __restrict__ s0,
+                                                        const cpx<T>* __restrict__ s1, const cpx<T>* __restrict__ s2,
+                                                        cpx<T>* __restrict__ d0, cpx<T>* __restrict__ d1,
+                                                        cpx<T>* __restrict__ d2) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)g.P * g.L;
+    if (e >= n) return;
+    const int fld = blockIdx.y;
+    const cpx<T>* src = fld == 0 ? s0 : (fld == 1 ? s1 : s2);
+    cpx<T>* dst = fld == 0 ? d0 : (fld == 1 ? d1 : d2);
+    if (!src) return;
+    const int m = (int)(e % g.L);
+    const int p = (int)(e / g.L);
+    const int kx = p / g.NYc, kyc = p - kx * g.NYc;
+    const int a = (g.NYc - 1) / 2;
+    const int row = kyc <= a ? kyc : g.N - (g.NYc - kyc);
+    const int mox = g.M / 2 + 1;
+    dst[e] = src[((size_t)m * g.N + row) * mox + kx];
+}
+
+// compact [p][m] -> reference layout, zeros in the removed band; one thread per reference
+// element, three fields
+template <typename T>
+__global__ void __launch_bounds__(256) k_compact_to_ref(DevGeom g, const cpx<T>* __restrict__ s0,
+                                                        const cpx<T>* __restrict__ s1, const cpx<T>* __restrict__ s2,
+                                                        cpx<T>* __restrict__ d0, cpx<T>* __restrict__ d1,
+                                                        cpx<T>* __restrict__ d2) {
+    const int mox = g.M / 2 + 1;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)mox * g.N * g.L;
+    if (e >= n) return;
+    const int fld = blockIdx.y;
+    const cpx<T>* src = fld == 0 ? s0 : (fld == 1 ? s1 : s2);
+    cpx<T>* dst = fld == 0 ? d0 : (fld == 1 ? d1 : d2);
+    if (!dst) return;
+    const int kx = (int)(e % mox);
+    const size_t r = e / mox;
+    const int row = (int)(r % g.N), m = (int)(r / g.N);
+    const int a = (g.NYc - 1) / 2;
+    const int kyc = row <= a ? row : (row >= g.N - a ? g.NYc - (g.N - row) : -1);
+    cpx<T> v = mk(T(0), T(0));
+    if (kx < g.KX && kyc >= 0) v = src[((size_t)kx * g.NYc + kyc) * g.L + m];
+    dst[e] = v;
+}
+
+// ---------------------------------------------------------------------------------------
+// Initial states on the kept modes (the removed band is never stored: the stepper dealiases on
+// entry, implicit_explicit_stepping.cu:40), one thread per compact (p, m), blockIdx.y = field.
+//   kind 1 (random): theta = amp (2u - 1) + i amp (2u' - 1), u, u' uniform from SplitMix64 on
+//          (seed, reference-layout index): the distribution of init_random (init.cu:9-20,
+//          348-365), whose cuRAND XORWOW stream cannot be reproduced; f, g, F, G zero.
+//   kind 2 (test): init_test_device (init.cu:263-301): theta and f at ky = 0, kx != 0 hold
+//          1e-6 sin(kx) in the real part, g = 0, F_m = 1e-6 cos(m), G = 0, in the real type T.
+//   kind 0: zeros.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t init_mix64(uint64_t x) {
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_init_state(DevGeom g, int kind, uint64_t seed, double amp,
+                                                    cpx<T>* __restrict__ X0, cpx<T>* __restrict__ X1,
+                                                    cpx<T>* __restrict__ X2, cpx<T>* __restrict__ XF,
+                                                    cpx<T>* __restrict__ XG) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int fld = blockIdx.y;
+    if (fld == 0 && e < (size_t)g.L) {  // mean flow
+        const int m = (int)e;
+        XF[m] = mk(kind == 2 ? T(1e-6) * cos(T(m)) : T(0), T(0));
+        XG[m] = mk(T(0), T(0));
+    }
+    if (e >= (size_t)g.P * g.L) return;
+    const int m = (int)(e % g.L), p = (int)(e / g.L);
+    const int kx = p / g.NYc, kyc = p - kx * g.NYc;
+    const int a = (g.NYc - 1) / 2;
+    const int row = kyc <= a ? kyc : g.N - (g.NYc - kyc);
+    cpx<T>* X = fld == 0 ? X0 : (fld == 1 ? X1 : X2);
+    cpx<T> v = mk(T(0), T(0));
+    if (kind == 1 && fld == 0) {
+        const uint64_t idx = ((uint64_t)m * g.N + row) * (uint64_t)(g.M / 2 + 1) + kx;
+        const uint64_t h0 = init_mix64(seed * 0x100000001B3ull + 2 * idx);
+        const uint64_t h1 = init_mix64(seed * 0x100000001B3ull + 2 * idx + 1);
+        const double u0 = (double)(h0 >> 11) * (1.0 / 9007199254740992.0);
+        const double u1 = (double)(h1 >> 11) * (1.0 / 9007199254740992.0);
+        v = mk(T(amp * (2.0 * u0 - 1.0)), T(amp * (2.0 * u1 - 1.0)));
+    } else if (kind == 2 && fld != 2 && row == 0 && kx != 0) {
+        v = mk(T(1e-6) * sin(T(kx)), T(0));
+    }
+    X[e] = v;
+}
+
+// force_mean (util/util.cu:74-81): F_0 = mean_F + 0 i
+template <typename T>
+__global__ void k_force_mean(cpx<T>* XF, double v) {
+    if (threadIdx.x == 0) XF[0] = mk(T(v), T(0));
+}
 
 // Nusselt reductions (FP64): blocks write partial sums of sum_p w_p sum_{m,n} Ia1(m,n)
 // Re(k^2 f_m conj theta_n) to red[blockIdx]; red[nblk] = plate sums from mode p = 0.
@@ -2228,7 +2354,7 @@ void Launch<T>::finish(const DevGeom& g, const Buffers& b, double Pr, double dt,
     hipLaunchKernelGGL(k_finish<T>, dim3((n + 255) / 256 + 1), dim3(256), 0, s, g, b.rows, (cpx<T>*)b.X[0],
                        (cpx<T>*)b.X[1], (cpx<T>*)b.X[2], (T)Pr, (T)dt, (T)sc.a_cur, (T)sc.a_prev, sc.use_prev,
                        (cpx<T>*)b.XF, (cpx<T>*)b.XG, (const cpx<T>*)b.NF, (const cpx<T>*)b.NG, (cpx<T>*)b.RpF,
-                       (cpx<T>*)b.RpG);
+                       (cpx<T>*)b.RpG, b.health);
 }
 
 template <typename T>
@@ -2236,6 +2362,34 @@ void Launch<T>::nusselt(const DevGeom& g, const Buffers& b, const void* ia1, hip
     const int nblk = (g.P + 255) / 256;
     hipLaunchKernelGGL(k_nusselt<T>, dim3(nblk), dim3(256), 0, s, g, (const cpx<T>*)b.X[0],
                        (const cpx<T>*)b.X[1], (const T*)b.k2, (const double*)ia1, b.red);
+}
+
+template <typename T>
+void Launch<T>::from_ref(const DevGeom& g, const void* const src[3], void* const dst[3], hipStream_t s) {
+    const size_t n = (size_t)g.P * g.L;
+    hipLaunchKernelGGL(k_ref_to_compact<T>, dim3((unsigned)((n + 255) / 256), 3), dim3(256), 0, s, g,
+                       (const cpx<T>*)src[0], (const cpx<T>*)src[1], (const cpx<T>*)src[2], (cpx<T>*)dst[0],
+                       (cpx<T>*)dst[1], (cpx<T>*)dst[2]);
+}
+
+template <typename T>
+void Launch<T>::to_ref(const DevGeom& g, const void* const src[3], void* const dst[3], hipStream_t s) {
+    const size_t n = (size_t)(g.M / 2 + 1) * g.N * g.L;
+    hipLaunchKernelGGL(k_compact_to_ref<T>, dim3((unsigned)((n + 255) / 256), 3), dim3(256), 0, s, g,
+                       (const cpx<T>*)src[0], (const cpx<T>*)src[1], (const cpx<T>*)src[2], (cpx<T>*)dst[0],
+                       (cpx<T>*)dst[1], (cpx<T>*)dst[2]);
+}
+
+template <typename T>
+void Launch<T>::init_state(const DevGeom& g, const Buffers& b, int kind, uint64_t seed, double amp, hipStream_t s) {
+    const size_t n = std::max((size_t)g.P * g.L, (size_t)g.L);
+    hipLaunchKernelGGL(k_init_state<T>, dim3((unsigned)((n + 255) / 256), 3), dim3(256), 0, s, g, kind, seed, amp,
+                       (cpx<T>*)b.X[0], (cpx<T>*)b.X[1], (cpx<T>*)b.X[2], (cpx<T>*)b.XF, (cpx<T>*)b.XG);
+}
+
+template <typename T>
+void Launch<T>::force_mean(const Buffers& b, double v, hipStream_t s) {
+    hipLaunchKernelGGL(k_force_mean<T>, dim3(1), dim3(64), 0, s, (cpx<T>*)b.XF, v);
 }
 
 template struct Launch<float>;
@@ -2365,7 +2519,7 @@ StepBatch* batch_create(const std::vector<BatchMember>& m, std::string& why) {
                       (const C*)b.Nn[0], (const C*)b.Nn[1], (const C*)b.Nn[2], (C*)b.Rp[0], (C*)b.Rp[1],
                       (C*)b.Rp[2]});
         fi.push_back({h, b.rows, (C*)b.X[0], (C*)b.X[1], (C*)b.X[2], (float)x.Pr, (float)x.dt, a_cur, a_prev, 1,
-                      (C*)b.XF, (C*)b.XG, (const C*)b.NF, (const C*)b.NG, (C*)b.RpF, (C*)b.RpG});
+                      (C*)b.XF, (C*)b.XG, (const C*)b.NF, (const C*)b.NG, (C*)b.RpF, (C*)b.RpG, b.health});
         sb->nchunk_max = std::max(sb->nchunk_max, b.nchunk);
     }
     try {

@@ -457,6 +457,41 @@ __global__ void __launch_bounds__(1024) k_tracer_moments(int np, const float* __
 // ---------------------------------------------------------------------------------------
 // Kinetic energy per mode, FP64: out[c][p] = sum_l wz_l |c_k(z_l)|^2 for c = u, v, w.
 // ---------------------------------------------------------------------------------------
+// Characteristic velocity <|u|> over a velocity grid (main_extract_vel.cu:182-218): per block
+// the sum of sqrt(u^2 + v^2 + w^2) over its points (the float components squared and summed in
+// FP64, points whose sum is NaN skipped) and the count of points summed; the host adds the
+// blocks in order.
+constexpr int kCharvelBlocks = 1024;
+__global__ void __launch_bounds__(256) k_charvel(size_t n, const float* __restrict__ grid, double* __restrict__ part) {
+    __shared__ double ss[256], sc[256];
+    double s = 0.0, cnt = 0.0;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        double val = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            const double v = grid[(size_t)c * n + e];
+            val += v * v;
+        }
+        if (val == val) {
+            cnt += 1.0;
+            s += sqrt(val);
+        }
+    }
+    ss[threadIdx.x] = s;
+    sc[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            ss[threadIdx.x] += ss[threadIdx.x + w];
+            sc[threadIdx.x] += sc[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = ss[0];
+        part[2 * blockIdx.x + 1] = sc[0];
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_energy_modes(DevGeom g, int nz, const double* __restrict__ zb,
                                                       const double* __restrict__ wz, const cpx<T>* __restrict__ Xf,
@@ -598,10 +633,11 @@ void velocity_grid(rbc_ctx* c, const GridWork& w) {
     HIPCHK(hipGetLastError());
 }
 
-// spectral velocity at np points (device arrays); part holds nslice*3*np floats
+// spectral velocity at np points (device arrays); part holds nslice*3*np floats; lam and
+// modes are the context's constants (VelWork)
 struct PointWork {
-    double* lam = nullptr;
-    float4* modes = nullptr;
+    const double* lam = nullptr;
+    const float4* modes = nullptr;
     float* part = nullptr;
     int nslice = 1, tiles_per_slice = 1;
 };
@@ -618,23 +654,66 @@ void plan_points(rbc_ctx* c, int np, PointWork& pw) {
     pw.nslice = (ntile + pw.tiles_per_slice - 1) / pw.tiles_per_slice;
 }
 
-void alloc_point_work(rbc_ctx* c, int np, PointWork& pw, std::vector<void*>& owned) {
-    auto al = [&](size_t bytes) {
-        void* p = nullptr;
-        HIPCHK(hipMalloc(&p, bytes > 0 ? bytes : 16));
-        owned.push_back(p);
-        return p;
-    };
+// floats of partial sums an evaluation at np points needs
+size_t part_floats(rbc_ctx* c, int np) {
+    PointWork pw;
     plan_points(c, np, pw);
-    const int L = c->dg.L;
-    pw.lam = (double*)al((size_t)L * sizeof(double));
-    pw.modes = (float4*)al((size_t)c->dg.P * sizeof(float4));
-    pw.part = (float*)al((size_t)pw.nslice * 3 * np * sizeof(float));
-    std::vector<double> lam(L);
-    for (int m = 0; m < L; ++m) lam[m] = lambda_k(m + 1);
-    const std::vector<float> mt = mode_table(c);
-    HIPCHK(hipMemcpy(pw.lam, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(pw.modes, mt.data(), mt.size() * sizeof(float), hipMemcpyHostToDevice));
+    return (size_t)pw.nslice * 3 * np;
+}
+
+// Per-context workspace of the velocity diagnostics.  lam, modes and the spectrum scratch are
+// made by velocity_setup (rbc_create); the grid work and the point partial sums of the
+// host-buffer calls grow on demand and are kept for later calls.
+struct VelWork {
+    double* lam = nullptr;    // L
+    float4* modes = nullptr;  // P
+    double* ezb = nullptr;    // spectrum: basis values [nz][3][L], nz <= max(nq, 1)
+    double* ew = nullptr;     // spectrum: level weights
+    double* ee = nullptr;     // spectrum: per-mode energies [3][P]
+    int ez_cap = 0;
+    GridWork gw;              // rbc_velocity_grid, capacity gw_cap heights
+    int gw_cap = 0;
+    std::vector<void*> gw_owned;
+    float* part = nullptr;    // rbc_velocity_at partial sums
+    size_t part_cap = 0;
+    float* io = nullptr;      // rbc_velocity_at points and results, 6 n floats
+    size_t io_cap = 0;
+    double* cpart = nullptr;  // characteristic velocity: per block (sum, count)
+    std::vector<void*> owned;
+    ~VelWork() {
+        for (void* p : gw_owned) hipFree(p);
+        if (part) hipFree(part);
+        if (io) hipFree(io);
+        for (void* p : owned) hipFree(p);
+    }
+};
+
+void free_vel_work(void* p) { delete (VelWork*)p; }
+
+VelWork* vel_work(rbc_ctx* c) { return (VelWork*)c->vel_ws; }
+
+void alloc_point_work(rbc_ctx* c, int np, PointWork& pw, std::vector<void*>& owned) {
+    plan_points(c, np, pw);
+    VelWork* vw = vel_work(c);
+    pw.lam = vw->lam;
+    pw.modes = vw->modes;
+    void* p = nullptr;
+    HIPCHK(hipMalloc(&p, std::max<size_t>(16, (size_t)pw.nslice * 3 * np * sizeof(float))));
+    owned.push_back(p);
+    pw.part = (float*)p;
+}
+
+// grows a device buffer to at least `bytes` (contents not kept)
+template <typename P>
+void grow(P*& buf, size_t& cap, size_t need) {
+    if (need <= cap && buf) return;
+    if (buf) HIPCHK(hipFree(buf));
+    buf = nullptr;
+    cap = 0;
+    void* p = nullptr;
+    HIPCHK(hipMalloc(&p, std::max<size_t>(16, need * sizeof(P))));
+    buf = (P*)p;
+    cap = need;
 }
 
 template <typename T, int LM>
@@ -669,6 +748,35 @@ void velocity_points(rbc_ctx* c, int np, const float* x, const float* y, const f
 }
 
 }  // namespace
+
+namespace rbc {
+
+void velocity_setup(rbc_ctx* c) {
+    VelWork* vw = new VelWork;
+    c->vel_ws = vw;
+    c->vel_ws_free = free_vel_work;
+    auto al = [&](size_t bytes) {
+        void* p = nullptr;
+        HIPCHK(hipMalloc(&p, std::max<size_t>(16, bytes)));
+        vw->owned.push_back(p);
+        return p;
+    };
+    const int L = c->dg.L, P = c->dg.P;
+    vw->lam = (double*)al((size_t)L * sizeof(double));
+    vw->modes = (float4*)al((size_t)P * sizeof(float4));
+    std::vector<double> lam(L);
+    for (int m = 0; m < L; ++m) lam[m] = lambda_k(m + 1);
+    const std::vector<float> mt = mode_table(c);
+    HIPCHK(hipMemcpy(vw->lam, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(vw->modes, mt.data(), mt.size() * sizeof(float), hipMemcpyHostToDevice));
+    vw->ez_cap = std::max(1, c->dg.nq);
+    vw->ezb = (double*)al((size_t)vw->ez_cap * 3 * L * sizeof(double));
+    vw->ew = (double*)al((size_t)vw->ez_cap * sizeof(double));
+    vw->ee = (double*)al((size_t)3 * P * sizeof(double));
+    vw->cpart = (double*)al((size_t)2 * kCharvelBlocks * sizeof(double));
+}
+
+}  // namespace rbc
 
 // ---- tracer object -----------------------------------------------------------------------
 struct rbc_tracer {
@@ -923,12 +1031,20 @@ int rbc_tracer_moments(rbc_tracer* t, double* out4) {
 int rbc_velocity_grid(rbc_ctx* c, int nz, const double* z, float* ux, float* uy, float* uz) {
     if (!c || nz < 1 || !z || !ux || !uy || !uz) return RBC_ERR_ARG;
     if (int rc = check_ctx_for_velocity(c)) return rc;
-    std::vector<void*> owned;
-    int rc = RBC_OK;
     try {
         hipSetDevice(c->p.device);
-        GridWork w;
-        alloc_grid_work(c, nz, w, owned);
+        VelWork* vw = vel_work(c);
+        if (nz > vw->gw_cap) {  // grow the grid work (kept for later calls)
+            HIPCHK(hipStreamSynchronize(c->stream));
+            for (void* p : vw->gw_owned) HIPCHK(hipFree(p));
+            vw->gw_owned.clear();
+            vw->gw = GridWork();
+            vw->gw_cap = 0;
+            alloc_grid_work(c, nz, vw->gw, vw->gw_owned);
+            vw->gw_cap = nz;
+        }
+        GridWork& w = vw->gw;
+        w.nz = nz;
         set_heights(c, w, std::vector<double>(z, z + nz));
         velocity_grid(c, w);
         const size_t n = (size_t)nz * c->dg.N * c->dg.M;
@@ -937,25 +1053,69 @@ int rbc_velocity_grid(rbc_ctx* c, int nz, const double* z, float* ux, float* uy,
         HIPCHK(hipMemcpyAsync(uz, w.grid + 2 * n, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     } catch (const HipError& e) {
-        rc = hip_fail(c, e);
+        return hip_fail(c, e);
     }
-    for (void* p : owned) hipFree(p);
-    return rc;
+    return RBC_OK;
+}
+
+int rbc_characteristic_velocity(rbc_ctx* c, int nz, double* U) {
+    if (!c || nz < 2 || !U) return RBC_ERR_ARG;
+    if (int rc = check_ctx_for_velocity(c)) return rc;
+    try {
+        hipSetDevice(c->p.device);
+        VelWork* vw = vel_work(c);
+        if (nz > vw->gw_cap) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            for (void* p : vw->gw_owned) HIPCHK(hipFree(p));
+            vw->gw_owned.clear();
+            vw->gw = GridWork();
+            vw->gw_cap = 0;
+            alloc_grid_work(c, nz, vw->gw, vw->gw_owned);
+            vw->gw_cap = nz;
+        }
+        GridWork& w = vw->gw;
+        w.nz = nz;
+        // linspace(-0.5, 0.5, nz) in the device real type (calculate_velocity_operator.cu:56-58, 320-327)
+        std::vector<double> z(nz);
+        for (int l = 0; l < nz; ++l) z[l] = (double)(-0.5f + l * (0.5f - -0.5f) / (nz - 1));
+        HIPCHK(hipStreamSynchronize(c->stream));  // the heights table may still be read
+        set_heights(c, w, z);
+        velocity_grid(c, w);
+        const size_t n = (size_t)nz * c->dg.N * c->dg.M;
+        hipLaunchKernelGGL(k_charvel, dim3(kCharvelBlocks), dim3(256), 0, c->stream, n, (const float*)w.grid,
+                           vw->cpart);
+        HIPCHK(hipGetLastError());
+        std::vector<double> part(2 * (size_t)kCharvelBlocks);
+        HIPCHK(hipMemcpyAsync(part.data(), vw->cpart, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        double s = 0.0, cnt = 0.0;
+        for (int b = 0; b < kCharvelBlocks; ++b) {
+            s += part[2 * b];
+            cnt += part[2 * b + 1];
+        }
+        *U = s / cnt;
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
 }
 
 int rbc_velocity_at(rbc_ctx* c, int n, const float* x, const float* y, const float* z, float* u, float* v, float* w) {
     if (!c || n < 0 || (n > 0 && (!x || !y || !z || !u || !v || !w))) return RBC_ERR_ARG;
     if (lmax_for(c->dg.L) == 0) return fail(c, RBC_ERR_ARG, "spectral evaluation supports L <= 32");
     if (n == 0) return RBC_OK;
-    std::vector<void*> owned;
-    int rc = RBC_OK;
     try {
         hipSetDevice(c->p.device);
+        VelWork* vw = vel_work(c);
+        HIPCHK(hipStreamSynchronize(c->stream));  // earlier calls may still use the buffers
+        grow(vw->io, vw->io_cap, (size_t)6 * n);
+        grow(vw->part, vw->part_cap, part_floats(c, n));
         PointWork pw;
-        alloc_point_work(c, n, pw, owned);
-        float* d = nullptr;
-        HIPCHK(hipMalloc(&d, (size_t)6 * n * sizeof(float)));
-        owned.push_back(d);
+        plan_points(c, n, pw);
+        pw.lam = vw->lam;
+        pw.modes = vw->modes;
+        pw.part = vw->part;
+        float* d = vw->io;
         const size_t b = (size_t)n * sizeof(float);
         HIPCHK(hipMemcpyAsync(d, x, b, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(d + n, y, b, hipMemcpyHostToDevice, c->stream));
@@ -967,49 +1127,69 @@ int rbc_velocity_at(rbc_ctx* c, int n, const float* x, const float* y, const flo
         HIPCHK(hipMemcpyAsync(w, o + 2 * (size_t)n, b, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     } catch (const HipError& e) {
-        rc = hip_fail(c, e);
+        return hip_fail(c, e);
     }
-    for (void* p : owned) hipFree(p);
-    return rc;
+    return RBC_OK;
 }
 
-// per-mode energies at heights z with weights wz (FP64, host result [3][P])
+int rbc_velocity_at_device(rbc_ctx* c, int n, const float* x, const float* y, const float* z, float* u, float* v,
+                           float* w) {
+    if (!c || n < 0 || (n > 0 && (!x || !y || !z || !u || !v || !w))) return RBC_ERR_ARG;
+    if (lmax_for(c->dg.L) == 0) return fail(c, RBC_ERR_ARG, "spectral evaluation supports L <= 32");
+    if (n == 0) return RBC_OK;
+    try {
+        hipSetDevice(c->p.device);
+        VelWork* vw = vel_work(c);
+        const size_t need = part_floats(c, n);
+        if (need > vw->part_cap) {
+            HIPCHK(hipStreamSynchronize(c->stream));  // the old buffer may be in use
+            grow(vw->part, vw->part_cap, need);
+        }
+        PointWork pw;
+        plan_points(c, n, pw);
+        pw.lam = vw->lam;
+        pw.modes = vw->modes;
+        pw.part = vw->part;
+        velocity_points(c, n, x, y, z, pw, u, v, w);
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+// per-mode energies at heights z with weights wz (FP64, host result [3][P]); nz <= the
+// context's level count (the spectrum scratch is made at create)
 static int mode_energies(rbc_ctx* c, const std::vector<double>& z, const std::vector<double>& wz,
                          std::vector<double>& e) {
-    std::vector<void*> owned;
-    int rc = RBC_OK;
     try {
         hipSetDevice(c->p.device);
         const DevGeom& g = c->dg;
-        const std::vector<double> tb = basis_table(g.L, z);
-        double *dzb = nullptr, *dw = nullptr, *de = nullptr;
-        HIPCHK(hipMalloc(&dzb, tb.size() * sizeof(double)));
-        owned.push_back(dzb);
-        HIPCHK(hipMalloc(&dw, wz.size() * sizeof(double)));
-        owned.push_back(dw);
-        HIPCHK(hipMalloc(&de, (size_t)3 * g.P * sizeof(double)));
-        owned.push_back(de);
-        HIPCHK(hipMemcpy(dzb, tb.data(), tb.size() * sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(dw, wz.data(), wz.size() * sizeof(double), hipMemcpyHostToDevice));
-        const Buffers& b = c->b;
+        VelWork* vw = vel_work(c);
         const int nz = (int)z.size();
+        if (nz > vw->ez_cap) return fail(c, RBC_ERR_ARG, "too many spectrum levels");
+        const std::vector<double> tb = basis_table(g.L, z);
+        HIPCHK(hipStreamSynchronize(c->stream));  // the scratch may still be read by an earlier call
+        HIPCHK(hipMemcpy(vw->ezb, tb.data(), tb.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(vw->ew, wz.data(), wz.size() * sizeof(double), hipMemcpyHostToDevice));
+        const Buffers& b = c->b;
         if (c->rsz == 8)
-            hipLaunchKernelGGL(k_energy_modes<double>, dim3((g.P + 255) / 256), dim3(256), 0, c->stream, g, nz, dzb, dw,
-                               (const cpx<double>*)b.X[1], (const cpx<double>*)b.X[2], (const cpx<double>*)b.XF,
-                               (const cpx<double>*)b.XG, (const double*)b.kx, (const double*)b.ky, de);
+            hipLaunchKernelGGL(k_energy_modes<double>, dim3((g.P + 255) / 256), dim3(256), 0, c->stream, g, nz,
+                               (const double*)vw->ezb, (const double*)vw->ew, (const cpx<double>*)b.X[1],
+                               (const cpx<double>*)b.X[2], (const cpx<double>*)b.XF, (const cpx<double>*)b.XG,
+                               (const double*)b.kx, (const double*)b.ky, vw->ee);
         else
-            hipLaunchKernelGGL(k_energy_modes<float>, dim3((g.P + 255) / 256), dim3(256), 0, c->stream, g, nz, dzb, dw,
-                               (const cpx<float>*)b.X[1], (const cpx<float>*)b.X[2], (const cpx<float>*)b.XF,
-                               (const cpx<float>*)b.XG, (const float*)b.kx, (const float*)b.ky, de);
+            hipLaunchKernelGGL(k_energy_modes<float>, dim3((g.P + 255) / 256), dim3(256), 0, c->stream, g, nz,
+                               (const double*)vw->ezb, (const double*)vw->ew, (const cpx<float>*)b.X[1],
+                               (const cpx<float>*)b.X[2], (const cpx<float>*)b.XF, (const cpx<float>*)b.XG,
+                               (const float*)b.kx, (const float*)b.ky, vw->ee);
         HIPCHK(hipGetLastError());
         e.resize((size_t)3 * g.P);
-        HIPCHK(hipMemcpyAsync(e.data(), de, e.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(e.data(), vw->ee, e.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     } catch (const HipError& err) {
-        rc = hip_fail(c, err);
+        return hip_fail(c, err);
     }
-    for (void* p : owned) hipFree(p);
-    return rc;
+    return RBC_OK;
 }
 
 int rbc_energy_spectrum(rbc_ctx* c, int cap, double* E, int* nbins, double* totals) {

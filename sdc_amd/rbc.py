@@ -30,6 +30,18 @@ class RbcError(RuntimeError):
     pass
 
 
+class RbcNumericError(RbcError):
+    """RBC_ERR_NUMERIC: a step ended with a non-finite value in the state (the reference's
+    find_NaN check, util/util.cu:83-105), or a factor could not be built."""
+
+
+ERR_NUMERIC = -4
+
+
+def _raise(rc, msg):
+    raise (RbcNumericError if rc == ERR_NUMERIC else RbcError)(msg)
+
+
 class rbc_params(ctypes.Structure):
     _fields_ = [("M", ctypes.c_int), ("N", ctypes.c_int), ("L", ctypes.c_int),
                 ("Lx", ctypes.c_double), ("Ly", ctypes.c_double),
@@ -55,7 +67,10 @@ EXPORTS = ["rbc_create", "rbc_destroy", "rbc_last_error", "rbc_get_info", "rbc_s
            "rbc_closure_fit_normalization", "rbc_closure_get_normalization", "rbc_closure_set_normalization",
            "rbc_closure_get_weights", "rbc_closure_set_weights", "rbc_closure_gradient", "rbc_closure_train",
            "rbc_closure_predict", "rbc_closure_eval", "rbc_closure_enable", "rbc_closure_interval", "rbc_closure_step",
-           "rbc_closure_status"]
+           "rbc_closure_status", "rbc_set_state_device", "rbc_get_state_device", "rbc_nonlinear_device",
+           "rbc_inverse_B_device", "rbc_implicit_device", "rbc_stream", "rbc_velocity_at_device",
+           "rbc_init_state", "rbc_force_mean", "rbc_characteristic_velocity", "rbc_run_main",
+           "rbc_extract_vel_main"]
 
 CLOSURE_NIN, CLOSURE_NOUT = 8, 3
 
@@ -88,6 +103,17 @@ def load_library(path=LIB_PATH):
     for fn in ("rbc_nonlinear", "rbc_inverse_B", "rbc_implicit"):
         getattr(L, fn).argtypes = [vp] + [dp] * 10
     L.rbc_nusselt.argtypes = [vp, dp]
+    for fn in ("rbc_set_state_device", "rbc_get_state_device"):
+        getattr(L, fn).argtypes = [vp] * 6
+    for fn in ("rbc_nonlinear_device", "rbc_inverse_B_device", "rbc_implicit_device"):
+        getattr(L, fn).argtypes = [vp] * 11
+    L.rbc_stream.argtypes = [vp]
+    L.rbc_stream.restype = vp
+    L.rbc_init_state.argtypes = [vp, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double]
+    L.rbc_characteristic_velocity.argtypes = [vp, ctypes.c_int, dp]
+    L.rbc_run_main.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]
+    L.rbc_extract_vel_main.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]
+    L.rbc_force_mean.argtypes = [vp, ctypes.c_double]
     L.rbc_profile_steps.argtypes = [vp, ctypes.c_int, dp]
     L.rbc_stage_name.argtypes = [ctypes.c_int]
     L.rbc_stage_name.restype = ctypes.c_char_p
@@ -100,6 +126,7 @@ def load_library(path=LIB_PATH):
     L.rbc_set_history.argtypes = [vp] + [dp] * 5
     L.rbc_velocity_grid.argtypes = [vp, ctypes.c_int, dp, fp, fp, fp]
     L.rbc_velocity_at.argtypes = [vp, ctypes.c_int, fp, fp, fp, fp, fp, fp]
+    L.rbc_velocity_at_device.argtypes = [vp, ctypes.c_int] + [vp] * 6
     L.rbc_tracer_create.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_ulonglong, ctypes.POINTER(vp)]
     L.rbc_tracer_destroy.argtypes = [vp]
@@ -196,7 +223,7 @@ class Stepper:
 
     def _check(self, rc, what):
         if rc != 0:
-            raise RbcError(f"{what} failed ({rc}): {self.lib.rbc_last_error(self.h).decode()}")
+            _raise(rc, f"{what} failed ({rc}): {self.lib.rbc_last_error(self.h).decode()}")
 
     def close(self):
         for ref in getattr(self, "_dependents", []):
@@ -259,6 +286,46 @@ class Stepper:
     def calculate_implicit(self, theta, f, g, F, G):
         return self._op(self.lib.rbc_implicit, "rbc_implicit", (theta, f, g, F, G))
 
+    # ---- device-resident exchange (device pointers in the reference layout) ----------------
+    # Each argument is an integer device address (e.g. torch.Tensor.data_ptr()) or None; the
+    # element type is complex128 for precision "f64", complex64 otherwise.  The work is queued
+    # on the context's stream: call synchronize() before reading the outputs.
+    @staticmethod
+    def _ptrs(ptrs):
+        return [ctypes.c_void_p(int(p)) if p else None for p in ptrs]
+
+    def set_state_device(self, theta, f, g, F, G):
+        self._check(self.lib.rbc_set_state_device(self.h, *self._ptrs((theta, f, g, F, G))), "rbc_set_state_device")
+
+    def get_state_device(self, theta, f, g, F, G):
+        self._check(self.lib.rbc_get_state_device(self.h, *self._ptrs((theta, f, g, F, G))), "rbc_get_state_device")
+
+    def calculate_nonlinear_device(self, ins, outs):
+        self._check(self.lib.rbc_nonlinear_device(self.h, *self._ptrs(ins), *self._ptrs(outs)), "rbc_nonlinear_device")
+
+    def calculate_inverse_device(self, ins, outs):
+        self._check(self.lib.rbc_inverse_B_device(self.h, *self._ptrs(ins), *self._ptrs(outs)), "rbc_inverse_B_device")
+
+    def calculate_implicit_device(self, ins, outs):
+        self._check(self.lib.rbc_implicit_device(self.h, *self._ptrs(ins), *self._ptrs(outs)), "rbc_implicit_device")
+
+    INIT_KINDS = {"zero": 0, "random": 1, "test": 2}
+
+    def init_state(self, kind="random", seed=0, amplitude=1e-6):
+        """Initial state on the device (rbc_init_state): "zero", "random" (init_random's
+        distribution, project RNG) or "test" (init_test); make_real applied, history cleared."""
+        k = self.INIT_KINDS[kind] if isinstance(kind, str) else int(kind)
+        self._check(self.lib.rbc_init_state(self.h, k, int(seed), float(amplitude)), "rbc_init_state")
+
+    def force_mean(self, mean_F):
+        """F_0 = mean_F (force_mean, util/util.cu:74-81), queued on the context's stream."""
+        self._check(self.lib.rbc_force_mean(self.h, float(mean_F)), "rbc_force_mean")
+
+    @property
+    def stream(self):
+        """The context's hipStream_t as an integer."""
+        return int(self.lib.rbc_stream(self.h) or 0)
+
     def nusselt(self):
         nu = np.zeros(3)
         self._check(self.lib.rbc_nusselt(self.h, _dp(nu)), "rbc_nusselt")
@@ -302,6 +369,19 @@ class Stepper:
         self._check(self.lib.rbc_velocity_at(self.h, len(x), _fp(x), _fp(y), _fp(z), *[_fp(a) for a in out]),
                     "rbc_velocity_at")
         return out
+
+    def velocity_at_device(self, n, x, y, z, u, v, w):
+        """Velocity at n points on device float32 arrays (integer addresses); queued on the
+        context's stream (synchronize() before reading u, v, w)."""
+        self._check(self.lib.rbc_velocity_at_device(self.h, int(n), *self._ptrs((x, y, z, u, v, w))),
+                    "rbc_velocity_at_device")
+
+    def characteristic_velocity(self, nz=10):
+        """<|u|> over the grid at nz heights linspace(-1/2, 1/2, nz) (main_extract_vel.cu)."""
+        U = ctypes.c_double()
+        self._check(self.lib.rbc_characteristic_velocity(self.h, int(nz), ctypes.byref(U)),
+                    "rbc_characteristic_velocity")
+        return U.value
 
     def energy_spectrum(self):
         """(E_h per |k| shell, (E_h total, E_z total)), FP64."""

@@ -74,3 +74,36 @@ def test_ensemble_fp32_member_vs_fp64_twin(singles):
     err = np.abs(singles[3][0][:, keep] - ref[0][:, keep]).max() / np.abs(ref[0][:, keep]).max()
     print("128^2 x 24 FP32 vs FP64 after", STEPS, "steps:", err)
     assert err < 1e-4, err
+
+
+@pytest.mark.parametrize("spl", [5, 0])
+def test_ensemble_mixed_history_steps_every_member_n_times(spl):
+    """A member reset to an Euler start beside members that keep their AB2 history: one
+    rbc_ensemble_step(n) advances every member by exactly n steps (the reset member by an
+    Euler step then AB2, the others by AB2 throughout), equal to solo contexts stepped the
+    same way."""
+    print("case: ensemble mixed history, steps_per_launch", spl, flush=True)
+    from sdc_amd import Ensemble, Stepper
+    cfgs = CONFIGS[1:3]
+    states = [random_theta(c["M"], c["N"], c["L"], seed=2000 + i, amp=1e-3) for i, c in enumerate(cfgs)]
+    first, second = 3, 11
+    solo = []
+    for i, (c, st) in enumerate(zip(cfgs, states)):
+        with Stepper(c["M"], c["N"], c["L"], c["Lx"], c["Ly"], c["Ra"], c["Pr"], c["dt"],
+                     precision=c["precision"]) as S:
+            S.set_state(*st)
+            S.step_time(first)
+            if i == 0:
+                S.reset_history()
+            S.step_time(second)
+            solo.append(S.get_state())
+    with Ensemble(cfgs) as E:
+        for m, st in zip(E.members, states):
+            m.set_state(*st)
+        E.step(first, steps_per_launch=spl)
+        E.members[0].reset_history()
+        E.step(second, steps_per_launch=spl)
+        got = [m.get_state() for m in E.members]
+    for i, (g, r) in enumerate(zip(got, solo)):
+        for a, b in zip(g, r):
+            assert np.array_equal(a, b), f"member {i}"

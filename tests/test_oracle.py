@@ -231,3 +231,17 @@ def test_bundled_state_fixture_invariants():
     for n in ("theta", "f", "g"):
         assert float(d[n + "_removed_band_maxabs"]) == 0.0
     assert d["F"].shape == (2,) and d["G"].shape == (2,)
+
+
+def test_init_test_state_values():
+    """init_test (util/init.cu:263-301) at a few entries, against values typed from a table of
+    sin and cos (sin 1 = 0.8414709848078965, sin 7 = 0.6569865987187891, cos 3 =
+    -0.9899924966004454), and its zero pattern."""
+    th, f, g, F, G = O.init_test_state(32, 32, 8)
+    assert th[0, 0, 1] == pytest.approx(0.8414709848078965e-6, rel=1e-15)
+    assert th[5, 0, 7] == pytest.approx(0.6569865987187891e-6, rel=1e-15)
+    assert f[3, 0, 7] == th[3, 0, 7] and F[3] == pytest.approx(-0.9899924966004454e-6, rel=1e-15)
+    assert np.all(th[:, 1:, :] == 0) and np.all(th[:, 0, 0] == 0) and np.all(th.imag == 0)
+    assert not np.any(g) and not np.any(G)
+    th32 = O.init_test_state(32, 32, 8, fp32=True)[0]
+    assert th32[0, 0, 1] == np.float64(np.float32(1e-6) * np.sin(np.float32(1)))
