@@ -531,14 +531,25 @@ def main():
     st = sdc_amd.Stepper(W["M"], W["N"], W["L"], W["Lx"], W["Ly"], W["Ra"], W["Pr"], W["dt"],
                          precision=a.precision, device=0)
     st.set_state(*random_theta(W["M"], W["N"], W["L"], seed=3, amp=1e-3))
+    # device-wide wait around the timed region (torch.cuda.synchronize where torch sees the GPU;
+    # the torch build of this image does not, so hipDeviceSynchronize through ctypes)
     tsync = lambda: None  # noqa: E731
     if not a.no_torch:
+        have_torch_gpu = False
         try:
             import torch
-            if torch.cuda.is_available():
-                tsync = torch.cuda.synchronize
+            have_torch_gpu = torch.cuda.is_available()
         except Exception:
             pass
+        if have_torch_gpu:
+            tsync = torch.cuda.synchronize
+        else:
+            try:
+                import ctypes
+                _hip = ctypes.CDLL("libamdhip64.so")
+                tsync = lambda: _hip.hipDeviceSynchronize()  # noqa: E731
+            except OSError:
+                pass
     seed_state = random_theta(W["M"], W["N"], W["L"], seed=3, amp=1e-3)
 
     def reseed(s):
@@ -612,25 +623,35 @@ def main():
     state = st.get_state()
     boundary_ms = 1e3 * (time.perf_counter() - tb)
     # the same exchange on device buffers in the reference layout (rbc_set_state_device /
-    # rbc_get_state_device): state in and out without a step, mean of 20
+    # rbc_get_state_device): state in and out without a step, mean of 20.  Device memory comes
+    # from hipMalloc (the torch build of this image does not see the GPU).
     dev_boundary_us = None
     try:
-        import torch
-        if torch.cuda.is_available():
-            ct = torch.complex128 if a.precision == "f64" else torch.complex64
-            bufs = [torch.from_numpy(x).to("cuda:0", dtype=ct) for x in state]
-            torch.cuda.synchronize()
-            ptrs = [t.data_ptr() for t in bufs]
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so")
+        hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        hip.hipFree.argtypes = [ctypes.c_void_p]
+        hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        ct = np.complex128 if a.precision == "f64" else np.complex64
+        ptrs = []
+        for x in state:
+            h = np.ascontiguousarray(x.astype(ct))
+            p = ctypes.c_void_p()
+            assert hip.hipMalloc(ctypes.byref(p), h.nbytes) == 0
+            assert hip.hipMemcpy(p, h.ctypes.data_as(ctypes.c_void_p), h.nbytes, 1) == 0
+            ptrs.append(p.value)
+        st.set_state_device(*ptrs)
+        st.get_state_device(*ptrs)
+        st.synchronize()
+        tb = time.perf_counter()
+        for _ in range(20):
             st.set_state_device(*ptrs)
             st.get_state_device(*ptrs)
-            st.synchronize()
-            tb = time.perf_counter()
-            for _ in range(20):
-                st.set_state_device(*ptrs)
-                st.get_state_device(*ptrs)
-            st.synchronize()
-            dev_boundary_us = 1e6 * (time.perf_counter() - tb) / 20
-    except ImportError:
+        st.synchronize()
+        dev_boundary_us = 1e6 * (time.perf_counter() - tb) / 20
+        for p in ptrs:
+            hip.hipFree(ctypes.c_void_p(p))
+    except OSError:
         pass
     st.close()
     # the other single-GPU precision forms on the same workload (plain step_time, no stage events)

@@ -51,6 +51,7 @@ def lib():
         L.orc_reset_history.argtypes = [ctypes.c_void_p]
         L.orc_nusselt.argtypes = [ctypes.c_void_p, dp]
         L.orc_implicit_factor.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, dp]
+        L.orc_set_penalization.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -191,6 +192,13 @@ class Stepper:
         nu = np.zeros(3)
         lib().orc_nusselt(self.h, _p(nu))
         return nu
+
+    MASKS = {"rectangle": 0, "circle": 1, "diskarray": 2, "everywhere": 3}
+
+    def set_penalization(self, eta, shape="rectangle"):
+        """Penalized build (PENALIZED): mask terms added to the nonlinear operator."""
+        sh = self.MASKS[shape] if isinstance(shape, str) else int(shape)
+        assert lib().orc_set_penalization(self.h, float(eta), sh) == 0
 
     def implicit_factor(self, col, row):
         out = np.zeros(4 * self.L * self.L)

@@ -362,6 +362,8 @@ struct orc_ctx {
     int have_hist;
     double *imp;  /* per (col,row): 4*L*L implicit blocks TT, Tf, fT, ff */
     double *invBf; /* per (col,row): L*L */
+    double eta;    /* penalization parameter; 0 = none (no PENALIZED build) */
+    double* mask;  /* M*N physical mask (x fastest) when eta > 0 */
 };
 
 #define TWO(c, t, i, j) ((c)->two[(size_t)(t) * (c)->L * (c)->L + (i) + (size_t)(j) * (c)->L])
@@ -498,7 +500,7 @@ void orc_destroy(orc_ctx* c) {
     if (!c) return;
     free(c->two); free(c->three); free(c->th); free(c->f); free(c->g); free(c->F); free(c->G);
     for (int i = 0; i < 5; ++i) free(c->r1[i]);
-    free(c->imp); free(c->invBf);
+    free(c->imp); free(c->invBf); free(c->mask);
     free(c);
 }
 
@@ -663,6 +665,127 @@ static double coef(const orc_ctx* c, const summand* s, int i, int j, int k) {
     }
 }
 
+/* ---- Penalization (PENALIZED builds) ---------------------------------------------------- */
+/* The masks of nonlinear_operator_rayleigh_noslip.cu:1652-1740 on the M x N grid (index
+ * col + row * M, get_current_matrix_indices :1416-1424): 1 inside the penalized region. */
+static void build_mask(int shape, int columns, int rows, double* mask) {
+    for (int row = 0; row < rows; ++row)
+        for (int col = 0; col < columns; ++col) {
+            double v = 0.0;
+            if (shape == ORC_MASK_RECTANGLE) { /* :1654-1671 */
+                if (col < 3 || col > columns - 4 || row < 3 || row > rows - 4) v = 1.0;
+            } else if (shape == ORC_MASK_CIRCLE) { /* :1673-1693 */
+                int d2 = (col - columns / 2) * (col - columns / 2) + (row - rows / 2) * (row - rows / 2);
+                int mn = columns;
+                if (rows < columns) mn = rows;
+                if (d2 > (mn / 2 - 3) * (mn / 2 - 3)) v = 1.0;
+            } else if (shape == ORC_MASK_DISKARRAY) { /* :1695-1727, 3 x 2 disks */
+                const int nx = 3, ny = 2;
+                for (int i = 0; i < nx; ++i)
+                    for (int j = 0; j < ny; ++j) {
+                        int cx = i * columns / nx + columns / (nx * 2);
+                        int cy = j * rows / ny + rows / (ny * 2);
+                        int d2 = (cx - col) * (cx - col) + (cy - row) * (cy - row);
+                        int r = columns / (nx * 2);
+                        if (r > rows / (ny * 2)) r = rows / (ny * 2);
+                        if (d2 < r) v = 1.0; /* squared distance against the radius, as there */
+                    }
+            } else { /* everywhere, :1729-1738 */
+                v = 1.0;
+            }
+            mask[(size_t)row * columns + col] = v;
+        }
+}
+
+int orc_set_penalization(orc_ctx* c, double eta, int shape) {
+    if (eta < 0.0 || shape < 0 || shape > ORC_MASK_EVERYWHERE) return -1;
+    free(c->mask);
+    c->mask = NULL;
+    c->eta = eta;
+    if (eta == 0.0) return 0;
+    c->mask = malloc(sizeof(double) * (size_t)c->M * c->N);
+    if (!c->mask) return -2;
+    build_mask(shape, c->M, c->N, c->mask);
+    return 0;
+}
+
+/* The penalization summands of nonlinear...cu:1100-1343 for every mode j, added into the
+ * unscaled sums (the caller scales theta by -1 and f, g, F, G by -1/Pr afterwards):
+ * h_ab_j = R2C(mask * u_ab_j / (M N)) for the mode-j physical fields u11, u21, u31, u12, u22,
+ * theta; then
+ *   f_k += Pr/eta [Iac(j,k) d_x h11 + Ia2(k,j) d_x h12 + Iac(j,k) d_y h21 + Ia2(k,j) d_y h22],
+ *   f_j += -Pr/eta Lap h31;
+ *   g_k += Pr/eta (-Ia2(j,k)) d_y h11 - Pr/eta (-Ia2(j,k)) d_x h21;
+ *   g_j += 0.5 Pr/eta d_y h12 - 0.5 Pr/eta d_y h22;
+ *   F_k += Pr/eta (-Ia2(j,k)) h11(0,0), F_j += 0.5 Pr/eta h12(0,0); G likewise with h21, h22;
+ *   theta_j += 0.5/eta h_theta. */
+static int penalty_terms(orc_ctx* c, const double* phys, cplx* acc, cplx* accF, cplx* accG) {
+    const int L = c->L, M = c->M, N = c->N;
+    const size_t plane = c->plane, X = (size_t)M * N;
+    const double scale = 1.0 / (double)X, pe = c->Pr / c->eta;
+    double* buf = malloc(sizeof(double) * X);
+    cplx* H = malloc(sizeof(cplx) * 6 * plane);
+    cplx* D = malloc(sizeof(cplx) * plane);
+    if (!buf || !H || !D) { free(buf); free(H); free(D); return -2; }
+    static const int src[6] = {U11, U21, U31, U12, U22, TH};
+    cplx* accT = acc;
+    cplx* accf = acc + (size_t)L * plane;
+    cplx* accg = acc + 2 * (size_t)L * plane;
+    for (int j = 0; j < L; ++j) {
+        for (int a = 0; a < 6; ++a) {
+            const double* u = phys + ((size_t)src[a] * L + j) * X;
+            for (size_t q = 0; q < X; ++q) buf[q] = scale * c->mask[q] * u[q];
+            orc_fft_r2c(M, N, buf, (double*)(H + (size_t)a * plane));
+        }
+        const cplx *h11 = H, *h21 = H + plane, *h31 = H + 2 * plane, *h12 = H + 3 * plane, *h22 = H + 4 * plane,
+                   *ht = H + 5 * plane;
+        /* f */
+        const cplx* fx[4] = {h11, h12, h21, h22};
+        const int fd[4] = {D_X, D_X, D_Y, D_Y};
+        for (int t = 0; t < 4; ++t) {
+            apply_deriv(c, fd[t], fx[t], D);
+            for (int k = 0; k < L; ++k) {
+                const double C = (t == 0 || t == 2) ? TWO(c, ORC_IAC, j, k) : TWO(c, ORC_IA2, k, j);
+                if (C == 0.0) continue;
+                cplx* out = accf + (size_t)k * plane;
+                for (size_t q = 0; q < plane; ++q) out[q] += D[q] * (pe * C);
+            }
+        }
+        apply_deriv(c, D_LAP, h31, D);
+        for (size_t q = 0; q < plane; ++q) accf[(size_t)j * plane + q] += D[q] * (-1.0 * pe);
+        /* g */
+        apply_deriv(c, D_Y, h11, D);
+        for (int k = 0; k < L; ++k) {
+            const double C = -TWO(c, ORC_IA2, j, k);
+            if (C == 0.0) continue;
+            for (size_t q = 0; q < plane; ++q) accg[(size_t)k * plane + q] += D[q] * (pe * C);
+        }
+        apply_deriv(c, D_Y, h12, D);
+        for (size_t q = 0; q < plane; ++q) accg[(size_t)j * plane + q] += D[q] * (0.5 * pe);
+        apply_deriv(c, D_X, h21, D);
+        for (int k = 0; k < L; ++k) {
+            const double C = -TWO(c, ORC_IA2, j, k);
+            if (C == 0.0) continue;
+            for (size_t q = 0; q < plane; ++q) accg[(size_t)k * plane + q] += D[q] * (-1.0 * pe * C);
+        }
+        apply_deriv(c, D_Y, h22, D);
+        for (size_t q = 0; q < plane; ++q) accg[(size_t)j * plane + q] += D[q] * (-0.5 * pe);
+        /* F, G */
+        for (int k = 0; k < L; ++k) {
+            const double C = -TWO(c, ORC_IA2, j, k);
+            if (C == 0.0) continue;
+            accF[k] += h11[0] * (pe * C);
+            accG[k] += h21[0] * (pe * C);
+        }
+        accF[j] += h12[0] * (0.5 * pe);
+        accG[j] += h22[0] * (0.5 * pe);
+        /* theta */
+        for (size_t q = 0; q < plane; ++q) accT[(size_t)j * plane + q] += ht[q] * (0.5 / c->eta);
+    }
+    free(buf); free(H); free(D);
+    return 0;
+}
+
 static int nonlinear_impl(orc_ctx* c, const double* th_, const double* f_, const double* g_, const double* F_,
                           const double* G_, double* oth, double* of, double* og, double* oF, double* oG,
                           int pair_stride, double* t_phys, double* t_pairs) {
@@ -758,8 +881,13 @@ static int nonlinear_impl(orc_ctx* c, const double* th_, const double* f_, const
         }
         free(buf); free(P); free(D);
     }
-    free(phys);
     if (t_pairs) *t_pairs = omp_wtime_() - t_pair0;
+    /* penalization (PENALIZED builds, nonlinear...cu:1097-1343), added to thread 0's sums */
+    if (c->eta > 0.0) {
+        int rc = penalty_terms(c, phys, acc, acc + 3 * (size_t)L * plane, acc + 3 * (size_t)L * plane + L);
+        if (rc) { free(phys); free(acc); return rc; }
+    }
+    free(phys);
     /* reduce in thread order, then scale (:1366-1370) */
     cplx* outs[3] = {(cplx*)oth, (cplx*)of, (cplx*)og};
     const double sc[3] = {-1.0, -1.0 / c->Pr, -1.0 / c->Pr};

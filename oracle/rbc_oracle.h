@@ -64,6 +64,12 @@ int orc_implicit(orc_ctx* c, const double* th, const double* f, const double* g,
 void orc_dealias(int mox, int N, int L, double* field);   /* implicit_explicit_stepping.cu:163-182 */
 void orc_make_real(int mox, int N, int L, double* field); /* util.cu make_real */
 
+/* Penalization of a PENALIZED build (nonlinear_operator_rayleigh_noslip.cu:1097-1343) with
+ * the mask shapes of :1652-1740 (rectangle: the default build, circle: -DCIRCLE, disk array:
+ * -DDISKARRAY, everywhere: the commented alternative); eta = 0 turns it off. */
+enum { ORC_MASK_RECTANGLE = 0, ORC_MASK_CIRCLE, ORC_MASK_DISKARRAY, ORC_MASK_EVERYWHERE };
+int orc_set_penalization(orc_ctx* c, double eta, int shape);
+
 /* One reference time step (step_time + make_real_signal), nsteps times. */
 int orc_step(orc_ctx* c, int nsteps);
 void orc_reset_history(orc_ctx* c);

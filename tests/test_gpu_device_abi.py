@@ -2,8 +2,8 @@
 device buffers in the reference layout, element type per precision.  They run the same
 kernels as the host-buffer calls, so their results must equal those bit for bit, and the
 host calls are pinned to the oracle; the operators are also compared with the oracle here.
-Device memory comes from torch (plumbing only); the library works on its own stream, so the
-test synchronises torch before a call and the context before reading."""
+Device memory comes from hipMalloc (tests/hipmem.py); the library works on its own stream, so
+the test synchronises the device after uploads and the context before reading."""
 import numpy as np
 import pytest
 
@@ -18,26 +18,25 @@ CASES = [(dict(M=32, N=32, L=8, Lx=4.032, Ly=4.032, Ra=5000.0, Pr=1.0, dt=1e-3),
 
 
 def _dev(arrays, precision):
-    import torch
-    dt = torch.complex128 if precision == "f64" else torch.complex64
-    npdt = np.complex128 if precision == "f64" else np.complex64
-    return [torch.from_numpy(np.ascontiguousarray(a.astype(npdt))).to("cuda:0", dtype=dt) for a in arrays]
+    from tests.hipmem import DeviceArray
+    dt = np.complex128 if precision == "f64" else np.complex64
+    return [DeviceArray(a.astype(dt)) for a in arrays]
 
 
-def _host(tensors):
-    return [t.cpu().numpy().astype(np.complex128) for t in tensors]
+def _host(bufs):
+    return [b.download().astype(np.complex128) for b in bufs]
 
 
 def _like(st, precision, fill=1.0):
-    import torch
-    dt = torch.complex128 if precision == "f64" else torch.complex64
-    return [torch.full(a.shape, fill, dtype=dt, device="cuda:0") for a in st]
+    from tests.hipmem import DeviceArray
+    dt = np.complex128 if precision == "f64" else np.complex64
+    return [DeviceArray.full(a.shape, dt, fill) for a in st]
 
 
 @pytest.mark.parametrize("cfg,precision,tol", CASES, ids=["32x32x8-f64", "64x32x16-f32"])
 def test_device_state_and_operators(cfg, precision, tol):
-    import torch
     from sdc_amd import Stepper
+    from tests.hipmem import synchronize
     M, N, L = cfg["M"], cfg["N"], cfg["L"]
     print(f"case: device ABI {M}x{N}x{L} {precision}", flush=True)
     st = random_state(M, N, L, seed=M + L, amp=1e-2, decay=0.05)
@@ -50,12 +49,12 @@ def test_device_state_and_operators(cfg, precision, tol):
     with Stepper(M, N, L, cfg["Lx"], cfg["Ly"], cfg["Ra"], cfg["Pr"], cfg["dt"], precision=precision) as S:
         host = [S.calculate_nonlinear(*st), S.calculate_inverse(*st), S.calculate_implicit(*st)]
         din = _dev(st, precision)
-        torch.cuda.synchronize()
+        synchronize()
         calls = [S.calculate_nonlinear_device, S.calculate_inverse_device, S.calculate_implicit_device]
         for k, call in enumerate(calls):
             out = _like(st, precision)  # the removed band must come back as zeros
-            torch.cuda.synchronize()
-            call([t.data_ptr() for t in din], [t.data_ptr() for t in out])
+            synchronize()
+            call([t.ptr for t in din], [t.ptr for t in out])
             S.synchronize()
             got = _host(out)
             for a, b in zip(got, host[k]):
@@ -65,10 +64,11 @@ def test_device_state_and_operators(cfg, precision, tol):
             errs = [rel_err(got[i][:, keep], ref[k][i][:, keep]) for i in range(3)]
             assert max(errs) < (tol if k == 0 else 10 * tol), (k, errs)
         # state round trip and steps on device buffers equal the host-buffer path
-        S.set_state_device(*[t.data_ptr() for t in din])
+        S.set_state_device(*[t.ptr for t in din])
         S.step_time(4)
         out = _like(st, precision, fill=0.0)
-        S.get_state_device(*[t.data_ptr() for t in out])
+        synchronize()
+        S.get_state_device(*[t.ptr for t in out])
         S.synchronize()
         dev_steps = _host(out)
         S.set_state(*st)
@@ -82,8 +82,8 @@ def test_device_state_and_operators(cfg, precision, tol):
 def test_velocity_at_device_equals_host_call():
     """rbc_velocity_at_device on device arrays equals rbc_velocity_at on host arrays bit for bit
     (same plan and kernels), across calls that grow and then reuse the context's scratch."""
-    import torch
     from sdc_amd import Stepper
+    from tests.hipmem import DeviceArray, synchronize
     print("case: velocity_at_device", flush=True)
     cfg = (64, 64, 16, 4.0, 3.0, 2e4, 0.7, 1e-3)
     st = random_state(64, 64, 16, seed=77, amp=5e-2, decay=0.1)
@@ -95,10 +95,10 @@ def test_velocity_at_device_equals_host_call():
             y = rng.uniform(0, 3.0, n).astype(np.float32)
             z = rng.uniform(-0.5, 0.5, n).astype(np.float32)
             host = S.velocity_at(x, y, z)
-            dx, dy, dz = (torch.from_numpy(a).to("cuda:0") for a in (x, y, z))
-            out = [torch.zeros(n, dtype=torch.float32, device="cuda:0") for _ in range(3)]
-            torch.cuda.synchronize()
-            S.velocity_at_device(n, dx.data_ptr(), dy.data_ptr(), dz.data_ptr(), *[o.data_ptr() for o in out])
+            dx, dy, dz = (DeviceArray(a) for a in (x, y, z))
+            out = [DeviceArray(np.zeros(n, np.float32)) for _ in range(3)]
+            synchronize()
+            S.velocity_at_device(n, dx.ptr, dy.ptr, dz.ptr, *[o.ptr for o in out])
             S.synchronize()
             for a, b in zip(out, host):
-                assert np.array_equal(a.cpu().numpy(), b), n
+                assert np.array_equal(a.download(), b), n

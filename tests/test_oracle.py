@@ -245,3 +245,80 @@ def test_init_test_state_values():
     assert not np.any(g) and not np.any(G)
     th32 = O.init_test_state(32, 32, 8, fp32=True)[0]
     assert th32[0, 0, 1] == np.float64(np.float32(1e-6) * np.sin(np.float32(1)))
+
+
+def test_penalization_everywhere_closed_forms():
+    """With the mask 1 everywhere the Brinkman terms of a PENALIZED build
+    (nonlinear...cu:1097-1343) reduce to closed forms in mode space (each h field is the
+    mode field itself): N_theta = -theta / (2 eta); N_f,k = -(1/eta) sum_j (k^4 delta_jk -
+    k^2 Iac(j,k)) f_j (the f mass operator); N_g = (ky^2 + kx ky) g / (2 eta) (the reference's
+    fourth g summand differentiates in y, :1265-1272); N_F = -F / (2 eta), N_G = -G / (2 eta)."""
+    M, N, L, Lx, Ly, Pr, eta = 32, 32, 8, 4.0, 3.0, 0.7, 0.05
+    st = random_state(M, N, L, seed=3, amp=1e-2)
+    o = O.Stepper(M, N, L, Lx, Ly, 2e4, Pr, 1e-3)
+    base = o.nonlinear(*st)
+    o.set_penalization(eta, "everywhere")
+    d = [p - b for p, b in zip(o.nonlinear(*st), base)]
+    o.close()
+    kx = (np.arange(M // 2 + 1) * 2 * np.pi / Lx)[None, :] * np.ones((N, 1))
+    r = np.arange(N)
+    ky = (np.where(r > N // 2, r - N, r) * 2 * np.pi / Ly)[:, None] * np.ones((1, M // 2 + 1))
+    k2 = kx ** 2 + ky ** 2
+    iac = O.coeff_tables(L)[0]["Iac"]
+    th, f, g, F, G = st
+    want = [-(0.5 / eta) * th,
+            -(1 / eta) * (k2[None] ** 2 * f - k2[None] * np.einsum("jk,jyx->kyx", iac, f)),
+            (0.5 / eta) * (ky ** 2 + kx * ky)[None] * g]
+    keep = kept_mask(M, N)
+    for a, b in zip(d[:3], want):
+        assert rel_err(a[:, keep], b[:, keep]) < 1e-13
+    assert rel_err(d[3], -(0.5 / eta) * F) < 1e-11 and rel_err(d[4], -(0.5 / eta) * G) < 1e-11
+
+
+def test_penalization_masks_and_linearity():
+    """The penalty (the difference the mask terms make to the nonlinear operator) is linear in
+    the state for every mask shape (the rest of the operator is quadratic) and not zero."""
+    M, N, L, Lx, Ly, Pr, eta = 32, 32, 4, 4.0, 4.0, 1.0, 0.1
+    a = random_state(M, N, L, seed=1, amp=1e-2)
+    b = random_state(M, N, L, seed=2, amp=1e-2)
+    o = O.Stepper(M, N, L, Lx, Ly, 2e4, Pr, 1e-3)
+
+    def pen(st, shape):
+        o.set_penalization(0.0, shape)
+        n0 = o.nonlinear(*st)
+        o.set_penalization(eta, shape)
+        return [x - y for x, y in zip(o.nonlinear(*st), n0)]
+    keep = kept_mask(M, N)
+    for shape in ("rectangle", "circle", "diskarray"):
+        pa, pb = pen(a, shape), pen(b, shape)
+        pab = pen([x + 2 * y for x, y in zip(a, b)], shape)
+        for i in range(3):
+            assert rel_err(pab[i][:, keep], pa[i][:, keep] + 2 * pb[i][:, keep]) < 1e-12
+        assert max(np.abs(x).max() for x in pa[:3]) > 0
+    o.close()
+
+
+@pytest.mark.parametrize("shape,inside,outside", [
+    ("rectangle", [(0, 0), (2, 10), (31, 5), (10, 29)], [(3, 3), (16, 16), (28, 28)]),
+    ("circle", [(0, 0), (31, 31), (16, 2)], [(16, 16), (16, 4), (4, 16)]),
+    ("diskarray", [(5, 8), (16, 24), (26, 8)], [(0, 0), (16, 16), (10, 8)]),
+])
+def test_penalization_mask_points(shape, inside, outside):
+    """Mask values at hand-picked grid points (col, row) of a 32 x 32 grid: rectangle border of
+    3 points; circle: outside radius 13 around (16, 16); disk array: 3 x 2 disk centres at
+    x = 5, 15, 26 and y = 8, 24 with squared distance below 5."""
+    M = N = 32
+    o = O.Stepper(M, N, 2, 4.0, 4.0, 2e4, 1.0, 1e-3)
+    # a single Fourier mode theta(kx=0, ky=0, m=0) = 1: the theta penalty is the mask's transform
+    th = np.zeros((2, N, M // 2 + 1), np.complex128)
+    th[0, 0, 0] = 1.0
+    st = [th, np.zeros_like(th), np.zeros_like(th), np.zeros(2, complex), np.zeros(2, complex)]
+    o.set_penalization(1.0, shape)
+    nt = o.nonlinear(*st)[0][0]
+    o.close()
+    # theta_0 physical = 1 everywhere, so N_theta,0 = -(1/2) R2C(mask / (M N)) (unnormalised R2C)
+    mask = -2.0 * (M * N) * np.fft.irfft2(nt, s=(N, M))
+    for col, row in inside:
+        assert abs(mask[row, col] - 1.0) < 1e-12, (col, row)
+    for col, row in outside:
+        assert abs(mask[row, col]) < 1e-12, (col, row)
