@@ -202,6 +202,14 @@ int rbc_velocity_at(rbc_ctx* c, int n, const float* x, const float* y, const flo
  * scratch, grown (with a wait) when n exceeds every earlier call's. */
 int rbc_velocity_at_device(rbc_ctx* c, int n, const float* x, const float* y, const float* z,
                            float* u, float* v, float* w);
+/* rbc_velocity_at_device without any allocation: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops).  The
+ * partial sums live in a file-scope device array of 16 MiB, one per GPU, held until the process
+ * ends; n above 1,398,101 points is rejected with RBC_ERR_ARG.  All contexts on one GPU share
+ * that array: calls that overlap in time on one GPU (other streams, other threads) race, so
+ * order them.  Results equal rbc_velocity_at_device's. */
+int rbc_velocity_at_device_noalloc(rbc_ctx* c, int n, const float* x, const float* y, const float* z,
+                                   float* u, float* v, float* w);
 
 /* ---- Passive tracers (≙ particle_tracer, particle/particle_tracer.h:21-89) ----------------
  * Positions are FP32, stored as a ring of buffer_size time slots (the reference's layout

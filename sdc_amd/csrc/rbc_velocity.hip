@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -661,6 +662,34 @@ size_t part_floats(rbc_ctx* c, int np) {
     return (size_t)pw.nslice * 3 * np;
 }
 
+// ---- scratch of rbc_velocity_at_device_noalloc ------------------------------------------------
+// The partial sums of the spectral point evaluation (part[slice][3][n]) for callers that cannot
+// allocate: a file-scope device array, one copy on every GPU the library runs on, held until
+// the process ends.  Its address on a device comes from hipGetSymbolAddress, taken once per
+// device by rbc_create with that device current; host code uses that address, never the
+// array's name.  16 MiB holds the partial sums of every n <= 1,398,101 points.
+constexpr size_t kNoAllocScratchFloats = (size_t)4 << 20;
+static __device__ __align__(256) float scratch[kNoAllocScratchFloats];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// address of this device's copy of `scratch` (the device is current); nullptr when the runtime
+// cannot give it (the no-allocation entry point then fails with RBC_ERR_HIP, nothing else does)
+void* noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        g_scratch_ptr[device] = ptr;
+    }
+    return g_scratch_ptr[device];
+}
+
 // Per-context workspace of the velocity diagnostics.  lam, modes and the spectrum scratch are
 // made by velocity_setup (rbc_create); the grid work and the point partial sums of the
 // host-buffer calls grow on demand and are kept for later calls.
@@ -679,6 +708,7 @@ struct VelWork {
     float* io = nullptr;      // rbc_velocity_at points and results, 6 n floats
     size_t io_cap = 0;
     double* cpart = nullptr;  // characteristic velocity: per block (sum, count)
+    float* noalloc = nullptr; // this device's copy of `scratch` (rbc_velocity_at_device_noalloc)
     std::vector<void*> owned;
     ~VelWork() {
         for (void* p : gw_owned) hipFree(p);
@@ -774,6 +804,7 @@ void velocity_setup(rbc_ctx* c) {
     vw->ew = (double*)al((size_t)vw->ez_cap * sizeof(double));
     vw->ee = (double*)al((size_t)3 * P * sizeof(double));
     vw->cpart = (double*)al((size_t)2 * kCharvelBlocks * sizeof(double));
+    vw->noalloc = (float*)noalloc_scratch(c->p.device);
 }
 
 }  // namespace rbc
@@ -1126,6 +1157,33 @@ int rbc_velocity_at(rbc_ctx* c, int n, const float* x, const float* y, const flo
         HIPCHK(hipMemcpyAsync(v, o + n, b, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(w, o + 2 * (size_t)n, b, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+int rbc_velocity_at_device_noalloc(rbc_ctx* c, int n, const float* x, const float* y, const float* z, float* u,
+                                   float* v, float* w) {
+    if (!c || n < 0 || (n > 0 && (!x || !y || !z || !u || !v || !w))) return RBC_ERR_ARG;
+    if (lmax_for(c->dg.L) == 0) return fail(c, RBC_ERR_ARG, "spectral evaluation supports L <= 32");
+    if (n == 0) return RBC_OK;
+    VelWork* vw = vel_work(c);
+    if (!vw->noalloc) return fail(c, RBC_ERR_HIP, "no-allocation scratch not available on this device");
+    PointWork pw;
+    plan_points(c, n, pw);
+    const size_t need = (size_t)pw.nslice * 3 * n;
+    if (need > kNoAllocScratchFloats)
+        return fail(c, RBC_ERR_ARG,
+                    std::to_string(n) + " points need " + std::to_string(need) +
+                        " floats of partial sums; the no-allocation scratch holds " +
+                        std::to_string(kNoAllocScratchFloats) + " (every n <= 1398101 fits)");
+    pw.lam = vw->lam;
+    pw.modes = vw->modes;
+    pw.part = vw->noalloc;  // every slot the sum reads is written first in the same call
+    try {
+        hipSetDevice(c->p.device);
+        velocity_points(c, n, x, y, z, pw, u, v, w);
     } catch (const HipError& e) {
         return hip_fail(c, e);
     }

@@ -70,7 +70,7 @@ EXPORTS = ["rbc_create", "rbc_destroy", "rbc_last_error", "rbc_get_info", "rbc_s
            "rbc_closure_status", "rbc_set_state_device", "rbc_get_state_device", "rbc_nonlinear_device",
            "rbc_inverse_B_device", "rbc_implicit_device", "rbc_stream", "rbc_velocity_at_device",
            "rbc_init_state", "rbc_force_mean", "rbc_characteristic_velocity", "rbc_run_main",
-           "rbc_extract_vel_main"]
+           "rbc_extract_vel_main", "rbc_velocity_at_device_noalloc"]
 
 CLOSURE_NIN, CLOSURE_NOUT = 8, 3
 
@@ -127,6 +127,7 @@ def load_library(path=LIB_PATH):
     L.rbc_velocity_grid.argtypes = [vp, ctypes.c_int, dp, fp, fp, fp]
     L.rbc_velocity_at.argtypes = [vp, ctypes.c_int, fp, fp, fp, fp, fp, fp]
     L.rbc_velocity_at_device.argtypes = [vp, ctypes.c_int] + [vp] * 6
+    L.rbc_velocity_at_device_noalloc.argtypes = [vp, ctypes.c_int] + [vp] * 6
     L.rbc_tracer_create.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_ulonglong, ctypes.POINTER(vp)]
     L.rbc_tracer_destroy.argtypes = [vp]
@@ -382,6 +383,12 @@ class Stepper:
         self._check(self.lib.rbc_characteristic_velocity(self.h, int(nz), ctypes.byref(U)),
                     "rbc_characteristic_velocity")
         return U.value
+
+    def velocity_at_device_noalloc(self, n, x, y, z, u, v, w):
+        """velocity_at_device with the partial sums in the library's per-GPU static scratch (no
+        allocation; n <= 1,398,101; overlapping calls on one GPU race)."""
+        self._check(self.lib.rbc_velocity_at_device_noalloc(self.h, int(n), *self._ptrs((x, y, z, u, v, w))),
+                    "rbc_velocity_at_device_noalloc")
 
     def energy_spectrum(self):
         """(E_h per |k| shell, (E_h total, E_z total)), FP64."""
