@@ -490,7 +490,7 @@ def main():
     ap.add_argument("--no-variants", action="store_true", help="skip timing the other precision form")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-torch", action="store_true", help="bracket the timed region with the library's "
-                    "stream synchronisation only (for profiler runs)")
+                    "stream synchronisation only, no device-wide wait (for profiler runs)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MB",
                     help="also run one device-to-device copy of MB mebibytes (a kernel of known traffic "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -531,25 +531,17 @@ def main():
     st = sdc_amd.Stepper(W["M"], W["N"], W["L"], W["Lx"], W["Ly"], W["Ra"], W["Pr"], W["dt"],
                          precision=a.precision, device=0)
     st.set_state(*random_theta(W["M"], W["N"], W["L"], seed=3, amp=1e-3))
-    # device-wide wait around the timed region (torch.cuda.synchronize where torch sees the GPU;
-    # the torch build of this image does not, so hipDeviceSynchronize through ctypes)
+    # device-wide wait around the timed region: hipDeviceSynchronize of the HIP runtime the
+    # library loaded (torch is not imported: its bundled HIP runtime does not see the GPU of the
+    # test machines, and a second runtime in the process breaks hipMalloc)
     tsync = lambda: None  # noqa: E731
     if not a.no_torch:
-        have_torch_gpu = False
         try:
-            import torch
-            have_torch_gpu = torch.cuda.is_available()
-        except Exception:
+            import ctypes
+            _hip = ctypes.CDLL("libamdhip64.so")
+            tsync = lambda: _hip.hipDeviceSynchronize()  # noqa: E731
+        except OSError:
             pass
-        if have_torch_gpu:
-            tsync = torch.cuda.synchronize
-        else:
-            try:
-                import ctypes
-                _hip = ctypes.CDLL("libamdhip64.so")
-                tsync = lambda: _hip.hipDeviceSynchronize()  # noqa: E731
-            except OSError:
-                pass
     seed_state = random_theta(W["M"], W["N"], W["L"], seed=3, amp=1e-3)
 
     def reseed(s):
@@ -624,7 +616,7 @@ def main():
     boundary_ms = 1e3 * (time.perf_counter() - tb)
     # the same exchange on device buffers in the reference layout (rbc_set_state_device /
     # rbc_get_state_device): state in and out without a step, mean of 20.  Device memory comes
-    # from hipMalloc (the torch build of this image does not see the GPU).
+    # from hipMalloc of the library's HIP runtime.
     dev_boundary_us = None
     try:
         import ctypes

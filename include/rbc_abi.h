@@ -54,6 +54,7 @@ typedef struct {
 } rbc_params;
 
 typedef struct rbc_ctx rbc_ctx;
+typedef struct rbc_tracer rbc_tracer;  /* passive tracers (below); destroy before its context */
 
 /* ≙ Coeff<double>(L) + linear_implicit_operator::init_operator + inverse_B_matrix::
  *   init_operator + nonlinear_operator_rayleigh_noslip::init_operator +
@@ -143,6 +144,56 @@ void* rbc_stream(rbc_ctx* c);
 #define RBC_INIT_RANDOM 1
 #define RBC_INIT_TEST 2
 int rbc_init_state(rbc_ctx* c, int kind, unsigned long long seed, double amplitude);
+/* ---- Penalization and plane operators ------------------------------------------------------
+ * rbc_set_penalization ≙ the PENALIZED builds (nonlinear_operator_rayleigh_noslip.cu:
+ * 1097-1343): Brinkman terms of a mask chi(x, y) added to the nonlinear term with penalization
+ * parameter eta (main.cu -eta; eta = 0 turns them off).  Masks (:1652-1740): RBC_MASK_RECTANGLE
+ * (3-point border; the default PENALIZED build), RBC_MASK_CIRCLE (-DCIRCLE), RBC_MASK_DISKARRAY
+ * (3 x 2 disks, -DDISKARRAY), RBC_MASK_EVERYWHERE, or RBC_MASK_CUSTOM with `mask` holding
+ * M x N values (x fastest).  A setup call (allocates; waits for the stream). */
+#define RBC_MASK_RECTANGLE 0
+#define RBC_MASK_CIRCLE 1
+#define RBC_MASK_DISKARRAY 2
+#define RBC_MASK_EVERYWHERE 3
+#define RBC_MASK_CUSTOM 4
+int rbc_set_penalization(rbc_ctx* c, double eta, int shape, const double* mask);
+/* Operators on the state's mode planes, applied between steps as main.cu:569-590 does:
+ *   rbc_freeze        ≙ freeze_operator::freeze_circle (radius) / freeze_rectangle (width):
+ *                       captures the physical planes of theta, f, g and the mask
+ *   rbc_apply_freeze  ≙ freeze_operator::calculate_operator: outside the mask the planes
+ *                       return to the captured ones (nothing before rbc_freeze)
+ *   rbc_temp_mask     ≙ masktemp::temp_circle / temp_rectangle (the heat-source region)
+ *   rbc_apply_temp_mask ≙ masktemp::calculate_operator: theta + 1.5/100 in the region
+ *   rbc_mask_theta    ≙ masking::set_mask_parameters + masker: theta times `factor` inside a
+ *                       circle at (cx, cy) (fractions of the box) of radius r (fraction of min/2)
+ * Shapes: RBC_PLANE_CIRCLE (param = radius as a fraction of min(M, N) / 2), RBC_PLANE_RECTANGLE
+ * (param = border width as a fraction of the side).  Enqueued on the context's stream. */
+#define RBC_PLANE_CIRCLE 0
+#define RBC_PLANE_RECTANGLE 1
+int rbc_freeze(rbc_ctx* c, int shape, double param);
+int rbc_apply_freeze(rbc_ctx* c);
+int rbc_temp_mask(rbc_ctx* c, int shape, double param);
+int rbc_apply_temp_mask(rbc_ctx* c);
+int rbc_mask_theta(rbc_ctx* c, double cx, double cy, double radius, double factor);
+
+/* ≙ calculate_temperature_operator::calculate_operator_at (calculate_temperature_operator.cu:
+ * 27-78): theta(x, y, z) = sum_i S_i(z) theta_i(x, y) on the M x N grid at nz heights, FP32,
+ * [l][y][x]. */
+int rbc_temperature_grid(rbc_ctx* c, int nz, const double* z, float* out);
+/* Packets of the visual stream (output/server_writer.cu:79-252, output/datastruct.h:4-72): the
+ * frame server_writer::Prepare builds for one request, written to `out` as NumPackets packets
+ * of 2048 bytes (packet 0: the simulation header; then the image sampled at Nx x Ny points,
+ * res_req <= 0 for the full grid): RBC_IMG_TEMPERATURE (theta at z = 0), RBC_IMG_PARTICLES
+ * (particle counts on a res_req^2 grid, 128^2 by default; tracer t), RBC_IMG_TEMP_PARTICLES
+ * (temperature with the particles' points set to 0).  Nt is the step index, eta and t_final go
+ * into the header.  *nbytes receives the size (out = NULL: query).  The socket that sends
+ * them (RBCServer) is the caller's. */
+#define RBC_IMG_TEMPERATURE 1
+#define RBC_IMG_PARTICLES 2
+#define RBC_IMG_TEMP_PARTICLES 3
+int rbc_stream_packets(rbc_ctx* c, rbc_tracer* t, int img_type, int res_req, int Nt, double eta,
+                       double t_final, void* out, size_t cap, size_t* nbytes);
+
 /* ≙ force_mean (util/util.cu:74-81), which main.cu:567-568 applies before every step under
  * -mean_F: F_0 = mean_F + 0i.  Enqueued on the context's stream. */
 int rbc_force_mean(rbc_ctx* c, double mean_F);
@@ -230,7 +281,6 @@ int rbc_velocity_at_device_noalloc(rbc_ctx* c, int n, const float* x, const floa
 #define RBC_PART_DIAGONAL 3
 #define RBC_PART_PAIRS 4
 #define RBC_PART_COORD_SIGMA 5
-typedef struct rbc_tracer rbc_tracer;  /* destroy before its context */
 /* ≙ particle_tracer(M, N, L, Np, buffer, cube_length, Le) (particle_tracer.cu:7-44): all
  * particles start at the centre. */
 int rbc_tracer_create(rbc_ctx* c, int num_particles, int buffer_size, int mode, double diff_coeff,

@@ -128,6 +128,7 @@ void nonlinear_into(rbc_ctx* c, void* const X[3], const void* XF, const void* XG
     Launch<T>::fft_y_fwd(c->dg, c->b, c->stream);
     mark(4);
     Launch<T>::project(c->dg, c->b, c->p.Pr, Nout, NF, NG, c->stream);
+    if (c->pen.eta > 0.0) Launch<T>::penalty(c->dg, c->b, c->pen, c->p.Pr, X, XF, XG, Nout, NF, NG, c->stream);
     mark(5);
 }
 
@@ -693,6 +694,222 @@ int rbc_init_state(rbc_ctx* c, int kind, unsigned long long seed, double amplitu
         HIPCHK(hipGetLastError());
         c->have_hist = false;
         clear_health(c);
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+int rbc_set_penalization(rbc_ctx* c, double eta, int shape, const double* mask) {
+    if (!c || !std::isfinite(eta) || eta < 0.0) return RBC_ERR_ARG;
+    if (eta > 0.0 && (shape < RBC_MASK_RECTANGLE || shape > RBC_MASK_CUSTOM || (shape == RBC_MASK_CUSTOM && !mask)))
+        return fail(c, RBC_ERR_ARG, "unknown mask shape (or no mask array for RBC_MASK_CUSTOM)");
+    try {
+        hipSetDevice(c->p.device);
+        HIPCHK(hipStreamSynchronize(c->stream));  // steps in flight may read the old arrays
+        for (void* a : c->pen_allocs) HIPCHK(hipFree(a));
+        c->pen_allocs.clear();
+        c->pen = Penalty();
+        if (eta == 0.0) return RBC_OK;
+        const DevGeom& g = c->dg;
+        auto al = [&](size_t bytes) {
+            void* ptr = nullptr;
+            HIPCHK(hipMalloc(&ptr, bytes > 0 ? bytes : 16));
+            c->pen_allocs.push_back(ptr);
+            return ptr;
+        };
+        std::vector<double> m;
+        if (shape == RBC_MASK_CUSTOM) m.assign(mask, mask + (size_t)g.M * g.N);
+        else penalty_mask(shape, g.M, g.N, m);
+        std::vector<double> iac, ia1, ia2;
+        closed_form_iac_ia1(g.L, iac, ia1);
+        quadrature_ia2(g.L, ia2);
+        Penalty& pen = c->pen;
+        pen.mask = al(m.size() * c->rsz);
+        pen.iac = al(iac.size() * c->rsz);
+        pen.ia2 = al(ia2.size() * c->rsz);
+        upload_reals(c, pen.mask, m);
+        upload_reals(c, pen.iac, iac);
+        upload_reals(c, pen.ia2, ia2);
+        // work arrays: the level-space buffers when they are large enough (the penalty runs after
+        // the projection has consumed them), else arrays of their own
+        const size_t cs = c->csz(), spec = (size_t)10 * g.L * g.P * cs, hyb = (size_t)10 * g.L * g.N * g.KXp * cs;
+        const bool reuse = 10 * g.L <= 6 * g.nq;
+        pen.spec = reuse ? c->b.lev : al(spec);
+        pen.spec_out = reuse ? c->b.prod : al(spec);
+        pen.hin = reuse ? c->b.hin : al(hyb);
+        pen.hout = reuse ? c->b.hout : al(hyb);
+        HIPCHK(hipDeviceSynchronize());
+        pen.eta = eta;
+    } catch (const HipError& e) {
+        c->pen = Penalty();
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// integer masks of the plane operators (1 = the operator leaves the point alone), index
+// col + row M; real arithmetic in the context's real type as the reference's kernels use
+// CUDA_FLOAT_REAL (freeze_operator.cu, masktemp.cu, masking.cu init_mask_physical_space_*)
+template <typename R>
+std::vector<int> plane_mask(int shape, int M, int N, double param, double cxf, double cyf) {
+    std::vector<int> m((size_t)M * N, 0);
+    for (int row = 0; row < N; ++row)
+        for (int col = 0; col < M; ++col) {
+            int v = 0;
+            if (shape == RBC_PLANE_RECTANGLE) {
+                const R w = (R)param;
+                if (col < w * M || col > (1 - w) * M || row < w * N || row > (1 - w) * N) v = 1;
+            } else {  // circle around (cxf M, cyf N), radius param * min(M, N) / 2
+                const R dx = (R)col - (R)cxf * M, dy = (R)row - (R)cyf * N;
+                const R d2 = dx * dx + dy * dy;
+                const int mn = N < M ? N : M;
+                const R r = (R)param * (R)0.5 * mn;
+                if (d2 > r * r) v = 1;
+            }
+            m[(size_t)row * M + col] = v;
+        }
+    return m;
+}
+
+int* upload_mask(rbc_ctx* c, int*& dst, const std::vector<int>& m) {
+    if (!dst) dst = (int*)c->dalloc(m.size() * sizeof(int));
+    HIPCHK(hipMemcpy(dst, m.data(), m.size() * sizeof(int), hipMemcpyHostToDevice));
+    return dst;
+}
+
+void run_plane_op(rbc_ctx* c, int op, int nf, const int* imask, double factor) {
+    Buffers& b = c->b;
+    if (c->rsz == 8)
+        Launch<double>::plane_op(c->dg, b, op, nf, imask, factor, c->frozen, b.X, b.lev, b.hin, b.hout, b.prod, c->stream);
+    else
+        Launch<float>::plane_op(c->dg, b, op, nf, imask, factor, c->frozen, b.X, b.lev, b.hin, b.hout, b.prod, c->stream);
+    HIPCHK(hipGetLastError());
+}
+
+bool plane_buffers_fit(const rbc_ctx* c, int nf) { return nf * c->dg.L <= 6 * c->dg.nq; }
+
+}  // namespace
+
+extern "C" {
+
+int rbc_freeze(rbc_ctx* c, int shape, double param) {
+    if (!c || (shape != RBC_PLANE_CIRCLE && shape != RBC_PLANE_RECTANGLE) || !std::isfinite(param))
+        return RBC_ERR_ARG;
+    if (!plane_buffers_fit(c, 3)) return fail(c, RBC_ERR_ARG, "plane operators need nq >= L / 2");
+    try {
+        hipSetDevice(c->p.device);
+        const DevGeom& g = c->dg;
+        const auto m = c->rsz == 8 ? plane_mask<double>(shape, g.M, g.N, param, 0.5, 0.5)
+                                   : plane_mask<float>(shape, g.M, g.N, param, 0.5, 0.5);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        upload_mask(c, c->freeze_mask, m);
+        if (!c->frozen) c->frozen = c->dalloc((size_t)3 * g.L * g.M * g.N * c->rsz);
+        run_plane_op(c, PLANE_CAPTURE, 3, c->freeze_mask, 1.0);
+        c->frozen_set = true;
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+int rbc_apply_freeze(rbc_ctx* c) {
+    if (!c) return RBC_ERR_ARG;
+    if (!c->frozen_set) return RBC_OK;  // freeze_operator::calculate_operator before freeze_*: nothing
+    try {
+        hipSetDevice(c->p.device);
+        run_plane_op(c, PLANE_FREEZE, 3, c->freeze_mask, 1.0);
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+int rbc_temp_mask(rbc_ctx* c, int shape, double param) {
+    if (!c || (shape != RBC_PLANE_CIRCLE && shape != RBC_PLANE_RECTANGLE) || !std::isfinite(param))
+        return RBC_ERR_ARG;
+    if (!plane_buffers_fit(c, 1)) return fail(c, RBC_ERR_ARG, "plane operators need nq >= L / 6");
+    try {
+        hipSetDevice(c->p.device);
+        const DevGeom& g = c->dg;
+        // masktemp's circle is centred at (0.2 M, 0.3 N) (masktemp.cu init_mask_physical_space_circle)
+        const auto m = c->rsz == 8 ? plane_mask<double>(shape, g.M, g.N, param, 0.2, 0.3)
+                                   : plane_mask<float>(shape, g.M, g.N, param, 0.2, 0.3);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        upload_mask(c, c->temp_mask, m);
+        c->temp_set = true;
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+int rbc_apply_temp_mask(rbc_ctx* c) {
+    if (!c) return RBC_ERR_ARG;
+    if (!c->temp_set) return RBC_OK;  // masktemp::calculate_operator before temp_*: nothing
+    try {
+        hipSetDevice(c->p.device);
+        run_plane_op(c, PLANE_HEAT, 1, c->temp_mask, 1.0);
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+int rbc_mask_theta(rbc_ctx* c, double cx, double cy, double radius, double factor) {
+    if (!c || !std::isfinite(cx) || !std::isfinite(cy) || !std::isfinite(radius) || !std::isfinite(factor))
+        return RBC_ERR_ARG;
+    if (!plane_buffers_fit(c, 1)) return fail(c, RBC_ERR_ARG, "plane operators need nq >= L / 6");
+    try {
+        hipSetDevice(c->p.device);
+        const DevGeom& g = c->dg;
+        // masking.cu: the circle centre is (c_y M, c_x N), the coordinates swapped as there
+        const auto m = c->rsz == 8 ? plane_mask<double>(RBC_PLANE_CIRCLE, g.M, g.N, radius, cy, cx)
+                                   : plane_mask<float>(RBC_PLANE_CIRCLE, g.M, g.N, radius, cy, cx);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        upload_mask(c, c->theta_mask, m);
+        run_plane_op(c, PLANE_SCALE, 1, c->theta_mask, factor);
+    } catch (const HipError& e) {
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
+
+int rbc_temperature_grid(rbc_ctx* c, int nz, const double* z, float* out) {
+    if (!c || nz < 1 || !z || !out) return RBC_ERR_ARG;
+    if (!plane_buffers_fit(c, 1)) return fail(c, RBC_ERR_ARG, "the temperature grid needs nq >= L / 6");
+    try {
+        hipSetDevice(c->p.device);
+        const DevGeom& g = c->dg;
+        const size_t nxy = (size_t)g.M * g.N;
+        // the physical mode planes of theta (the reference's C2R per mode) into the context's
+        // product buffer, then per height the S-weighted sum into a float plane of the hybrid buffer
+        Buffers& b = c->b;
+        if (c->rsz == 8)
+            Launch<double>::plane_op(g, b, PLANE_CAPTURE, 1, nullptr, 1.0, b.prod, b.X, b.lev, b.hin, b.hout, nullptr,
+                                     c->stream);
+        else
+            Launch<float>::plane_op(g, b, PLANE_CAPTURE, 1, nullptr, 1.0, b.prod, b.X, b.lev, b.hin, b.hout, nullptr,
+                                    c->stream);
+        float* dS = (float*)b.fgw;  // L floats; fgw holds 2 nq complex >= L floats
+        float* dout = (float*)b.hout;
+        std::vector<float> S(g.L), plane(nxy);
+        for (int k = 0; k < nz; ++k) {
+            // S_i = sin(pi (i + 1) (z + 1/2)) with z in the build's real type (:59-65)
+            const float zk = (float)z[k];
+            for (int i = 0; i < g.L; ++i) S[i] = (float)std::sin(M_PI * (i + 1) * (zk + 0.5));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            HIPCHK(hipMemcpy(dS, S.data(), S.size() * sizeof(float), hipMemcpyHostToDevice));
+            if (c->rsz == 8) Launch<double>::temperature_sum(g, b.prod, dS, dout, c->stream);
+            else Launch<float>::temperature_sum(g, b.prod, dS, dout, c->stream);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(out + (size_t)k * nxy, dout, nxy * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
     } catch (const HipError& e) {
         return hip_fail(c, e);
     }

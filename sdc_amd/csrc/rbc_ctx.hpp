@@ -55,6 +55,15 @@ struct rbc_ctx {
     // demand (a call allocates only when it needs more than any earlier call)
     void* vel_ws = nullptr;
     void (*vel_ws_free)(void*) = nullptr;
+    // penalization (rbc_set_penalization); its own device arrays, freed when it is set again
+    rbc::Penalty pen;
+    std::vector<void*> pen_allocs;
+    // plane operators: frozen planes and masks (rbc_freeze, rbc_temp_mask, rbc_mask_theta)
+    void* frozen = nullptr;     // 3 L M N reals
+    int* freeze_mask = nullptr; // M N
+    int* temp_mask = nullptr;   // M N
+    int* theta_mask = nullptr;  // M N
+    bool frozen_set = false, temp_set = false;
 
     size_t csz() const { return 2 * rsz; }
     void* dalloc(size_t bytes) {
@@ -66,6 +75,7 @@ struct rbc_ctx {
     ~rbc_ctx() {
         if (stream) hipStreamSynchronize(stream);
         if (vel_ws && vel_ws_free) vel_ws_free(vel_ws);
+        for (void* a : pen_allocs) hipFree(a);
         for (void* a : allocs) hipFree(a);
         if (events)
             for (auto& row : ev_ring)

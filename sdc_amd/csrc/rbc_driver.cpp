@@ -18,10 +18,12 @@
 //   - -write_history also writes the AB2 history next to each state file, -history_files
 //     restores one, so a restart continues with AB2 instead of an Euler step;
 //   - -tracer_mode spectral selects the spectral RK2 tracer (default: the reference's RK4).
-// Not available in this build (they exit with an error naming the option): the initial
-// patterns other than random and test (-init_rectangle, -init_dislocation, -init_isr, -init_sv,
-// -init_cr, -init_giantspiral), -freeze_circle and -temp_circle.  -port (USE_SERVER) and
-// -snapshot_freq (WRITE_SNAPSHOT) are accepted and ignored, as in the reference's default build.
+// -freeze_circle n and -temp_circle n run the freeze and heat-source operators before every step
+// from step n on (main.cu:569-590).  Not available in this build (they exit with an error naming
+// the option): the initial patterns other than random and test (-init_rectangle,
+// -init_dislocation, -init_isr, -init_sv, -init_cr, -init_giantspiral).  -port (USE_SERVER)
+// and -snapshot_freq (WRITE_SNAPSHOT) are accepted and ignored, as in the reference's default
+// build.
 #include <hip/hip_runtime.h>
 #include <sys/stat.h>
 
@@ -322,8 +324,6 @@ extern "C" int rbc_run_main(int argc, const char* const* argv) {
     if (!A.get1<int>(freeze_circle, "-freeze_circle", false) || !A.get1<int>(temp_circle, "-temp_circle", false) ||
         !A.get1<int>(port, "-port", false))
         return bad();
-    if (freeze_circle) return die("-freeze_circle is not available in this build");
-    if (temp_circle) return die("-temp_circle is not available in this build");
     const bool write_hist = A.opt("-write_history");
     int nq = 0;
     if (!A.get1<int>(nq, "-nq", false)) return bad();
@@ -459,7 +459,7 @@ extern "C" int rbc_run_main(int argc, const char* const* argv) {
     }
 
     std::cout << "Start time loop" << std::endl;
-    const bool per_step = ntr > 0 || mean_F != 0.0;
+    const bool per_step = ntr > 0 || mean_F != 0.0 || freeze_circle || temp_circle;
     for (int i = 1; i <= iterations;) {
         // steps until the next output (one at a time when anything happens between steps)
         int n = 1;
@@ -469,6 +469,15 @@ extern "C" int rbc_run_main(int argc, const char* const* argv) {
         }
         if (do_step) {
             if (mean_F != 0.0 && rbc_force_mean(c, mean_F) != RBC_OK) return cfail("force_mean");
+            if (freeze_circle) {  // main.cu:569-577: the operator, then the capture at step n
+                if (rbc_apply_freeze(c) != RBC_OK) return cfail("freeze operator");
+                if (i == freeze_circle && rbc_freeze(c, RBC_PLANE_CIRCLE, 0.8) != RBC_OK) return cfail("freeze_circle");
+            }
+            if (temp_circle) {  // main.cu:580-588: the heat source, prepared (a rectangle) at step n
+                if (rbc_apply_temp_mask(c) != RBC_OK) return cfail("heat source");
+                if (i == temp_circle && rbc_temp_mask(c, RBC_PLANE_RECTANGLE, 0.1) != RBC_OK)
+                    return cfail("temp_rectangle");
+            }
             if (rbc_step(c, n) != RBC_OK) return cfail("step");
         }
         const int last = i + n - 1;

@@ -84,12 +84,13 @@ constexpr size_t kBatchMemberBytes = (size_t)8 << 20;
 constexpr size_t kBatchBytes = (size_t)64 << 20;
 
 // size classes of two or more members with the same launch shapes become batches; the rest,
-// and members with a hook after the nonlinear term, step on their own
+// and members with a hook after the nonlinear term or penalization, step on their own
 void build_batches(rbc_ensemble* e) {
     std::map<std::tuple<int, int, int, int, int, int, int>, std::vector<int>> cls;
     for (size_t i = 0; i < e->m.size(); ++i) {
         rbc_ctx* c = e->m[i];
-        if (!e->batching || c->post_nonlinear || c->rsz != 4 || stage_bytes(c) > kBatchMemberBytes) {
+        if (!e->batching || c->post_nonlinear || c->pen.eta > 0.0 || c->rsz != 4 ||
+            stage_bytes(c) > kBatchMemberBytes) {
             e->solo.push_back((int)i);
             continue;
         }

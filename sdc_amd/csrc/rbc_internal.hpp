@@ -132,12 +132,36 @@ struct Buffers {
     int tune_rows_threads = 0;  // threads per block of the row kernel (64, 128 or 256)
 };
 
+// Penalization of a PENALIZED build (nonlinear_operator_rayleigh_noslip.cu:1097-1343): the
+// Brinkman terms of a z-independent mask chi(x, y) added to the nonlinear term.  Device arrays
+// in the context's real type; the four work arrays may alias the context's level-space buffers
+// (the penalty runs after the projection has consumed them).
+struct Penalty {
+    double eta = 0.0;        // 0 = off
+    void* mask = nullptr;    // M * N reals, x fastest
+    void* iac = nullptr;     // L * L reals, Iac(j, k) at j + k L (coeff.cu:77-116, quirk kept)
+    void* ia2 = nullptr;     // L * L reals, Ia2(i, j) = <C_i S'_j> at i + j L
+    void* spec = nullptr;    // [10][L][P] complex: the mode-space fields before the transforms
+    void* hin = nullptr;     // [10][L][N][KXp] complex (row-major hybrid rows)
+    void* hout = nullptr;    // same, after the mask products
+    void* spec_out = nullptr;  // [10][L][P] complex: the transforms of the mask products
+};
+
+// Ia2(i, j) = <C_i S'_j> by a 4 L + 64 point Gauss-Legendre rule (coeff.cu's ra2), i + j L.
+void quadrature_ia2(int L, std::vector<double>& Ia2);
+
+// The mask shapes of nonlinear_operator_rayleigh_noslip.cu:1652-1740 on the M x N grid.
+void penalty_mask(int shape, int M, int N, std::vector<double>& mask);
+
 struct StepCoef {
     double a_cur, a_prev;  // X += a_cur*R ; X += a_prev*R_prev
     int use_prev;
 };
 
 enum UpdateMode { UPD_STEP = 0, UPD_INVB = 1, UPD_IMPLICIT = 2 };
+
+// plane operators on the state's mode planes (freeze_operator.cu, masktemp.cu, masking.cu)
+enum PlaneOp { PLANE_CAPTURE = 0, PLANE_FREEZE = 1, PLANE_HEAT = 2, PLANE_SCALE = 3 };
 
 template <typename T>
 struct Launch {
@@ -175,6 +199,17 @@ struct Launch {
     static void init_state(const DevGeom& g, const Buffers& b, int kind, uint64_t seed, double amp, hipStream_t s);
     // F_0 = v (force_mean)
     static void force_mean(const Buffers& b, double v, hipStream_t s);
+    // plane operator `op` on the first nf fields of X (theta, f, g) in place (PLANE_CAPTURE
+    // writes the physical planes to `frozen` and leaves X); spec / spec_out hold nf L P complex,
+    // hin / hout nf L N KXp complex
+    static void plane_op(const DevGeom& g, const Buffers& b, int op, int nf, const int* imask, double factor,
+                         void* frozen, void* const X[3], void* spec, void* hin, void* hout, void* spec_out,
+                         hipStream_t s);
+    // out[x + y M] = sum_i S[i] planes[i][y][x] (float), planes: L physical mode planes of theta
+    static void temperature_sum(const DevGeom& g, const void* planes, const float* S, float* out, hipStream_t s);
+    // adds the penalization terms of state X to N (already scaled as the projection scales)
+    static void penalty(const DevGeom& g, const Buffers& b, const Penalty& pen, double Pr, void* const X[3],
+                        const void* XF, const void* XG, void* const Nout[3], void* NF, void* NG, hipStream_t s);
 };
 
 // Batched steps of the members of one ensemble size class (rbc_kernels.hip).  A member is

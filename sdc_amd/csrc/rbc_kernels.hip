@@ -962,6 +962,231 @@ __global__ void k_force_mean(cpx<T>* XF, double v) {
     if (threadIdx.x == 0) XF[0] = mk(T(v), T(0));
 }
 
+// ---------------------------------------------------------------------------------------
+// Penalization (PENALIZED builds, nonlinear_operator_rayleigh_noslip.cu:1097-1343).  The mask
+// chi(x, y) does not depend on z, so every sum over the source mode j of a summand commutes with
+// the mask product and the transforms: the summands need, per output mode k, the transforms
+// H = R2C(chi u / (M N)) of ten mode-space fields u (the reference forms six per j and sums
+// after the transforms):
+//   0: i kx a_k, 1: i ky a_k                 a_k = sum_j Iac(j, k) f_j   (f summands 1, 3)
+//   2: i ky b_k + bF_k d_k0,                 b_k = sum_j Ia2(k, j) g_j, bF_k, bG_k from F, G
+//   3: -i kx b_k + bG_k d_k0                                             (f summands 2, 4)
+//   4: k^2 f_k                                                           (f summand 5)
+//   5: i kx c_k, 6: i ky c_k                 c_k = sum_j Ia2(j, k) f_j   (g 1, 3; F, G 1)
+//   7: i ky g_k + F_k d_k0, 8: -i kx g_k + G_k d_k0   (u12_k, u22_k: g 2, 4; F, G 2)
+//   9: theta_k                                                           (theta)
+// (d_k0: at kx = ky = 0 only, add_dc_part).  After the mask products and the transforms:
+//   N_theta += -(1/(2 eta)) H9
+//   N_f     += -(1/eta) (i kx (H0 + H2) + i ky (H1 + H3) + k^2 H4)
+//   N_g     += -(1/eta) (-i ky H5 + i kx H6 + i ky (H7 - H8) / 2)   (summand 4 is d_y, as there)
+//   N_F     += -(1/eta) (-H5 + H7 / 2) at k = 0,  N_G += -(1/eta) (-H6 + H8 / 2) at k = 0
+// which are the reference's summands scaled by -1 (theta) and -1/Pr (:1366-1370).
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_pen_fields(DevGeom g, const T* __restrict__ iac, const T* __restrict__ ia2,
+                                                    const cpx<T>* __restrict__ Xt, const cpx<T>* __restrict__ Xf,
+                                                    const cpx<T>* __restrict__ Xg, const cpx<T>* __restrict__ XF,
+                                                    const cpx<T>* __restrict__ XG, const T* __restrict__ kxa,
+                                                    const T* __restrict__ kya, cpx<T>* __restrict__ out) {
+    const int L = g.L, P = g.P;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)L * P) return;
+    const int k = (int)(e / P), p = (int)(e % P);
+    const cpx<T>* f = Xf + (size_t)p * L;
+    const cpx<T>* gg = Xg + (size_t)p * L;
+    cpx<T> a = mk(T(0), T(0)), b = a, c = a, bF = a, bG = a;
+    for (int j = 0; j < L; ++j) {
+        const T cac = iac[j + (size_t)k * L], ckj = ia2[k + (size_t)j * L], cjk = ia2[j + (size_t)k * L];
+        const cpx<T> fj = f[j], gj = gg[j];
+        a = a + cscale(fj, cac);
+        b = b + cscale(gj, ckj);
+        c = c + cscale(fj, cjk);
+        if (p == 0) {
+            bF = bF + cscale(XF[j], ckj);
+            bG = bG + cscale(XG[j], ckj);
+        }
+    }
+    const T kx = kxa[p], ky = kya[p];
+    const cpx<T> fk = f[k], gk = gg[k];
+    cpx<T> v[10];
+    v[0] = imul(a, kx);
+    v[1] = imul(a, ky);
+    v[2] = imul(b, ky);
+    v[3] = imul(b, -kx);
+    v[4] = cscale(fk, kx * kx + ky * ky);
+    v[5] = imul(c, kx);
+    v[6] = imul(c, ky);
+    v[7] = imul(gk, ky);
+    v[8] = imul(gk, -kx);
+    v[9] = Xt[(size_t)p * L + k];
+    if (p == 0) {
+        v[2] = v[2] + bF;
+        v[3] = v[3] + bG;
+        v[7] = v[7] + XF[k];
+        v[8] = v[8] + XG[k];
+    }
+    const size_t fs = (size_t)L * P;
+#pragma unroll
+    for (int q = 0; q < 10; ++q) out[q * fs + e] = v[q];
+}
+
+// one (y row, mode k, field group) per block: the group's one or two hybrid rows as one complex
+// M-point transform (X1 + i X2, the kx = 0 imaginary parts dropped as a C2R does), the product
+// with chi(x, y) / (M N), the forward transform and the unpacking to kx < KX.  Groups pair
+// fields of like size: (0, 1), (2, 3), (5, 6), (7, 8), (4), (9).
+template <typename T>
+__global__ void __launch_bounds__(256) k_rows_mask(DevGeom g, const T* __restrict__ mask,
+                                                   const cpx<T>* __restrict__ hin, const cpx<T>* __restrict__ twM,
+                                                   cpx<T>* __restrict__ hout) {
+    extern __shared__ unsigned char smem_raw[];
+    cpx<T>* a = reinterpret_cast<cpx<T>*>(smem_raw);
+    cpx<T>* bb = a + g.M;
+    const int M = g.M, N = g.N, KX = g.KX, KXp = g.KXp, L = g.L;
+    const int y = blockIdx.x, k = blockIdx.y, grp = blockIdx.z;
+    const int f1 = grp < 4 ? (grp == 0 ? 0 : grp == 1 ? 2 : grp == 2 ? 5 : 7) : (grp == 4 ? 4 : 9);
+    const int f2 = grp < 4 ? f1 + 1 : -1;
+    const size_t fstride = (size_t)L * N * KXp;
+    const size_t roff = ((size_t)k * N + y) * KXp;
+    for (int e = threadIdx.x; e < M; e += blockDim.x) a[e] = mk(T(0), T(0));
+    __syncthreads();
+    for (int kk = threadIdx.x; kk < KX; kk += blockDim.x) {
+        cpx<T> x1 = hin[(size_t)f1 * fstride + roff + kk];
+        cpx<T> x2 = f2 >= 0 ? hin[(size_t)f2 * fstride + roff + kk] : mk(T(0), T(0));
+        if (kk == 0) {
+            x1.y = T(0);
+            x2.y = T(0);
+        }
+        a[kk] = mk(x1.x - x2.y, x1.y + x2.x);                  // X1 + i X2
+        if (kk > 0) a[M - kk] = mk(x1.x + x2.y, -x1.y + x2.x);  // conj(X1) + i conj(X2)
+    }
+    __syncthreads();
+    cpx<T>* r = lds_fft(a, bb, g.logM, 1, twM, true);
+    const T s = T(1) / (T(M) * T(N));
+    for (int x = threadIdx.x; x < M; x += blockDim.x) r[x] = cscale(r[x], s * mask[(size_t)y * M + x]);
+    __syncthreads();
+    cpx<T>* R = lds_fft(r, r == a ? bb : a, g.logM, 1, twM, false);
+    for (int kk = threadIdx.x; kk < KX; kk += blockDim.x) {
+        const cpx<T> zk = R[kk], zm = R[(M - kk) & (M - 1)];
+        hout[(size_t)f1 * fstride + roff + kk] = mk(T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y));
+        if (f2 >= 0) hout[(size_t)f2 * fstride + roff + kk] = mk(T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x));
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_pen_combine(DevGeom g, T ie, const cpx<T>* __restrict__ H,
+                                                     const T* __restrict__ kxa, const T* __restrict__ kya,
+                                                     cpx<T>* __restrict__ Nt, cpx<T>* __restrict__ Nf,
+                                                     cpx<T>* __restrict__ Ng, cpx<T>* __restrict__ NF,
+                                                     cpx<T>* __restrict__ NG) {
+    const int L = g.L, P = g.P;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)L * P) return;
+    const int p = (int)(e / L), k = (int)(e % L);
+    const size_t fs = (size_t)L * P, at = (size_t)k * P + p;
+    cpx<T> h[10];
+#pragma unroll
+    for (int q = 0; q < 10; ++q) h[q] = H[q * fs + at];
+    const T kx = kxa[p], ky = kya[p];
+    const cpx<T> ft = imul(h[0] + h[2], kx) + imul(h[1] + h[3], ky) + cscale(h[4], kx * kx + ky * ky);
+    const cpx<T> gt = imul(h[5], -ky) + imul(h[6], kx) + imul(h[7] - h[8], T(0.5) * ky);
+    Nt[e] = Nt[e] + cscale(h[9], T(-0.5) * ie);
+    Nf[e] = Nf[e] + cscale(ft, -ie);
+    Ng[e] = Ng[e] + cscale(gt, -ie);
+    if (p == 0) {
+        NF[k] = NF[k] + cscale(cscale(h[7], T(0.5)) - h[5], -ie);
+        NG[k] = NG[k] + cscale(cscale(h[8], T(0.5)) - h[6], -ie);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Plane operators on the state's mode planes (operator/freeze_operator.cu, masktemp.cu,
+// masking.cu): each mode plane m of theta (and f, g for the freeze) goes to physical space
+// (C2R), is combined point by point under an integer mask, scaled by 1/(M N) and transformed
+// back (R2C):
+//   PLANE_CAPTURE  store the physical plane (freeze_mask, the frozen copy)
+//   PLANE_FREEZE   mask ? u : frozen                  (freeze_operator apply_mask)
+//   PLANE_HEAT     mask ? u : u + 1.5/100             (masktemp apply_mask)
+//   PLANE_SCALE    mask ? u : u * factor              (masking julie_do_the_thing)
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_modes_to_planes(DevGeom g, int nf, const cpx<T>* __restrict__ X0,
+                                                         const cpx<T>* __restrict__ X1,
+                                                         const cpx<T>* __restrict__ X2, cpx<T>* __restrict__ out) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = blockIdx.y;
+    if (e >= (size_t)g.L * g.P || f >= nf) return;
+    const int m = (int)(e / g.P), p = (int)(e % g.P);
+    const cpx<T>* X = f == 0 ? X0 : (f == 1 ? X1 : X2);
+    out[(size_t)f * g.L * g.P + e] = X[(size_t)p * g.L + m];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_planes_to_modes(DevGeom g, int nf, const cpx<T>* __restrict__ in,
+                                                         cpx<T>* __restrict__ X0, cpx<T>* __restrict__ X1,
+                                                         cpx<T>* __restrict__ X2) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = blockIdx.y;
+    if (e >= (size_t)g.L * g.P || f >= nf) return;
+    const int p = (int)(e / g.L), m = (int)(e % g.L);
+    cpx<T>* X = f == 0 ? X0 : (f == 1 ? X1 : X2);
+    X[e] = in[((size_t)f * g.L + m) * g.P + p];
+}
+
+// one (y row, mode m, field) per block; g.nq = L (the mode planes stand where the levels do)
+template <typename T>
+__global__ void __launch_bounds__(256) k_rows_planeop(DevGeom g, int op, const int* __restrict__ imask, T factor,
+                                                      T* __restrict__ frozen, const cpx<T>* __restrict__ hin,
+                                                      const cpx<T>* __restrict__ twM, cpx<T>* __restrict__ hout) {
+    extern __shared__ unsigned char smem_raw[];
+    cpx<T>* a = reinterpret_cast<cpx<T>*>(smem_raw);
+    cpx<T>* bb = a + g.M;
+    const int M = g.M, N = g.N, KX = g.KX, KXp = g.KXp, L = g.L;
+    const int y = blockIdx.x, m = blockIdx.y, f = blockIdx.z;
+    const size_t roff = (((size_t)f * L + m) * N + y) * KXp;
+    for (int e = threadIdx.x; e < M; e += blockDim.x) a[e] = mk(T(0), T(0));
+    __syncthreads();
+    for (int kk = threadIdx.x; kk < KX; kk += blockDim.x) {
+        cpx<T> x = hin[roff + kk];
+        if (kk == 0) x.y = T(0);
+        a[kk] = x;
+        if (kk > 0) a[M - kk] = conjg(x);
+    }
+    __syncthreads();
+    cpx<T>* r = lds_fft(a, bb, g.logM, 1, twM, true);  // the plane's row: real parts
+    T* fr = frozen + (((size_t)f * L + m) * N + y) * M;
+    if (op == PLANE_CAPTURE) {
+        for (int x = threadIdx.x; x < M; x += blockDim.x) fr[x] = r[x].x;
+        return;
+    }
+    const T s = T(1) / (T(M) * T(N));
+    for (int x = threadIdx.x; x < M; x += blockDim.x) {
+        const T u = r[x].x;
+        const bool keep = imask[(size_t)y * M + x] == 1;
+        T v = u;
+        if (!keep) {
+            if (op == PLANE_FREEZE) v = fr[x];
+            else if (op == PLANE_HEAT) v = u + T(1.5) / T(100);
+            else v = u * factor;
+        }
+        r[x] = mk(s * v, T(0));
+    }
+    __syncthreads();
+    cpx<T>* R = lds_fft(r, r == a ? bb : a, g.logM, 1, twM, false);
+    for (int kk = threadIdx.x; kk < KX; kk += blockDim.x) hout[roff + kk] = R[kk];
+}
+
+// temperature at one height from the physical mode planes of theta: sum_i S_i theta_i(x, y),
+// accumulated in mode order (calculate_temperature_operator.cu:56-68)
+template <typename T>
+__global__ void __launch_bounds__(256) k_temp_sum(int L, size_t nxy, const T* __restrict__ planes,
+                                                  const float* __restrict__ S, float* __restrict__ out) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nxy) return;
+    T acc = T(0);
+    for (int i = 0; i < L; ++i) acc += T(S[i]) * planes[(size_t)i * nxy + e];
+    out[e] = (float)acc;
+}
+
 // Nusselt reductions (FP64): blocks write partial sums of sum_p w_p sum_{m,n} Ia1(m,n)
 // Re(k^2 f_m conj theta_n) to red[blockIdx]; red[nblk] = plate sums from mode p = 0.
 template <typename T>
@@ -2390,6 +2615,63 @@ void Launch<T>::init_state(const DevGeom& g, const Buffers& b, int kind, uint64_
 template <typename T>
 void Launch<T>::force_mean(const Buffers& b, double v, hipStream_t s) {
     hipLaunchKernelGGL(k_force_mean<T>, dim3(1), dim3(64), 0, s, (cpx<T>*)b.XF, v);
+}
+
+template <typename T>
+void Launch<T>::penalty(const DevGeom& g, const Buffers& b, const Penalty& pen, double Pr, void* const X[3],
+                        const void* XF, const void* XG, void* const Nout[3], void* NF, void* NG, hipStream_t s) {
+    (void)Pr;
+    const size_t n = (size_t)g.L * g.P;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_pen_fields<T>, dim3(nb), dim3(256), 0, s, g, (const T*)pen.iac, (const T*)pen.ia2,
+                       (const cpx<T>*)X[0], (const cpx<T>*)X[1], (const cpx<T>*)X[2], (const cpx<T>*)XF,
+                       (const cpx<T>*)XG, (const T*)b.kx, (const T*)b.ky, (cpx<T>*)pen.spec);
+    // the generic y-transforms with the L modes in place of the levels, ten fields
+    DevGeom gp = g;
+    gp.nq = g.L;
+    const int TX = fft_tile(g.N, sizeof(cpx<T>));
+    const size_t sm = (size_t)2 * TX * g.N * sizeof(cpx<T>);
+    const dim3 gy((g.KX + TX - 1) / TX, g.L, 10);
+    hipLaunchKernelGGL(k_fft_y_inv<T>, gy, dim3(256), sm, s, gp, TX, (const cpx<T>*)pen.spec, b.rows,
+                       (const cpx<T>*)b.twN, (cpx<T>*)pen.hin);
+    hipLaunchKernelGGL(k_rows_mask<T>, dim3(g.N, g.L, 6), dim3(256), (size_t)2 * g.M * sizeof(cpx<T>), s, gp,
+                       (const T*)pen.mask, (const cpx<T>*)pen.hin, (const cpx<T>*)b.twM, (cpx<T>*)pen.hout);
+    hipLaunchKernelGGL(k_fft_y_fwd<T>, gy, dim3(256), sm, s, gp, TX, (const cpx<T>*)pen.hout, b.rows,
+                       (const cpx<T>*)b.twN, (cpx<T>*)pen.spec_out);
+    hipLaunchKernelGGL(k_pen_combine<T>, dim3(nb), dim3(256), 0, s, g, (T)(1.0 / pen.eta),
+                       (const cpx<T>*)pen.spec_out, (const T*)b.kx, (const T*)b.ky, (cpx<T>*)Nout[0],
+                       (cpx<T>*)Nout[1], (cpx<T>*)Nout[2], (cpx<T>*)NF, (cpx<T>*)NG);
+}
+
+template <typename T>
+void Launch<T>::plane_op(const DevGeom& g, const Buffers& b, int op, int nf, const int* imask, double factor,
+                         void* frozen, void* const X[3], void* spec, void* hin, void* hout, void* spec_out,
+                         hipStream_t s) {
+    const size_t n = (size_t)g.L * g.P;
+    const dim3 gm((unsigned)((n + 255) / 256), nf);
+    hipLaunchKernelGGL(k_modes_to_planes<T>, gm, dim3(256), 0, s, g, nf, (const cpx<T>*)X[0], (const cpx<T>*)X[1],
+                       (const cpx<T>*)X[2], (cpx<T>*)spec);
+    DevGeom gp = g;
+    gp.nq = g.L;
+    const int TX = fft_tile(g.N, sizeof(cpx<T>));
+    const size_t sm = (size_t)2 * TX * g.N * sizeof(cpx<T>);
+    const dim3 gy((g.KX + TX - 1) / TX, g.L, nf);
+    hipLaunchKernelGGL(k_fft_y_inv<T>, gy, dim3(256), sm, s, gp, TX, (const cpx<T>*)spec, b.rows,
+                       (const cpx<T>*)b.twN, (cpx<T>*)hin);
+    hipLaunchKernelGGL(k_rows_planeop<T>, dim3(g.N, g.L, nf), dim3(256), (size_t)2 * g.M * sizeof(cpx<T>), s, gp,
+                       op, imask, (T)factor, (T*)frozen, (const cpx<T>*)hin, (const cpx<T>*)b.twM, (cpx<T>*)hout);
+    if (op == PLANE_CAPTURE) return;
+    hipLaunchKernelGGL(k_fft_y_fwd<T>, gy, dim3(256), sm, s, gp, TX, (const cpx<T>*)hout, b.rows,
+                       (const cpx<T>*)b.twN, (cpx<T>*)spec_out);
+    hipLaunchKernelGGL(k_planes_to_modes<T>, gm, dim3(256), 0, s, g, nf, (const cpx<T>*)spec_out, (cpx<T>*)X[0],
+                       (cpx<T>*)X[1], (cpx<T>*)X[2]);
+}
+
+template <typename T>
+void Launch<T>::temperature_sum(const DevGeom& g, const void* planes, const float* S, float* out, hipStream_t s) {
+    const size_t n = (size_t)g.M * g.N;
+    hipLaunchKernelGGL(k_temp_sum<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g.L, n, (const T*)planes,
+                       S, out);
 }
 
 template struct Launch<float>;

@@ -119,6 +119,47 @@ void closed_form_iac_ia1(int L, std::vector<double>& Iac, std::vector<double>& I
     }
 }
 
+void quadrature_ia2(int L, std::vector<double>& Ia2) {
+    const int n = 4 * L + 64;
+    std::vector<double> z, w, b;
+    gauss_legendre(n, z, w);
+    basis_values(L, z, b);
+    Ia2.assign((size_t)L * L, 0.0);
+    for (int i = 0; i < L; ++i)
+        for (int j = 0; j < L; ++j) {
+            double s = 0.0;
+            for (int q = 0; q < n; ++q) s += w[q] * b[((size_t)0 * L + i) * n + q] * b[((size_t)5 * L + j) * n + q];
+            Ia2[i + (size_t)j * L] = s;
+        }
+}
+
+void penalty_mask(int shape, int M, int N, std::vector<double>& mask) {
+    // columns = M (x), rows = N (y); index col + row * M (get_current_matrix_indices, :1416-1424)
+    mask.assign((size_t)M * N, 0.0);
+    for (int row = 0; row < N; ++row)
+        for (int col = 0; col < M; ++col) {
+            double v = 0.0;
+            if (shape == RBC_MASK_RECTANGLE) {  // three grid points along every side (:1654-1671)
+                if (col < 3 || col > M - 4 || row < 3 || row > N - 4) v = 1.0;
+            } else if (shape == RBC_MASK_CIRCLE) {  // outside the inscribed circle less 3 points (:1673-1693)
+                const int dx = col - M / 2, dy = row - N / 2, r = std::min(M, N) / 2 - 3;
+                if (dx * dx + dy * dy > r * r) v = 1.0;
+            } else if (shape == RBC_MASK_DISKARRAY) {  // 3 x 2 disks (:1695-1727)
+                const int nx = 3, ny = 2;
+                const int r = std::min(M / (nx * 2), N / (ny * 2));
+                for (int i = 0; i < nx; ++i)
+                    for (int j = 0; j < ny; ++j) {
+                        const int cx = i * M / nx + M / (nx * 2), cy = j * N / ny + N / (ny * 2);
+                        // the squared distance is compared with the radius itself, as there
+                        if ((cx - col) * (cx - col) + (cy - row) * (cy - row) < r) v = 1.0;
+                    }
+            } else {  // RBC_MASK_EVERYWHERE (:1729-1738)
+                v = 1.0;
+            }
+            mask[(size_t)row * M + col] = v;
+        }
+}
+
 int exact_level_count(int L, double tol, double* achieved) {
     // Compare the products the nonlinear term projects (C', C, C'', S, S' combinations)
     // integrated with n points against a 4L+64 point rule.

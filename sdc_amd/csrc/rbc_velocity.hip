@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -1315,3 +1316,128 @@ int rbc_energy_spectrum_ref(rbc_ctx* c, double z, int cap, double* E3, int* nbin
 }
 
 }  // extern "C"
+
+// ---- packets of the visual stream (output/server_writer.cu:79-252, output/datastruct.h) -----
+// The frames the reference's server_writer sends over UDP to its viewer, built in caller
+// memory; sending them is left to the caller.  Layout (packed as the reference's structs):
+// every packet is 2048 bytes, a 16-byte header {uint16 Type, uint16 NumPackets, uint32 Nt,
+// uint32 Nx, uint32 Ny} and a payload; packet 0 (Type 0x491C) carries the simulation header,
+// packets 1.. (Type 0x08B1) {int32 iStart, int32 iEnd, float Data[506]}: the image sampled at
+// Nx x Ny points, x outer, y inner.
+namespace {
+constexpr int kUPack = 2048, kUData = kUPack / 4 - 6, kUDescr = 32;
+constexpr uint16_t kUTypeH = 0x491C, kUTypeP = 0x08B1;
+struct UHeader {
+    uint16_t Type, NumPackets;
+    uint32_t Nt, Nx, Ny;
+};
+struct USimHeader {
+    float uVersion, cVersion, tFinal, dt, Ra, Pr, eta, clx, cly, clz;
+    uint16_t nmx, nmy, nmz;
+    char device[kUDescr], descr[kUDescr], Err1[kUDescr], Err2[kUDescr], Err3[kUDescr], Err4[kUDescr];
+};
+struct USimPayload {
+    int32_t iStart, iEnd;
+    float Data[kUData];
+};
+struct UPacket {
+    UHeader H;
+    USimPayload P;
+};
+static_assert(sizeof(UPacket) == kUPack, "visual-stream packet size");
+static_assert(sizeof(USimHeader) <= sizeof(USimPayload), "simulation header fits the payload");
+}  // namespace
+
+extern "C" int rbc_stream_packets(rbc_ctx* c, rbc_tracer* t, int img_type, int res_req, int Nt, double eta,
+                                  double t_final, void* out, size_t cap, size_t* nbytes) {
+    if (!c || !nbytes || img_type < RBC_IMG_TEMPERATURE || img_type > RBC_IMG_TEMP_PARTICLES) return RBC_ERR_ARG;
+    const bool want_t = img_type != RBC_IMG_PARTICLES, want_p = img_type != RBC_IMG_TEMPERATURE;
+    if (want_p && !t) return fail(c, RBC_ERR_ARG, "particle images need a tracer");
+    const int M = c->dg.M, N = c->dg.N;
+    // the image (x fastest): temperature at z = 0, or particle counts on a res x res grid
+    int xs = M, ys = N;
+    std::vector<float> img;
+    std::string err;
+    if (want_t) {
+        img.resize((size_t)M * N);
+        const double z0 = 0.0;
+        if (int rc = rbc_temperature_grid(c, 1, &z0, img.data())) return rc;
+    } else {
+        xs = ys = res_req <= 0 ? 128 : res_req;
+        img.assign((size_t)xs * ys, 0.f);
+    }
+    if (want_p) {
+        std::vector<float> px((size_t)(t->buf - 1) * t->np), py(px.size()), pz(px.size());
+        int nt = 0;
+        if (int rc = rbc_tracer_get_positions(t, 0, px.data(), py.data(), pz.data(), &nt)) return rc;
+        const size_t off = (size_t)(nt - 1) * t->np;  // the last slot read out (noClear)
+        const float clx = (float)c->p.Lx, cly = (float)c->p.Ly;
+        for (int i = t->np - 1; i >= 0; --i) {  // reversed so that particle 0 is drawn last
+            const float x = px[off + i], y = py[off + i];
+            const float xg = (xs * x) / clx, yg = (ys * y) / cly;
+            const int ix = (int)(xg - std::floor(xg / xs) * xs), iy = (int)(yg - std::floor(yg / ys) * ys);
+            if (ix < 0 || ix >= xs || iy < 0 || iy >= ys) {
+                char b[96];
+                snprintf(b, sizeof b, "Particle %d at (%f,%f) out of bounds.", i, x, y);
+                err = b;
+                continue;
+            }
+            float& e = img[(size_t)iy * xs + ix];
+            e = img_type == RBC_IMG_TEMP_PARTICLES ? 0.f : e + 1.f;
+        }
+    }
+    int Nx = res_req, Ny = res_req;
+    if (Nx <= 0 || Nx > xs) Nx = xs;
+    if (Ny <= 0 || Ny > ys) Ny = ys;
+    const int npack = 1 + (Nx * Ny + kUData - 1) / kUData;
+    *nbytes = (size_t)npack * kUPack;
+    if (!out) return RBC_OK;
+    if (cap < *nbytes) return fail(c, RBC_ERR_ARG, "packet buffer too small");
+    UPacket* pk = (UPacket*)out;
+    std::memset(out, 0, *nbytes);
+    for (int i = 0; i < npack; ++i) {
+        pk[i].H.Type = i == 0 ? kUTypeH : kUTypeP;
+        pk[i].H.NumPackets = (uint16_t)npack;
+        pk[i].H.Nt = (uint32_t)Nt;
+        pk[i].H.Nx = (uint32_t)Nx;
+        pk[i].H.Ny = (uint32_t)Ny;
+    }
+    USimHeader h;
+    std::memset(&h, 0, sizeof h);
+    h.uVersion = 1.1f;
+    h.cVersion = 1.0f;
+    h.tFinal = (float)t_final;
+    h.dt = (float)c->p.dt;
+    h.Ra = (float)c->p.Ra;
+    h.Pr = (float)c->p.Pr;
+    h.eta = (float)eta;
+    h.clx = (float)c->p.Lx;
+    h.cly = (float)c->p.Ly;
+    h.clz = 1.f;
+    h.nmx = (uint16_t)M;
+    h.nmy = (uint16_t)N;
+    h.nmz = (uint16_t)c->dg.L;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->p.device) == hipSuccess) strncpy(h.device, prop.name, kUDescr);
+    const char* descr = img_type == RBC_IMG_TEMPERATURE ? "Temperature field at z = 0"
+                        : img_type == RBC_IMG_TEMP_PARTICLES ? "Temperature at z = 0 w/ Part" : "Particles";
+    strncpy(h.descr, descr, kUDescr);
+    strncpy(h.Err1, err.c_str(), kUDescr);
+    std::memcpy(&pk[0].P, &h, sizeof h);
+    int tot = 0, ip = 1, in = 0;
+    USimPayload* P = &pk[ip].P;
+    P->iStart = 0;
+    for (int ix = 0; ix < Nx; ++ix)
+        for (int iy = 0; iy < Ny; ++iy, ++in, ++tot) {
+            if (in >= kUData) {
+                P->iEnd = tot;
+                P = &pk[++ip].P;
+                P->iStart = tot;
+                in = 0;
+            }
+            const int sx = ix * xs / Nx, sy = iy * ys / Ny;
+            P->Data[in] = img[(size_t)sy * xs + sx];
+        }
+    P->iEnd = tot;
+    return RBC_OK;
+}

@@ -70,7 +70,9 @@ EXPORTS = ["rbc_create", "rbc_destroy", "rbc_last_error", "rbc_get_info", "rbc_s
            "rbc_closure_status", "rbc_set_state_device", "rbc_get_state_device", "rbc_nonlinear_device",
            "rbc_inverse_B_device", "rbc_implicit_device", "rbc_stream", "rbc_velocity_at_device",
            "rbc_init_state", "rbc_force_mean", "rbc_characteristic_velocity", "rbc_run_main",
-           "rbc_extract_vel_main", "rbc_velocity_at_device_noalloc"]
+           "rbc_extract_vel_main", "rbc_velocity_at_device_noalloc", "rbc_set_penalization", "rbc_freeze",
+           "rbc_apply_freeze", "rbc_temp_mask", "rbc_apply_temp_mask", "rbc_mask_theta", "rbc_temperature_grid",
+           "rbc_stream_packets"]
 
 CLOSURE_NIN, CLOSURE_NOUT = 8, 3
 
@@ -111,6 +113,15 @@ def load_library(path=LIB_PATH):
     L.rbc_stream.restype = vp
     L.rbc_init_state.argtypes = [vp, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double]
     L.rbc_characteristic_velocity.argtypes = [vp, ctypes.c_int, dp]
+    L.rbc_set_penalization.argtypes = [vp, ctypes.c_double, ctypes.c_int, dp]
+    L.rbc_freeze.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+    L.rbc_apply_freeze.argtypes = [vp]
+    L.rbc_temp_mask.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+    L.rbc_apply_temp_mask.argtypes = [vp]
+    L.rbc_mask_theta.argtypes = [vp] + [ctypes.c_double] * 4
+    L.rbc_temperature_grid.argtypes = [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_float)]
+    L.rbc_stream_packets.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_double, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
     L.rbc_run_main.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]
     L.rbc_extract_vel_main.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]
     L.rbc_force_mean.argtypes = [vp, ctypes.c_double]
@@ -318,6 +329,35 @@ class Stepper:
         k = self.INIT_KINDS[kind] if isinstance(kind, str) else int(kind)
         self._check(self.lib.rbc_init_state(self.h, k, int(seed), float(amplitude)), "rbc_init_state")
 
+    MASKS = {"rectangle": 0, "circle": 1, "diskarray": 2, "everywhere": 3, "custom": 4}
+    PLANES = {"circle": 0, "rectangle": 1}
+
+    def set_penalization(self, eta, shape="rectangle", mask=None):
+        """Brinkman penalization of a PENALIZED build (eta = 0: off); mask (N, M) for "custom"."""
+        sh = self.MASKS[shape] if isinstance(shape, str) else int(shape)
+        m = None
+        if mask is not None:
+            m = np.ascontiguousarray(mask, np.float64)
+            assert m.shape == (self.N, self.M)
+        self._check(self.lib.rbc_set_penalization(self.h, float(eta), sh, _dp(m) if m is not None else None),
+                    "rbc_set_penalization")
+
+    def freeze(self, shape="circle", param=0.8):
+        self._check(self.lib.rbc_freeze(self.h, self.PLANES[shape], float(param)), "rbc_freeze")
+
+    def apply_freeze(self):
+        self._check(self.lib.rbc_apply_freeze(self.h), "rbc_apply_freeze")
+
+    def temp_mask(self, shape="rectangle", param=0.1):
+        self._check(self.lib.rbc_temp_mask(self.h, self.PLANES[shape], float(param)), "rbc_temp_mask")
+
+    def apply_temp_mask(self):
+        self._check(self.lib.rbc_apply_temp_mask(self.h), "rbc_apply_temp_mask")
+
+    def mask_theta(self, cx, cy, radius, factor):
+        self._check(self.lib.rbc_mask_theta(self.h, float(cx), float(cy), float(radius), float(factor)),
+                    "rbc_mask_theta")
+
     def force_mean(self, mean_F):
         """F_0 = mean_F (force_mean, util/util.cu:74-81), queued on the context's stream."""
         self._check(self.lib.rbc_force_mean(self.h, float(mean_F)), "rbc_force_mean")
@@ -389,6 +429,27 @@ class Stepper:
         allocation; n <= 1,398,101; overlapping calls on one GPU race)."""
         self._check(self.lib.rbc_velocity_at_device_noalloc(self.h, int(n), *self._ptrs((x, y, z, u, v, w))),
                     "rbc_velocity_at_device_noalloc")
+
+    def temperature_grid(self, z):
+        """theta(x, y, z) at heights z, float32 (nz, N, M) (calculate_temperature_operator)."""
+        z = np.ascontiguousarray(z, np.float64)
+        out = np.zeros((len(z), self.N, self.M), np.float32)
+        self._check(self.lib.rbc_temperature_grid(self.h, len(z), _dp(z), _fp(out)), "rbc_temperature_grid")
+        return out
+
+    IMG = {"temperature": 1, "particles": 2, "temp_particles": 3}
+
+    def stream_packets(self, img="temperature", res=0, Nt=0, eta=0.0, t_final=0.0, tracer=None):
+        """The visual-stream packets of one frame (server_writer::Prepare) as bytes."""
+        k = self.IMG[img] if isinstance(img, str) else int(img)
+        th = tracer.h if tracer is not None else None
+        n = ctypes.c_size_t()
+        self._check(self.lib.rbc_stream_packets(self.h, th, k, int(res), int(Nt), float(eta), float(t_final), None, 0,
+                                                ctypes.byref(n)), "rbc_stream_packets")
+        buf = ctypes.create_string_buffer(n.value)
+        self._check(self.lib.rbc_stream_packets(self.h, th, k, int(res), int(Nt), float(eta), float(t_final), buf,
+                                                n.value, ctypes.byref(n)), "rbc_stream_packets")
+        return buf.raw[:n.value]
 
     def energy_spectrum(self):
         """(E_h per |k| shell, (E_h total, E_z total)), FP64."""

@@ -29,7 +29,6 @@ BASE = ["-nmodes", "32", "32", "8", "-clength", "4", "4", "-ra", "5000", "-pr", 
     BASE[:-4] + ["-t", "0.01"],                    # -dt missing
     BASE + ["-precision", "f16"],                  # unknown precision
     BASE + ["-init_rectangle"],                    # initial pattern not in this build
-    BASE + ["-freeze_circle", "10"],               # freeze operator not in this build
     BASE + ["-nfiles", "x"],                       # value that does not parse
 ])
 def test_driver_rejects(lib, args):

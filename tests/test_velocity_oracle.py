@@ -165,3 +165,39 @@ def test_bundled_state_header_layout():
     d = np.load(os.path.join(os.path.dirname(__file__), "golden", "data_state_kept.npz"))
     assert d["theta_dims"].tolist() == [257, 512, 2]
     assert d["F_dims"].tolist() == [1, 1, 2]
+
+
+def test_plane_operators_special_masks():
+    """oracle/planes.py pinned by what the operators do under uniform masks: a mask of ones
+    leaves a band-limited state unchanged; with zeros the freeze gives the frozen state, the heat
+    source adds 0.015 to every mode's mean (the transform of a constant 0.015 over M N points,
+    scaled by 1/(M N)), and the scale multiplies by the factor."""
+    from oracle import planes as PL
+    from tests.helpers import random_state
+    M, N, L = 32, 16, 3
+    st = random_state(M, N, L, seed=4, amp=1e-2)
+    other = random_state(M, N, L, seed=5, amp=1e-2)
+    ones = np.ones((N, M), np.int32)
+    zeros = np.zeros((N, M), np.int32)
+    for a, b in zip(PL.apply(st[:3], "heat", ones, M, N), st[:3]):
+        assert np.abs(a - b).max() < 1e-15
+    frozen = PL.capture(other[:3], M, N)
+    for a, b in zip(PL.apply(st[:3], "freeze", zeros, M, N, frozen=frozen), other[:3]):
+        assert np.abs(a - b).max() < 1e-15
+    heated = PL.apply(st[:1], "heat", zeros, M, N)[0]
+    d = heated - st[0]
+    assert np.allclose(d[:, 0, 0], 0.015, atol=1e-15) and np.abs(d[:, 1:, :]).max() < 1e-15
+    scaled = PL.apply(st[:1], "scale", zeros, M, N, factor=1.1)[0]
+    assert np.abs(scaled - 1.1 * st[0]).max() < 1e-15
+
+
+def test_plane_masks():
+    """Mask geometry: circle of radius 0.8 * min/2 around the centre (freeze), its complement
+    kept; the rectangle of width 0.1 keeps a border of ceil(0.1 M) points (masktemp)."""
+    from oracle import planes as PL
+    m = PL.plane_mask("circle", 64, 32, 0.8)
+    assert m[16, 32] == 0 and m[0, 0] == 1 and m[16, 32 + 13] == 1 and m[16, 32 + 12] == 0
+    r = PL.plane_mask("rectangle", 64, 32, 0.1)
+    assert r[16, 6] == 1 and r[16, 7] == 0 and r[3, 32] == 1 and r[4, 32] == 0 and r[16, 58] == 1
+    t = PL.plane_mask("circle", 64, 32, 0.2, 0.2, 0.3)
+    assert t[int(0.3 * 32), int(0.2 * 64)] == 0 and t[31, 63] == 1
