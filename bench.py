@@ -353,6 +353,62 @@ def _max_rel(a, b, keep):
     return max(errs)
 
 
+def tracer_flops(P, L, npart):
+    """Algorithmic FP32 operations of one spectral velocity evaluation (k_vel_points) at npart
+    particles over P kept modes: per (particle, mode) six length-L dot products (12 L), the
+    phase and its sine and cosine (about 20) and the three velocity components (about 20)."""
+    return npart * P * (12 * L + 40)
+
+
+def run_tracers(a):
+    """Tracer throughput at 128^2 x 16 (Ra 1e5, Pr 0.7, box 4 x 4) with 1e5 particles: the
+    spectral midpoint RK2 (two exact velocity evaluations per step) and the reference's RK4
+    with trilinear interpolation on the 128-level velocity grid."""
+    import ctypes
+    import sdc_amd
+    from sdc_amd.initial import random_theta
+    hip = ctypes.CDLL("libamdhip64.so")
+    cfg = (128, 128, 16, 4.0, 4.0, 1e5, 0.7, 2e-4)
+    npart, steps = 100_000, a.steps
+    S = sdc_amd.Stepper(*cfg, precision="f32")
+    S.set_state(*random_theta(128, 128, 16, seed=2, amp=1e-3))
+    S.step_time(20)
+    out = {"metric": "tracer steps/s (1e5 tracers, 128^2x16 RBC state)", "unit": "tracer-steps/s", "n_gpus": 1,
+           "steps": steps, "warmup": a.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic (seeded theta ~ 1e-3 N(0,1), 20 steps)",
+           "config": {"workload": "1e5 passive tracers in a 128x128 grid x 16 mode state, Ra=1e5, Pr=0.7, 4x4x1",
+                      "particles": npart}}
+    modes = {}
+    for mode in ("spectral_rk2", "rk4_trilinear"):
+        T = sdc_amd.Tracer(S, npart, buffer_size=steps + a.warmup + 2, mode=mode, seed=2)
+        T.init_positions("random")
+        for _ in range(a.warmup):
+            T.step_particles(1e-3)
+        S.synchronize()
+        hip.hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            T.step_particles(1e-3)
+        S.synchronize()
+        hip.hipDeviceSynchronize()
+        dt = time.perf_counter() - t0
+        T.close()
+        modes[mode] = {"tracer_steps_per_s": round(steps / dt, 2), "ms_per_step": round(1e3 * dt / steps, 4),
+                       "particle_steps_per_s": round(npart * steps / dt, 1)}
+    P = S.kx_kept * S.ky_kept
+    fl = 2 * tracer_flops(P, 16, npart)
+    tf = fl / (modes["spectral_rk2"]["ms_per_step"] * 1e-3) / 1e12
+    out["value"] = modes["spectral_rk2"]["tracer_steps_per_s"]
+    out["ms_per_step"] = modes["spectral_rk2"]["ms_per_step"]
+    out["modes"] = modes
+    out["roofline"] = {"bound": "valu", "achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": round(tf / FP32_PEAK_TFLOPS, 4), "kernel": "k_vel_points (two per spectral RK2 step)",
+                       "algorithmic_flops_per_step": fl, "kept_modes": P,
+                       "note": "whole tracer step time (both evaluations, the slice sums and the position updates)"}
+    S.close()
+    print(json.dumps(out))
+
+
 def run_validate(a):
     """The accuracy forms, each against an FP64 solver on the same seeded input:
     C1 (32^2 x 8, Ra 5000) FP64 on the GPU against the FP64 CPU oracle, 1000 steps;
@@ -497,6 +553,7 @@ def main():
     ap.add_argument("--ensemble", action="store_true", help="run the batched-ensemble workload instead")
     ap.add_argument("--closure", action="store_true", help="run the learned-closure experiment instead")
     ap.add_argument("--validate", action="store_true", help="run the accuracy forms (C1, C2, C3) instead")
+    ap.add_argument("--tracers", action="store_true", help="run the tracer workload (1e5 tracers at 128^2 x 16)")
     ap.add_argument("--closure-T", type=float, default=1.0)
     ap.add_argument("--closure-every", default="1", help="network intervals of the corrected runs, e.g. 1,2,4 "
                     "(the first is reported)")
@@ -507,7 +564,7 @@ def main():
     ap.add_argument("--ens-k", type=int, default=10, help="steps captured per graph launch")
     a = ap.parse_args()
     if a.steps is None:
-        a.steps = 500 if a.ensemble else 200
+        a.steps = 500 if a.ensemble else (50 if a.tracers else 200)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU layout of this "
               "path is described in DESIGN.md and not measured here.", file=sys.stderr)
@@ -521,6 +578,9 @@ def main():
         return
     if a.validate:
         run_validate(a)
+        return
+    if a.tracers:
+        run_tracers(a)
         return
 
     import sdc_amd
