@@ -365,6 +365,8 @@ int rbc_ensemble_batched_members(const rbc_ensemble* e);
  *                             steps (rbc_step then runs the corrected method)
  *   rbc_closure_interval      evaluate the network on every `every`-th corrected step and add
  *                             the last evaluated correction on the steps between (default 1)
+ *   rbc_closure_damping       add gain times the correction on the modes with |k| <= kfrac k_max
+ *                             (k_max: the largest kept |k|) and none above (default 1, 1)
  *   rbc_closure_step          corrected steps in chunks of check_every; a chunk whose last
  *                             correction has |dN| / |N| above max_ratio (or not finite) is redone
  *                             with the plain method, which then continues (fallback)
@@ -393,6 +395,7 @@ int rbc_closure_predict(rbc_closure* cl, int n, const float* X, float* Y);
 int rbc_closure_eval(rbc_closure* cl, int first, int n, double* mse_model, double* mse_zero);
 int rbc_closure_enable(rbc_closure* cl, int on, double max_ratio);
 int rbc_closure_interval(rbc_closure* cl, int every);
+int rbc_closure_damping(rbc_closure* cl, double gain, double kfrac);
 int rbc_closure_step(rbc_closure* cl, int nsteps, int check_every);
 int rbc_closure_status(const rbc_closure* cl, int* active, int* fallback, double* last_ratio);
 

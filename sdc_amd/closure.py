@@ -20,9 +20,10 @@ def _mk(cfg, precision):
 
 def _record(st, nsteps, every, advance, log=None, tag=""):
     """Advance nsteps in chunks of `every` with advance(k); returns (times of the samples,
-    volume Nu, plate Nu (mean of both plates), summed E_h spectra, number of spectra, wall s)."""
+    volume Nu, plate Nu (mean of both plates), the E_h spectra of the samples (rows, cut to a
+    common length), number of spectra, wall s)."""
     t_nu, nu_v, nu_p = [], [], []
-    spec, nspec = None, 0
+    spec, nspec = [], 0
     wall = 0.0
     done = 0
     while done < nsteps:
@@ -38,24 +39,27 @@ def _record(st, nsteps, every, advance, log=None, tag=""):
         nu_v.append(nu[2])
         nu_p.append(0.5 * (nu[0] + nu[1]))
         E, _ = st.energy_spectrum()
-        if spec is None:
-            spec = E.copy()
-        else:
-            n = min(len(spec), len(E))
-            spec = spec[:n] + E[:n]
+        spec.append(np.array(E, copy=True))
         nspec += 1
         if log and nspec % 10 == 0:
             log(tag, "step", done, "of", nsteps, "Nu", nu.tolist(), "wall", round(wall, 2))
-    return np.array(t_nu), np.array(nu_v), np.array(nu_p), spec, nspec, wall
+    n = min(len(E) for E in spec)
+    return np.array(t_nu), np.array(nu_v), np.array(nu_p), np.array([E[:n] for E in spec]), nspec, wall
 
 
 def run(fine, coarse, *, T=1.0, spinup_steps=2000, samples=10, train_steps=2000, batch=65536, lr=1e-3,
         nz=0, seed=3, tol=0.02, every=200, fine_precision="f32", coarse_precision="f32", max_ratio=10.0,
-        check_every=200, closure_every=1, log=None):
+        check_every=200, closure_every=1, variants=None, log=None):
     """The whole experiment.  fine / coarse: dicts with M, N, L, Lx, Ly, Ra, Pr, dt (same box,
     Ra, Pr).  closure_every: the network is evaluated on every closure_every-th coarse step and
-    the steps between reuse its last correction (a list runs one corrected coarse run per value;
-    the first is the reported one).  Returns a dict of the measurements."""
+    the steps between reuse its last correction (a list runs one corrected coarse run per value).
+    variants: corrected runs as dicts {"every", "gain", "kfrac"} (gain times the correction on
+    the modes with |k| <= kfrac k_max, none above), in place of closure_every.
+
+    Errors are measured on both halves of the time unit: the first half (whose fine states also
+    supply the training data) selects the reported corrected run -- the smallest of its Nu and
+    spectrum errors relative to the plain coarse run's -- and the second half is reported, so
+    that the choice does not see the reported figures.  Returns a dict of the measurements."""
     say = log or (lambda *a: None)
     out = {}
     F = _mk(fine, fine_precision)
@@ -124,34 +128,54 @@ def run(fine, coarse, *, T=1.0, spinup_steps=2000, samples=10, train_steps=2000,
         cl.enable(False)
         return r, status
 
+    first = lambda a: a[:len(a) // 2]  # noqa: E731
     half = lambda a: a[len(a) // 2:]  # noqa: E731
-    nu_f = float(np.mean(half(nuf)))
-    ef = specf / nsf
+    nu_f, nu_f1 = float(np.mean(half(nuf))), float(np.mean(first(nuf)))
 
-    def spec_err(e):
-        nb = min(len(ef), len(e))
+    def spec_err(e, ref):
+        nb = min(len(ref), len(e))
         kcut = max(2, nb - 1)
-        return float(np.linalg.norm(e[1:kcut] - ef[1:kcut]) / np.linalg.norm(ef[1:kcut]))
+        return float(np.linalg.norm(e[1:kcut] - ref[1:kcut]) / np.linalg.norm(ref[1:kcut]))
+
+    def errors(nu, spec):
+        """(first-half, second-half) relative errors of the Nu mean and the mean spectrum."""
+        nb = min(spec.shape[1], specf.shape[1])
+        e1 = (abs(np.mean(first(nu)) - nu_f1) / nu_f1,
+              spec_err(first(spec[:, :nb]).mean(0), first(specf[:, :nb]).mean(0)))
+        e2 = (abs(np.mean(half(nu)) - nu_f) / nu_f, spec_err(half(spec[:, :nb]).mean(0), half(specf[:, :nb]).mean(0)))
+        return e1, e2
 
     (tp, nup, nupp, specp, nsp, wall_p), _ = coarse_run(False)
     nu_p = float(np.mean(half(nup)))
-    out["plain_coarse"] = dict(steps=n_c, wall_s=wall_p, nu=nu_p, nu_rel_err=abs(nu_p - nu_f) / nu_f,
-                               spectrum_rel_err=spec_err(specp / nsp),
+    (p1_nu, p1_sp), (p2_nu, p2_sp) = errors(nup, specp)
+    out["plain_coarse"] = dict(steps=n_c, wall_s=wall_p, nu=nu_p, nu_rel_err=p2_nu, spectrum_rel_err=p2_sp,
+                               first_half=dict(nu_rel_err=p1_nu, spectrum_rel_err=p1_sp),
                                speedup_wall_per_time_unit=wall_f / wall_p if wall_p > 0 else None)
-    # one corrected run per network interval; the first is the reported one
-    variants = []
-    for k in (closure_every if isinstance(closure_every, (list, tuple)) else [closure_every]):
-        (tc, nuc, nucp, specc, nsc, wall_c), status = coarse_run(True, k)
+    if variants is None:
+        variants = [dict(every=k, gain=1.0, kfrac=1.0)
+                    for k in (closure_every if isinstance(closure_every, (list, tuple)) else [closure_every])]
+    runs = []
+    for v in variants:
+        cl.set_damping(v.get("gain", 1.0), v.get("kfrac", 1.0))
+        (tc, nuc, nucp, specc, nsc, wall_c), status = coarse_run(True, v.get("every", 1))
         nu_c = float(np.mean(half(nuc)))
-        variants.append(dict(network_every=k, steps=n_c, wall_s=wall_c, nu=nu_c, nu_rel_err=abs(nu_c - nu_f) / nu_f,
-                             spectrum_rel_err=spec_err(specc / nsc), runtime_fallback=status[1],
-                             last_correction_ratio=status[2],
-                             speedup_wall_per_time_unit=wall_f / wall_c if wall_c > 0 else None))
-        say("closure run", variants[-1])
-    out["closure_coarse"] = variants[0]
-    out["closure_variants"] = variants
+        (c1_nu, c1_sp), (c2_nu, c2_sp) = errors(nuc, specc)
+        # selection score on the first half: the larger of the two errors relative to plain's
+        score = max(c1_nu / max(p1_nu, 1e-12), c1_sp / max(p1_sp, 1e-12)) if np.isfinite(nu_c) else np.inf
+        runs.append(dict(network_every=v.get("every", 1), gain=v.get("gain", 1.0), kfrac=v.get("kfrac", 1.0),
+                         steps=n_c, wall_s=wall_c, nu=nu_c, nu_rel_err=c2_nu, spectrum_rel_err=c2_sp,
+                         first_half=dict(nu_rel_err=c1_nu, spectrum_rel_err=c1_sp, score_vs_plain=score),
+                         beats_plain=bool(c2_nu < p2_nu and c2_sp < p2_sp),
+                         runtime_fallback=status[1], last_correction_ratio=status[2],
+                         speedup_wall_per_time_unit=wall_f / wall_c if wall_c > 0 else None))
+        say("closure run", runs[-1])
+    cl.set_damping(1.0, 1.0)
+    best = min(range(len(runs)), key=lambda i: runs[i]["first_half"]["score_vs_plain"])
+    out["closure_coarse"] = runs[best]
+    out["closure_variants"] = runs
+    out["selected_variant"] = best
     out["nu_fine"] = nu_f
-    nu_c, wall_c, fell_back = variants[0]["nu"], variants[0]["wall_s"], variants[0]["runtime_fallback"]
+    nu_c, wall_c, fell_back = runs[best]["nu"], runs[best]["wall_s"], runs[best]["runtime_fallback"]
     out["speedup_wall_per_time_unit"] = wall_f / wall_c if wall_c > 0 else None
     use_closure = (not fell_back) and np.isfinite(nu_c) and abs(nu_c - nu_f) / nu_f <= tol
     out["tolerance"] = tol

@@ -347,7 +347,9 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_closure_project(DevGeom g, int nz, const T* __restrict__ pw,
                                                          const cpx<float>* __restrict__ spec, cpx<T>* __restrict__ N0,
                                                          cpx<T>* __restrict__ N1, cpx<T>* __restrict__ N2,
-                                                         cpx<T>* __restrict__ dN, double* __restrict__ guard) {
+                                                         cpx<T>* __restrict__ dN, double* __restrict__ guard,
+                                                         T gain, T k2cut, const T* __restrict__ kxa,
+                                                         const T* __restrict__ kya) {
     __shared__ double sd[256], sn[256];
     const int L = g.L, P = g.P;
     const size_t PL = (size_t)P * L;
@@ -378,6 +380,11 @@ __global__ void __launch_bounds__(256) k_closure_project(DevGeom g, int nz, cons
                     ay += wl[u] * (T)v[u].y;
                 }
         }
+        // damping: the correction times gain on the modes with |k|^2 <= k2cut, zero above
+        const T k2 = kxa[p] * kxa[p] + kya[p] * kya[p];
+        const T fac = k2 <= k2cut ? gain : T(0);
+        ax *= fac;
+        ay *= fac;
         cpx<T> n = out[e];
         nn = (double)n.x * n.x + (double)n.y * n.y;
         dd = (double)ax * ax + (double)ay * ay;
@@ -756,7 +763,7 @@ struct rbc_closure {
     // run-time state
     bool enabled = false, fallback = false;
     double max_ratio = 1.0, last_ratio = 0.0;
-    float gain = 1.0f;
+    double gain = 1.0, k2cut = HUGE_VAL;  // damping (rbc_closure_damping): all modes at full gain
     std::vector<void*> snap;          // state + history copy for the fallback
     ~rbc_closure() {
         if (c) {
@@ -843,7 +850,8 @@ void launch_project(rbc_closure* cl) {
     const DevGeom& g = cl->gz;
     hipLaunchKernelGGL(k_closure_project<T>, dim3(cl->nguard), dim3(256), 0, c->stream, g, cl->nz,
                        (const T*)cl->pw, (const cpx<float>*)cl->spec, (cpx<T>*)c->b.Nn[0], (cpx<T>*)c->b.Nn[1],
-                       (cpx<T>*)c->b.Nn[2], (cpx<T>*)cl->dN, cl->guard);
+                       (cpx<T>*)c->b.Nn[2], (cpx<T>*)cl->dN, cl->guard, (T)cl->gain, (T)cl->k2cut,
+                       (const T*)c->b.kx, (const T*)c->b.ky);
 }
 
 template <typename T>
@@ -1451,6 +1459,17 @@ int rbc_closure_enable(rbc_closure* cl, int on, double max_ratio) {
         }
         cl->enabled = false;
     }
+    return RBC_OK;
+}
+
+int rbc_closure_damping(rbc_closure* cl, double gain, double kfrac) {
+    if (!cl) return RBC_ERR_ARG;
+    if (!std::isfinite(gain) || gain < 0 || !(kfrac > 0)) return cfail(cl, RBC_ERR_ARG, "gain >= 0 and kfrac > 0 required");
+    const rbc::Factors& f = cl->c->fac;
+    double k2max = 0;
+    for (double k2 : f.k2) k2max = std::max(k2max, k2);
+    cl->gain = gain;
+    cl->k2cut = kfrac >= 1 ? HUGE_VAL : kfrac * kfrac * k2max;
     return RBC_OK;
 }
 

@@ -66,7 +66,8 @@ EXPORTS = ["rbc_create", "rbc_destroy", "rbc_last_error", "rbc_get_info", "rbc_s
            "rbc_closure_dataset", "rbc_closure_load_dataset", "rbc_closure_clear_dataset",
            "rbc_closure_fit_normalization", "rbc_closure_get_normalization", "rbc_closure_set_normalization",
            "rbc_closure_get_weights", "rbc_closure_set_weights", "rbc_closure_gradient", "rbc_closure_train",
-           "rbc_closure_predict", "rbc_closure_eval", "rbc_closure_enable", "rbc_closure_interval", "rbc_closure_step",
+           "rbc_closure_predict", "rbc_closure_eval", "rbc_closure_enable", "rbc_closure_interval",
+           "rbc_closure_damping", "rbc_closure_step",
            "rbc_closure_status", "rbc_set_state_device", "rbc_get_state_device", "rbc_nonlinear_device",
            "rbc_inverse_B_device", "rbc_implicit_device", "rbc_stream", "rbc_velocity_at_device",
            "rbc_init_state", "rbc_force_mean", "rbc_characteristic_velocity", "rbc_run_main",
@@ -190,6 +191,7 @@ def load_library(path=LIB_PATH):
     L.rbc_closure_eval.argtypes = [vp, i_, i_, dp, dp]
     L.rbc_closure_enable.argtypes = [vp, i_, ctypes.c_double]
     L.rbc_closure_interval.argtypes = [vp, i_]
+    L.rbc_closure_damping.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     L.rbc_closure_step.argtypes = [vp, i_, i_]
     L.rbc_closure_status.argtypes = [vp, ip, ip, dp]
     _lib = L
@@ -737,6 +739,10 @@ class Closure:
         """Evaluate the network on every `every`-th corrected step; the steps between reuse the
         last evaluated correction."""
         self._check(self.lib.rbc_closure_interval(self.h, int(every)), "rbc_closure_interval")
+
+    def set_damping(self, gain=1.0, kfrac=1.0):
+        """Add gain times the correction on the modes with |k| <= kfrac * k_max, none above."""
+        self._check(self.lib.rbc_closure_damping(self.h, float(gain), float(kfrac)), "rbc_closure_damping")
 
     def step(self, nsteps, check_every=100):
         self._check(self.lib.rbc_closure_step(self.h, int(nsteps), int(check_every)), "rbc_closure_step")

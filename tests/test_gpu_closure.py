@@ -208,6 +208,57 @@ def test_correction_interval_reuses_the_last_correction():
     assert all(np.isfinite(a).all() for a in two2)
 
 
+def test_damping_scales_and_filters_the_correction():
+    """rbc_closure_damping(gain, kfrac): gain 0 gives the plain step bit for bit; one corrected
+    Euler step is linear in the added correction, so the change a gain of 0.5 makes is half the
+    change of gain 1 (to FP32 rounding of the update); with kfrac = 0.5 the change vanishes on
+    every mode with |k| > 0.5 k_max and equals the undamped change on the others."""
+    print("case: closure damping", flush=True)
+    from sdc_amd import Closure
+    from sdc_amd.initial import kept_mask
+    st = random_state(COARSE["M"], COARSE["N"], COARSE["L"], seed=15, amp=1e-1)
+    w, norm, _, _ = _net(6)
+
+    def run(gain, kfrac, closure=True):
+        with _stepper(COARSE, "f64") as A:
+            A.set_state(*st)
+            cl = Closure(A, nz=8, capacity_points=0)
+            cl.set_weights(w)
+            cl.set_normalization(*norm)
+            cl.set_damping(gain, kfrac)
+            cl.enable(closure, max_ratio=1e30)
+            if closure:
+                cl.step(1, check_every=1)
+            else:
+                A.step_time(1)
+            out = A.get_state()
+            cl.close()
+        return out
+
+    plain = run(1.0, 1.0, closure=False)
+    zero = run(0.0, 1.0)
+    assert all(np.array_equal(a, b) for a, b in zip(zero, plain))
+    full, halfg, low = run(1.0, 1.0), run(0.5, 1.0), run(1.0, 0.5)
+    d_full = [a - b for a, b in zip(full[:3], plain[:3])]
+    d_half = [a - b for a, b in zip(halfg[:3], plain[:3])]
+    d_low = [a - b for a, b in zip(low[:3], plain[:3])]
+    scale = max(float(np.abs(d).max()) for d in d_full)
+    assert scale > 0
+    assert max(float(np.abs(h - 0.5 * f).max()) for h, f in zip(d_half, d_full)) < 1e-9 * max(scale, 1.0)
+    # |k| per (ky row, kx column) of the reference layout
+    M, N = COARSE["M"], COARSE["N"]
+    kx = 2 * np.pi / COARSE["Lx"] * np.arange(M // 2 + 1)
+    ky = 2 * np.pi / COARSE["Ly"] * np.fft.fftfreq(N, 1.0 / N)
+    k = np.sqrt(kx[None, :] ** 2 + ky[:, None] ** 2)
+    keep = kept_mask(M, N)
+    high = keep & (k > 0.5 * k[keep].max() * (1 + 1e-12))
+    lowm = keep & (k < 0.5 * k[keep].max() * (1 - 1e-12))
+    assert high.any() and lowm.any()
+    for dl, df in zip(d_low, d_full):
+        assert np.array_equal(dl[:, high], np.zeros_like(dl[:, high]))
+        assert np.abs(dl[:, lowm] - df[:, lowm]).max() < 1e-9 * max(scale, 1.0)
+
+
 def test_fallback_to_the_plain_step():
     print("case: closure fallback", flush=True)
     from sdc_amd import Closure
