@@ -1,7 +1,8 @@
 """Learned-closure workflow on one GPU: a plain fine run supplies training data and is the
 judge; a coarse run with the trained closure is compared with it and with the plain coarse
 run, and the closure is dropped (fallback to the plain coarse method) when its Nusselt
-number misses the fine run's by more than a tolerance.
+number misses the fine run's by more than a tolerance or when it does no better than the plain
+coarse run on the half of the time unit that selects it.
 
 All compute runs in libsdc_rbc.so (rbc_step, rbc_closure_*); this module only sequences it.
 """
@@ -177,7 +178,10 @@ def run(fine, coarse, *, T=1.0, spinup_steps=2000, samples=10, train_steps=2000,
     out["nu_fine"] = nu_f
     nu_c, wall_c, fell_back = runs[best]["nu"], runs[best]["wall_s"], runs[best]["runtime_fallback"]
     out["speedup_wall_per_time_unit"] = wall_f / wall_c if wall_c > 0 else None
-    use_closure = (not fell_back) and np.isfinite(nu_c) and abs(nu_c - nu_f) / nu_f <= tol
+    # the closure is kept only when it is within the Nu tolerance of the fine run and better
+    # than the plain coarse run on the selection half (score below 1)
+    use_closure = ((not fell_back) and np.isfinite(nu_c) and abs(nu_c - nu_f) / nu_f <= tol
+                   and runs[best]["first_half"]["score_vs_plain"] < 1.0)
     out["tolerance"] = tol
     out["fallback_to_plain"] = not use_closure
     out["result_nu"] = nu_c if use_closure else nu_p
