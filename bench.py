@@ -317,12 +317,38 @@ CLOSURE_COARSE = dict(M=128, N=128, L=16, Lx=4.0, Ly=4.0, Ra=1e6, Pr=0.7, dt=1e-
 
 def closure_variants(a):
     """Corrected runs from --closure-variants "every:gain:kfrac,..." (default: the network on every
-    step and every 2nd step undamped, then damped and low-pass-filtered corrections)."""
+    step and every 2nd step undamped, then damped and low-pass-filtered corrections), plus the
+    controls of --closure-controls (same syntax; reported, never selected)."""
     out = []
-    for item in a.closure_variants.split(","):
-        e, g, k = (item.split(":") + ["1", "1"])[:3]
-        out.append(dict(every=int(e), gain=float(g), kfrac=float(k)))
+    for spec, control in ((a.closure_variants, False), (a.closure_controls, True)):
+        for item in filter(None, spec.split(",")):
+            e, g, k = (item.split(":") + ["1", "1"])[:3]
+            out.append(dict(every=int(e), gain=float(g), kfrac=float(k), control=control))
     return out
+
+
+SMALL_FINE = dict(M=128, N=128, L=16, Lx=4.0, Ly=4.0, Ra=1e5, Pr=0.7, dt=2.5e-5)
+SMALL_COARSE = [dict(M=32, N=32, L=8, Lx=4.0, Ly=4.0, Ra=1e5, Pr=0.7, dt=1e-4),
+                dict(M=64, N=64, L=8, Lx=4.0, Ly=4.0, Ra=1e5, Pr=0.7, dt=5e-5)]
+
+
+def run_closure_small(a):
+    """The closure experiment at a small size where the coarse grids are far from resolved:
+    plain 128^2 x 16 at Ra = 1e5 (judge and data source) against 32^2 x 8 and 64^2 x 8 runs with
+    and without the closure over one time unit; one JSON line per coarse grid."""
+    from sdc_amd import closure
+    for coarse in SMALL_COARSE:
+        r = closure.run(SMALL_FINE, coarse, T=a.closure_T, spinup_steps=8000, samples=a.closure_samples,
+                        train_steps=a.closure_train_steps, batch=65536, lr=1e-3, every=400,
+                        fine_precision=a.precision, variants=closure_variants(a),
+                        log=lambda *m: print("#", *m, file=sys.stderr, flush=True))
+        print(json.dumps({"metric": "closure vs plain Nu err & spectrum err (small grids, Ra=1e5)",
+                          "config": {"fine": SMALL_FINE, "coarse": coarse, "T": a.closure_T},
+                          "nu_fine": r["nu_fine"], "plain_coarse": r["plain_coarse"],
+                          "closure_coarse": r["closure_coarse"], "closure_variants": r["closure_variants"],
+                          "selected_variant": r["selected_variant"], "fallback_to_plain": r["fallback_to_plain"],
+                          "training": r["training"],
+                          "fine": {k: v for k, v in r["fine"].items() if k != "nu_series"}}), flush=True)
 
 
 def run_closure(a):
@@ -562,6 +588,8 @@ def main():
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--ensemble", action="store_true", help="run the batched-ensemble workload instead")
     ap.add_argument("--closure", action="store_true", help="run the learned-closure experiment instead")
+    ap.add_argument("--closure-small", action="store_true",
+                    help="run the closure experiment on small grids (128^2 x 16 fine, 32^2 x 8 and 64^2 x 8 coarse)")
     ap.add_argument("--validate", action="store_true", help="run the accuracy forms (C1, C2, C3) instead")
     ap.add_argument("--tracers", action="store_true", help="run the tracer workload (1e5 tracers at 128^2 x 16)")
     ap.add_argument("--closure-T", type=float, default=1.0)
@@ -569,6 +597,9 @@ def main():
                     help="corrected runs as every:gain:kfrac (network interval, gain of the correction, "
                          "fraction of k_max below which it is added); the first half of the time unit "
                          "selects the reported one")
+    ap.add_argument("--closure-controls", default="1:0.001:1",
+                    help="control runs (every:gain:kfrac): a tiny gain makes the plain coarse run under a small "
+                         "perturbation, which shows the trajectory-to-trajectory spread of the error measures")
     ap.add_argument("--closure-spinup", type=int, default=8000)
     ap.add_argument("--closure-samples", type=int, default=10)
     ap.add_argument("--closure-train-steps", type=int, default=2000)
@@ -584,6 +615,9 @@ def main():
 
     if a.ensemble:
         run_ensemble(a)
+        return
+    if a.closure_small:
+        run_closure_small(a)
         return
     if a.closure:
         run_closure(a)

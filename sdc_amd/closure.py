@@ -55,7 +55,10 @@ def run(fine, coarse, *, T=1.0, spinup_steps=2000, samples=10, train_steps=2000,
     Ra, Pr).  closure_every: the network is evaluated on every closure_every-th coarse step and
     the steps between reuse its last correction (a list runs one corrected coarse run per value).
     variants: corrected runs as dicts {"every", "gain", "kfrac"} (gain times the correction on
-    the modes with |k| <= kfrac k_max, none above), in place of closure_every.
+    the modes with |k| <= kfrac k_max, none above), in place of closure_every.  A variant with
+    "control": True is run and reported but never selected: with a tiny gain it is the plain
+    coarse run under a small perturbation, whose errors show how much a single chaotic
+    trajectory's time means vary.
 
     Errors are measured on both halves of the time unit: the first half (whose fine states also
     supply the training data) selects the reported corrected run -- the smallest of its Nu and
@@ -164,6 +167,7 @@ def run(fine, coarse, *, T=1.0, spinup_steps=2000, samples=10, train_steps=2000,
         # selection score on the first half: the larger of the two errors relative to plain's
         score = max(c1_nu / max(p1_nu, 1e-12), c1_sp / max(p1_sp, 1e-12)) if np.isfinite(nu_c) else np.inf
         runs.append(dict(network_every=v.get("every", 1), gain=v.get("gain", 1.0), kfrac=v.get("kfrac", 1.0),
+                         control=bool(v.get("control", False)),
                          steps=n_c, wall_s=wall_c, nu=nu_c, nu_rel_err=c2_nu, spectrum_rel_err=c2_sp,
                          first_half=dict(nu_rel_err=c1_nu, spectrum_rel_err=c1_sp, score_vs_plain=score),
                          beats_plain=bool(c2_nu < p2_nu and c2_sp < p2_sp),
@@ -171,7 +175,8 @@ def run(fine, coarse, *, T=1.0, spinup_steps=2000, samples=10, train_steps=2000,
                          speedup_wall_per_time_unit=wall_f / wall_c if wall_c > 0 else None))
         say("closure run", runs[-1])
     cl.set_damping(1.0, 1.0)
-    best = min(range(len(runs)), key=lambda i: runs[i]["first_half"]["score_vs_plain"])
+    eligible = [i for i in range(len(runs)) if not runs[i]["control"]] or list(range(len(runs)))
+    best = min(eligible, key=lambda i: runs[i]["first_half"]["score_vs_plain"])
     out["closure_coarse"] = runs[best]
     out["closure_variants"] = runs
     out["selected_variant"] = best
