@@ -336,6 +336,158 @@ __global__ void __launch_bounds__(256) k_vel_points(DevGeom g, int np, const flo
     }
 }
 
+// The same sums on the FP32 matrix cores (FP32 state, L <= 32).  Block = 8 waves x 16
+// particles.  For a tile of 16 modes the six vertical sums of every (mode, particle) pair,
+// A = sum f_m C'_m, Cc = sum f_m C_m and B = sum g_m S_m with their real and imaginary parts,
+// are products D[mode][particle] = sum_m coef[mode][m] basis[m][particle] with k = m:
+// v_mfma_f32_16x16x4_f32 (K = 4), A operand = staged coefficients (row = mode = lane % 16,
+// k = lane / 16), B operand = basis values at the lane's particle (column = particle = lane % 16,
+// k = lane / 16).  The phase and the velocity combination then run on the VALU in the MFMA output
+// layout: lane l holds column l % 16 (its particle) and the mode rows 4 (l / 16) + r, r < 4, so the
+// four 16-lane groups hold complementary rows of the same particles and are added at the end.
+// The weight w_k (1 at kx = 0, else 2) is folded into the staged coefficients.  The modes come
+// from the padded list (padded_mode_table), where the 4 rows of a lane group share kx and have
+// consecutive ky: one sincospi per group gives exp(2 pi i k.x) of its first row, and the others
+// follow by a complex product with exp(2 pi i r y / Ly), r = 1, 2, 3, taken once per particle.
+// The coefficients of the next 64-mode tile are loaded into registers while the current tile is
+// computed, across LDS-only barriers.
+typedef float vpf32x4 __attribute__((ext_vector_type(4)));
+constexpr int kVelMmaThreads = 512;  // 8 waves
+template <int LMAX>
+__global__ void __launch_bounds__(kVelMmaThreads) k_vel_points_mma(
+    DevGeom g, int np, const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
+    const cpx<float>* __restrict__ Xf, const cpx<float>* __restrict__ Xg, const cpx<float>* __restrict__ XF,
+    const cpx<float>* __restrict__ XG, const double* __restrict__ lam, const float4* __restrict__ modes,
+    const int* __restrict__ pmap, int npad, int tiles_per_slice, float* __restrict__ part) {
+    static_assert(LMAX % 4 == 0 && LMAX <= 32, "k-steps of 4 modes");
+    constexpr int KS = LMAX / 4;
+    constexpr int PER = (kTile * LMAX + kVelMmaThreads - 1) / kVelMmaThreads;  // staged values per thread
+    __shared__ float4 sc[kTile][LMAX + 1];  // w_k (f.re, f.im, g.re, g.im); odd row length spreads the banks
+    __shared__ float4 sm[kTile];            // (kx index, ky index, kx, ky)
+    const int L = g.L, P = g.P, NYc = g.NYc;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j = lane & 15, q = lane >> 4;
+    const int i = blockIdx.x * (kVelMmaThreads / 4) + wave * 16 + j;
+    const int slice = blockIdx.y;
+    const bool live = i < np;
+    const float x = live ? px[i] : 0.f, y = live ? py[i] : 0.f, z = live ? pz[i] : 0.f;
+    // basis values of the lane's k index m = 4 s + q at the particle's height
+    float bcp[KS], bcc[KS], bss[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        const int m = 4 * s + q;
+        double C = 0, Cp = 0, S = 0;
+        if (m < L) basis_at(m, lam[m], (double)z, C, Cp, S);
+        bcp[s] = (float)Cp;
+        bcc[s] = (float)C;
+        bss[s] = (float)S;
+    }
+    const float xr = x / (float)g.Lx - floorf(x / (float)g.Lx);
+    const float yr = y / (float)g.Ly - floorf(y / (float)g.Ly);
+    // exp(2 pi i r yr), r = 1, 2, 3
+    float ds[4], dc[4];
+    sincospif(2.0f * yr, &ds[1], &dc[1]);
+    dc[2] = dc[1] * dc[1] - ds[1] * ds[1];
+    ds[2] = 2.f * ds[1] * dc[1];
+    dc[3] = dc[2] * dc[1] - ds[2] * ds[1];
+    ds[3] = ds[2] * dc[1] + dc[2] * ds[1];
+    float u = 0.f, v = 0.f, w = 0.f;
+    if (slice == 0) {  // mean flow (k = 0), each lane group its residue of m modulo 4
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int m = 4 * s + q;
+            if (m < L) {
+                u += bss[s] * XF[m].x;
+                v += bss[s] * XG[m].x;
+            }
+        }
+    }
+    const int ntile = (npad + kTile - 1) / kTile;
+    const int t0 = slice * tiles_per_slice, t1 = min(ntile, t0 + tiles_per_slice);
+    // the coefficients of tile t into registers: value e = threadIdx.x + u * threads of the tile
+    float4 pre[PER], pmd = make_float4(0.f, 0.f, 0.f, 0.f);
+    auto fetch = [&](int t) {
+        const int pbase = t * kTile, nm = min(kTile, npad - pbase);
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int e = threadIdx.x + u * kVelMmaThreads, jj = e / LMAX, m = e - jj * LMAX;
+            float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
+            const int p = (e < kTile * LMAX && jj < nm) ? pmap[pbase + jj] : -1;
+            if (p >= 0 && p < P && m < L) {
+                const cpx<float> f = Xf[(size_t)p * L + m], gg = Xg[(size_t)p * L + m];
+                const float wk = p < NYc ? 1.f : 2.f;  // compact p < NYc <=> kx = 0
+                val = make_float4(wk * f.x, wk * f.y, wk * gg.x, wk * gg.y);
+            }
+            pre[u] = val;
+        }
+        if (threadIdx.x < kTile)
+            pmd = (int)threadIdx.x < nm ? modes[pbase + threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
+    };
+    if (t0 < t1) fetch(t0);
+    for (int t = t0; t < t1; ++t) {
+        const int nm = min(kTile, npad - t * kTile);
+        lds_barrier();  // the previous tile is consumed
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int e = threadIdx.x + u * kVelMmaThreads;
+            if (e < kTile * LMAX) sc[e / LMAX][e % LMAX] = pre[u];
+        }
+        if (threadIdx.x < kTile) sm[threadIdx.x] = pmd;
+        lds_barrier();
+        if (t + 1 < t1) fetch(t + 1);  // in flight while tile t is computed
+        const int nsub = (nm + 15) / 16;
+#pragma unroll 1
+        for (int sub = 0; sub < nsub; ++sub) {
+            vpf32x4 aR = {}, aI = {}, cR = {}, cI = {}, bR = {}, bI = {};
+            const float4* row = sc[sub * 16 + j];
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const float4 c4 = row[4 * s + q];
+                aR = __builtin_amdgcn_mfma_f32_16x16x4f32(c4.x, bcp[s], aR, 0, 0, 0);
+                aI = __builtin_amdgcn_mfma_f32_16x16x4f32(c4.y, bcp[s], aI, 0, 0, 0);
+                cR = __builtin_amdgcn_mfma_f32_16x16x4f32(c4.x, bcc[s], cR, 0, 0, 0);
+                cI = __builtin_amdgcn_mfma_f32_16x16x4f32(c4.y, bcc[s], cI, 0, 0, 0);
+                bR = __builtin_amdgcn_mfma_f32_16x16x4f32(c4.z, bss[s], bR, 0, 0, 0);
+                bI = __builtin_amdgcn_mfma_f32_16x16x4f32(c4.w, bss[s], bI, 0, 0, 0);
+            }
+            const int b0 = sub * 16 + 4 * q;
+            float s0, c0;
+            {
+                const float4 m0 = sm[b0];
+                float ph = m0.x * xr + m0.y * yr;
+                ph -= rintf(ph);
+                sincospif(2.0f * ph, &s0, &c0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float4 md = sm[b0 + r];
+                const float cs = r == 0 ? c0 : c0 * dc[r] - s0 * ds[r];
+                const float sn = r == 0 ? s0 : s0 * dc[r] + c0 * ds[r];
+                const float kx = md.z, ky = md.w;
+                // u_k = i kx A + i ky B, v_k = i ky A - i kx B, w_k = k^2 Cc; with
+                // PA = Im(e A) = cs A.im + sn A.re and PB likewise, Re(e u_k) = -kx PA - ky PB and
+                // Re(e v_k) = -ky PA + kx PB (e = cs + i sn)
+                const float pa = cs * aI[r] + sn * aR[r], pb = cs * bI[r] + sn * bR[r];
+                const float k2 = kx * kx + ky * ky;
+                u -= kx * pa + ky * pb;
+                v += kx * pb - ky * pa;
+                w += k2 * (cs * cR[r] - sn * cI[r]);
+            }
+        }
+    }
+    u += __shfl_xor(u, 16);
+    v += __shfl_xor(v, 16);
+    w += __shfl_xor(w, 16);
+    u += __shfl_xor(u, 32);
+    v += __shfl_xor(v, 32);
+    w += __shfl_xor(w, 32);
+    if (live && q == 0) {
+        float* o = part + (size_t)slice * 3 * np;
+        o[i] = u;
+        o[np + i] = v;
+        o[2 * (size_t)np + i] = w;
+    }
+}
+
 __global__ void k_sum_slices(int np, int nslice, const float* __restrict__ part, float* __restrict__ u,
                              float* __restrict__ v, float* __restrict__ w) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -572,6 +724,36 @@ std::vector<float> mode_table(const rbc_ctx* c) {
     return t;
 }
 
+// Mode list of k_vel_points_mma: per kx column, every run of consecutive ky (the ky >= 0 and
+// ky < 0 halves of the kept rows) padded to a multiple of 4 with slots of zero coefficient whose
+// ky continues the run, so that every aligned group of 4 slots has one kx and consecutive ky.
+// t: (kx index, ky index, kx, ky) per slot; map: compact mode of the slot or -1 (padding).
+void padded_mode_table(const rbc_ctx* c, std::vector<float>& t, std::vector<int>& map) {
+    const Geometry& g = c->geo;
+    const int NYc = g.NYc();
+    const double Lx = c->p.Lx, Ly = c->p.Ly;
+    t.clear();
+    map.clear();
+    auto ky_of = [&](int kyc) { const int row = g.rows[kyc]; return row <= g.N / 2 ? row : row - g.N; };
+    auto push = [&](int kx, int ky, int p) {
+        t.push_back((float)kx);
+        t.push_back((float)ky);
+        t.push_back((float)((M_PI + M_PI) * (kx / Lx)));
+        t.push_back((float)((M_PI + M_PI) * (ky / Ly)));
+        map.push_back(p);
+    };
+    for (int kx = 0; kx < g.KX; ++kx) {
+        int kyc = 0;
+        while (kyc < NYc) {
+            int len = 1;
+            while (kyc + len < NYc && ky_of(kyc + len) == ky_of(kyc) + len) ++len;
+            for (int e = 0; e < ((len + 3) & ~3); ++e)
+                push(kx, ky_of(kyc) + e, e < len ? kx * NYc + kyc + e : -1);
+            kyc += len;
+        }
+    }
+}
+
 template <typename T>
 void launch_vel_levels(rbc_ctx* c, const DevGeom& gt, int nz, const double* zb, void* spec) {
     const Buffers& b = c->b;
@@ -639,18 +821,26 @@ void velocity_grid(rbc_ctx* c, const GridWork& w) {
 // modes are the context's constants (VelWork)
 struct PointWork {
     const double* lam = nullptr;
-    const float4* modes = nullptr;
+    const float4* modes = nullptr;   // compact modes (FP64 kernel)
+    const float4* pmodes = nullptr;  // padded mode list (matrix-core kernel)
+    const int* pmap = nullptr;
+    int npad = 0;
     float* part = nullptr;
     int nslice = 1, tiles_per_slice = 1;
 };
 
 int lmax_for(int L) { return L <= 8 ? 8 : L <= 16 ? 16 : L <= 24 ? 24 : L <= 32 ? 32 : 0; }
 
+int padded_modes(rbc_ctx* c);
+
 void plan_points(rbc_ctx* c, int np, PointWork& pw) {
-    const int ntile = (c->dg.P + kTile - 1) / kTile;
-    const int nblk = (np + 255) / 256;
-    // enough blocks to give every SIMD of the chip several waves
-    int ns = (8192 + 4 * nblk - 1) / (4 * nblk);
+    const int nmodes = c->rsz == 4 ? padded_modes(c) : c->dg.P;
+    const int ntile = (nmodes + kTile - 1) / kTile;
+    // particles per block: 128 for an FP32 state (k_vel_points_mma), 256 for FP64 (k_vel_points)
+    const int per = c->rsz == 4 ? kVelMmaThreads / 4 : 256;
+    const int nblk = (np + per - 1) / per;
+    // enough blocks (2048) to give every SIMD of the chip several waves
+    int ns = (2048 + nblk - 1) / nblk;
     ns = std::max(1, std::min(std::min(16, ns), ntile));
     pw.tiles_per_slice = (ntile + ns - 1) / ns;
     pw.nslice = (ntile + pw.tiles_per_slice - 1) / pw.tiles_per_slice;
@@ -697,6 +887,9 @@ void* noalloc_scratch(int device) {
 struct VelWork {
     double* lam = nullptr;    // L
     float4* modes = nullptr;  // P
+    float4* pmodes = nullptr; // padded mode list of the matrix-core point evaluation
+    int* pmap = nullptr;
+    int npad = 0;
     double* ezb = nullptr;    // spectrum: basis values [nz][3][L], nz <= max(nq, 1)
     double* ew = nullptr;     // spectrum: level weights
     double* ee = nullptr;     // spectrum: per-mode energies [3][P]
@@ -723,11 +916,21 @@ void free_vel_work(void* p) { delete (VelWork*)p; }
 
 VelWork* vel_work(rbc_ctx* c) { return (VelWork*)c->vel_ws; }
 
-void alloc_point_work(rbc_ctx* c, int np, PointWork& pw, std::vector<void*>& owned) {
-    plan_points(c, np, pw);
+int padded_modes(rbc_ctx* c) { return vel_work(c)->npad; }
+
+// the context's constants of a point evaluation
+void point_constants(rbc_ctx* c, PointWork& pw) {
     VelWork* vw = vel_work(c);
     pw.lam = vw->lam;
     pw.modes = vw->modes;
+    pw.pmodes = vw->pmodes;
+    pw.pmap = vw->pmap;
+    pw.npad = vw->npad;
+}
+
+void alloc_point_work(rbc_ctx* c, int np, PointWork& pw, std::vector<void*>& owned) {
+    plan_points(c, np, pw);
+    point_constants(c, pw);
     void* p = nullptr;
     HIPCHK(hipMalloc(&p, std::max<size_t>(16, (size_t)pw.nslice * 3 * np * sizeof(float))));
     owned.push_back(p);
@@ -751,10 +954,18 @@ template <typename T, int LM>
 void launch_points_t(rbc_ctx* c, int np, const float* x, const float* y, const float* z, const PointWork& pw,
                      float* u, float* v, float* w) {
     const Buffers& b = c->b;
-    dim3 grid((np + 255) / 256, pw.nslice);
-    hipLaunchKernelGGL((k_vel_points<T, LM>), grid, dim3(256), 0, c->stream, c->dg, np, x, y, z,
-                       (const cpx<T>*)b.X[1], (const cpx<T>*)b.X[2], (const cpx<T>*)b.XF, (const cpx<T>*)b.XG,
-                       (const double*)pw.lam, (const float4*)pw.modes, pw.tiles_per_slice, pw.part);
+    if constexpr (sizeof(T) == 4) {  // FP32 state: the vertical sums on the matrix cores
+        dim3 grid((np + kVelMmaThreads / 4 - 1) / (kVelMmaThreads / 4), pw.nslice);
+        hipLaunchKernelGGL((k_vel_points_mma<LM>), grid, dim3(kVelMmaThreads), 0, c->stream, c->dg, np, x, y, z,
+                           (const cpx<float>*)b.X[1], (const cpx<float>*)b.X[2], (const cpx<float>*)b.XF,
+                           (const cpx<float>*)b.XG, (const double*)pw.lam, pw.pmodes, pw.pmap, pw.npad,
+                           pw.tiles_per_slice, pw.part);
+    } else {
+        dim3 grid((np + 255) / 256, pw.nslice);
+        hipLaunchKernelGGL((k_vel_points<T, LM>), grid, dim3(256), 0, c->stream, c->dg, np, x, y, z,
+                           (const cpx<T>*)b.X[1], (const cpx<T>*)b.X[2], (const cpx<T>*)b.XF, (const cpx<T>*)b.XG,
+                           (const double*)pw.lam, (const float4*)pw.modes, pw.tiles_per_slice, pw.part);
+    }
     hipLaunchKernelGGL(k_sum_slices, dim3((np + 255) / 256), dim3(256), 0, c->stream, np, pw.nslice,
                        (const float*)pw.part, u, v, w);
 }
@@ -800,6 +1011,16 @@ void velocity_setup(rbc_ctx* c) {
     const std::vector<float> mt = mode_table(c);
     HIPCHK(hipMemcpy(vw->lam, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(vw->modes, mt.data(), mt.size() * sizeof(float), hipMemcpyHostToDevice));
+    {
+        std::vector<float> pt;
+        std::vector<int> pm;
+        padded_mode_table(c, pt, pm);
+        vw->npad = (int)pm.size();
+        vw->pmodes = (float4*)al(pm.size() * sizeof(float4));
+        vw->pmap = (int*)al(pm.size() * sizeof(int));
+        HIPCHK(hipMemcpy(vw->pmodes, pt.data(), pt.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(vw->pmap, pm.data(), pm.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
     vw->ez_cap = std::max(1, c->dg.nq);
     vw->ezb = (double*)al((size_t)vw->ez_cap * 3 * L * sizeof(double));
     vw->ew = (double*)al((size_t)vw->ez_cap * sizeof(double));
@@ -1144,8 +1365,7 @@ int rbc_velocity_at(rbc_ctx* c, int n, const float* x, const float* y, const flo
         grow(vw->part, vw->part_cap, part_floats(c, n));
         PointWork pw;
         plan_points(c, n, pw);
-        pw.lam = vw->lam;
-        pw.modes = vw->modes;
+        point_constants(c, pw);
         pw.part = vw->part;
         float* d = vw->io;
         const size_t b = (size_t)n * sizeof(float);
@@ -1179,8 +1399,7 @@ int rbc_velocity_at_device_noalloc(rbc_ctx* c, int n, const float* x, const floa
                     std::to_string(n) + " points need " + std::to_string(need) +
                         " floats of partial sums; the no-allocation scratch holds " +
                         std::to_string(kNoAllocScratchFloats) + " (every n <= 1398101 fits)");
-    pw.lam = vw->lam;
-    pw.modes = vw->modes;
+    point_constants(c, pw);
     pw.part = vw->noalloc;  // every slot the sum reads is written first in the same call
     try {
         hipSetDevice(c->p.device);
@@ -1206,8 +1425,7 @@ int rbc_velocity_at_device(rbc_ctx* c, int n, const float* x, const float* y, co
         }
         PointWork pw;
         plan_points(c, n, pw);
-        pw.lam = vw->lam;
-        pw.modes = vw->modes;
+        point_constants(c, pw);
         pw.part = vw->part;
         velocity_points(c, n, x, y, z, pw, u, v, w);
     } catch (const HipError& e) {
