@@ -363,7 +363,8 @@ __global__ void __launch_bounds__(kVelMmaThreads) k_vel_points_mma(
     constexpr int KS = LMAX / 4;
     constexpr int PER = (kTile * LMAX + kVelMmaThreads - 1) / kVelMmaThreads;  // staged values per thread
     __shared__ float4 sc[kTile][LMAX + 1];  // w_k (f.re, f.im, g.re, g.im); odd row length spreads the banks
-    __shared__ float4 sm[kTile];            // (kx index, ky index, kx, ky)
+    __shared__ float4 sm[kTile];            // (kx, ky, k^2, 0)
+    __shared__ float2 sg[kTile / 4];        // (kx index, ky index) of the first mode of each group of 4
     const int L = g.L, P = g.P, NYc = g.NYc;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j = lane & 15, q = lane >> 4;
     const int i = blockIdx.x * (kVelMmaThreads / 4) + wave * 16 + j;
@@ -431,28 +432,35 @@ __global__ void __launch_bounds__(kVelMmaThreads) k_vel_points_mma(
             const int e = threadIdx.x + u * kVelMmaThreads;
             if (e < kTile * LMAX) sc[e / LMAX][e % LMAX] = pre[u];
         }
-        if (threadIdx.x < kTile) sm[threadIdx.x] = pmd;
+        if (threadIdx.x < kTile) {
+            sm[threadIdx.x] = make_float4(pmd.z, pmd.w, pmd.z * pmd.z + pmd.w * pmd.w, 0.f);
+            if ((threadIdx.x & 3) == 0) sg[threadIdx.x >> 2] = make_float2(pmd.x, pmd.y);
+        }
         lds_barrier();
         if (t + 1 < t1) fetch(t + 1);  // in flight while tile t is computed
         const int nsub = (nm + 15) / 16;
-#pragma unroll 1
-        for (int sub = 0; sub < nsub; ++sub) {
-            vpf32x4 aR = {}, aI = {}, cR = {}, cI = {}, bR = {}, bI = {};
+        // the six products of a 16-mode sub-tile on the matrix cores
+        auto products = [&](int sub, vpf32x4 (&acc)[6]) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc[k] = vpf32x4{0.f, 0.f, 0.f, 0.f};
             const float4* row = sc[sub * 16 + j];
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
                 const float4 c4 = row[4 * s + q];
-                aR = __builtin_amdgcn_mfma_f32_16x16x4f32(c4.x, bcp[s], aR, 0, 0, 0);
-                aI = __builtin_amdgcn_mfma_f32_16x16x4f32(c4.y, bcp[s], aI, 0, 0, 0);
-                cR = __builtin_amdgcn_mfma_f32_16x16x4f32(c4.x, bcc[s], cR, 0, 0, 0);
-                cI = __builtin_amdgcn_mfma_f32_16x16x4f32(c4.y, bcc[s], cI, 0, 0, 0);
-                bR = __builtin_amdgcn_mfma_f32_16x16x4f32(c4.z, bss[s], bR, 0, 0, 0);
-                bI = __builtin_amdgcn_mfma_f32_16x16x4f32(c4.w, bss[s], bI, 0, 0, 0);
+                acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(c4.x, bcp[s], acc[0], 0, 0, 0);  // A.re
+                acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(c4.y, bcp[s], acc[1], 0, 0, 0);  // A.im
+                acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(c4.x, bcc[s], acc[2], 0, 0, 0);  // Cc.re
+                acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(c4.y, bcc[s], acc[3], 0, 0, 0);  // Cc.im
+                acc[4] = __builtin_amdgcn_mfma_f32_16x16x4f32(c4.z, bss[s], acc[4], 0, 0, 0);  // B.re
+                acc[5] = __builtin_amdgcn_mfma_f32_16x16x4f32(c4.w, bss[s], acc[5], 0, 0, 0);  // B.im
             }
+        };
+        // phases and the velocity combination of a sub-tile on the VALU
+        auto combine = [&](int sub, const vpf32x4 (&acc)[6]) {
             const int b0 = sub * 16 + 4 * q;
             float s0, c0;
             {
-                const float4 m0 = sm[b0];
+                const float2 m0 = sg[b0 >> 2];
                 float ph = m0.x * xr + m0.y * yr;
                 ph -= rintf(ph);
                 sincospif(2.0f * ph, &s0, &c0);
@@ -462,16 +470,26 @@ __global__ void __launch_bounds__(kVelMmaThreads) k_vel_points_mma(
                 const float4 md = sm[b0 + r];
                 const float cs = r == 0 ? c0 : c0 * dc[r] - s0 * ds[r];
                 const float sn = r == 0 ? s0 : s0 * dc[r] + c0 * ds[r];
-                const float kx = md.z, ky = md.w;
+                const float kx = md.x, ky = md.y, k2 = md.z;
                 // u_k = i kx A + i ky B, v_k = i ky A - i kx B, w_k = k^2 Cc; with
                 // PA = Im(e A) = cs A.im + sn A.re and PB likewise, Re(e u_k) = -kx PA - ky PB and
                 // Re(e v_k) = -ky PA + kx PB (e = cs + i sn)
-                const float pa = cs * aI[r] + sn * aR[r], pb = cs * bI[r] + sn * bR[r];
-                const float k2 = kx * kx + ky * ky;
+                const float pa = cs * acc[1][r] + sn * acc[0][r], pb = cs * acc[5][r] + sn * acc[4][r];
                 u -= kx * pa + ky * pb;
                 v += kx * pb - ky * pa;
-                w += k2 * (cs * cR[r] - sn * cI[r]);
+                w += k2 * (cs * acc[2][r] - sn * acc[3][r]);
             }
+        };
+        // software pipeline: the products of sub-tile sub + 1 are issued before the combination of
+        // sub-tile sub, so that the matrix cores and the VALU work at the same time
+        vpf32x4 acc0[6], acc1[6];
+        products(0, acc0);
+#pragma unroll
+        for (int sub = 0; sub < kTile / 16; sub += 2) {
+            if (sub + 1 < nsub) products(sub + 1, acc1);
+            if (sub < nsub) combine(sub, acc0);
+            if (sub + 2 < nsub) products(sub + 2, acc0);
+            if (sub + 1 < nsub) combine(sub + 1, acc1);
         }
     }
     u += __shfl_xor(u, 16);
@@ -839,8 +857,9 @@ void plan_points(rbc_ctx* c, int np, PointWork& pw) {
     // particles per block: 128 for an FP32 state (k_vel_points_mma), 256 for FP64 (k_vel_points)
     const int per = c->rsz == 4 ? kVelMmaThreads / 4 : 256;
     const int nblk = (np + per - 1) / per;
-    // enough blocks (2048) to give every SIMD of the chip several waves
-    int ns = (2048 + nblk - 1) / nblk;
+    // enough blocks to give every SIMD of the chip several waves: 2048 blocks
+    const int want = 2048;
+    int ns = (want + nblk - 1) / nblk;
     ns = std::max(1, std::min(std::min(16, ns), ntile));
     pw.tiles_per_slice = (ntile + ns - 1) / ns;
     pw.nslice = (ntile + pw.tiles_per_slice - 1) / pw.tiles_per_slice;
