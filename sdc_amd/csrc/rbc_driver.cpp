@@ -111,7 +111,9 @@ struct TrackWriter {
     uint32_t ntracks = 0;
     std::vector<uint32_t> times;
     std::vector<float> xyz;  // per frame: x, y, z of every track, interleaved per track
-    // ≙ track_writer::append: frames [slot][particle] of one read-out of the tracer ring
+    // ≙ track_writer::append: frames [slot][particle] of one read-out of the tracer ring.  As in
+    // the reference (track_writer.cu:45-61), offset changes only when a frame is taken: a read-out
+    // of fewer slots than the offset leaves it as it was.
     void append(const float* x, const float* y, const float* z, int np, int nframes) {
         if (times.empty()) ntracks = (uint32_t)np;
         for (int t = offset; t < nframes; t += period) {
