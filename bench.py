@@ -398,8 +398,9 @@ def tracer_flops(P, L, npart):
 
 def run_tracers(a):
     """Tracer throughput at 128^2 x 16 (Ra 1e5, Pr 0.7, box 4 x 4) with 1e5 particles: the
-    spectral midpoint RK2 (two exact velocity evaluations per step) and the reference's RK4
-    with trilinear interpolation on the 128-level velocity grid."""
+    spectral midpoint RK2 (two exact velocity evaluations per step) on the FP32 state (the
+    reported value) and on a BF16X3 copy of it, and the reference's RK4 with trilinear
+    interpolation on the 128-level velocity grid."""
     import ctypes
     import sdc_amd
     from sdc_amd.initial import random_theta
@@ -415,6 +416,25 @@ def run_tracers(a):
            "config": {"workload": "1e5 passive tracers in a 128x128 grid x 16 mode state, Ra=1e5, Pr=0.7, 4x4x1",
                       "particles": npart}}
     modes = {}
+    # the BF16X3 state: the spectral evaluation's vertical sums as BF16 hi/lo split products
+    Sb = sdc_amd.Stepper(*cfg, precision="bf16x3")
+    Sb.set_state(*S.get_state())
+    Tb = sdc_amd.Tracer(Sb, npart, buffer_size=steps + a.warmup + 2, mode="spectral_rk2", seed=2)
+    Tb.init_positions("random")
+    for _ in range(a.warmup):
+        Tb.step_particles(1e-3)
+    Sb.synchronize()
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        Tb.step_particles(1e-3)
+    Sb.synchronize()
+    hip.hipDeviceSynchronize()
+    dt_b = time.perf_counter() - t0
+    Tb.close()
+    Sb.close()
+    modes["spectral_rk2_bf16x3"] = {"tracer_steps_per_s": round(steps / dt_b, 2), "ms_per_step": round(1e3 * dt_b / steps, 4),
+                                    "particle_steps_per_s": round(npart * steps / dt_b, 1)}
     for mode in ("spectral_rk2", "rk4_trilinear"):
         T = sdc_amd.Tracer(S, npart, buffer_size=steps + a.warmup + 2, mode=mode, seed=2)
         T.init_positions("random")

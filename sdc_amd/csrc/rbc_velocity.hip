@@ -353,6 +353,35 @@ __global__ void __launch_bounds__(256) k_vel_points(DevGeom g, int np, const flo
 // computed, across LDS-only barriers.
 typedef float vpf32x4 __attribute__((ext_vector_type(4)));
 constexpr int kVelMmaThreads = 512;  // 8 waves
+
+// phases and velocity combination of one 16-mode sub-tile in the MFMA output layout of
+// v_mfma_f32_16x16x*: lane l holds its particle's (column l % 16) mode rows b0 + r, r < 4, with
+// b0 = 16 sub + 4 (l / 16); acc = A.re, A.im, Cc.re, Cc.im, B.re, B.im
+__device__ __forceinline__ void vel_combine(const float2* sg, const float4* sm, int b0, float xr, float yr,
+                                            const float (&dc)[4], const float (&ds)[4], const vpf32x4 (&acc)[6],
+                                            float& u, float& v, float& w) {
+    float s0, c0;
+    {
+        const float2 m0 = sg[b0 >> 2];
+        float ph = m0.x * xr + m0.y * yr;
+        ph -= rintf(ph);
+        sincospif(2.0f * ph, &s0, &c0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const float4 md = sm[b0 + r];
+        const float cs = r == 0 ? c0 : c0 * dc[r] - s0 * ds[r];
+        const float sn = r == 0 ? s0 : s0 * dc[r] + c0 * ds[r];
+        const float kx = md.x, ky = md.y, k2 = md.z;
+        // u_k = i kx A + i ky B, v_k = i ky A - i kx B, w_k = k^2 Cc; with
+        // PA = Im(e A) = cs A.im + sn A.re and PB likewise, Re(e u_k) = -kx PA - ky PB and
+        // Re(e v_k) = -ky PA + kx PB (e = cs + i sn)
+        const float pa = cs * acc[1][r] + sn * acc[0][r], pb = cs * acc[5][r] + sn * acc[4][r];
+        u -= kx * pa + ky * pb;
+        v += kx * pb - ky * pa;
+        w += k2 * (cs * acc[2][r] - sn * acc[3][r]);
+    }
+}
 template <int LMAX>
 __global__ void __launch_bounds__(kVelMmaThreads) k_vel_points_mma(
     DevGeom g, int np, const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
@@ -455,33 +484,193 @@ __global__ void __launch_bounds__(kVelMmaThreads) k_vel_points_mma(
                 acc[5] = __builtin_amdgcn_mfma_f32_16x16x4f32(c4.w, bss[s], acc[5], 0, 0, 0);  // B.im
             }
         };
-        // phases and the velocity combination of a sub-tile on the VALU
         auto combine = [&](int sub, const vpf32x4 (&acc)[6]) {
-            const int b0 = sub * 16 + 4 * q;
-            float s0, c0;
-            {
-                const float2 m0 = sg[b0 >> 2];
-                float ph = m0.x * xr + m0.y * yr;
-                ph -= rintf(ph);
-                sincospif(2.0f * ph, &s0, &c0);
-            }
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const float4 md = sm[b0 + r];
-                const float cs = r == 0 ? c0 : c0 * dc[r] - s0 * ds[r];
-                const float sn = r == 0 ? s0 : s0 * dc[r] + c0 * ds[r];
-                const float kx = md.x, ky = md.y, k2 = md.z;
-                // u_k = i kx A + i ky B, v_k = i ky A - i kx B, w_k = k^2 Cc; with
-                // PA = Im(e A) = cs A.im + sn A.re and PB likewise, Re(e u_k) = -kx PA - ky PB and
-                // Re(e v_k) = -ky PA + kx PB (e = cs + i sn)
-                const float pa = cs * acc[1][r] + sn * acc[0][r], pb = cs * acc[5][r] + sn * acc[4][r];
-                u -= kx * pa + ky * pb;
-                v += kx * pb - ky * pa;
-                w += k2 * (cs * acc[2][r] - sn * acc[3][r]);
-            }
+            vel_combine(sg, sm, sub * 16 + 4 * q, xr, yr, dc, ds, acc, u, v, w);
         };
         // software pipeline: the products of sub-tile sub + 1 are issued before the combination of
         // sub-tile sub, so that the matrix cores and the VALU work at the same time
+        vpf32x4 acc0[6], acc1[6];
+        products(0, acc0);
+#pragma unroll
+        for (int sub = 0; sub < kTile / 16; sub += 2) {
+            if (sub + 1 < nsub) products(sub + 1, acc1);
+            if (sub < nsub) combine(sub, acc0);
+            if (sub + 2 < nsub) products(sub + 2, acc0);
+            if (sub + 1 < nsub) combine(sub + 1, acc1);
+        }
+    }
+    u += __shfl_xor(u, 16);
+    v += __shfl_xor(v, 16);
+    w += __shfl_xor(w, 16);
+    u += __shfl_xor(u, 32);
+    v += __shfl_xor(v, 32);
+    w += __shfl_xor(w, 32);
+    if (live && q == 0) {
+        float* o = part + (size_t)slice * 3 * np;
+        o[i] = u;
+        o[np + i] = v;
+        o[2 * (size_t)np + i] = w;
+    }
+}
+
+// BF16X3 form of k_vel_points_mma (contexts of precision RBC_BF16X3): the same six vertical sums
+// with every FP32 operand split into BF16 hi + lo halves and a.b ~ ah.bh + ah.bl + al.bh
+// accumulated in FP32 (v_mfma_f32_16x16x32_bf16, K = 32 in one instruction).  Lane (j, q) takes the
+// KPL = LMAX / 4 modes m = KPL q + e, e < KPL, at k = 8 q + e and zeros at the other k of its
+// slice, so that the basis values per lane stay balanced for every L.  The coefficients are split
+// once, when a tile is staged ([component][hi, lo][mode][m] BF16 in LDS); the basis values once per
+// particle.  Phases, combination, slices and prefetch as in k_vel_points_mma.
+typedef __bf16 vpbf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int vpu32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ unsigned short bf16_bits(float x) { return __builtin_bit_cast(unsigned short, (__bf16)x); }
+__device__ __forceinline__ float bf16_value(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+template <int LMAX>
+__global__ void __launch_bounds__(kVelMmaThreads) k_vel_points_bf3(
+    DevGeom g, int np, const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
+    const cpx<float>* __restrict__ Xf, const cpx<float>* __restrict__ Xg, const cpx<float>* __restrict__ XF,
+    const cpx<float>* __restrict__ XG, const double* __restrict__ lam, const float4* __restrict__ modes,
+    const int* __restrict__ pmap, int npad, int tiles_per_slice, float* __restrict__ part) {
+    static_assert(LMAX % 8 == 0 && LMAX <= 32, "KPL = LMAX / 4 even, at most 8");
+    constexpr int KPL = LMAX / 4;
+    constexpr int RS = LMAX == 24 ? 40 : LMAX + 8;  // LDS row (BF16 values): 8-byte aligned, spreads the banks
+    constexpr int NPAIR = kTile * LMAX / 2;         // (mode, pair of m) entries of a tile
+    constexpr int PER = (NPAIR + kVelMmaThreads - 1) / kVelMmaThreads;
+    __shared__ unsigned int sa[4][2][kTile][RS / 2];  // [f.re, f.im, g.re, g.im][hi, lo][mode][m pair]
+    __shared__ float4 sm[kTile];                      // (kx, ky, k^2, 0)
+    __shared__ float2 sg[kTile / 4];                  // (kx index, ky index) of the first mode of each group of 4
+    const int L = g.L, P = g.P, NYc = g.NYc;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j = lane & 15, q = lane >> 4;
+    const int i = blockIdx.x * (kVelMmaThreads / 4) + wave * 16 + j;
+    const int slice = blockIdx.y;
+    const bool live = i < np;
+    const float x = live ? px[i] : 0.f, y = live ? py[i] : 0.f, z = live ? pz[i] : 0.f;
+    // basis values of m = KPL q + e at the particle's height, split into BF16 hi and lo
+    vpbf16x8 bh[3], bl[3];  // C', C, S
+    float ssf[KPL];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int m = KPL * q + e;
+        double C = 0, Cp = 0, S = 0;
+        if (e < KPL && m < L) basis_at(m, lam[m], (double)z, C, Cp, S);
+        const float val[3] = {(float)Cp, (float)C, (float)S};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const unsigned short h = bf16_bits(val[k]);
+            bh[k][e] = __builtin_bit_cast(__bf16, h);
+            bl[k][e] = (__bf16)(val[k] - bf16_value(h));
+        }
+        if (e < KPL) ssf[e] = (float)S;
+    }
+    const float xr = x / (float)g.Lx - floorf(x / (float)g.Lx);
+    const float yr = y / (float)g.Ly - floorf(y / (float)g.Ly);
+    float ds[4], dc[4];
+    sincospif(2.0f * yr, &ds[1], &dc[1]);
+    dc[2] = dc[1] * dc[1] - ds[1] * ds[1];
+    ds[2] = 2.f * ds[1] * dc[1];
+    dc[3] = dc[2] * dc[1] - ds[2] * ds[1];
+    ds[3] = ds[2] * dc[1] + dc[2] * ds[1];
+    float u = 0.f, v = 0.f, w = 0.f;
+    if (slice == 0) {  // mean flow (k = 0), each lane group its KPL modes
+#pragma unroll
+        for (int e = 0; e < KPL; ++e) {
+            const int m = KPL * q + e;
+            if (m < L) {
+                u += ssf[e] * XF[m].x;
+                v += ssf[e] * XG[m].x;
+            }
+        }
+    }
+    const int ntile = (npad + kTile - 1) / kTile;
+    const int t0 = slice * tiles_per_slice, t1 = min(ntile, t0 + tiles_per_slice);
+    // pairs (m, m + 1) of the next tile's coefficients, times w_k, into registers
+    float4 pf[PER], pg[PER];
+    float4 pmd = make_float4(0.f, 0.f, 0.f, 0.f);
+    auto fetch = [&](int t) {
+        const int pbase = t * kTile, nm = min(kTile, npad - pbase);
+#pragma unroll
+        for (int uu = 0; uu < PER; ++uu) {
+            const int e = threadIdx.x + uu * kVelMmaThreads, jj = e / (LMAX / 2), m = 2 * (e - jj * (LMAX / 2));
+            float4 fv = make_float4(0.f, 0.f, 0.f, 0.f), gv = fv;
+            const int p = (e < NPAIR && jj < nm) ? pmap[pbase + jj] : -1;
+            if (p >= 0 && p < P && m < L) {
+                const float wk = p < NYc ? 1.f : 2.f;  // compact p < NYc <=> kx = 0
+                const cpx<float> f0 = Xf[(size_t)p * L + m], g0 = Xg[(size_t)p * L + m];
+                const cpx<float> f1 = m + 1 < L ? Xf[(size_t)p * L + m + 1] : mk(0.f, 0.f);
+                const cpx<float> g1 = m + 1 < L ? Xg[(size_t)p * L + m + 1] : mk(0.f, 0.f);
+                fv = make_float4(wk * f0.x, wk * f0.y, wk * f1.x, wk * f1.y);
+                gv = make_float4(wk * g0.x, wk * g0.y, wk * g1.x, wk * g1.y);
+            }
+            pf[uu] = fv;
+            pg[uu] = gv;
+        }
+        if (threadIdx.x < kTile)
+            pmd = (int)threadIdx.x < nm ? modes[pbase + threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
+    };
+    // hi and lo BF16 of (a, b) packed into two 32-bit words (a in the low half)
+    auto split2 = [](float a, float b, unsigned int& hi, unsigned int& lo) {
+        const unsigned short ha = bf16_bits(a), hb = bf16_bits(b);
+        const unsigned short la = bf16_bits(a - bf16_value(ha)), lb = bf16_bits(b - bf16_value(hb));
+        hi = (unsigned int)ha | ((unsigned int)hb << 16);
+        lo = (unsigned int)la | ((unsigned int)lb << 16);
+    };
+    if (t0 < t1) fetch(t0);
+    for (int t = t0; t < t1; ++t) {
+        const int nm = min(kTile, npad - t * kTile);
+        lds_barrier();  // the previous tile is consumed
+#pragma unroll
+        for (int uu = 0; uu < PER; ++uu) {
+            const int e = threadIdx.x + uu * kVelMmaThreads;
+            if (e < NPAIR) {
+                const int jj = e / (LMAX / 2), mp = e - jj * (LMAX / 2);
+                unsigned int hi, lo;
+                split2(pf[uu].x, pf[uu].z, hi, lo);  // f.re at m, m + 1
+                sa[0][0][jj][mp] = hi;
+                sa[0][1][jj][mp] = lo;
+                split2(pf[uu].y, pf[uu].w, hi, lo);  // f.im
+                sa[1][0][jj][mp] = hi;
+                sa[1][1][jj][mp] = lo;
+                split2(pg[uu].x, pg[uu].z, hi, lo);  // g.re
+                sa[2][0][jj][mp] = hi;
+                sa[2][1][jj][mp] = lo;
+                split2(pg[uu].y, pg[uu].w, hi, lo);  // g.im
+                sa[3][0][jj][mp] = hi;
+                sa[3][1][jj][mp] = lo;
+            }
+        }
+        if (threadIdx.x < kTile) {
+            sm[threadIdx.x] = make_float4(pmd.z, pmd.w, pmd.z * pmd.z + pmd.w * pmd.w, 0.f);
+            if ((threadIdx.x & 3) == 0) sg[threadIdx.x >> 2] = make_float2(pmd.x, pmd.y);
+        }
+        lds_barrier();
+        if (t + 1 < t1) fetch(t + 1);  // in flight while tile t is computed
+        const int nsub = (nm + 15) / 16;
+        auto products = [&](int sub, vpf32x4 (&acc)[6]) {
+            const int row = sub * 16 + j;
+            vpbf16x8 ah[4], al[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                vpu32x4 wh = {0u, 0u, 0u, 0u}, wl = {0u, 0u, 0u, 0u};
+#pragma unroll
+                for (int e2 = 0; e2 < KPL / 2; ++e2) {
+                    wh[e2] = sa[c][0][row][(KPL * q) / 2 + e2];
+                    wl[e2] = sa[c][1][row][(KPL * q) / 2 + e2];
+                }
+                ah[c] = __builtin_bit_cast(vpbf16x8, wh);
+                al[c] = __builtin_bit_cast(vpbf16x8, wl);
+            }
+            static constexpr int comp[6] = {0, 1, 0, 1, 2, 3}, kind[6] = {0, 0, 1, 1, 2, 2};
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                vpf32x4 a4 = {0.f, 0.f, 0.f, 0.f};
+                a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[comp[k]], bh[kind[k]], a4, 0, 0, 0);
+                a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[comp[k]], bl[kind[k]], a4, 0, 0, 0);
+                a4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[comp[k]], bh[kind[k]], a4, 0, 0, 0);
+                acc[k] = a4;
+            }
+        };
+        auto combine = [&](int sub, const vpf32x4 (&acc)[6]) {
+            vel_combine(sg, sm, sub * 16 + 4 * q, xr, yr, dc, ds, acc, u, v, w);
+        };
         vpf32x4 acc0[6], acc1[6];
         products(0, acc0);
 #pragma unroll
@@ -975,10 +1164,16 @@ void launch_points_t(rbc_ctx* c, int np, const float* x, const float* y, const f
     const Buffers& b = c->b;
     if constexpr (sizeof(T) == 4) {  // FP32 state: the vertical sums on the matrix cores
         dim3 grid((np + kVelMmaThreads / 4 - 1) / (kVelMmaThreads / 4), pw.nslice);
-        hipLaunchKernelGGL((k_vel_points_mma<LM>), grid, dim3(kVelMmaThreads), 0, c->stream, c->dg, np, x, y, z,
-                           (const cpx<float>*)b.X[1], (const cpx<float>*)b.X[2], (const cpx<float>*)b.XF,
-                           (const cpx<float>*)b.XG, (const double*)pw.lam, pw.pmodes, pw.pmap, pw.npad,
-                           pw.tiles_per_slice, pw.part);
+        if (c->p.precision == RBC_BF16X3)  // BF16 hi/lo split products, as the step's vertical transforms
+            hipLaunchKernelGGL((k_vel_points_bf3<LM>), grid, dim3(kVelMmaThreads), 0, c->stream, c->dg, np, x, y, z,
+                               (const cpx<float>*)b.X[1], (const cpx<float>*)b.X[2], (const cpx<float>*)b.XF,
+                               (const cpx<float>*)b.XG, (const double*)pw.lam, pw.pmodes, pw.pmap, pw.npad,
+                               pw.tiles_per_slice, pw.part);
+        else
+            hipLaunchKernelGGL((k_vel_points_mma<LM>), grid, dim3(kVelMmaThreads), 0, c->stream, c->dg, np, x, y, z,
+                               (const cpx<float>*)b.X[1], (const cpx<float>*)b.X[2], (const cpx<float>*)b.XF,
+                               (const cpx<float>*)b.XG, (const double*)pw.lam, pw.pmodes, pw.pmap, pw.npad,
+                               pw.tiles_per_slice, pw.part);
     } else {
         dim3 grid((np + 255) / 256, pw.nslice);
         hipLaunchKernelGGL((k_vel_points<T, LM>), grid, dim3(256), 0, c->stream, c->dg, np, x, y, z,

@@ -1,9 +1,10 @@
 """Tracers at the 128^2 x 16 configuration and at L = 24, 32, tracer diffusion and the lid
 reflection on the GPU (through the C-ABI) against oracle/velocity.py.
 
-- Spectral RK2 at 128^2 x 16 with 1e5 tracers (the k_vel_points instantiation for L <= 16 at
-  the size the configuration names); particles move independently, so a seeded subset of
-  2,000 is followed by the FP64 oracle.  L = 24 and 32 run the other two instantiations.
+- Spectral RK2 at 128^2 x 16 with 1e5 tracers (the point-evaluation instantiation for L <= 16
+  at the size the configuration names), FP32 and BF16X3 states; particles move independently,
+  so a seeded subset of 2,000 is followed by the FP64 oracle.  L = 24 and 32 run the other two
+  instantiations.
 - Diffusion: in a zero flow a step adds sigma * N(0, 1) per axis with sigma = sqrt(2 D dt)
   (particle_tracer.cu:196-200, 692-717); after n steps the mean-square displacement per axis
   is 2 D dt n.  Checked on 1e5 tracers (relative standard error of a sample variance:
@@ -52,13 +53,16 @@ def _build():
                                    (dict(M=64, N=64, L=24, Lx=4.0, Ly=3.0, Ra=1e5, Pr=0.7, dt=2e-4), 4000),
                                    (dict(M=64, N=64, L=32, Lx=3.0, Ly=4.0, Ra=1e6, Pr=0.7, dt=5e-5), 4000)],
                          ids=["128x128x16-1e5", "64x64x24", "64x64x32"])
-def test_spectral_rk2_tracers_vs_oracle(cfg, n):
+@pytest.mark.parametrize("precision", ["f32", "bf16x3"])
+def test_spectral_rk2_tracers_vs_oracle(cfg, n, precision):
+    """FP32 states run the vertical sums on the FP32 matrix cores, BF16X3 states as BF16 hi/lo
+    split products (relative error about 2^-16 per product); both against the FP64 oracle."""
     M, N, L = cfg["M"], cfg["N"], cfg["L"]
-    print(f"case: spectral rk2 {n} tracers {M}x{N}x{L}", flush=True)
+    print(f"case: spectral rk2 {n} tracers {M}x{N}x{L} {precision}", flush=True)
     from sdc_amd import Tracer
     state = _gentle(cfg, seed=L + 1)
     steps, dt = 3, 2e-3
-    with _gpu(cfg) as G:
+    with _gpu(cfg, precision) as G:
         G.set_state(*state)
         T = Tracer(G, n, buffer_size=8, mode="spectral_rk2", seed=6)
         T.init_positions("random")
