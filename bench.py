@@ -516,10 +516,14 @@ def run_validate(a):
     T = sdc_amd.Stepper(128, 128, 16, 4.0, 4.0, 1e5, 0.7, 2e-4, precision="f64")
     T.set_state(*st)
     blowup = None
+    from sdc_amd import RbcNumericError
     for k in range(20):
         T.step_time(100)
-        if not np.all(np.isfinite(T.nusselt())):
+        try:
+            T.synchronize()
+        except RbcNumericError as e:  # the end-of-step health word names the first non-finite step
             blowup = 100 * (k + 1)
+            out["C2_stated_dt_health"] = str(e)
             break
     T.close()
     c2 = (128, 128, 16, 4.0, 4.0, 1e5, 0.7, 2.5e-5)

@@ -860,51 +860,70 @@ __global__ void __launch_bounds__(256) k_finish_batch(const k_finish_args<T>* __
 // follows from the row itself.
 // ---------------------------------------------------------------------------------------
 
-// reference layout -> compact [p][m] (kept modes only); one thread per (p, m), three fields
+// Reference layout [m][row][kx] (kx fastest, M/2 + 1 columns) <-> compact [p = kx NYc + kyc][m],
+// three fields, as 32 x 32 tiles of (kx, m) of one y row staged in LDS: the reference side is read
+// or written in runs of 32 kx and the compact side in runs of 32 m, so both sides are coalesced.
+// blockIdx = (kx tile, kept row kyc or reference row, field * m tiles + m tile).
+__device__ __forceinline__ int kept_row_of(const DevGeom& g, int kyc) {
+    const int a = (g.NYc - 1) / 2;
+    return kyc <= a ? kyc : g.N - (g.NYc - kyc);
+}
+__device__ __forceinline__ int kept_index_of(const DevGeom& g, int row) {  // -1: removed row
+    const int a = (g.NYc - 1) / 2;
+    return row <= a ? row : (row >= g.N - a ? g.NYc - (g.N - row) : -1);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_ref_to_compact(DevGeom g, const cpx<T>* __restrict__ s0,
                                                         const cpx<T>* __restrict__ s1, const cpx<T>* __restrict__ s2,
                                                         cpx<T>* __restrict__ d0, cpx<T>* __restrict__ d1,
                                                         cpx<T>* __restrict__ d2) {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t n = (size_t)g.P * g.L;
-    if (e >= n) return;
-    const int fld = blockIdx.y;
+    __shared__ cpx<T> tile[32][33];  // [m][kx]
+    const int mtn = (g.L + 31) / 32, fld = blockIdx.z / mtn, m0 = (blockIdx.z - fld * mtn) * 32;
     const cpx<T>* src = fld == 0 ? s0 : (fld == 1 ? s1 : s2);
     cpx<T>* dst = fld == 0 ? d0 : (fld == 1 ? d1 : d2);
-    if (!src) return;
-    const int m = (int)(e % g.L);
-    const int p = (int)(e / g.L);
-    const int kx = p / g.NYc, kyc = p - kx * g.NYc;
-    const int a = (g.NYc - 1) / 2;
-    const int row = kyc <= a ? kyc : g.N - (g.NYc - kyc);
-    const int mox = g.M / 2 + 1;
-    dst[e] = src[((size_t)m * g.N + row) * mox + kx];
+    if (!src) return;  // the whole block
+    const int kyc = blockIdx.y, row = kept_row_of(g, kyc), kx0 = blockIdx.x * 32, mox = g.M / 2 + 1;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int m = m0 + ty + 8 * i, kx = kx0 + tx;
+        if (m < g.L && kx < g.KX) tile[ty + 8 * i][tx] = src[((size_t)m * g.N + row) * mox + kx];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int kx = kx0 + ty + 8 * i, m = m0 + tx;
+        if (m < g.L && kx < g.KX) dst[((size_t)kx * g.NYc + kyc) * g.L + m] = tile[tx][ty + 8 * i];
+    }
 }
 
-// compact [p][m] -> reference layout, zeros in the removed band; one thread per reference
-// element, three fields
+// compact [p][m] -> reference layout, zeros in the removed band (columns kx >= KX and removed rows)
 template <typename T>
 __global__ void __launch_bounds__(256) k_compact_to_ref(DevGeom g, const cpx<T>* __restrict__ s0,
                                                         const cpx<T>* __restrict__ s1, const cpx<T>* __restrict__ s2,
                                                         cpx<T>* __restrict__ d0, cpx<T>* __restrict__ d1,
                                                         cpx<T>* __restrict__ d2) {
-    const int mox = g.M / 2 + 1;
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t n = (size_t)mox * g.N * g.L;
-    if (e >= n) return;
-    const int fld = blockIdx.y;
+    __shared__ cpx<T> tile[32][33];  // [kx][m]
+    const int mtn = (g.L + 31) / 32, fld = blockIdx.z / mtn, m0 = (blockIdx.z - fld * mtn) * 32;
     const cpx<T>* src = fld == 0 ? s0 : (fld == 1 ? s1 : s2);
     cpx<T>* dst = fld == 0 ? d0 : (fld == 1 ? d1 : d2);
-    if (!dst) return;
-    const int kx = (int)(e % mox);
-    const size_t r = e / mox;
-    const int row = (int)(r % g.N), m = (int)(r / g.N);
-    const int a = (g.NYc - 1) / 2;
-    const int kyc = row <= a ? row : (row >= g.N - a ? g.NYc - (g.N - row) : -1);
-    cpx<T> v = mk(T(0), T(0));
-    if (kx < g.KX && kyc >= 0) v = src[((size_t)kx * g.NYc + kyc) * g.L + m];
-    dst[e] = v;
+    if (!dst) return;  // the whole block
+    const int row = blockIdx.y, kyc = kept_index_of(g, row), kx0 = blockIdx.x * 32, mox = g.M / 2 + 1;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int kx = kx0 + ty + 8 * i, m = m0 + tx;
+        cpx<T> v = mk(T(0), T(0));
+        if (kyc >= 0 && kx < g.KX && m < g.L) v = src[((size_t)kx * g.NYc + kyc) * g.L + m];
+        tile[ty + 8 * i][tx] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int m = m0 + ty + 8 * i, kx = kx0 + tx;
+        if (m < g.L && kx < mox) dst[((size_t)m * g.N + row) * mox + kx] = tile[tx][ty + 8 * i];
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -2357,6 +2376,8 @@ void Launch<T>::eval(const DevGeom& g, const Buffers& b, void* const X[3], const
         const int K = b.mma == 0 ? 2 : 16;
         const size_t sx = (size_t)3 * 128 * (b.ev_nks * K + 1) * sizeof(float);
         const size_t sa = (size_t)4 * b.ev_nks * 64 * (K / 2) * sizeof(float);
+        // the A operands go to LDS with the modes only while the block stays within 64 KiB: at
+        // 66.7 KiB (512^2 x 32; 2 blocks per CU instead of 3) the evaluation took 103 us against 97
         const bool alds = sx + sa <= 65536;
         const size_t smm = alds ? sx + sa : sx;
         const int nblk = ((g.P + 63) / 64) * b.ev_nlt, per = (nblk + 7) / 8;
@@ -2591,16 +2612,16 @@ void Launch<T>::nusselt(const DevGeom& g, const Buffers& b, const void* ia1, hip
 
 template <typename T>
 void Launch<T>::from_ref(const DevGeom& g, const void* const src[3], void* const dst[3], hipStream_t s) {
-    const size_t n = (size_t)g.P * g.L;
-    hipLaunchKernelGGL(k_ref_to_compact<T>, dim3((unsigned)((n + 255) / 256), 3), dim3(256), 0, s, g,
+    const int mtn = (g.L + 31) / 32;
+    hipLaunchKernelGGL(k_ref_to_compact<T>, dim3((g.KX + 31) / 32, g.NYc, 3 * mtn), dim3(256), 0, s, g,
                        (const cpx<T>*)src[0], (const cpx<T>*)src[1], (const cpx<T>*)src[2], (cpx<T>*)dst[0],
                        (cpx<T>*)dst[1], (cpx<T>*)dst[2]);
 }
 
 template <typename T>
 void Launch<T>::to_ref(const DevGeom& g, const void* const src[3], void* const dst[3], hipStream_t s) {
-    const size_t n = (size_t)(g.M / 2 + 1) * g.N * g.L;
-    hipLaunchKernelGGL(k_compact_to_ref<T>, dim3((unsigned)((n + 255) / 256), 3), dim3(256), 0, s, g,
+    const int mtn = (g.L + 31) / 32;
+    hipLaunchKernelGGL(k_compact_to_ref<T>, dim3((g.M / 2 + 1 + 31) / 32, g.N, 3 * mtn), dim3(256), 0, s, g,
                        (const cpx<T>*)src[0], (const cpx<T>*)src[1], (const cpx<T>*)src[2], (cpx<T>*)dst[0],
                        (cpx<T>*)dst[1], (cpx<T>*)dst[2]);
 }
