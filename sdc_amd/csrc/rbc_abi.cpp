@@ -33,60 +33,26 @@ void upload_reals(rbc_ctx* c, void* dst, const std::vector<double>& src) {
     else upload_real<float>(c, dst, src);
 }
 
-// reference layout (mox x N x L complex doubles) -> compact [p][m] in device precision
-template <typename T>
-void pack_field(const rbc_ctx* c, const double* src, T* dst) {
-    const Geometry& g = c->geo;
-    const int L = g.L, NYc = g.NYc(), mox = g.mox;
-    for (int kx = 0; kx < g.KX; ++kx)
-        for (int kc = 0; kc < NYc; ++kc) {
-            const size_t p = (size_t)kx * NYc + kc;
-            const int row = g.rows[kc];
-            for (int m = 0; m < L; ++m) {
-                const size_t s = ((size_t)m * g.N + row) * mox + kx;
-                dst[2 * (p * L + m)] = (T)src[2 * s];
-                dst[2 * (p * L + m) + 1] = (T)src[2 * s + 1];
-            }
-        }
-}
-
-template <typename T>
-void unpack_field(const rbc_ctx* c, const T* src, double* dst) {
-    const Geometry& g = c->geo;
-    const int L = g.L, NYc = g.NYc(), mox = g.mox;
-    std::memset(dst, 0, sizeof(double) * 2 * (size_t)mox * g.N * L);
-    for (int kx = 0; kx < g.KX; ++kx)
-        for (int kc = 0; kc < NYc; ++kc) {
-            const size_t p = (size_t)kx * NYc + kc;
-            const int row = g.rows[kc];
-            for (int m = 0; m < L; ++m) {
-                const size_t s = ((size_t)m * g.N + row) * mox + kx;
-                dst[2 * s] = (double)src[2 * (p * L + m)];
-                dst[2 * s + 1] = (double)src[2 * (p * L + m) + 1];
-            }
-        }
-}
-
 }  // namespace
 
 namespace rbc {
 
+// host fields in the reference layout: copied as they are into the context's FP64 staging
+// buffer and converted to (from) the compact layout on the GPU
 void put_field(rbc_ctx* c, const double* host, void* dev) {
-    const size_t n = (size_t)c->geo.P() * c->geo.L * c->csz();
-    c->stage.resize(n);
-    if (c->rsz == 8) pack_field<double>(c, host, (double*)c->stage.data());
-    else pack_field<float>(c, host, (float*)c->stage.data());
-    HIPCHK(hipMemcpyAsync(dev, c->stage.data(), n, hipMemcpyHostToDevice, c->stream));
+    const size_t n = (size_t)c->geo.mox * c->geo.N * c->geo.L * 16;
+    HIPCHK(hipMemcpyAsync(c->b.refstage, host, n, hipMemcpyHostToDevice, c->stream));
+    field_from_ref64(c->dg, c->rsz, c->b.refstage, dev, c->stream);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
 }
 
 void get_field(rbc_ctx* c, const void* dev, double* host) {
-    const size_t n = (size_t)c->geo.P() * c->geo.L * c->csz();
-    c->stage.resize(n);
-    HIPCHK(hipMemcpyAsync(c->stage.data(), dev, n, hipMemcpyDeviceToHost, c->stream));
+    const size_t n = (size_t)c->geo.mox * c->geo.N * c->geo.L * 16;
+    field_to_ref64(c->dg, c->rsz, dev, c->b.refstage, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(host, c->b.refstage, n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (c->rsz == 8) unpack_field<double>(c, (const double*)c->stage.data(), host);
-    else unpack_field<float>(c, (const float*)c->stage.data(), host);
 }
 
 void put_vec(rbc_ctx* c, const double* host, void* dev) {
@@ -300,6 +266,7 @@ int rbc_create(const rbc_params* p, rbc_ctx** out) {
         b.hout = c->dalloc((size_t)7 * nq * d.N * d.KXp * cs);
         b.prod = c->dalloc((size_t)7 * nq * P * cs);
         b.fgw = c->dalloc((size_t)2 * nq * cs);
+        b.refstage = c->dalloc((size_t)(d.M / 2 + 1) * d.N * L * 16);
         // zero-initialise state and padding once (not in any step: nothing is memset later)
         for (int i = 0; i < 3; ++i) {
             HIPCHK(hipMemset(b.X[i], 0, P * L * cs));

@@ -41,7 +41,6 @@ struct rbc_ctx {
     bool have_hist = false;
     int bad_step = 0;  // host copy of the first non-finite step (health[0]) once read; 0 = none
     std::string err;
-    std::vector<unsigned char> stage;  // host staging for one compact field
     static constexpr int kRing = 64;                 // steps of stage events in flight
     hipEvent_t ev_ring[kRing][RBC_NSTAGES + 1]{};
     hipEvent_t* ev = ev_ring[0];                     // events of the step being recorded

@@ -98,6 +98,7 @@ struct Buffers {
     // the projection's five level combinations live in `lev` (free once the inverse y-FFT has
     // consumed it): [5][l][p] complex; fgw[l] = (P_uw, P_vw) at k = 0 for the mean flow
     void* fgw;    // nq * 2 complex
+    void* refstage;  // (M/2+1) * N * L complex FP64: one field of the host boundary in the reference layout
     // tables
     void* evtab;  // nq * 4 * L : C', C, S, S' at level l   [l][t][m]
     void* prtab;  // nq * 5 * L : w*C', w*C'', w*C, w*S, w*S'  [l][t][m]
@@ -162,6 +163,11 @@ enum UpdateMode { UPD_STEP = 0, UPD_INVB = 1, UPD_IMPLICIT = 2 };
 
 // plane operators on the state's mode planes (freeze_operator.cu, masktemp.cu, masking.cu)
 enum PlaneOp { PLANE_CAPTURE = 0, PLANE_FREEZE = 1, PLANE_HEAT = 2, PLANE_SCALE = 3 };
+
+// one field between an FP64 reference-layout device buffer and the compact field of the
+// context's real type (rsz = 4 or 8); enqueued on s
+void field_from_ref64(const DevGeom& g, int rsz, const void* ref, void* dst, hipStream_t s);
+void field_to_ref64(const DevGeom& g, int rsz, const void* src, void* ref, hipStream_t s);
 
 template <typename T>
 struct Launch {

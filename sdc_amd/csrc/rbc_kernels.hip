@@ -873,15 +873,15 @@ __device__ __forceinline__ int kept_index_of(const DevGeom& g, int row) {  // -1
     return row <= a ? row : (row >= g.N - a ? g.NYc - (g.N - row) : -1);
 }
 
-template <typename T>
+template <typename T, typename TD = T>
 __global__ void __launch_bounds__(256) k_ref_to_compact(DevGeom g, const cpx<T>* __restrict__ s0,
                                                         const cpx<T>* __restrict__ s1, const cpx<T>* __restrict__ s2,
-                                                        cpx<T>* __restrict__ d0, cpx<T>* __restrict__ d1,
-                                                        cpx<T>* __restrict__ d2) {
+                                                        cpx<TD>* __restrict__ d0, cpx<TD>* __restrict__ d1,
+                                                        cpx<TD>* __restrict__ d2) {
     __shared__ cpx<T> tile[32][33];  // [m][kx]
     const int mtn = (g.L + 31) / 32, fld = blockIdx.z / mtn, m0 = (blockIdx.z - fld * mtn) * 32;
     const cpx<T>* src = fld == 0 ? s0 : (fld == 1 ? s1 : s2);
-    cpx<T>* dst = fld == 0 ? d0 : (fld == 1 ? d1 : d2);
+    cpx<TD>* dst = fld == 0 ? d0 : (fld == 1 ? d1 : d2);
     if (!src) return;  // the whole block
     const int kyc = blockIdx.y, row = kept_row_of(g, kyc), kx0 = blockIdx.x * 32, mox = g.M / 2 + 1;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
@@ -894,28 +894,34 @@ __global__ void __launch_bounds__(256) k_ref_to_compact(DevGeom g, const cpx<T>*
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int kx = kx0 + ty + 8 * i, m = m0 + tx;
-        if (m < g.L && kx < g.KX) dst[((size_t)kx * g.NYc + kyc) * g.L + m] = tile[tx][ty + 8 * i];
+        if (m < g.L && kx < g.KX) {
+            const cpx<T> v = tile[tx][ty + 8 * i];
+            dst[((size_t)kx * g.NYc + kyc) * g.L + m] = mk((TD)v.x, (TD)v.y);
+        }
     }
 }
 
 // compact [p][m] -> reference layout, zeros in the removed band (columns kx >= KX and removed rows)
-template <typename T>
+template <typename T, typename TD = T>
 __global__ void __launch_bounds__(256) k_compact_to_ref(DevGeom g, const cpx<T>* __restrict__ s0,
                                                         const cpx<T>* __restrict__ s1, const cpx<T>* __restrict__ s2,
-                                                        cpx<T>* __restrict__ d0, cpx<T>* __restrict__ d1,
-                                                        cpx<T>* __restrict__ d2) {
-    __shared__ cpx<T> tile[32][33];  // [kx][m]
+                                                        cpx<TD>* __restrict__ d0, cpx<TD>* __restrict__ d1,
+                                                        cpx<TD>* __restrict__ d2) {
+    __shared__ cpx<TD> tile[32][33];  // [kx][m]
     const int mtn = (g.L + 31) / 32, fld = blockIdx.z / mtn, m0 = (blockIdx.z - fld * mtn) * 32;
     const cpx<T>* src = fld == 0 ? s0 : (fld == 1 ? s1 : s2);
-    cpx<T>* dst = fld == 0 ? d0 : (fld == 1 ? d1 : d2);
+    cpx<TD>* dst = fld == 0 ? d0 : (fld == 1 ? d1 : d2);
     if (!dst) return;  // the whole block
     const int row = blockIdx.y, kyc = kept_index_of(g, row), kx0 = blockIdx.x * 32, mox = g.M / 2 + 1;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int kx = kx0 + ty + 8 * i, m = m0 + tx;
-        cpx<T> v = mk(T(0), T(0));
-        if (kyc >= 0 && kx < g.KX && m < g.L) v = src[((size_t)kx * g.NYc + kyc) * g.L + m];
+        cpx<TD> v = mk(TD(0), TD(0));
+        if (kyc >= 0 && kx < g.KX && m < g.L) {
+            const cpx<T> a = src[((size_t)kx * g.NYc + kyc) * g.L + m];
+            v = mk((TD)a.x, (TD)a.y);
+        }
         tile[ty + 8 * i][tx] = v;
     }
     __syncthreads();
@@ -2624,6 +2630,29 @@ void Launch<T>::to_ref(const DevGeom& g, const void* const src[3], void* const d
     hipLaunchKernelGGL(k_compact_to_ref<T>, dim3((g.M / 2 + 1 + 31) / 32, g.N, 3 * mtn), dim3(256), 0, s, g,
                        (const cpx<T>*)src[0], (const cpx<T>*)src[1], (const cpx<T>*)src[2], (cpx<T>*)dst[0],
                        (cpx<T>*)dst[1], (cpx<T>*)dst[2]);
+}
+
+// one field between an FP64 reference-layout buffer and the compact field of the context's real
+// type (the host-buffer boundary: the host arrays are copied as they are and converted here)
+void field_from_ref64(const DevGeom& g, int rsz, const void* ref, void* dst, hipStream_t s) {
+    const int mtn = (g.L + 31) / 32;
+    const dim3 grid((g.KX + 31) / 32, g.NYc, mtn);
+    if (rsz == 8)
+        hipLaunchKernelGGL((k_ref_to_compact<double, double>), grid, dim3(256), 0, s, g, (const cpx<double>*)ref,
+                           nullptr, nullptr, (cpx<double>*)dst, nullptr, nullptr);
+    else
+        hipLaunchKernelGGL((k_ref_to_compact<double, float>), grid, dim3(256), 0, s, g, (const cpx<double>*)ref,
+                           nullptr, nullptr, (cpx<float>*)dst, nullptr, nullptr);
+}
+void field_to_ref64(const DevGeom& g, int rsz, const void* src, void* ref, hipStream_t s) {
+    const int mtn = (g.L + 31) / 32;
+    const dim3 grid((g.M / 2 + 1 + 31) / 32, g.N, mtn);
+    if (rsz == 8)
+        hipLaunchKernelGGL((k_compact_to_ref<double, double>), grid, dim3(256), 0, s, g, (const cpx<double>*)src,
+                           nullptr, nullptr, (cpx<double>*)ref, nullptr, nullptr);
+    else
+        hipLaunchKernelGGL((k_compact_to_ref<float, double>), grid, dim3(256), 0, s, g, (const cpx<float>*)src,
+                           nullptr, nullptr, (cpx<double>*)ref, nullptr, nullptr);
 }
 
 template <typename T>
