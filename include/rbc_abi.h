@@ -144,6 +144,21 @@ void* rbc_stream(rbc_ctx* c);
 #define RBC_INIT_RANDOM 1
 #define RBC_INIT_TEST 2
 int rbc_init_state(rbc_ctx* c, int kind, unsigned long long seed, double amplitude);
+/* ≙ the pattern starts of main.cu:326-352 (-init_rectangle, -init_dislocation, -init_isr, -init_sv,
+ * -init_cr, -init_giantspiral; util/init.cu:22-261): theta starts random (RBC_INIT_RANDOM,
+ * amplitude 1e-6, all levels) and its vertical mode 0 is replaced by the unnormalised 2-D R2C of a
+ * real pattern of amplitude 1e-6 on the M x N grid, then make_real.  init_mode ≙ -init_mode (the
+ * reference's default is Lx / 2), truncated to int as the init_* parameters are; the rectangle
+ * takes its x mode from Lx and M as main.cu:330 does.  The patterns are evaluated in FP64; the
+ * rectangle's noise columns and the isr phase come from the project RNG (the reference uses C
+ * rand()).  A setup call (allocates one M x N plane for its duration; waits for the stream). */
+#define RBC_PATTERN_RECTANGLE 1
+#define RBC_PATTERN_DISLOCATION 2
+#define RBC_PATTERN_ISR 3
+#define RBC_PATTERN_SV 4
+#define RBC_PATTERN_CR 5
+#define RBC_PATTERN_GIANTSPIRAL 6
+int rbc_init_pattern(rbc_ctx* c, int kind, double init_mode, unsigned long long seed);
 /* ---- Penalization and plane operators ------------------------------------------------------
  * rbc_set_penalization ≙ the PENALIZED builds (nonlinear_operator_rayleigh_noslip.cu:
  * 1097-1343): Brinkman terms of a mask chi(x, y) added to the nonlinear term with penalization

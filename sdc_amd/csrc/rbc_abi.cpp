@@ -749,12 +749,13 @@ int* upload_mask(rbc_ctx* c, int*& dst, const std::vector<int>& m) {
     return dst;
 }
 
-void run_plane_op(rbc_ctx* c, int op, int nf, const int* imask, double factor) {
+void run_plane_op(rbc_ctx* c, int op, int nf, const int* imask, double factor, void* planes = nullptr) {
     Buffers& b = c->b;
+    void* fr = planes ? planes : c->frozen;
     if (c->rsz == 8)
-        Launch<double>::plane_op(c->dg, b, op, nf, imask, factor, c->frozen, b.X, b.lev, b.hin, b.hout, b.prod, c->stream);
+        Launch<double>::plane_op(c->dg, b, op, nf, imask, factor, fr, b.X, b.lev, b.hin, b.hout, b.prod, c->stream);
     else
-        Launch<float>::plane_op(c->dg, b, op, nf, imask, factor, c->frozen, b.X, b.lev, b.hin, b.hout, b.prod, c->stream);
+        Launch<float>::plane_op(c->dg, b, op, nf, imask, factor, fr, b.X, b.lev, b.hin, b.hout, b.prod, c->stream);
     HIPCHK(hipGetLastError());
 }
 
@@ -763,6 +764,35 @@ bool plane_buffers_fit(const rbc_ctx* c, int nf) { return nf * c->dg.L <= 6 * c-
 }  // namespace
 
 extern "C" {
+
+int rbc_init_pattern(rbc_ctx* c, int kind, double init_mode, unsigned long long seed) {
+    if (!c || kind < RBC_PATTERN_RECTANGLE || kind > RBC_PATTERN_GIANTSPIRAL || !std::isfinite(init_mode))
+        return RBC_ERR_ARG;
+    if (!plane_buffers_fit(c, 1)) return fail(c, RBC_ERR_ARG, "initial patterns need nq >= L / 6");
+    int rc = rbc_init_state(c, RBC_INIT_RANDOM, seed, 1e-6);  // the random start of every level (main.cu:300)
+    if (rc != RBC_OK) return rc;
+    void* plane = nullptr;
+    try {
+        hipSetDevice(c->p.device);
+        const DevGeom& g = c->dg;
+        // the mode arguments are int parameters of the init_* functions (main.cu:329-345)
+        const int mode = kind == RBC_PATTERN_RECTANGLE ? (int)((c->p.Lx / (g.M - 1.0)) * (g.M - 5.0) / 2.0)
+                                                       : (int)init_mode;
+        HIPCHK(hipMalloc(&plane, (size_t)g.M * g.N * c->rsz));
+        if (c->rsz == 8) Launch<double>::init_pattern(g, kind, mode, 1e-6, seed, plane, c->stream);
+        else Launch<float>::init_pattern(g, kind, mode, 1e-6, seed, plane, c->stream);
+        run_plane_op(c, PLANE_SET, 1, nullptr, 1.0, plane);
+        if (c->rsz == 8) Launch<double>::make_real(g, c->b, c->b.X, c->stream);
+        else Launch<float>::make_real(g, c->b, c->b.X, c->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(plane));
+    } catch (const HipError& e) {
+        if (plane) (void)hipFree(plane);
+        return hip_fail(c, e);
+    }
+    return RBC_OK;
+}
 
 int rbc_freeze(rbc_ctx* c, int shape, double param) {
     if (!c || (shape != RBC_PLANE_CIRCLE && shape != RBC_PLANE_RECTANGLE) || !std::isfinite(param))

@@ -19,9 +19,9 @@
 //     restores one, so a restart continues with AB2 instead of an Euler step;
 //   - -tracer_mode spectral selects the spectral RK2 tracer (default: the reference's RK4).
 // -freeze_circle n and -temp_circle n run the freeze and heat-source operators before every step
-// from step n on (main.cu:569-590).  Not available in this build (they exit with an error naming
-// the option): the initial patterns other than random and test (-init_rectangle,
-// -init_dislocation, -init_isr, -init_sv, -init_cr, -init_giantspiral).  -port (USE_SERVER)
+// from step n on (main.cu:569-590).  The pattern starts (-init_rectangle, -init_dislocation,
+// -init_isr, -init_sv, -init_cr, -init_giantspiral with -init_mode) go through rbc_init_pattern.
+// -port (USE_SERVER)
 // and -snapshot_freq (WRITE_SNAPSHOT) are accepted and ignored, as in the reference's default
 // build.
 #include <hip/hip_runtime.h>
@@ -255,11 +255,20 @@ extern "C" int rbc_run_main(int argc, const char* const* argv) {
     std::vector<double> clen;
     if (!reals(clen, 2, "-clength", true)) return bad();
 
-    enum { INIT_RANDOM, INIT_TEST } init = INIT_RANDOM;
-    for (const char* o : {"-init_rectangle", "-init_dislocation", "-init_isr", "-init_sv", "-init_cr",
-                          "-init_giantspiral"})
-        if (A.opt(o)) return die(std::string(o) + " is not available in this build (use -init_test, -files or the "
-                                                  "random default)");
+    // the theta start (main.cu:90-96: a later option in this list overrides an earlier one)
+    enum { INIT_RANDOM, INIT_TEST, INIT_PATTERN } init = INIT_RANDOM;
+    int pattern = 0;
+    {
+        static const struct { const char* opt; int kind; } pats[] = {
+            {"-init_rectangle", RBC_PATTERN_RECTANGLE}, {"-init_dislocation", RBC_PATTERN_DISLOCATION},
+            {"-init_isr", RBC_PATTERN_ISR},             {"-init_sv", RBC_PATTERN_SV},
+            {"-init_cr", RBC_PATTERN_CR},               {"-init_giantspiral", RBC_PATTERN_GIANTSPIRAL}};
+        for (const auto& pt : pats)
+            if (A.opt(pt.opt)) {
+                init = INIT_PATTERN;
+                pattern = pt.kind;
+            }
+    }
     if (A.opt("-init_test")) init = INIT_TEST;
     double init_mode = clen[0] / 2.0;
     if (!A.get1<double>(init_mode, "-init_mode", false)) return bad();
@@ -389,6 +398,8 @@ extern "C" int rbc_run_main(int argc, const char* const* argv) {
         paths5(infiles, full, ptr);
         if (rbc_read_state_files(c, ptr.data(), 0) != RBC_OK) return cfail("reading the state files");
         std::cout << "...reading init data from binary file finished" << std::endl;
+    } else if (init == INIT_PATTERN) {
+        if (rbc_init_pattern(c, pattern, init_mode, (unsigned long long)seed) != RBC_OK) return cfail("initial pattern");
     } else if (rbc_init_state(c, init == INIT_TEST ? RBC_INIT_TEST : RBC_INIT_RANDOM, (unsigned long long)seed,
                               1e-6) != RBC_OK) {
         return cfail("initial state");

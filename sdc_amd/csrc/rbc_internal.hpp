@@ -162,7 +162,7 @@ struct StepCoef {
 enum UpdateMode { UPD_STEP = 0, UPD_INVB = 1, UPD_IMPLICIT = 2 };
 
 // plane operators on the state's mode planes (freeze_operator.cu, masktemp.cu, masking.cu)
-enum PlaneOp { PLANE_CAPTURE = 0, PLANE_FREEZE = 1, PLANE_HEAT = 2, PLANE_SCALE = 3 };
+enum PlaneOp { PLANE_CAPTURE = 0, PLANE_FREEZE = 1, PLANE_HEAT = 2, PLANE_SCALE = 3, PLANE_SET = 4 };
 
 // one field between an FP64 reference-layout device buffer and the compact field of the
 // context's real type (rsz = 4 or 8); enqueued on s
@@ -203,6 +203,9 @@ struct Launch {
     static void to_ref(const DevGeom& g, const void* const src[3], void* const dst[3], hipStream_t s);
     // initial state of the context's X, XF, XG (kind: 0 zeros, 1 random theta, 2 init_test)
     static void init_state(const DevGeom& g, const Buffers& b, int kind, uint64_t seed, double amp, hipStream_t s);
+    // the real plane of an initial pattern (util/init.cu), [row][col] of M x N, in the real type
+    static void init_pattern(const DevGeom& g, int kind, int mode, double amp, uint64_t seed, void* plane,
+                             hipStream_t s);
     // F_0 = v (force_mean)
     static void force_mean(const Buffers& b, double v, hipStream_t s);
     // plane operator `op` on the first nf fields of X (theta, f, g) in place (PLANE_CAPTURE

@@ -1186,9 +1186,12 @@ __global__ void __launch_bounds__(256) k_rows_planeop(DevGeom g, int op, const i
     const T s = T(1) / (T(M) * T(N));
     for (int x = threadIdx.x; x < M; x += blockDim.x) {
         const T u = r[x].x;
-        const bool keep = imask[(size_t)y * M + x] == 1;
         T v = u;
-        if (!keep) {
+        if (op == PLANE_SET) {
+            // mode plane 0 of field 0 becomes the given real plane (frozen = [row][col]) under the
+            // unnormalised R2C of util/init.cu fft(); the other planes pass through
+            if (m == 0 && f == 0) v = frozen[(size_t)y * M + x] * (T(M) * T(N));
+        } else if (imask[(size_t)y * M + x] != 1) {
             if (op == PLANE_FREEZE) v = fr[x];
             else if (op == PLANE_HEAT) v = u + T(1.5) / T(100);
             else v = u * factor;
@@ -1198,6 +1201,63 @@ __global__ void __launch_bounds__(256) k_rows_planeop(DevGeom g, int op, const i
     __syncthreads();
     cpx<T>* R = lds_fft(r, r == a ? bb : a, g.logM, 1, twM, false);
     for (int kk = threadIdx.x; kk < KX; kk += blockDim.x) hout[roff + kk] = R[kk];
+}
+
+// ---------------------------------------------------------------------------------------
+// Initial patterns of theta (util/init.cu, main.cu:326-352): one real plane, amplitude amp, on
+// the num_x = M by num_y = N grid, point (i, j) at x = i / M, y = j / N (fractions of the box),
+// evaluated in FP64.  kind: 1 rectangle (init_rectangle: x_mode = mode, y_mode 0; the columns
+// i < M / 8 hold uniform noise in [-amp, amp]), 2 dislocation (init_dislocation, x_mode = mode),
+// 3 isr (init_isr, x_mode = mode, one random phase), 4 sv (init_sv, y_mode = mode, x_frac 0.2),
+// 5 cr (init_cr, y_mode = mode, x_frac 0.8), 6 giantspiral (init_giantspiral, mode).  The
+// reference draws the noise and the phase from the C library's rand(); here they come from
+// SplitMix64 on (seed, point) as the random start does.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ double init_uniform(uint64_t seed, uint64_t idx) {
+    const uint64_t h = init_mix64(seed * 0x100000001B3ull + idx);
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_init_pattern(int M, int N, int kind, int mode, double amp, uint64_t seed,
+                                                      T* __restrict__ plane) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)M * N) return;
+    const int i = (int)(e % M), j = (int)(e / M);
+    const double twopi = 2.0 * M_PI, dx = 1.0 / M, dy = 1.0 / N, x = i * dx, y = j * dy;
+    double v = 0.0;
+    if (kind == 1) {  // init.cu:70-100 (x_mode = mode, y_mode = 0)
+        if (i < M / 8) v = (2.0 * init_uniform(seed, 2 * e) - 1.0) * amp;
+        else v = amp * cos(twopi * (i * ((double)mode / M)));
+    } else if (kind == 2) {  // init.cu:22-68: two opposite dislocations on the line x = 1/2 + dx/2
+        const double x1 = (M / 2.0) * dx + dx / 2.0, y1 = (N / 4.0) * dy + dy / 2.0;
+        const double y2 = (3.0 * N / 4.0) * dy + dy / 2.0;
+        v = amp * cos(mode * twopi * x - atan2(y - y1, x - x1) + atan2(y - y2, x - x1));
+    } else if (kind == 3) {  // init.cu:102-129 (x_mode = mode, y_mode = 0)
+        const double phase = twopi * init_uniform(seed, (uint64_t)-1);
+        v = amp * cos(twopi * (i * ((double)mode / M)) + phase);
+    } else if (kind == 4) {  // init.cu:131-165 (x_frac = 0.2)
+        const double xf = 0.2, xl = 0.5 * (1.0 - xf), xr = 1.0 - xl;
+        double ph = -0.5 * sin(twopi * y) - x * x;
+        if (x > xr) ph += 1.0;
+        else if (x > xl) ph += 0.5 * (1.0 - cos(M_PI * (x - xl) / xf));
+        v = amp * cos(twopi * (j * ((double)mode / N) + ph));
+    } else if (kind == 5) {  // init.cu:167-213 (x_frac = 0.8)
+        const double xf = 0.8, x1 = 0.5 * ((1.0 - xf) * M + 1) * dx, x2 = 1.0 - x1, y1 = 0.5 * (N + 1) * dy;
+        v = amp * cos(mode * twopi * y - atan2(y - y1, x - x1) + atan2(y - y1, x - x2));
+    } else if (kind == 6) {  // init.cu:215-242: centre (0.5, 0.5), second core (0.8, 0.5)
+        const double r = sqrt((x - 0.5) * (x - 0.5) + (y - 0.5) * (y - 0.5));
+        v = amp * cos(twopi * mode * r - atan2(y - 0.5, x - 0.5) + atan2(y - 0.5, x - 0.8));
+    }
+    plane[e] = (T)v;
+}
+
+template <typename T>
+void Launch<T>::init_pattern(const DevGeom& g, int kind, int mode, double amp, uint64_t seed, void* plane,
+                             hipStream_t s) {
+    const size_t n = (size_t)g.M * g.N;
+    hipLaunchKernelGGL(k_init_pattern<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g.M, g.N, kind, mode,
+                       amp, seed, (T*)plane);
 }
 
 // temperature at one height from the physical mode planes of theta: sum_i S_i theta_i(x, y),

@@ -70,7 +70,7 @@ EXPORTS = ["rbc_create", "rbc_destroy", "rbc_last_error", "rbc_get_info", "rbc_s
            "rbc_closure_damping", "rbc_closure_step",
            "rbc_closure_status", "rbc_set_state_device", "rbc_get_state_device", "rbc_nonlinear_device",
            "rbc_inverse_B_device", "rbc_implicit_device", "rbc_stream", "rbc_velocity_at_device",
-           "rbc_init_state", "rbc_force_mean", "rbc_characteristic_velocity", "rbc_run_main",
+           "rbc_init_state", "rbc_init_pattern", "rbc_force_mean", "rbc_characteristic_velocity", "rbc_run_main",
            "rbc_extract_vel_main", "rbc_velocity_at_device_noalloc", "rbc_set_penalization", "rbc_freeze",
            "rbc_apply_freeze", "rbc_temp_mask", "rbc_apply_temp_mask", "rbc_mask_theta", "rbc_temperature_grid",
            "rbc_stream_packets"]
@@ -113,6 +113,7 @@ def load_library(path=LIB_PATH):
     L.rbc_stream.argtypes = [vp]
     L.rbc_stream.restype = vp
     L.rbc_init_state.argtypes = [vp, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double]
+    L.rbc_init_pattern.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong]
     L.rbc_characteristic_velocity.argtypes = [vp, ctypes.c_int, dp]
     L.rbc_set_penalization.argtypes = [vp, ctypes.c_double, ctypes.c_int, dp]
     L.rbc_freeze.argtypes = [vp, ctypes.c_int, ctypes.c_double]
@@ -330,6 +331,16 @@ class Stepper:
         distribution, project RNG) or "test" (init_test); make_real applied, history cleared."""
         k = self.INIT_KINDS[kind] if isinstance(kind, str) else int(kind)
         self._check(self.lib.rbc_init_state(self.h, k, int(seed), float(amplitude)), "rbc_init_state")
+
+    PATTERNS = {"rectangle": 1, "dislocation": 2, "isr": 3, "sv": 4, "cr": 5, "giantspiral": 6}
+
+    def init_pattern(self, kind, init_mode=None, seed=0):
+        """Pattern start of -init_rectangle / -init_dislocation / -init_isr / -init_sv / -init_cr /
+        -init_giantspiral (rbc_init_pattern): random theta at 1e-6 with vertical mode 0 replaced by
+        the pattern's unnormalised R2C; init_mode defaults to Lx / 2 as in the reference."""
+        k = self.PATTERNS[kind] if isinstance(kind, str) else int(kind)
+        mode = self.params.Lx / 2.0 if init_mode is None else float(init_mode)
+        self._check(self.lib.rbc_init_pattern(self.h, k, mode, int(seed)), "rbc_init_pattern")
 
     MASKS = {"rectangle": 0, "circle": 1, "diskarray": 2, "everywhere": 3, "custom": 4}
     PLANES = {"circle": 0, "rectangle": 1}

@@ -1,6 +1,6 @@
 """CPU checks of the simulation driver's argument handling (rbc_run_main): the reference
-parser's errors for missing mandatory arguments and the options this build does not offer end
-the run with status 1 before any GPU work."""
+parser's errors for missing mandatory arguments and values this build does not accept end the
+run with status 1 before any GPU work."""
 import ctypes
 
 import pytest
@@ -28,7 +28,7 @@ BASE = ["-nmodes", "32", "32", "8", "-clength", "4", "4", "-ra", "5000", "-pr", 
     [a for a in BASE if a not in ("-ra", "5000")],  # -ra missing
     BASE[:-4] + ["-t", "0.01"],                    # -dt missing
     BASE + ["-precision", "f16"],                  # unknown precision
-    BASE + ["-init_rectangle"],                    # initial pattern not in this build
+    BASE + ["-tracer_mode", "euler"],              # unknown tracer integrator
     BASE + ["-nfiles", "x"],                       # value that does not parse
 ])
 def test_driver_rejects(lib, args):
