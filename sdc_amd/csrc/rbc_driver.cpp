@@ -19,7 +19,8 @@
 //     restores one, so a restart continues with AB2 instead of an Euler step;
 //   - -tracer_mode spectral selects the spectral RK2 tracer (default: the reference's RK4).
 // -freeze_circle n and -temp_circle n run the freeze and heat-source operators before every step
-// from step n on (main.cu:569-590).  The pattern starts (-init_rectangle, -init_dislocation,
+// from step n on (main.cu:569-590).  -penalization rectangle|circle|diskarray|everywhere (an option
+// of this driver) runs the Brinkman terms the PENALIZED builds compile in, with -eta.  The pattern starts (-init_rectangle, -init_dislocation,
 // -init_isr, -init_sv, -init_cr, -init_giantspiral with -init_mode) go through rbc_init_pattern.
 // -port (USE_SERVER)
 // and -snapshot_freq (WRITE_SNAPSHOT) are accepted and ignored, as in the reference's default
@@ -282,7 +283,21 @@ extern "C" int rbc_run_main(int argc, const char* const* argv) {
     if (!A.get1<int>(nfiles, "-nfiles", false)) return bad();
     double eta = 0.01;
     if (!real(eta, "-eta", false)) return bad();
-    eta = 0;  // no PENALIZED build (main.cu:120-122)
+    // -penalization <shape> runs what the PENALIZED builds compile in (run/Makefile: the default
+    // rectangle, -DCIRCLE, -DDISKARRAY) with the -eta given; without it eta is unused, as in the
+    // reference's default build (main.cu:120-122)
+    int pen_shape = -1;
+    {
+        std::vector<std::string> sh;
+        if (A.strings(sh, 1, "-penalization")) {
+            if (sh[0] == "rectangle") pen_shape = RBC_MASK_RECTANGLE;
+            else if (sh[0] == "circle") pen_shape = RBC_MASK_CIRCLE;
+            else if (sh[0] == "diskarray") pen_shape = RBC_MASK_DISKARRAY;
+            else if (sh[0] == "everywhere") pen_shape = RBC_MASK_EVERYWHERE;
+            else return die("-penalization must be rectangle, circle, diskarray or everywhere");
+        }
+    }
+    if (pen_shape < 0) eta = 0;
     std::vector<std::string> infiles;
     const bool have_files = A.strings(infiles, 5, "-files");
     std::vector<std::string> histfiles;
@@ -410,6 +425,7 @@ extern "C" int rbc_run_main(int argc, const char* const* argv) {
         paths5(histfiles, full, ptr);
         if (rbc_read_state_files(c, ptr.data(), 1) != RBC_OK) return cfail("reading the history files");
     }
+    if (pen_shape >= 0 && rbc_set_penalization(c, eta, pen_shape, nullptr) != RBC_OK) return cfail("penalization");
 
     {  // simulation.log (main.cu:401-424)
         std::ofstream log(at("simulation.log"));

@@ -104,3 +104,34 @@ def test_penalized_l32_fp32_vs_oracle():
             e = _cmp(S.calculate_nonlinear(*st), ref, keep)
         print(precision, e)
         assert max(e) < tol, e
+
+
+def test_driver_penalization_option_equals_library(tmp_path):
+    """rbc_run -penalization circle -eta 0.05 (the -DCIRCLE PENALIZED build) writes the same state
+    files as rbc_set_penalization(0.05, circle) and the same steps through the library."""
+    import os
+    import subprocess
+    from sdc_amd import Stepper
+    print("case: driver penalization", flush=True)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    d = str(tmp_path / "run")
+    args = ["-nmodes", "32", "32", "8", "-clength", "4.032", "4.032", "-ra", "5000", "-pr", "1", "-dt", "0.001",
+            "-t", "0.0045", "-nfiles", "1", "-init_test", "-eta", "0.05", "-penalization", "circle",
+            "-precision", "f64", "-dir", d]
+    r = subprocess.run([os.path.join(root, "sdc_amd", "rbc_run")] + args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    with Stepper(32, 32, 8, 4.032, 4.032, 5000.0, 1.0, 1e-3, precision="f64") as S:
+        S.init_state("test")
+        S.set_penalization(0.05, "circle")
+        S.step_time(5)
+        names = [str(tmp_path / f"api_{n}.bin") for n in ("theta", "f", "g", "F", "G")]
+        S.write_state_files(names)
+        plain = Stepper(32, 32, 8, 4.032, 4.032, 5000.0, 1.0, 1e-3, precision="f64")
+        plain.init_state("test")
+        plain.step_time(5)
+        th_plain = plain.get_state()[0]
+        plain.close()
+        th_pen = S.get_state()[0]
+    for n, ref in zip(("theta", "f", "g", "F", "G"), names):
+        assert open(os.path.join(d, f"{n}_00000005.bin"), "rb").read() == open(ref, "rb").read(), n
+    assert np.abs(th_pen - th_plain).max() > 0  # the penalization acted
