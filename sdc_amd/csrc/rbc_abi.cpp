@@ -307,6 +307,13 @@ int rbc_create(const rbc_params* p, rbc_ctx** out) {
             const bool off = no && no[0] == '1';
             if (p->precision == RBC_BF16X3) b.mma = 1;
             else if (p->precision == RBC_F32 && p->L <= 32 && !off) b.mma = 0;
+            // the matrix-core projection addresses one level combination (2 nq P floats) with
+            // 32-bit buffer offsets
+            if (b.mma >= 0 && (double)nq * P * 8.0 >= 2147483648.0) {
+                if (p->precision == RBC_BF16X3)
+                    return fail(nullptr, RBC_ERR_ARG, "BF16X3: grid too large for the 32-bit projection offsets");
+                b.mma = -1;
+            }
             if (b.mma >= 0) {
                 const int K = b.mma == 0 ? 2 : 16, KPL = K / 2;
                 b.ev_nlt = (nq + 31) / 32;

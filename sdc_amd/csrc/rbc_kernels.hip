@@ -1991,27 +1991,35 @@ __device__ __forceinline__ void k_project_mma_body(DevGeom g, int nks, const flo
     const int pc = pvalid ? p : 0;
     const float kx = kxa[pc], ky = kya[pc], k2 = kx * kx + ky * ky;
     const size_t ffs = (size_t)2 * nq * P;  // floats per combination
-    const float* dcol = Dfl + (size_t)2 * pc + comp;
     f32x16 cF1 = {}, cF2 = {}, cG = {}, cT = {};
     const int spw = (nks + KS - 1) / KS, sbeg = wk * spw, send = min(nks, sbeg + spw);
-    // B operands of k-step s; levels past nq are clamped to nq - 1 (their table entries are
-    // zero), so that no load is predicated
-    auto fetch = [&](int s, float (&b)[5][MT::KPL]) {
+    // Operands through buffer loads: a descriptor per level combination (its 2 nq P floats; a
+    // level past nq, whose table entries are zero, lies outside the range and reads as 0) and
+    // one for the tables, so that each load is a 32-bit offset that grows by a constant per
+    // k-step, with no 64-bit address arithmetic.  Byte offsets stay below 2^31 (the launcher
+    // checks the sizes).
+    __amdgpu_buffer_rsrc_t rD[5];
+#pragma unroll
+    for (int t = 0; t < 5; ++t)
+        rD[t] = __builtin_amdgcn_make_buffer_rsrc((void*)(Dfl + t * ffs), (short)0, (int)(ffs * 4), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rA =
+        __builtin_amdgcn_make_buffer_rsrc((void*)prA, (short)0, 5 * nks * 64 * MT::KPL * 4, 0x00020000);
+    const int dbase = (2 * pc + comp) * 4, dstep = 2 * P * 4;  // bytes: column, one level
+    auto fetch = [&](int s, float (&a)[5][MT::KPL], float (&b)[5][MT::KPL]) {
 #pragma unroll
         for (int e = 0; e < MT::KPL; ++e) {
-            const int l = min(s * MT::K + half * MT::KPL + e, nq - 1);
+            const int off = dbase + (s * MT::K + half * MT::KPL + e) * dstep;
 #pragma unroll
-            for (int t = 0; t < 5; ++t) b[t][e] = dcol[t * ffs + (size_t)2 * l * P];
+            for (int t = 0; t < 5; ++t) b[t][e] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rD[t], off, 0, 0));
         }
-    };
-    auto step = [&](int s, const float (&b)[5][MT::KPL]) {
-        float a[5][MT::KPL];
 #pragma unroll
         for (int t = 0; t < 5; ++t) {
-            const float* src = prA + (((size_t)t * nks + s) * 64 + lane) * MT::KPL;
+            const int off = ((t * nks + s) * 64 + lane) * MT::KPL * 4;
 #pragma unroll
-            for (int e = 0; e < MT::KPL; ++e) a[t][e] = src[e];
+            for (int e = 0; e < MT::KPL; ++e) a[t][e] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rA, off + 4 * e, 0, 0));
         }
+    };
+    auto step = [&](const float (&a)[5][MT::KPL], const float (&b)[5][MT::KPL]) {
         MT::run(cF1, a[0], b[0]);  // wC'  . D0
         MT::run(cF1, a[1], b[1]);  // wC'' . D1
         MT::run(cF2, a[2], b[1]);  // wC   . D1   (times k^2 below)
@@ -2019,22 +2027,45 @@ __device__ __forceinline__ void k_project_mma_body(DevGeom g, int nks, const flo
         MT::run(cG, a[4], b[3]);   // wS'  . D3
         MT::run(cT, a[3], b[4]);   // wS   . D4
     };
-    // U k-steps whose operand loads are issued together while whole batches last, then singly
-    constexpr int U = MM == 0 ? 4 : 1;
+    // FP32: batches of U k-steps, the next batch's operands (tables and data, issued together)
+    // loading while the current batch runs on the matrix cores (loads return in issue order,
+    // so a batch waits for its own only).  Two buffers taken in turn over a loop unrolled twice,
+    // so that no buffer is copied (a copy would wait for its loads).  A load past the wave's
+    // range is harmless and unused: a step of another wave's range, or past the end of a
+    // descriptor, which reads as 0.
+    constexpr int U = 2;
     int s = sbeg;
-    for (; s + U <= send; s += U) {
-        float b[U][5][MT::KPL];
+    if constexpr (MM == 0) {
+        float a0[U][5][MT::KPL], b0[U][5][MT::KPL], a1[U][5][MT::KPL], b1[U][5][MT::KPL];
+        auto load = [&](int s0, float (&a)[U][5][MT::KPL], float (&b)[U][5][MT::KPL]) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) fetch(s + u, b[u]);
+            for (int u = 0; u < U; ++u) fetch(s0 + u, a[u], b[u]);
+        };
+        auto run = [&](const float (&a)[U][5][MT::KPL], const float (&b)[U][5][MT::KPL]) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) step(s + u, b[u]);
-    }
-    if constexpr (U > 1)
-        for (; s < send; ++s) {
-            float b[5][MT::KPL];
-            fetch(s, b);
-            step(s, b);
+            for (int u = 0; u < U; ++u) step(a[u], b[u]);
+        };
+        load(s, a0, b0);
+        for (; s + 2 * U <= send; s += 2 * U) {
+            __builtin_amdgcn_sched_barrier(0);
+            load(s + U, a1, b1);
+            __builtin_amdgcn_sched_barrier(0);
+            run(a0, b0);
+            __builtin_amdgcn_sched_barrier(0);
+            load(s + 2 * U, a0, b0);
+            __builtin_amdgcn_sched_barrier(0);
+            run(a1, b1);
         }
+        if (s + U <= send) {  // one more whole batch, already loaded
+            run(a0, b0);
+            s += U;
+        }
+    }
+    for (; s < send; ++s) {
+        float a[5][MT::KPL], b[5][MT::KPL];
+        fetch(s, a, b);
+        step(a, b);
+    }
     if (KS > 1) {
         if (wk > 0) {
             f32x16* dst = sPart[(wk - 1) * WPB + wave][0];
