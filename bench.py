@@ -2,8 +2,9 @@
 
 Workload (one "step" = one IMEX time step of the whole field): 512 x 512 horizontal grid,
 32 vertical Galerkin modes per field, Ra = 1e6, Pr = 0.7, box 4 x 4 x 1, dt = 5e-5, from a
-seeded theta ~ 1e-3 N(0,1) in every kept mode (synthetic input, resident in HBM before
-timing).  Prints one JSON line.
+seeded theta ~ 1e-6 N(0,1) in the lowest 4 x 4 x 2 modes, the other fields zero (synthetic input,
+resident in HBM before timing).  Prints one JSON line; its roofline object also carries the
+rates of a plain 1 GiB fill and copy on the same GPU (--no-hbm-probe skips them).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f32|bf16x3|f64]
                   [--no-cpu] [--cpu-threads T]
@@ -126,6 +127,46 @@ def copy_probe(mib):
     assert hip.hipDeviceSynchronize() == 0
     hip.hipFree(src)
     hip.hipFree(dst)
+
+
+def hbm_probe(mib=1024, reps=20):
+    """What this GPU's HBM does for plain streams: a fill (hipMemsetD32Async, writes only) and a
+    device-to-device copy (reads and writes) of `mib` MiB each, timed with HIP events over `reps`
+    calls; GB/s of bytes moved.  Context for the per-kernel rates beside the published 8 TB/s."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    hip.hipMemsetD32Async.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p]
+    hip.hipMemcpyDtoDAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
+    hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+    hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+    n = mib << 20
+    src, dst, e0, e1 = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), n) == 0 and hip.hipMalloc(ctypes.byref(dst), n) == 0
+    assert hip.hipEventCreate(ctypes.byref(e0)) == 0 and hip.hipEventCreate(ctypes.byref(e1)) == 0
+
+    def timed(call):
+        assert call() == 0 and hip.hipDeviceSynchronize() == 0
+        assert hip.hipEventRecord(e0, None) == 0
+        for _ in range(reps):
+            assert call() == 0
+        assert hip.hipEventRecord(e1, None) == 0 and hip.hipEventSynchronize(e1) == 0
+        ms = ctypes.c_float()
+        assert hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1) == 0
+        return ms.value / reps
+
+    fill = timed(lambda: hip.hipMemsetD32Async(src, 0, n // 4, None))
+    copy = timed(lambda: hip.hipMemcpyDtoDAsync(dst, src, n, None))
+    for p in (src, dst):
+        hip.hipFree(p)
+    for e in (e0, e1):
+        hip.hipEventDestroy(e)
+    return {"MiB": mib, "fill_GBs": round(n / fill / 1e6, 1), "copy_GBs": round(2 * n / copy / 1e6, 1),
+            "method": "hipMemsetD32Async and hipMemcpyDtoDAsync, HIP events, %d calls each" % reps}
 
 
 def host_cpu_info():
@@ -613,6 +654,8 @@ def main():
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MB",
                     help="also run one device-to-device copy of MB mebibytes (a kernel of known traffic "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--no-hbm-probe", action="store_true",
+                    help="skip timing a 1 GiB fill and copy (the measured HBM rates beside the published peak)")
     ap.add_argument("--ensemble", action="store_true", help="run the batched-ensemble workload instead")
     ap.add_argument("--closure", action="store_true", help="run the learned-closure experiment instead")
     ap.add_argument("--closure-small", action="store_true",
@@ -805,6 +848,8 @@ def main():
             sv.close()
     if a.copy_calibrate > 0:
         copy_probe(a.copy_calibrate)
+    if not a.no_hbm_probe:
+        roofline["hbm_measured"] = hbm_probe()
 
     cpu = None
     if not a.no_cpu:
