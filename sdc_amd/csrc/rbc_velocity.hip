@@ -67,6 +67,19 @@ __host__ __device__ inline void basis_at(int m, double lam, double z, double& C,
     S = sin(M_PI * k * (z + 0.5));
 }
 
+// Basis values at the points' heights for the FP32 point evaluations, once per evaluation:
+// zb[i][m] = (C'_m, C_m, S_m, 0) at z_i, basis_at in FP64 rounded to FP32 -- the values the
+// point kernels would otherwise compute in every mode slice.
+__global__ void __launch_bounds__(256) k_point_basis(int np, int L, const float* __restrict__ pz,
+                                                     const double* __restrict__ lam, float4* __restrict__ zb) {
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (size_t)np * L) return;
+    const int i = (int)(e / L), m = (int)(e % L);
+    double C = 0, Cp = 0, S = 0;
+    basis_at(m, lam[m], (double)pz[i], C, Cp, S);
+    zb[e] = make_float4((float)Cp, (float)C, (float)S, 0.f);
+}
+
 // ---------------------------------------------------------------------------------------
 // Spectral velocity at nz heights: spec[c][l][p] for c = u, v, w (complex), thread per mode.
 // zb[l][3][L] = C'_m, C_m, S_m at height l.
@@ -387,7 +400,7 @@ __global__ void __launch_bounds__(kVelMmaThreads) k_vel_points_mma(
     DevGeom g, int np, const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
     const cpx<float>* __restrict__ Xf, const cpx<float>* __restrict__ Xg, const cpx<float>* __restrict__ XF,
     const cpx<float>* __restrict__ XG, const double* __restrict__ lam, const float4* __restrict__ modes,
-    const int* __restrict__ pmap, int npad, int tiles_per_slice, float* __restrict__ part) {
+    const int* __restrict__ pmap, int npad, int tiles_per_slice, float* __restrict__ part, const float4* __restrict__ zb) {
     static_assert(LMAX % 4 == 0 && LMAX <= 32, "k-steps of 4 modes");
     constexpr int KS = LMAX / 4;
     constexpr int PER = (kTile * LMAX + kVelMmaThreads - 1) / kVelMmaThreads;  // staged values per thread
@@ -405,11 +418,23 @@ __global__ void __launch_bounds__(kVelMmaThreads) k_vel_points_mma(
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
         const int m = 4 * s + q;
-        double C = 0, Cp = 0, S = 0;
-        if (m < L) basis_at(m, lam[m], (double)z, C, Cp, S);
-        bcp[s] = (float)Cp;
-        bcc[s] = (float)C;
-        bss[s] = (float)S;
+        bcp[s] = bcc[s] = bss[s] = 0.f;
+        if (m < L) {
+            if (zb) {  // computed once per evaluation by k_point_basis
+                if (live) {
+                    const float4 t = zb[(size_t)i * L + m];
+                    bcp[s] = t.x;
+                    bcc[s] = t.y;
+                    bss[s] = t.z;
+                }
+            } else {
+                double C = 0, Cp = 0, S = 0;
+                basis_at(m, lam[m], (double)z, C, Cp, S);
+                bcp[s] = (float)Cp;
+                bcc[s] = (float)C;
+                bss[s] = (float)S;
+            }
+        }
     }
     const float xr = x / (float)g.Lx - floorf(x / (float)g.Lx);
     const float yr = y / (float)g.Ly - floorf(y / (float)g.Ly);
@@ -529,7 +554,7 @@ __global__ void __launch_bounds__(kVelMmaThreads) k_vel_points_bf3(
     DevGeom g, int np, const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
     const cpx<float>* __restrict__ Xf, const cpx<float>* __restrict__ Xg, const cpx<float>* __restrict__ XF,
     const cpx<float>* __restrict__ XG, const double* __restrict__ lam, const float4* __restrict__ modes,
-    const int* __restrict__ pmap, int npad, int tiles_per_slice, float* __restrict__ part) {
+    const int* __restrict__ pmap, int npad, int tiles_per_slice, float* __restrict__ part, const float4* __restrict__ zb) {
     static_assert(LMAX % 8 == 0 && LMAX <= 32, "KPL = LMAX / 4 even, at most 8");
     constexpr int KPL = LMAX / 4;
     constexpr int RS = LMAX == 24 ? 40 : LMAX + 8;  // LDS row (BF16 values): 8-byte aligned, spreads the banks
@@ -550,16 +575,30 @@ __global__ void __launch_bounds__(kVelMmaThreads) k_vel_points_bf3(
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         const int m = KPL * q + e;
-        double C = 0, Cp = 0, S = 0;
-        if (e < KPL && m < L) basis_at(m, lam[m], (double)z, C, Cp, S);
-        const float val[3] = {(float)Cp, (float)C, (float)S};
+        float val[3] = {0.f, 0.f, 0.f};
+        if (e < KPL && m < L) {
+            if (zb) {  // computed once per evaluation by k_point_basis
+                if (live) {
+                    const float4 t = zb[(size_t)i * L + m];
+                    val[0] = t.x;
+                    val[1] = t.y;
+                    val[2] = t.z;
+                }
+            } else {
+                double C = 0, Cp = 0, S = 0;
+                basis_at(m, lam[m], (double)z, C, Cp, S);
+                val[0] = (float)Cp;
+                val[1] = (float)C;
+                val[2] = (float)S;
+            }
+        }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const unsigned short h = bf16_bits(val[k]);
             bh[k][e] = __builtin_bit_cast(__bf16, h);
             bl[k][e] = (__bf16)(val[k] - bf16_value(h));
         }
-        if (e < KPL) ssf[e] = (float)S;
+        if (e < KPL) ssf[e] = val[2];
     }
     const float xr = x / (float)g.Lx - floorf(x / (float)g.Lx);
     const float yr = y / (float)g.Ly - floorf(y / (float)g.Ly);
@@ -1033,6 +1072,7 @@ struct PointWork {
     const int* pmap = nullptr;
     int npad = 0;
     float* part = nullptr;
+    float4* zb = nullptr;  // FP32 states: np x L basis values at the points (null: in the kernel)
     int nslice = 1, tiles_per_slice = 1;
 };
 
@@ -1046,8 +1086,10 @@ void plan_points(rbc_ctx* c, int np, PointWork& pw) {
     // particles per block: 128 for an FP32 state (k_vel_points_mma), 256 for FP64 (k_vel_points)
     const int per = c->rsz == 4 ? kVelMmaThreads / 4 : 256;
     const int nblk = (np + per - 1) / per;
-    // enough blocks to give every SIMD of the chip several waves: 2048 blocks
-    const int want = 2048;
+    // enough blocks to give every SIMD of the chip several waves: 4096 blocks for an FP32 state
+    // (1e5 tracers at 128^2 x 16, basis values computed once per evaluation: 2048 blocks 585,
+    // 4096 610, 8192 599 tracer steps/s), 2048 for FP64
+    const int want = c->rsz == 4 ? 4096 : 2048;
     int ns = (want + nblk - 1) / nblk;
     ns = std::max(1, std::min(std::min(16, ns), ntile));
     pw.tiles_per_slice = (ntile + ns - 1) / ns;
@@ -1107,6 +1149,8 @@ struct VelWork {
     std::vector<void*> gw_owned;
     float* part = nullptr;    // rbc_velocity_at partial sums
     size_t part_cap = 0;
+    float4* zbas = nullptr;   // rbc_velocity_at basis values at the points (FP32 states)
+    size_t zbas_cap = 0;
     float* io = nullptr;      // rbc_velocity_at points and results, 6 n floats
     size_t io_cap = 0;
     double* cpart = nullptr;  // characteristic velocity: per block (sum, count)
@@ -1115,6 +1159,7 @@ struct VelWork {
     ~VelWork() {
         for (void* p : gw_owned) hipFree(p);
         if (part) hipFree(part);
+        if (zbas) hipFree(zbas);
         if (io) hipFree(io);
         for (void* p : owned) hipFree(p);
     }
@@ -1143,6 +1188,12 @@ void alloc_point_work(rbc_ctx* c, int np, PointWork& pw, std::vector<void*>& own
     HIPCHK(hipMalloc(&p, std::max<size_t>(16, (size_t)pw.nslice * 3 * np * sizeof(float))));
     owned.push_back(p);
     pw.part = (float*)p;
+    if (c->rsz == 4) {
+        void* q = nullptr;
+        HIPCHK(hipMalloc(&q, std::max<size_t>(16, (size_t)np * c->dg.L * sizeof(float4))));
+        owned.push_back(q);
+        pw.zb = (float4*)q;
+    }
 }
 
 // grows a device buffer to at least `bytes` (contents not kept)
@@ -1163,17 +1214,23 @@ void launch_points_t(rbc_ctx* c, int np, const float* x, const float* y, const f
                      float* u, float* v, float* w) {
     const Buffers& b = c->b;
     if constexpr (sizeof(T) == 4) {  // FP32 state: the vertical sums on the matrix cores
+        const float4* zb = pw.zb;
+        if (zb) {
+            const size_t ne = (size_t)np * c->dg.L;
+            hipLaunchKernelGGL(k_point_basis, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, c->stream, np, c->dg.L,
+                               z, (const double*)pw.lam, pw.zb);
+        }
         dim3 grid((np + kVelMmaThreads / 4 - 1) / (kVelMmaThreads / 4), pw.nslice);
         if (c->p.precision == RBC_BF16X3)  // BF16 hi/lo split products, as the step's vertical transforms
             hipLaunchKernelGGL((k_vel_points_bf3<LM>), grid, dim3(kVelMmaThreads), 0, c->stream, c->dg, np, x, y, z,
                                (const cpx<float>*)b.X[1], (const cpx<float>*)b.X[2], (const cpx<float>*)b.XF,
                                (const cpx<float>*)b.XG, (const double*)pw.lam, pw.pmodes, pw.pmap, pw.npad,
-                               pw.tiles_per_slice, pw.part);
+                               pw.tiles_per_slice, pw.part, zb);
         else
             hipLaunchKernelGGL((k_vel_points_mma<LM>), grid, dim3(kVelMmaThreads), 0, c->stream, c->dg, np, x, y, z,
                                (const cpx<float>*)b.X[1], (const cpx<float>*)b.X[2], (const cpx<float>*)b.XF,
                                (const cpx<float>*)b.XG, (const double*)pw.lam, pw.pmodes, pw.pmap, pw.npad,
-                               pw.tiles_per_slice, pw.part);
+                               pw.tiles_per_slice, pw.part, zb);
     } else {
         dim3 grid((np + 255) / 256, pw.nslice);
         hipLaunchKernelGGL((k_vel_points<T, LM>), grid, dim3(256), 0, c->stream, c->dg, np, x, y, z,
@@ -1577,10 +1634,12 @@ int rbc_velocity_at(rbc_ctx* c, int n, const float* x, const float* y, const flo
         HIPCHK(hipStreamSynchronize(c->stream));  // earlier calls may still use the buffers
         grow(vw->io, vw->io_cap, (size_t)6 * n);
         grow(vw->part, vw->part_cap, part_floats(c, n));
+        if (c->rsz == 4) grow(vw->zbas, vw->zbas_cap, (size_t)n * c->dg.L);
         PointWork pw;
         plan_points(c, n, pw);
         point_constants(c, pw);
         pw.part = vw->part;
+        pw.zb = vw->zbas;
         float* d = vw->io;
         const size_t b = (size_t)n * sizeof(float);
         HIPCHK(hipMemcpyAsync(d, x, b, hipMemcpyHostToDevice, c->stream));
@@ -1633,14 +1692,17 @@ int rbc_velocity_at_device(rbc_ctx* c, int n, const float* x, const float* y, co
         hipSetDevice(c->p.device);
         VelWork* vw = vel_work(c);
         const size_t need = part_floats(c, n);
-        if (need > vw->part_cap) {
-            HIPCHK(hipStreamSynchronize(c->stream));  // the old buffer may be in use
+        const size_t nzb = c->rsz == 4 ? (size_t)n * c->dg.L : 0;
+        if (need > vw->part_cap || nzb > vw->zbas_cap) {
+            HIPCHK(hipStreamSynchronize(c->stream));  // the old buffers may be in use
             grow(vw->part, vw->part_cap, need);
+            if (nzb) grow(vw->zbas, vw->zbas_cap, nzb);
         }
         PointWork pw;
         plan_points(c, n, pw);
         point_constants(c, pw);
         pw.part = vw->part;
+        pw.zb = vw->zbas;
         velocity_points(c, n, x, y, z, pw, u, v, w);
     } catch (const HipError& e) {
         return hip_fail(c, e);
