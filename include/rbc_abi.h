@@ -297,7 +297,11 @@ int rbc_velocity_at_device_noalloc(rbc_ctx* c, int n, const float* x, const floa
 #define RBC_PART_PAIRS 4
 #define RBC_PART_COORD_SIGMA 5
 /* ≙ particle_tracer(M, N, L, Np, buffer, cube_length, Le) (particle_tracer.cu:7-44): all
- * particles start at the centre. */
+ * particles start at the centre.  Device memory is allocated here, once: the position ring
+ * (3 buffer_size num_particles floats) and the mode's work -- RK4: the 128-level velocity grid;
+ * spectral RK2: velocities and midpoints (6 num_particles floats), the partial sums of the mode
+ * slices and, for FP32 / BF16X3 contexts, the basis values at the particle heights
+ * (16 L num_particles bytes). */
 int rbc_tracer_create(rbc_ctx* c, int num_particles, int buffer_size, int mode, double diff_coeff,
                       unsigned long long seed, rbc_tracer** out);
 void rbc_tracer_destroy(rbc_tracer* t);

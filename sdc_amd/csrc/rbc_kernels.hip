@@ -1996,8 +1996,8 @@ __device__ __forceinline__ void k_project_mma_body(DevGeom g, int nks, const flo
     // Operands through buffer loads: a descriptor per level combination (its 2 nq P floats; a
     // level past nq, whose table entries are zero, lies outside the range and reads as 0) and
     // one for the tables, so that each load is a 32-bit offset that grows by a constant per
-    // k-step, with no 64-bit address arithmetic.  Byte offsets stay below 2^31 (the launcher
-    // checks the sizes).
+    // k-step, with no 64-bit address arithmetic.  Byte offsets stay below 2^31 (rbc_create
+    // keeps the matrix-core forms only for grids where 2 nq P floats fit).
     __amdgpu_buffer_rsrc_t rD[5];
 #pragma unroll
     for (int t = 0; t < 5; ++t)
