@@ -50,21 +50,41 @@ __device__ inline float rng_normal(uint64_t seed, uint64_t ctr) {
 
 // ---- basis values at one height, FP64 (coeff.h:17-36; the clamp of get_z_coeff_at,
 //      calculate_velocity_operator.cu:462-466) --------------------------------------------
-__host__ __device__ inline void basis_at(int m, double lam, double z, double& C, double& Cp, double& S) {
+// The mode's constants: cosh and cos of lam / 2 (odd k = m + 1), sinh and sin (even k).
+__host__ __device__ inline void basis_consts(int m, double lam, double& c1, double& c2) {
+    if ((m + 1) % 2) {
+        c1 = cosh(0.5 * lam);
+        c2 = cos(0.5 * lam);
+    } else {
+        c1 = sinh(0.5 * lam);
+        c2 = sin(0.5 * lam);
+    }
+}
+// The height-dependent part: cosh and sinh of lam z from one exponential (|lam z| <= lam / 2, no
+// overflow), cos and sin from one sincos.
+__host__ __device__ inline void basis_at_c(int m, double lam, double c1, double c2, double z, double& C, double& Cp,
+                                           double& S) {
     if (z < -0.5) z = -0.5;
     else if (z > 0.5) z = 0.5;
     const int k = m + 1;
     const double lz = lam * z;
+    const double ex = exp(lz), iex = 1.0 / ex;
+    const double chz = 0.5 * (ex + iex), shz = 0.5 * (ex - iex);
+    double sz, cz;
+    sincos(lz, &sz, &cz);
     if (k % 2) {
-        const double ch = cosh(0.5 * lam), c = cos(0.5 * lam);
-        C = cosh(lz) / ch - cos(lz) / c;
-        Cp = lam * (sinh(lz) / ch + sin(lz) / c);
+        C = chz / c1 - cz / c2;
+        Cp = lam * (shz / c1 + sz / c2);
     } else {
-        const double sh = sinh(0.5 * lam), s = sin(0.5 * lam);
-        C = sinh(lz) / sh - sin(lz) / s;
-        Cp = lam * (cosh(lz) / sh - cos(lz) / s);
+        C = shz / c1 - sz / c2;
+        Cp = lam * (chz / c1 - cz / c2);
     }
     S = sin(M_PI * k * (z + 0.5));
+}
+__host__ __device__ inline void basis_at(int m, double lam, double z, double& C, double& Cp, double& S) {
+    double c1, c2;
+    basis_consts(m, lam, c1, c2);
+    basis_at_c(m, lam, c1, c2, z, C, Cp, S);
 }
 
 // Basis values at the points' heights for the FP32 point evaluations, once per evaluation:
@@ -72,12 +92,23 @@ __host__ __device__ inline void basis_at(int m, double lam, double z, double& C,
 // point kernels would otherwise compute in every mode slice.
 __global__ void __launch_bounds__(256) k_point_basis(int np, int L, const float* __restrict__ pz,
                                                      const double* __restrict__ lam, float4* __restrict__ zb) {
+    // thread: mode m = e % L of the points g * 8 .. g * 8 + 7, g = e / L (the mode's constants once)
+    constexpr int R = 8;
     const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= (size_t)np * L) return;
-    const int i = (int)(e / L), m = (int)(e % L);
-    double C = 0, Cp = 0, S = 0;
-    basis_at(m, lam[m], (double)pz[i], C, Cp, S);
-    zb[e] = make_float4((float)Cp, (float)C, (float)S, 0.f);
+    const size_t ng = ((size_t)np + R - 1) / R;
+    if (e >= ng * L) return;
+    const int m = (int)(e % L);
+    const size_t g0 = e / L * R;
+    const double lm = lam[m];
+    double c1, c2;
+    basis_consts(m, lm, c1, c2);
+    for (int r = 0; r < R; ++r) {
+        const size_t i = g0 + r;
+        if (i >= (size_t)np) break;
+        double C, Cp, S;
+        basis_at_c(m, lm, c1, c2, (double)pz[i], C, Cp, S);
+        zb[i * L + m] = make_float4((float)Cp, (float)C, (float)S, 0.f);
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1216,7 +1247,7 @@ void launch_points_t(rbc_ctx* c, int np, const float* x, const float* y, const f
     if constexpr (sizeof(T) == 4) {  // FP32 state: the vertical sums on the matrix cores
         const float4* zb = pw.zb;
         if (zb) {
-            const size_t ne = (size_t)np * c->dg.L;
+            const size_t ne = ((size_t)np + 7) / 8 * c->dg.L;  // threads: (8 points, mode)
             hipLaunchKernelGGL(k_point_basis, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, c->stream, np, c->dg.L,
                                z, (const double*)pw.lam, pw.zb);
         }
