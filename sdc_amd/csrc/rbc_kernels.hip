@@ -599,44 +599,40 @@ __device__ __forceinline__ void k_update32_body(DevGeom g, int nchunk, int L0, i
     const bool tf = r < Lc, gl = lane < L;
     const int m = tf ? cls + 2 * r : 0;  // theta/f mode of this lane (element 0 outside the class)
     const int mg = gl ? lane : 0;        // g mode of this lane
-    const cpx<T> z = mk<T>(0, 0);
-    // all loads of one member are issued together; the next member's are in flight while
-    // the current one is computed
-    cpx<T> nf = z, xt = z, xf = z, nt = z, rt0 = z, rf0 = z, ng = z, xg = z, rg0 = z;
-    T k2 = T(0);
-    size_t base = 0;
-    auto fetch = [&](int mi, cpx<T>& nf_, cpx<T>& xt_, cpx<T>& xf_, cpx<T>& nt_, cpx<T>& rt_, cpx<T>& rf_,
-                     cpx<T>& ng_, cpx<T>& xg_, cpx<T>& rg_, T& k2_, size_t& base_) {
-        const int p = gmem[mi];
-        base_ = (size_t)p * L;
-        k2_ = k2a[p];
-        nf_ = Nf[base_ + m];
-        xt_ = Xt[base_ + m];
-        xf_ = Xf[base_ + m];
-        nt_ = Nt[base_ + m];
-        rt_ = Rt[base_ + m];  // read also without history (then unused)
-        rf_ = Rf[base_ + m];
-        ng_ = Ng[base_ + mg];
-        xg_ = Xg[base_ + mg];
-        rg_ = Rg[base_ + mg];
+    // one member's loads (issued together)
+    struct Member {
+        cpx<T> nf, xt, xf, nt, rt, rf, ng, xg, rg;
+        T k2;
+        size_t base;
     };
-    if (b0 < b1) fetch(b0, nf, xt, xf, nt, rt0, rf0, ng, xg, rg0, k2, base);
-    for (int mi = b0; mi < b1; ++mi) {
-        cpx<T> nnf = z, nxt = z, nxf = z, nnt = z, nrt = z, nrf = z, nng = z, nxg = z, nrg = z;
-        T nk2 = T(0);
-        size_t nbase = 0;
-        if (mi + 1 < b1) fetch(mi + 1, nnf, nxt, nxf, nnt, nrt, nrf, nng, nxg, nrg, nk2, nbase);
-        if (tf) snf[r] = nf;
+    auto fetch = [&](int mi, Member& d) {
+        const int p = gmem[mi];
+        d.base = (size_t)p * L;
+        d.k2 = k2a[p];
+        d.nf = Nf[d.base + m];
+        d.xt = Xt[d.base + m];
+        d.xf = Xf[d.base + m];
+        d.nt = Nt[d.base + m];
+        d.rt = Rt[d.base + m];  // read also without history (then unused)
+        d.rf = Rf[d.base + m];
+        d.ng = Ng[d.base + mg];
+        d.xg = Xg[d.base + mg];
+        d.rg = Rg[d.base + mg];
+    };
+    auto update = [&](const Member& d) {
+        const size_t base = d.base;
+        const T k2 = d.k2;
+        if (tf) snf[r] = d.nf;
         wave_sync();
         if (tf) {
             cpx<T> rf = mk<T>(0, 0);
 #pragma unroll
             for (int j = 0; j < NC / 2; ++j) rf = rf + cscale(snf[j], brow[j]);
-            const cpx<T> rt = cscale(nt, T(2));
-            cpx<T> x1 = xt + cscale(rt, a_cur), x2 = xf + cscale(rf, a_cur);
+            const cpx<T> rt = cscale(d.nt, T(2));
+            cpx<T> x1 = d.xt + cscale(rt, a_cur), x2 = d.xf + cscale(rf, a_cur);
             if (use_prev) {
-                x1 = x1 + cscale(rt0, a_prev);
-                x2 = x2 + cscale(rf0, a_prev);
+                x1 = x1 + cscale(d.rt, a_prev);
+                x2 = x2 + cscale(d.rf, a_prev);
             }
             Rt[base + m] = rt;
             Rf[base + m] = rf;
@@ -655,16 +651,32 @@ __device__ __forceinline__ void k_update32_body(DevGeom g, int nchunk, int L0, i
         // g: diagonal factors, one vertical mode per lane
         if (gl) {
             const T mpi = T(lane + 1) * T(M_PI);
-            const cpx<T> rg = (k2 != T(0)) ? cscale(ng, T(-2) * Pr / k2) : mk<T>(0, 0);
-            cpx<T> x3 = xg + cscale(rg, a_cur);
-            if (use_prev) x3 = x3 + cscale(rg0, a_prev);
+            const cpx<T> rg = (k2 != T(0)) ? cscale(d.ng, T(-2) * Pr / k2) : mk<T>(0, 0);
+            cpx<T> x3 = d.xg + cscale(rg, a_cur);
+            if (use_prev) x3 = x3 + cscale(d.rg, a_prev);
             Rg[base + lane] = rg;
             Xg[base + lane] = (k2 != T(0)) ? cscale(x3, T(1) / (T(1) + dt * Pr * (k2 + mpi * mpi))) : mk<T>(0, 0);
         }
         wave_sync();
-        nf = nnf; xt = nxt; xf = nxf; nt = nnt; rt0 = nrt; rf0 = nrf; ng = nng; xg = nxg; rg0 = nrg;
-        k2 = nk2;
-        base = nbase;
+    };
+    // Members of a chunk (at most kMaxChunk) two ahead: member mi + 2's loads are in flight while
+    // member mi is computed.  Three register sets taken in turn over a fully unrolled loop, so
+    // that no set is copied (a copy would wait for its loads); loads return in issue order, so
+    // member mi waits for its own only.
+    constexpr int kMaxChunk = 4;
+    Member d[3];
+    if (b0 < b1) fetch(b0, d[0]);
+    if (b0 + 1 < b1) fetch(b0 + 1, d[1]);
+#pragma unroll
+    for (int u = 0; u < kMaxChunk; ++u) {
+        if (b0 + u >= b1) break;
+        if (b0 + u + 2 < b1 && u + 2 < kMaxChunk) fetch(b0 + u + 2, d[(u + 2) % 3]);
+        update(d[u % 3]);
+    }
+    for (int mi = b0 + kMaxChunk; mi < b1; ++mi) {  // longer chunks (the host makes none): in turn
+        Member e;
+        fetch(mi, e);
+        update(e);
     }
 }
 
