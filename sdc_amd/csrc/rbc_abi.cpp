@@ -327,9 +327,12 @@ int rbc_create(const rbc_params* p, rbc_ctx** out) {
                             for (int ln = 0; ln < 64; ++ln)
                                 for (int e = 0; e < KPL; ++e) {
                                     const int l = lt * 32 + (ln & 31), m = s * K + KPL * (ln >> 5) + e;
-                                    if (l < nq && m < (int)L)
-                                        eA[((((size_t)t * b.ev_nlt + lt) * b.ev_nks + s) * 64 + ln) * KPL + e] =
-                                            B(evt[t], m, l);
+                                    // FP32 (KPL = 1): the four tables interleaved, one float4 per (tile,
+                                    // k-step, lane); BF16X3: [table][tile][k-step][lane][KPL]
+                                    const size_t at =
+                                        b.mma == 0 ? ((((size_t)lt * b.ev_nks + s) * 64 + ln) * 4 + t)
+                                                   : ((((size_t)t * b.ev_nlt + lt) * b.ev_nks + s) * 64 + ln) * KPL + e;
+                                    if (l < nq && m < (int)L) eA[at] = B(evt[t], m, l);
                                 }
                 std::vector<double> pA((size_t)5 * b.pr_nks * 64 * KPL, 0.0);
                 static const int prt[5] = {1, 2, 0, 4, 5};  // w C', w C'', w C, w S, w S'

@@ -1793,8 +1793,9 @@ struct Mma<1> {
 // Evaluation on the matrix cores: D[level][column] = sum_m A_t[level][m] B_f[m][column], with
 // columns = (mode, re/im) of 16 modes per wave and 4 tables C', C, S, S' applied to f, f, g,
 // theta, theta (5 chains).  Block = 64 modes (4 waves); the block's mode data is staged in
-// LDS as [field][m][128 columns].  evA: [4][NLT][NKS][64][KPL] floats; ALDS: the block's A
-// operands are copied to LDS with the modes (when both fit in 64 KiB).
+// LDS as [field][m][128 columns].  evA: [NLT][NKS][64][4 tables] floats for FP32 (KPL = 1, one
+// float4 per lane and k-step), [4][NLT][NKS][64][KPL] for BF16X3; ALDS: the block's A operands are
+// copied to LDS with the modes (when both fit in 64 KiB).
 template <int MM, bool ALDS>
 __device__ __forceinline__ void k_eval_mma_body(DevGeom g, int nlt, int nks, int per,
                                                 const cpx<float>* __restrict__ Xt,
@@ -1820,11 +1821,16 @@ __device__ __forceinline__ void k_eval_mma_body(DevGeom g, int nlt, int nks, int
     // A operands of the block's level tile, [4][nks][64][KPL], copied next to the modes
     float* sA = sX + 3 * 128 * LP;
     if (ALDS) {
-        const int na = nks * 64 * MT::KPL / 4;  // float4 per table
-        for (int e = threadIdx.x; e < 4 * na; e += 256) {
-            const int t = e / na, i = e - t * na;
-            reinterpret_cast<float4*>(sA)[e] =
-                reinterpret_cast<const float4*>(evA + ((size_t)t * nlt + lt_block) * nks * 64 * MT::KPL)[i];
+        if constexpr (MM == 0) {  // the tile's interleaved tables are one contiguous piece
+            const float4* src = reinterpret_cast<const float4*>(evA) + (size_t)lt_block * nks * 64;
+            for (int e = threadIdx.x; e < nks * 64; e += 256) reinterpret_cast<float4*>(sA)[e] = src[e];
+        } else {
+            const int na = nks * 64 * MT::KPL / 4;  // float4 per table
+            for (int e = threadIdx.x; e < 4 * na; e += 256) {
+                const int t = e / na, i = e - t * na;
+                reinterpret_cast<float4*>(sA)[e] =
+                    reinterpret_cast<const float4*>(evA + ((size_t)t * nlt + lt_block) * nks * 64 * MT::KPL)[i];
+            }
         }
     }
     // mean flow u, v (sum_m C_m F_m, sum_m C_m G_m) at the block's 32 levels, for mode p = 0
@@ -1882,12 +1888,21 @@ __device__ __forceinline__ void k_eval_mma_body(DevGeom g, int nlt, int nks, int
         f32x16 cFp = {}, cFc = {}, cGs = {}, cTs = {}, cTz = {};
         for (int s = 0; s < nks; ++s) {
             float a[4][MT::KPL], bt[MT::KPL], bf[MT::KPL], bg[MT::KPL];
+            if constexpr (MM == 0) {  // FP32: one float4 holds the lane's value of the four tables
+                const float4 av = ALDS ? reinterpret_cast<const float4*>(sA)[s * 64 + lane]
+                                       : reinterpret_cast<const float4*>(evA)[((size_t)lt * nks + s) * 64 + lane];
+                a[0][0] = av.x;
+                a[1][0] = av.y;
+                a[2][0] = av.z;
+                a[3][0] = av.w;
+            } else {
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const float* src = ALDS ? sA + (((size_t)t * nks + s) * 64 + lane) * MT::KPL
-                                        : evA + ((((size_t)t * nlt + lt) * nks + s) * 64 + lane) * MT::KPL;
+                for (int t = 0; t < 4; ++t) {
+                    const float* src = ALDS ? sA + (((size_t)t * nks + s) * 64 + lane) * MT::KPL
+                                            : evA + ((((size_t)t * nlt + lt) * nks + s) * 64 + lane) * MT::KPL;
 #pragma unroll
-                for (int e = 0; e < MT::KPL; ++e) a[t][e] = src[e];
+                    for (int e = 0; e < MT::KPL; ++e) a[t][e] = src[e];
+                }
             }
 #pragma unroll
             for (int e = 0; e < MT::KPL; ++e) {
