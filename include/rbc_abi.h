@@ -39,7 +39,9 @@ extern "C" {
  *   RBC_F64    : every GPU stage in FP64 (parity form; PRECISION_DOUBLE, cuda_defines.h:9-27)
  *   RBC_F32    : FFTs, products, vertical transforms and implicit apply in FP32
  *   RBC_BF16X3 : as RBC_F32, but the vertical transforms run as BF16 hi/lo split MFMA
- *                products accumulated in FP32
+ *                products accumulated in FP32 (L <= 32, and one level combination of
+ *                2 nq P floats below 2^31 bytes; rbc_create refuses larger grids with
+ *                RBC_ERR_ARG, where RBC_F32 falls back to VALU vertical transforms)
  * Per-wavenumber factors are always built in FP64 on the host. */
 typedef enum { RBC_F64 = 0, RBC_F32 = 1, RBC_BF16X3 = 2 } rbc_precision;
 
