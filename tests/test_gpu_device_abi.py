@@ -14,7 +14,9 @@ from tests.helpers import random_state, rel_err
 pytestmark = pytest.mark.gpu
 
 CASES = [(dict(M=32, N=32, L=8, Lx=4.032, Ly=4.032, Ra=5000.0, Pr=1.0, dt=1e-3), "f64", 1e-12),
-         (dict(M=64, N=32, L=16, Lx=4.0, Ly=2.5, Ra=3e4, Pr=0.7, dt=2e-4), "f32", 2e-5)]
+         (dict(M=64, N=32, L=16, Lx=4.0, Ly=2.5, Ra=3e4, Pr=0.7, dt=2e-4), "f32", 2e-5),
+         # BF16 hi/lo split vertical transforms (relative error about 2^-16 per product)
+         (dict(M=64, N=32, L=16, Lx=4.0, Ly=2.5, Ra=3e4, Pr=0.7, dt=2e-4), "bf16x3", 2e-4)]
 
 
 def _dev(arrays, precision):
@@ -33,7 +35,7 @@ def _like(st, precision, fill=1.0):
     return [DeviceArray.full(a.shape, dt, fill) for a in st]
 
 
-@pytest.mark.parametrize("cfg,precision,tol", CASES, ids=["32x32x8-f64", "64x32x16-f32"])
+@pytest.mark.parametrize("cfg,precision,tol", CASES, ids=["32x32x8-f64", "64x32x16-f32", "64x32x16-bf16x3"])
 def test_device_state_and_operators(cfg, precision, tol):
     from sdc_amd import Stepper
     from tests.hipmem import synchronize

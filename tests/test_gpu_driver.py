@@ -105,7 +105,7 @@ def _api_files(tmp, precision, steps, every, mean_F=0.0, init="test"):
     return out
 
 
-@pytest.mark.parametrize("precision,mean_F", [("f64", 0.0), ("f32", 0.05)])
+@pytest.mark.parametrize("precision,mean_F", [("f64", 0.0), ("f32", 0.05), ("bf16x3", 0.0)])
 def test_driver_files_equal_library_run(tmp_path, precision, mean_F):
     print("case: driver files", precision, mean_F, flush=True)
     d = str(tmp_path / "run")

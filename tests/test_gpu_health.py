@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 CFG = (32, 32, 8, 4.032, 4.032, 5000.0, 1.0, 1e-3)
 
 
-@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("precision", ["f64", "f32", "bf16x3"])
 def test_nan_in_one_theta_coefficient_is_reported(precision):
     print("case: NaN injection", precision, flush=True)
     from sdc_amd import RbcNumericError, Stepper

@@ -2,7 +2,8 @@
 vertical mode 0 of theta against oracle/patterns.py (the unnormalised R2C of the restated plane),
 the other modes against the plain random start of the same seed, f, g, F, G zero; and the
 driver's -init_<pattern> -init_mode options against the same start through the library.
-Tolerances: FP64 1e-12, FP32 2e-6, relative to the largest kept coefficient."""
+Tolerances: FP64 1e-12, FP32 and BF16X3 (FP32 storage; the start involves no vertical
+transform) 2e-6, relative to the largest kept coefficient."""
 import os
 import subprocess
 
@@ -19,7 +20,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CFG = (64, 32, 8, 4.0, 2.0, 2e4, 0.7, 1e-3)
 
 
-@pytest.mark.parametrize("precision,tol", [("f64", 1e-12), ("f32", 2e-6)])
+@pytest.mark.parametrize("precision,tol", [("f64", 1e-12), ("f32", 2e-6), ("bf16x3", 2e-6)])
 @pytest.mark.parametrize("kind", list(PT.KINDS))
 def test_pattern_start_vs_oracle(kind, precision, tol):
     print("case: pattern start", kind, precision, flush=True)
