@@ -71,3 +71,57 @@ def test_single_modes_evolve_as_on_a_small_grid(precision, tol):
     print("largest other coefficient", leak)
     assert leak < 1e-4 * np.abs(a).max(), leak
     assert np.abs(out[3]).max() < 1e-4 * AMP and np.abs(out[4]).max() < 1e-4 * AMP
+
+
+# The nonlinear term of a state made of two horizontal modes lives on the wavenumbers k1 +- k2,
+# 2 k1, 2 k2 and 0, all of which both grids keep, and the collocation form is exact for such
+# band-limited input: the 512^2 x 32 term must equal the oracle's at 32^2 x 32 on every
+# wavenumber of the small grid and vanish elsewhere.  Tolerances relative to the largest
+# coefficient of each field: FP64 1e-12, FP32 2e-5, BF16X3 2e-4 (as the operator tests); the
+# "elsewhere" bound is 10 x that in FP32 and BF16X3: the rounding of the 512-point transforms
+# spreads over every wavenumber, and the f and g equations weight it by wavenumber factors up
+# to |k| ~ 270 against |k| ~ 5 of the signal (measured: f 1.9e-5, g 8.1e-5 in FP32).
+TWO = [(1, 1), (-2, 3)]  # (ky, kx), both kx > 0
+
+
+def _two_mode_state(M, N, seed=5):
+    mox = M // 2 + 1
+    fields = []
+    for c in range(3):
+        a = np.zeros((L, N, mox), np.complex128)
+        z = gaussian_complex(seed + c, L * len(TWO)).reshape(len(TWO), L) * 1e-2
+        for i, (ky, kx) in enumerate(TWO):
+            a[:, ky % N, kx] = z[i]
+        fields.append(a)
+    mean = gaussian_complex(seed + 3, 2 * L).reshape(2, L).real * 1e-2
+    return fields + [mean[0].astype(np.complex128), mean[1].astype(np.complex128)]
+
+
+@pytest.mark.parametrize("precision,tol", [("f64", 1e-12), ("f32", 2e-5), ("bf16x3", 2e-4)])
+def test_two_mode_nonlinear_term_matches_the_oracle_on_a_small_grid(precision, tol):
+    from oracle import oracle as O
+    from sdc_amd import Stepper
+    print(f"case: two-mode nonlinear term 512x512x32 {precision} vs oracle 32x32x32", flush=True)
+    O.build()
+    o = O.Stepper(32, 32, L, *PHYS, nthreads=16)
+    ref = o.nonlinear(*_two_mode_state(32, 32))
+    o.close()
+    with Stepper(512, 512, L, *PHYS, precision=precision) as S:
+        out = S.calculate_nonlinear(*_two_mode_state(512, 512))
+    keep32 = kept_mask(32, 32)
+    rows = np.arange(32)
+    ky = np.where(rows < 16, rows, rows - 32)
+    errs, leaks = [], []
+    for c in range(3):
+        big = out[c][:, ky % 512, :17]  # the small grid's wavenumbers inside the large grid
+        scale = np.abs(ref[c]).max()
+        assert scale > 0
+        errs.append(np.abs(big[:, keep32] - ref[c][:, keep32]).max() / scale)
+        rest = out[c].copy()
+        rest[:, (ky % 512)[:, None], np.arange(17)[None, :]] = 0
+        leaks.append(np.abs(rest).max() / scale)
+    for c in (3, 4):
+        errs.append(np.abs(out[c] - ref[c]).max() / max(np.abs(ref[c]).max(), 1e-300))
+    print("errors (theta, f, g, F, G)", errs, "elsewhere", leaks)
+    assert max(errs) < tol, errs
+    assert max(leaks) < (tol if precision == "f64" else 10 * tol), leaks
