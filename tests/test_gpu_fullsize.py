@@ -125,3 +125,47 @@ def test_two_mode_nonlinear_term_matches_the_oracle_on_a_small_grid(precision, t
     print("errors (theta, f, g, F, G)", errs, "elsewhere", leaks)
     assert max(errs) < tol, errs
     assert max(leaks) < (tol if precision == "f64" else 10 * tol), leaks
+
+
+# The spectral velocity at arbitrary points (the tracer evaluation, matrix-core kernels with
+# L = 32) of a state band-limited to the small grid's kept modes is the same Fourier sum on
+# either grid: rbc_velocity_at at 512^2 x 32 against oracle/velocity.py on the 32^2 x 32 state,
+# 5,000 random points.  Tolerances relative to the largest velocity component: 2e-5 for FP64
+# and FP32 states (the point evaluation runs in FP32 for both: float positions and velocities,
+# as the tracers keep them; measured 2e-6), 2e-4 for BF16X3.
+@pytest.mark.parametrize("precision,tol", [("f64", 2e-5), ("f32", 2e-5), ("bf16x3", 2e-4)])
+def test_point_velocity_matches_the_oracle_on_a_small_grid(precision, tol):
+    from oracle import velocity as V
+    from sdc_amd import Stepper
+    print(f"case: point velocity 512x512x32 {precision} vs oracle 32x32x32", flush=True)
+    keep = kept_mask(32, 32)
+    rows, cols = np.nonzero(keep)
+    pick = np.random.default_rng(8).choice(rows.size, 24, replace=False)
+    sel = [(int(rows[i]), int(cols[i])) for i in pick if cols[i] > 0]  # kx > 0: no mirrored partner
+    ky = [r if r < 16 else r - 32 for r, _ in sel]
+
+    def state(M, N):
+        mox = M // 2 + 1
+        out = []
+        for c in range(3):
+            a = np.zeros((L, N, mox), np.complex128)
+            z = gaussian_complex(40 + c, L * len(sel)).reshape(len(sel), L) * 1e-2
+            for i, (k, (_, kx)) in enumerate(zip(ky, sel)):
+                a[:, k % N, kx] = z[i]
+            out.append(a)
+        mean = gaussian_complex(43, 2 * L).reshape(2, L).real * 1e-2
+        return out + [mean[0].astype(np.complex128), mean[1].astype(np.complex128)]
+
+    rng = np.random.default_rng(9)
+    n = 5000
+    x = rng.uniform(0, PHYS[0], n).astype(np.float32)
+    y = rng.uniform(0, PHYS[1], n).astype(np.float32)
+    z = rng.uniform(-0.5, 0.5, n).astype(np.float32)
+    ref = V.velocity_points(state(32, 32), 32, 32, PHYS[0], PHYS[1], x.astype(np.float64), y.astype(np.float64),
+                            z.astype(np.float64))
+    with Stepper(512, 512, L, *PHYS, precision=precision) as S:
+        S.set_state(*state(512, 512))
+        got = S.velocity_at(x, y, z)
+    errs = [np.abs(a - b).max() / np.abs(b).max() for a, b in zip(got, ref)]
+    print("u, v, w errors", errs)
+    assert max(errs) < tol, errs
