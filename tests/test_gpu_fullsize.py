@@ -1,4 +1,7 @@
-"""Parity at the full 512 x 512 x 32 workload through a size-independent property.
+"""Parity at the full 512 x 512 x 32 workload through size-independent properties: states
+band-limited to a small grid's wavenumbers give the same operators, nonlinear term, step and
+point velocity on any grid that keeps those wavenumbers, so the oracle at 32^2 x 32 is the
+reference for the 512^2 x 32 GPU path.
 
 A horizontal Fourier mode of amplitude 1e-12 evolves linearly: its products with itself and
 with the other seeded modes are smaller by many orders of magnitude (also with the velocity
@@ -169,3 +172,76 @@ def test_point_velocity_matches_the_oracle_on_a_small_grid(precision, tol):
     errs = [np.abs(a - b).max() / np.abs(b).max() for a, b in zip(got, ref)]
     print("u, v, w errors", errs)
     assert max(errs) < tol, errs
+
+
+# The linear operators act mode by mode, and one step of a state band-limited to |kx|, |ky| <= 5
+# forms products up to |k| <= 10, which the 32^2 grid keeps: inverse B, the implicit operator
+# and one (Euler) step at 512^2 x 32 must equal the oracle's at 32^2 x 32 on every wavenumber of
+# the small grid (the 512^2 factor records group many more wavenumbers per |k|^2: this pins
+# their grouping at the full size) and vanish elsewhere.  Tolerances relative to each field's
+# largest coefficient: operators FP64 1e-13, FP32 and BF16X3 2e-6 (factors rounded to FP32);
+# one step FP64 1e-10, FP32 2e-5, BF16X3 1e-4.
+def _band_limited_32(seed=12):
+    from tests.helpers import random_state
+    st = random_state(32, 32, L, seed=seed, amp=1e-2, decay=0.05)
+    rows = np.arange(32)
+    ky = np.where(rows < 16, rows, rows - 32)
+    cut = (np.abs(ky)[:, None] > 5) | (np.arange(17)[None, :] > 5)
+    for a in st[:3]:
+        a[:, cut] = 0
+    return st
+
+
+def _to_512(st32):
+    rows = np.arange(32)
+    ky = np.where(rows < 16, rows, rows - 32)
+    out = []
+    for a in st32[:3]:
+        b = np.zeros((L, 512, 257), np.complex128)
+        b[:, ky % 512, :17] = a
+        out.append(b)
+    return out + [st32[3].copy(), st32[4].copy()]
+
+
+def _cmp_512_32(out, ref):
+    rows = np.arange(32)
+    ky = np.where(rows < 16, rows, rows - 32)
+    keep = kept_mask(32, 32)
+    errs, leaks = [], []
+    for c in range(3):
+        scale = np.abs(ref[c]).max()
+        big = out[c][:, ky % 512, :17]
+        errs.append(np.abs(big[:, keep] - ref[c][:, keep]).max() / scale)
+        rest = out[c].copy()
+        rest[:, (ky % 512)[:, None], np.arange(17)[None, :]] = 0
+        leaks.append(np.abs(rest).max() / scale)
+    for c in (3, 4):
+        errs.append(np.abs(out[c] - ref[c]).max() / max(np.abs(ref[c]).max(), 1e-300))
+    return errs, leaks
+
+
+@pytest.mark.parametrize("precision,tol_op,tol_step", [("f64", 1e-13, 1e-10), ("f32", 2e-6, 2e-5),
+                                                       ("bf16x3", 2e-6, 1e-4)])
+def test_operators_and_one_step_match_the_oracle_on_a_small_grid(precision, tol_op, tol_step):
+    from oracle import oracle as O
+    from sdc_amd import Stepper
+    print(f"case: operators and one step 512x512x32 {precision} vs oracle 32x32x32", flush=True)
+    O.build()
+    st32 = _band_limited_32()
+    o = O.Stepper(32, 32, L, *PHYS, nthreads=16)
+    ref_b, ref_i = o.inverse_B(*st32), o.implicit(*st32)
+    o.set_state(*st32)
+    o.step(1)
+    ref_s = o.get_state()
+    o.close()
+    st = _to_512(st32)
+    with Stepper(512, 512, L, *PHYS, precision=precision) as S:
+        eb, lb = _cmp_512_32(S.calculate_inverse(*st), ref_b)
+        ei, li = _cmp_512_32(S.calculate_implicit(*st), ref_i)
+        S.set_state(*st)
+        S.step_time(1)
+        es, ls = _cmp_512_32(S.get_state(), ref_s)
+    print("inverse B", eb, lb, "implicit", ei, li, "one step", es, ls)
+    assert max(eb + ei) < tol_op and max(lb + li) == 0, (eb, ei, lb, li)
+    assert max(es) < tol_step, es
+    assert max(ls) < (tol_step if precision == "f64" else 10 * tol_step), ls
