@@ -364,6 +364,18 @@ int rbc_create(const rbc_params* p, rbc_ctx** out) {
         b.twN = c->dalloc(tN.size() * c->rsz);
         upload_reals(c.get(), b.twM, tM);
         upload_reals(c.get(), b.twN, tN);
+        if (c->rsz == 4 && p->M == 512 && p->N == 512) {
+            // the per-pass table of the 512-point kernels: w^(j r) at 64 r + j (r < 8, j < 64),
+            // w^(8 j r) at 512 + 8 r + j (r, j < 8), w = exp(-2 pi i / 512)
+            std::vector<double> tP((size_t)2 * 576);
+            for (int e = 0; e < 576; ++e) {
+                const int k = e < 512 ? (e & 63) * (e >> 6) : 8 * (e & 7) * ((e - 512) >> 3);
+                tP[2 * e] = tM[2 * k];
+                tP[2 * e + 1] = tM[2 * k + 1];
+            }
+            b.twP = c->dalloc(tP.size() * c->rsz);
+            upload_reals(c.get(), b.twP, tP);
+        }
         b.kx = c->dalloc(P * c->rsz);
         b.ky = c->dalloc(P * c->rsz);
         b.k2 = c->dalloc(P * c->rsz);

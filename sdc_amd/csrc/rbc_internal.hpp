@@ -104,6 +104,8 @@ struct Buffers {
     void* prtab;  // nq * 5 * L : w*C', w*C'', w*C, w*S, w*S'  [l][t][m]
     void* twM;    // M complex: exp(-2 pi i k / M), k < M
     void* twN;    // N complex
+    void* twP = nullptr;  // at M = N = 512 (FP32 storage): 576 complex, the 512-point twiddles in the
+                          // per-pass order of the 512-point kernels (TwByPass)
     void* kx;     // P real
     void* ky;     // P real
     void* k2;     // P real

@@ -1434,7 +1434,7 @@ __device__ __forceinline__ int hyb512(int y, int kx) { return (kx >> 3) * (512 *
 // d formed before any rounding of the larger ww (which would swamp uu - vv).  The second
 // field of every pair is scaled to the size of the first for the transform (pair_scale).
 __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, int nblk, int per, const cpx<float>* __restrict__ hin,
-                                                  const cpx<float>* __restrict__ twM, cpx<float>* __restrict__ hout) {
+                                                  const cpx<float>* __restrict__ twp, cpx<float>* __restrict__ hout) {
     __shared__ cpx<float> tw[TwByPass::size];
     __shared__ cpx<float> zs[3][XorMap::slots];
     // one (level, y) row per block; consecutive rows (which share the 128-byte lines of the
@@ -1447,7 +1447,9 @@ __global__ void __launch_bounds__(192) k_rows512t(DevGeom g, int nblk, int per, 
     const int y = lb % 512, l = lb / 512;
     const size_t fstride = (size_t)nq * 512 * KXT512;
     const size_t roff = (size_t)l * 512 * KXT512 + (size_t)y * 8;
-    for (int e = threadIdx.x; e < TwByPass::size; e += 192) tw[e] = twM[TwByPass::source(e)];
+    // twp: the twiddles already in TwByPass order (built at create), so the copy is plain: the
+    // index arithmetic of TwByPass::source on every block cost 1.7 % of the kernel
+    for (int e = threadIdx.x; e < TwByPass::size; e += 192) tw[e] = twp[e];
     __syncthreads();
     {
         const cpx<float>* x1 = hin + (size_t)(2 * pr) * fstride + roff;
@@ -1701,7 +1703,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) k
     constexpr int NYc = NYC512;
     constexpr int a = (NYc - 1) / 2, shift = 512 - NYc;  // kept rows [0, a] U [512 - a, 511]
     const int P = g.P, KX = g.KX;
-    for (int e = threadIdx.x; e < TwByPass::size; e += 512) twl[e] = tw[TwByPass::source(e)];
+    for (int e = threadIdx.x; e < TwByPass::size; e += 512) twl[e] = tw[e];  // tw: in TwByPass order
     lds_barrier();
     const int range = (nblk + 7) / 8, xcd = blockIdx.x & 7;
     const int end = min(nblk, (xcd + 1) * range);
@@ -2268,7 +2270,7 @@ __global__ void __launch_bounds__(64 * CT) k_fft_y_fwd512c(DevGeom g, int ntile,
     const int lb = xcd_logical_block(blockIdx.x, per);
     if (lb >= nblk) return;
     if (TWL)
-        for (int e = threadIdx.x; e < TwByPass::size; e += 64 * CT) twl[e] = tw[TwByPass::source(e)];
+        for (int e = threadIdx.x; e < TwByPass::size; e += 64 * CT) twl[e] = tw[e];  // tw: in TwByPass order
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     constexpr int NYc = NYC512;
     const int P = g.P, nq = g.nq, KX = g.KX;
@@ -2316,7 +2318,7 @@ __global__ void __launch_bounds__(64 * CT) k_fft_y_fwd512c(DevGeom g, int ntile,
 #pragma unroll
         for (int r = 0; r < 8; ++r) v[r] = tile[XorMap::col(lane, r) * CS + c];
         if constexpr (TWL) fft512_wave<float, false, CS, XorMap, TwByPass>(v, tile + c, twl, lane);
-        else fft512_wave<float, false, CS, XorMap>(v, tile + c, tw, lane);
+        else fft512_wave<float, false, CS, XorMap, TwByPass>(v, tile + c, tw, lane);
 #pragma unroll
         for (int r = 0; r < 8; ++r) G::add(iq, v[r], kxv, kyv[r], d0[r], d1[r], d2[r]);
         if ((GRP == 1 || GRP == 3) && iq < 2 && kxc == 0 && lane == 0) fgw[2 * l + iq] = v[0];  // P_uw, P_vw, row 0
@@ -2537,7 +2539,7 @@ void Launch<T>::fft_y_inv(const DevGeom& g, const Buffers& b, hipStream_t s) {
         static const int resident = resident_blocks_per_xcd((const void*)k_fft_y_inv512, 512);
         const int bpx = std::max(1, std::min(resident, (nblk + 7) / 8));
         hipLaunchKernelGGL(k_fft_y_inv512, dim3(8 * bpx), dim3(512), 0, s, g, ntile, nblk, bpx,
-                           (const cpx<float>*)b.lev, (const cpx<float>*)b.twN, (cpx<float>*)b.hin);
+                           (const cpx<float>*)b.lev, (const cpx<float>*)b.twP, (cpx<float>*)b.hin);
         return;
     }
     if (wave_y_log<T>(g) && launch_y_inv_group(g, 6, (const cpx<float>*)b.lev, (const cpx<float>*)b.twN,
@@ -2555,7 +2557,7 @@ void Launch<T>::rows(const DevGeom& g, const Buffers& b, hipStream_t s) {
     if (fast512_rows<T>(g)) {
         const int nblk = g.N * g.nq, per = (nblk + 7) / 8;
         hipLaunchKernelGGL(k_rows512t, dim3(8 * per), dim3(192), 0, s, g, nblk, per, (const cpx<float>*)b.hin,
-                           (const cpx<float>*)b.twM, (cpx<float>*)b.hout);
+                           (const cpx<float>*)b.twP, (cpx<float>*)b.hout);
         return;
     }
     if (const int lg = wave_rows_log<T>(g)) {
@@ -2605,7 +2607,7 @@ void Launch<T>::fft_y_fwd(const DevGeom& g, const Buffers& b, hipStream_t s) {
         const int ntile = (g.KX + CT - 1) / CT, nblk = ntile * g.nq, per = (nblk + 7) / 8;
 #define RBC_YF512(GRPV)                                                                                          \
     hipLaunchKernelGGL((k_fft_y_fwd512c<GRPV, CT>), dim3(8 * per), dim3(64 * CT), 0, s, g, ntile, nblk, per,     \
-                       (const cpx<float>*)b.hout, (const cpx<float>*)b.twN, (const float*)b.kx,                  \
+                       (const cpx<float>*)b.hout, (const cpx<float>*)b.twP, (const float*)b.kx,                  \
                        (const float*)b.ky, (cpx<float>*)b.lev, (cpx<float>*)b.fgw)
         RBC_YF512(0);
         RBC_YF512(3);
