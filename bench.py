@@ -82,7 +82,7 @@ def fft_product_stage(st, stages, cs):
 # kernels of each stage on the 512 x 512 FP32 path (BF16X3 runs the <1> variants of the two
 # matrix-core kernels, with the same memory traffic)
 STAGE_KERNELS = {"eval": ["k_eval_mma<0, false>"], "fft_y_inv": ["k_fft_y_inv512"], "rows_products": ["k_rows512t"],
-                 "fft_y_fwd": ["k_fft_y_fwd512c<0, 8>", "k_fft_y_fwd512c<3, 8>"],
+                 "fft_y_fwd": ["k_fft_y_fwd512c2<8>"],
                  "project": ["k_project_mma<0, 4, 1>"], "update": ["k_update32<float, 32>"],
                  "finish": ["k_finish<float>"]}
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_512x512x32_f32.md")

@@ -2254,21 +2254,21 @@ struct YGroup<3> {  // groups 1 and 2 in one pass: uw, vw, u.grad(theta) -> D1, 
     }
 };
 
+// One block of the fused forward y-FFT: logical block lb = (level, 8-column tile) of group GRP.
+// tile: the block's LDS tile (XorMap::slots * (CT + 1) complex); twl: LDS room for the
+// per-pass twiddle table (used by groups of several products).
 template <int GRP, int CT>
-__global__ void __launch_bounds__(64 * CT) k_fft_y_fwd512c(DevGeom g, int ntile, int nblk, int per,
-                                                       const cpx<float>* __restrict__ hout,
-                                                       const cpx<float>* __restrict__ tw,
-                                                       const float* __restrict__ kxa, const float* __restrict__ kya,
-                                                       cpx<float>* __restrict__ D, cpx<float>* __restrict__ fgw) {
+__device__ __forceinline__ void y_fwd512c_body(int lb, const DevGeom& g, int ntile,
+                                               const cpx<float>* __restrict__ hout,
+                                               const cpx<float>* __restrict__ tw,
+                                               const float* __restrict__ kxa, const float* __restrict__ kya,
+                                               cpx<float>* __restrict__ D, cpx<float>* __restrict__ fgw,
+                                               cpx<float>* tile, cpx<float>* twl) {
     using G = YGroup<GRP>;
     constexpr int CS = CT + 1;  // tile row stride
     // groups of several products read the twiddles from LDS: the barrier between products keeps
     // the compiler from holding the table entries in registers across the transforms
     constexpr bool TWL = G::n > 1;
-    __shared__ cpx<float> tile[XorMap::slots * CS];
-    __shared__ cpx<float> twl[TWL ? TwByPass::size : 1];
-    const int lb = xcd_logical_block(blockIdx.x, per);
-    if (lb >= nblk) return;
     if (TWL)
         for (int e = threadIdx.x; e < TwByPass::size; e += 64 * CT) twl[e] = tw[e];  // tw: in TwByPass order
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -2335,6 +2335,26 @@ __global__ void __launch_bounds__(64 * CT) k_fft_y_fwd512c(DevGeom g, int ntile,
                 if (G::t2 >= 0) o[G::t2 * fs] = d2[r];
             }
     }
+}
+
+// Both groups of three products in one launch (group 0: h, d, uv -> D0, D2; group 3: uw, vw,
+// u.grad(theta) -> D1, D3, D4): logical blocks [0, nblk) are group 0, [nblk, 2 nblk) group 3.
+// At 116-120 registers two 8-wave blocks fit a CU, so one launch of a group is 4.1 rounds of
+// blocks; one grid over both leaves a single partial round instead of two and no drain between.
+template <int CT>
+__global__ void __launch_bounds__(64 * CT) k_fft_y_fwd512c2(DevGeom g, int ntile, int nblk, int per,
+                                                        const cpx<float>* __restrict__ hout,
+                                                        const cpx<float>* __restrict__ tw,
+                                                        const float* __restrict__ kxa,
+                                                        const float* __restrict__ kya, cpx<float>* __restrict__ D,
+                                                        cpx<float>* __restrict__ fgw) {
+    static_assert(YGroup<0>::n > 1 && YGroup<3>::n > 1, "both groups keep the twiddles in LDS");
+    __shared__ cpx<float> tile[XorMap::slots * (CT + 1)];
+    __shared__ cpx<float> twl[TwByPass::size];
+    const int lb = xcd_logical_block(blockIdx.x, per);
+    if (lb >= 2 * nblk) return;
+    if (lb < nblk) y_fwd512c_body<0, CT>(lb, g, ntile, hout, tw, kxa, kya, D, fgw, tile, twl);
+    else y_fwd512c_body<3, CT>(lb - nblk, g, ntile, hout, tw, kxa, kya, D, fgw, tile, twl);
 }
 
 // Forward y-FFT at N = 64, 128, 256 fused with the level combinations (FP32): blockIdx.z = the
@@ -2604,14 +2624,10 @@ void Launch<T>::fft_y_fwd(const DevGeom& g, const Buffers& b, hipStream_t s) {
         // launches of three products each: group 0 (h, d, uv -> D0, D2) and group 3 (uw, vw,
         // u.grad(theta) -> D1, D3, D4)
         constexpr int CT = 8;
-        const int ntile = (g.KX + CT - 1) / CT, nblk = ntile * g.nq, per = (nblk + 7) / 8;
-#define RBC_YF512(GRPV)                                                                                          \
-    hipLaunchKernelGGL((k_fft_y_fwd512c<GRPV, CT>), dim3(8 * per), dim3(64 * CT), 0, s, g, ntile, nblk, per,     \
-                       (const cpx<float>*)b.hout, (const cpx<float>*)b.twP, (const float*)b.kx,                  \
-                       (const float*)b.ky, (cpx<float>*)b.lev, (cpx<float>*)b.fgw)
-        RBC_YF512(0);
-        RBC_YF512(3);
-#undef RBC_YF512
+        const int ntile = (g.KX + CT - 1) / CT, nblk = ntile * g.nq, per = (2 * nblk + 7) / 8;
+        hipLaunchKernelGGL((k_fft_y_fwd512c2<CT>), dim3(8 * per), dim3(64 * CT), 0, s, g, ntile, nblk, per,
+                           (const cpx<float>*)b.hout, (const cpx<float>*)b.twP, (const float*)b.kx,
+                           (const float*)b.ky, (cpx<float>*)b.lev, (cpx<float>*)b.fgw);
         return;
     }
     if (wave_y_log<T>(g) && g.NYc == kept_rows(g.N)) {
