@@ -2338,7 +2338,9 @@ __device__ __forceinline__ void y_fwd512c_body(int lb, const DevGeom& g, int nti
 }
 
 // Both groups of three products in one launch (group 0: h, d, uv -> D0, D2; group 3: uw, vw,
-// u.grad(theta) -> D1, D3, D4): logical blocks [0, nblk) are group 0, [nblk, 2 nblk) group 3.
+// u.grad(theta) -> D1, D3, D4): even logical blocks are group 0, odd ones group 3, so that every
+// XCD (a contiguous range of logical blocks) carries both groups, whose traffic differs (306
+// and 353 MB at 512^2 x 32).
 // At 116-120 registers two 8-wave blocks fit a CU, so one launch of a group is 4.1 rounds of
 // blocks; one grid over both leaves a single partial round instead of two and no drain between.
 template <int CT>
@@ -2353,8 +2355,8 @@ __global__ void __launch_bounds__(64 * CT) k_fft_y_fwd512c2(DevGeom g, int ntile
     __shared__ cpx<float> twl[TwByPass::size];
     const int lb = xcd_logical_block(blockIdx.x, per);
     if (lb >= 2 * nblk) return;
-    if (lb < nblk) y_fwd512c_body<0, CT>(lb, g, ntile, hout, tw, kxa, kya, D, fgw, tile, twl);
-    else y_fwd512c_body<3, CT>(lb - nblk, g, ntile, hout, tw, kxa, kya, D, fgw, tile, twl);
+    if (lb & 1) y_fwd512c_body<3, CT>(lb >> 1, g, ntile, hout, tw, kxa, kya, D, fgw, tile, twl);
+    else y_fwd512c_body<0, CT>(lb >> 1, g, ntile, hout, tw, kxa, kya, D, fgw, tile, twl);
 }
 
 // Forward y-FFT at N = 64, 128, 256 fused with the level combinations (FP32): blockIdx.z = the
